@@ -1,0 +1,211 @@
+"""Benchmark of the Lloyd hot path on one MI355X.
+
+Workload (the configuration the metric is quoted on): plain Lloyd, FP32,
+n=10,000,000 samples, d=32, k=256, 256 Gaussian blobs with per-blob std
+uniform in [0.5, 2.0] and centers uniform in [-20, 20]^32, seeded
+random-sample init, tol=0.  A "step" is one Lloyd iteration (half-norms,
+fused assignment+update, reduction, division, shifts) over the full sample
+set resident in HBM.  The timed region is one kmeans_lloyd_driver_ex call
+with max_iter = --steps (tol = 0, so exactly that many iterations run, plus
+the driver's epilogue: final labels and inertia), bracketed by device
+synchronisations.
+
+Output: one JSON line with the metric, the roofline of the dominant kernel
+(the fused assignment+update step, timed with HIP events inside the timed
+region on the driver's stream) and a CPU baseline (the oracle's C
+restatement on a bounded sample of the same workload, all host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = "Lloyd iters/s + % HBM/MFMA roofline; served fits/s with p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
+
+N, D, K = 10_000_000, 32, 256
+SEED = 1001
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "bf16"])
+    p.add_argument("--n", type=int, default=N)
+    p.add_argument("--d", type=int, default=D)
+    p.add_argument("--k", type=int, default=K)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    return p.parse_args()
+
+
+def make_data(n, d, k, seed, device):
+    """Blobs generated on the GPU in the reference's transposed layout."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    centers = torch.rand((k, d), generator=g, device=device) * 40.0 - 20.0
+    std = torch.rand((k,), generator=g, device=device) * 1.5 + 0.5
+    blob = torch.randint(0, k, (n,), generator=g, device=device)
+    X_t = torch.empty((d, n), dtype=torch.float32, device=device)
+    chunk = 1 << 22
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        b = blob[s:e]
+        z = torch.randn((e - s, d), generator=g, device=device)
+        X_t[:, s:e] = (centers[b] + z * std[b, None]).T
+    init_idx = torch.randperm(n, generator=g, device=device)[:k]
+    init_t = X_t[:, init_idx].contiguous()
+    return X_t, init_t
+
+
+def traffic_from_profile(path):
+    """HBM bytes per fused-step launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(X_t, init_t, n, d, k, budget_s):
+    """The oracle's C restatement (OpenMP, all host cores) on a bounded sample
+    of the same workload; Lloyd iterations/s scaled to the full n."""
+    import numpy as np
+    here = os.path.dirname(os.path.abspath(__file__))
+    if here not in sys.path:
+        sys.path.insert(0, here)
+    from oracle import c_oracle
+    cores = len(os.sched_getaffinity(0))
+    threads = int(os.environ.get("OMP_NUM_THREADS", cores))
+    threads = max(1, min(threads, cores))
+    n_s = min(n, 1_000_000)
+    Xs = X_t[:, :n_s].cpu().numpy().copy()
+    Cs = init_t.cpu().numpy().copy()
+    w = np.ones(n_s, dtype=np.float32)
+    t0 = time.perf_counter()
+    c_oracle.lloyd(Xs, w, Cs, 0.0, 1, nthreads=threads)
+    t1 = time.perf_counter() - t0          # one iteration + epilogue (~2 passes)
+    iters = max(1, min(50, int(budget_s / max(t1 / 2, 1e-3)) - 1))
+    t0 = time.perf_counter()
+    _, _, it, _ = c_oracle.lloyd(Xs, w, Cs, 0.0, iters, nthreads=threads)
+    el = time.perf_counter() - t0
+    per_iter_full = (el / (it + 1)) * (n / n_s)   # epilogue costs ~one pass
+    return {
+        "value": 1.0 / per_iter_full,
+        "unit": "iters/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"C restatement of the reference (oracle/restate.c), FP32, "
+                   f"{it} Lloyd iterations + final assignment on the first "
+                   f"{n_s:,} of the {n:,} samples (d={d}, k={k}) in {el:.1f} s; "
+                   f"scaled linearly to n={n:,}"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the Lloyd "
+              f"path is single-GPU in this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import kmeans_dpcpp_amd as kd
+
+    dev = torch.device("cuda", 0)
+    n, d, k = args.n, args.d, args.k
+    X_t, init_t = make_data(n, d, k, SEED, dev)
+    w = torch.ones(n, dtype=torch.float32, device=dev)
+    labels = torch.empty(n, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    # warmup: W iterations (untimed)
+    c_a = init_t.clone()
+    c_b = torch.empty_like(c_a)
+    if args.warmup > 0:
+        kd.kmeans_lloyd_driver_ex(X_t, w, c_a, labels, c_b, 0.0, False, args.warmup,
+                                  args.precision, stream)
+    torch.cuda.synchronize()
+
+    # timed: exactly --steps iterations from the seeded init, tol = 0
+    c_a = init_t.clone()
+    c_b = torch.empty_like(c_a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n_iter, inertia, stats = kd.kmeans_lloyd_driver_ex(
+        X_t, w, c_a, labels, c_b, 0.0, False, args.steps, args.precision, stream)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    assert n_iter == args.steps, (n_iter, args.steps)
+
+    iters_per_s = n_iter / el
+    ms_step = 1e3 * el / n_iter
+    kern_ms = stats["fused_kernel_ms"] / max(1, stats["fused_launches"])
+    bytes_per_launch = n * d * 4 + n * 4 + n * 4      # X, weights, labels
+    flops_per_launch = 2.0 * n * k * d + 2.0 * n * d
+    hbm_gbs = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    tflops = flops_per_launch / (kern_ms * 1e-3) / 1e12
+    if args.precision == "fp32":
+        compute_peak = FP32_PEAK_TFLOPS
+    else:
+        compute_peak = BF16_PEAK_TFLOPS
+    t_hbm = bytes_per_launch / (HBM_PEAK_GBS * 1e9)
+    t_cmp = flops_per_launch / (compute_peak * 1e12)
+    here = os.path.dirname(os.path.abspath(__file__))
+    traffic = traffic_from_profile(os.path.join(here, "profiles",
+                                                "fused_step_n10M_d32_k256_pmc.md"))
+    if t_cmp >= t_hbm:
+        roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": compute_peak,
+                "unit": "TFLOP/s", "frac": round(tflops / compute_peak, 4)}
+    else:
+        roof = {"bound": "hbm", "achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(hbm_gbs / HBM_PEAK_GBS, 4)}
+    roof["traffic"] = traffic
+    roof["kernel"] = "fused assignment+update step"
+    roof["kernel_ms"] = round(kern_ms, 4)
+    roof["hbm_frac"] = round(hbm_gbs / HBM_PEAK_GBS, 4)
+    roof["algorithmic_bytes"] = bytes_per_launch
+    roof["algorithmic_flops"] = flops_per_launch
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(X_t, init_t, n, d, k, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": round(iters_per_s, 3),
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain Lloyd, n={n:,} d={d} k={k} Gaussian blobs, "
+                               f"tol=0, one iteration per step",
+                   "n_samples": n, "n_features": d, "n_clusters": k,
+                   "precision": args.precision},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "inertia": inertia,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

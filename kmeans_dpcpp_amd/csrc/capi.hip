@@ -1,0 +1,597 @@
+// extern "C" entry points of libkmeans_dpcpp_amd.so (include/kmeans_dpcpp_amd.h).
+// Each one validates its scalar arguments, dispatches on (dtype, itype) and
+// enqueues the kernels on the caller's stream.  Exceptions never cross the C
+// boundary: ValueError -> KMD_EVALUE, anything else -> KMD_ERUNTIME, with the
+// message kept in a thread-local buffer for kmd_last_error().
+#include <cmath>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "lloyd_fast.hpp"
+
+namespace kmd {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+int device_cu_count() {
+  static std::mutex mu;
+  static std::vector<int> cache;
+  int dev = 0;
+  KMD_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+  if (cache[dev] == 0) {
+    int cus = 0;
+    KMD_HIP_CHECK(
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    cache[dev] = cus > 0 ? cus : 1;
+  }
+  return cache[dev];
+}
+
+int64_t device_l2_bytes(int device) {
+  int bytes = 0;
+  KMD_HIP_CHECK(
+      hipDeviceGetAttribute(&bytes, hipDeviceAttributeL2CacheSize, device));
+  return bytes;
+}
+
+template <typename T, typename I>
+void driver_lloyd(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
+                  T *init, I *labels, T *res, double tol, bool verbose,
+                  int64_t max_iter, int64_t *n_iters_out, double *inertia_out,
+                  kmd_print_fn print_fn, void *print_user, hipStream_t s,
+                  DriverStats *stats);
+
+}  // namespace kmd
+
+using namespace kmd;
+
+namespace {
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    f();
+    return KMD_OK;
+  } catch (const ValueError &e) {
+    set_last_error(e.what());
+    return KMD_EVALUE;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return KMD_ERUNTIME;
+  } catch (...) {
+    set_last_error("unknown error");
+    return KMD_ERUNTIME;
+  }
+}
+
+inline hipStream_t as_stream(kmd_stream_t s) {
+  return reinterpret_cast<hipStream_t>(s);
+}
+
+inline void check_dtype(int dtype) {
+  require(dtype == KMD_F32 || dtype == KMD_F64,
+          "Unsupported elemental data type. Expecting single or double "
+          "precision floating point numbers");
+}
+inline void check_types(int dtype, int itype) {
+  require((dtype == KMD_F32 || dtype == KMD_F64) &&
+              (itype == KMD_I32 || itype == KMD_I64),
+          "Unsupported data types");
+}
+inline void check_ptr(const void *p, int64_t count, const char *name) {
+  if (count > 0 && p == nullptr)
+    throw ValueError(std::string("null pointer for non-empty array ") + name);
+}
+
+// Calls fn.template operator()<T>() for the runtime dtype.
+template <typename F>
+void dispatch_t(int dtype, F &&fn) {
+  if (dtype == KMD_F32)
+    fn((float *)nullptr);
+  else
+    fn((double *)nullptr);
+}
+template <typename F>
+void dispatch_ti(int dtype, int itype, F &&fn) {
+  if (dtype == KMD_F32 && itype == KMD_I32)
+    fn((float *)nullptr, (int32_t *)nullptr);
+  else if (dtype == KMD_F32 && itype == KMD_I64)
+    fn((float *)nullptr, (int64_t *)nullptr);
+  else if (dtype == KMD_F64 && itype == KMD_I32)
+    fn((double *)nullptr, (int32_t *)nullptr);
+  else
+    fn((double *)nullptr, (int64_t *)nullptr);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *kmd_last_error(void) { return g_last_error.c_str(); }
+
+const char *kmd_version(void) { return "kmeans_dpcpp_amd 0.1.0 (gfx950)"; }
+
+int kmd_device_count(int *count) {
+  return guarded([&] {
+    require(count != nullptr, "count must not be null");
+    KMD_HIP_CHECK(hipGetDeviceCount(count));
+  });
+}
+
+int kmd_set_device(int device) {
+  return guarded([&] { KMD_HIP_CHECK(hipSetDevice(device)); });
+}
+
+int kmd_event_create(kmd_event_t *event) {
+  return guarded([&] {
+    require(event != nullptr, "event must not be null");
+    hipEvent_t e;
+    KMD_HIP_CHECK(hipEventCreate(&e));
+    *event = reinterpret_cast<kmd_event_t>(e);
+  });
+}
+
+int kmd_event_record(kmd_event_t event, kmd_stream_t stream) {
+  return guarded([&] {
+    KMD_HIP_CHECK(
+        hipEventRecord(reinterpret_cast<hipEvent_t>(event), as_stream(stream)));
+  });
+}
+
+int kmd_event_synchronize(kmd_event_t event) {
+  return guarded([&] {
+    KMD_HIP_CHECK(hipEventSynchronize(reinterpret_cast<hipEvent_t>(event)));
+  });
+}
+
+int kmd_event_query(kmd_event_t event, int *done) {
+  return guarded([&] {
+    hipError_t e = hipEventQuery(reinterpret_cast<hipEvent_t>(event));
+    if (e == hipErrorNotReady) {
+      *done = 0;
+      return;
+    }
+    KMD_HIP_CHECK(e);
+    *done = 1;
+  });
+}
+
+int kmd_event_elapsed_ms(kmd_event_t start, kmd_event_t stop, float *ms) {
+  return guarded([&] {
+    KMD_HIP_CHECK(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start),
+                                      reinterpret_cast<hipEvent_t>(stop)));
+  });
+}
+
+int kmd_event_destroy(kmd_event_t event) {
+  return guarded([&] {
+    KMD_HIP_CHECK(hipEventDestroy(reinterpret_cast<hipEvent_t>(event)));
+  });
+}
+
+int kmd_stream_wait_event(kmd_stream_t stream, kmd_event_t event) {
+  return guarded([&] {
+    KMD_HIP_CHECK(hipStreamWaitEvent(as_stream(stream),
+                                     reinterpret_cast<hipEvent_t>(event), 0));
+  });
+}
+
+int kmd_stream_synchronize(kmd_stream_t stream) {
+  return guarded([&] { KMD_HIP_CHECK(hipStreamSynchronize(as_stream(stream))); });
+}
+
+int kmd_stream_create(kmd_stream_t *stream) {
+  return guarded([&] {
+    hipStream_t s;
+    KMD_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = reinterpret_cast<kmd_stream_t>(s);
+  });
+}
+
+int kmd_stream_destroy(kmd_stream_t stream) {
+  return guarded([&] { KMD_HIP_CHECK(hipStreamDestroy(as_stream(stream))); });
+}
+
+int kmd_broadcast_divide(int dtype, int64_t n_rows, int64_t n_cols,
+                         int64_t work_group_size, void *divident,
+                         const void *divisor, kmd_stream_t stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(n_rows >= 0 && n_cols >= 0, "negative array dimension");
+    (void)work_group_size;
+    check_ptr(divident, n_rows * n_cols, "divident");
+    check_ptr(divisor, n_cols, "divisor");
+    dispatch_t(dtype, [&](auto *tp) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      launch_broadcast_divide<T>(n_rows, n_cols, (T *)divident,
+                                 (const T *)divisor, as_stream(stream));
+    });
+  });
+}
+
+int kmd_half_l2_norm_squared(int dtype, int64_t n_features, int64_t n_clusters,
+                             int64_t work_group_size, const void *centroids,
+                             void *out, kmd_stream_t stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(n_features >= 0 && n_clusters >= 0, "negative array dimension");
+    (void)work_group_size;
+    check_ptr(centroids, n_features * n_clusters, "centroids");
+    check_ptr(out, n_clusters, "out");
+    dispatch_t(dtype, [&](auto *tp) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      launch_half_l2_norm<T>(n_features, n_clusters, (const T *)centroids,
+                             (T *)out, as_stream(stream));
+    });
+  });
+}
+
+int kmd_reduce_centroids_data(int dtype, int itype, int64_t n_copies,
+                              int64_t n_features, int64_t n_clusters,
+                              int64_t work_group_size,
+                              const void *cluster_sizes_private_copies,
+                              const void *centroids_t_private_copies,
+                              void *out_cluster_sizes, void *out_centroids_t,
+                              void *out_empty_clusters_list,
+                              void *out_n_empty_clusters, kmd_stream_t stream) {
+  return guarded([&] {
+    check_types(dtype, itype);
+    require(n_copies >= 0 && n_features >= 0 && n_clusters >= 0,
+            "negative array dimension");
+    (void)work_group_size;
+    check_ptr(out_n_empty_clusters, 1, "out_n_empty_clusters");
+    dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      using I = std::remove_pointer_t<decltype(ip)>;
+      launch_reduce_centroids<T, I>(
+          n_copies, n_features, n_clusters,
+          (const T *)cluster_sizes_private_copies,
+          (const T *)centroids_t_private_copies, (T *)out_cluster_sizes,
+          (T *)out_centroids_t, (I *)out_empty_clusters_list,
+          (I *)out_n_empty_clusters, n_clusters, as_stream(stream));
+    });
+  });
+}
+
+int kmd_compute_threshold(int dtype, int64_t n, const void *data, int64_t topk,
+                          void *threshold, kmd_stream_t stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(n >= 0 && topk >= 0, "negative size");
+    if (topk == 0) return;
+    require(topk <= n, "topk must not exceed the number of elements");
+    check_ptr(data, n, "data");
+    check_ptr(threshold, 1, "threshold");
+    dispatch_t(dtype, [&](auto *tp) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      launch_threshold<T>(n, (const T *)data, topk, (T *)threshold,
+                          as_stream(stream));
+    });
+  });
+}
+
+int kmd_select_samples_far_from_centroid(
+    int dtype, int itype, int64_t n_selected, int64_t n_samples,
+    int64_t work_group_size, const void *distance_to_centroid,
+    const void *threshold, void *selected_samples_idx,
+    void *n_selected_gt_threshold, void *n_selected_eq_threshold,
+    kmd_stream_t stream) {
+  return guarded([&] {
+    check_types(dtype, itype);
+    require(n_selected > 0, "Argument `n_selected` must be positive");
+    require(n_samples >= n_selected, "Argument `n_select` is too large");
+    (void)work_group_size;
+    check_ptr(distance_to_centroid, n_samples, "distance_to_centroid");
+    check_ptr(selected_samples_idx, n_samples, "selected_samples_idx");
+    check_ptr(threshold, 1, "threshold");
+    check_ptr(n_selected_gt_threshold, 1, "n_selected_gt_threshold");
+    check_ptr(n_selected_eq_threshold, 1, "n_selected_eq_threshold");
+    dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      using I = std::remove_pointer_t<decltype(ip)>;
+      launch_select_far<T, I>(n_selected, n_samples,
+                              (const T *)distance_to_centroid,
+                              (const T *)threshold, (I *)selected_samples_idx,
+                              (I *)n_selected_gt_threshold,
+                              (I *)n_selected_eq_threshold, as_stream(stream));
+    });
+  });
+}
+
+int kmd_relocate_empty_clusters(
+    int dtype, int itype, int64_t n_samples, int64_t n_features,
+    int64_t n_clusters, int64_t n_empty_clusters, int64_t work_group_size,
+    const void *X_t, const void *sample_weights, const void *assignment_id,
+    const void *empty_clusters_list, const void *sq_dist_to_nearest_centroid,
+    void *centroid_t, void *cluster_sizes, void *per_sample_inertia,
+    kmd_stream_t stream) {
+  return guarded([&] {
+    check_types(dtype, itype);
+    require(n_empty_clusters != 0, "n_empty_clusters must be non-zero.");
+    require(n_empty_clusters > 0 && n_empty_clusters <= n_clusters,
+            "n_empty_clusters must not exceed the number of clusters");
+    require(n_samples > 0 && n_features > 0 && n_clusters > 0,
+            "Input dimensions are inconsistent");
+    (void)work_group_size;
+    dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      using I = std::remove_pointer_t<decltype(ip)>;
+      launch_relocate<T, I>(n_samples, n_features, n_clusters, n_empty_clusters,
+                            (const T *)X_t, (const T *)sample_weights,
+                            (const I *)assignment_id,
+                            (const I *)empty_clusters_list,
+                            (const T *)sq_dist_to_nearest_centroid,
+                            (T *)centroid_t, (T *)cluster_sizes,
+                            (T *)per_sample_inertia, as_stream(stream));
+    });
+  });
+}
+
+int kmd_compute_centroid_shifts_squared(int dtype, int64_t n_features,
+                                        int64_t n_clusters,
+                                        const void *old_centroid_t,
+                                        const void *new_centroid_t,
+                                        void *out_centroid_shifts,
+                                        kmd_stream_t stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(n_features >= 0 && n_clusters >= 0, "negative array dimension");
+    dispatch_t(dtype, [&](auto *tp) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      launch_centroid_shifts<T>(n_features, n_clusters,
+                                (const T *)old_centroid_t,
+                                (const T *)new_centroid_t,
+                                (T *)out_centroid_shifts, as_stream(stream));
+    });
+  });
+}
+
+int kmd_compute_centroid_to_sample_distances(
+    int dtype, int64_t n_samples, int64_t n_features, int64_t n_clusters,
+    int64_t work_group_size, int64_t centroids_window_height, const void *X_t,
+    const void *centroid_t, void *euclidean_distances_t, kmd_stream_t stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(n_samples >= 0 && n_features >= 0 && n_clusters >= 0,
+            "negative array dimension");
+    require(work_group_size > 0 && centroids_window_height > 0,
+            "work_group_size and centroids_window_height must be positive");
+    if (n_features == 0 || n_clusters == 0 || n_samples == 0) {
+      if (n_features == 0 && n_clusters > 0 && n_samples > 0) {
+        KMD_HIP_CHECK(hipMemsetAsync(euclidean_distances_t, 0,
+                                     (dtype == KMD_F32 ? 4 : 8) * n_clusters *
+                                         n_samples,
+                                     as_stream(stream)));
+      }
+      return;
+    }
+    dispatch_t(dtype, [&](auto *tp) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      launch_distances<T>(n_samples, n_features, n_clusters,
+                          centroids_window_height, (const T *)X_t,
+                          (const T *)centroid_t, (T *)euclidean_distances_t,
+                          as_stream(stream));
+    });
+  });
+}
+
+int kmd_assignment(int dtype, int itype, int64_t n_samples, int64_t n_features,
+                   int64_t n_clusters, int64_t centroids_window_height,
+                   int64_t work_group_size, const void *X_t,
+                   const void *centroids_t, const void *centroids_half_l2_norm,
+                   void *assignment_id, kmd_stream_t stream) {
+  return guarded([&] {
+    check_types(dtype, itype);
+    require(n_samples >= 0 && n_features >= 1 && n_clusters >= 1,
+            "Inputs have inconsistent dimensions.");
+    require(work_group_size > 0 && centroids_window_height > 0,
+            "work_group_size and centroids_window_height must be positive");
+    require(n_clusters <= INT32_MAX && n_features <= INT32_MAX,
+            "n_clusters and n_features must fit in 32 bits");
+    dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      using I = std::remove_pointer_t<decltype(ip)>;
+      launch_assignment<T, I>(n_samples, n_features, n_clusters,
+                              centroids_window_height, (const T *)X_t,
+                              (const T *)centroids_t,
+                              (const T *)centroids_half_l2_norm,
+                              (I *)assignment_id, as_stream(stream));
+    });
+  });
+}
+
+int kmd_compute_inertia(int dtype, int itype, int64_t n_samples,
+                        int64_t n_features, int64_t n_clusters,
+                        int64_t work_group_size, const void *X_t,
+                        const void *sample_weight, const void *centroid_t,
+                        const void *assignment_id, void *out_per_sample_inertia,
+                        kmd_stream_t stream) {
+  return guarded([&] {
+    check_types(dtype, itype);
+    require(n_samples >= 0 && n_features >= 0 && n_clusters >= 0,
+            "Array dimensions are not consistent");
+    (void)work_group_size;
+    dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      using I = std::remove_pointer_t<decltype(ip)>;
+      launch_inertia<T, I>(n_samples, n_features, n_clusters, (const T *)X_t,
+                           (const T *)sample_weight, (const T *)centroid_t,
+                           (const I *)assignment_id,
+                           (T *)out_per_sample_inertia, as_stream(stream));
+    });
+  });
+}
+
+int kmd_reduce_vector_blocking(int dtype, int64_t n, const void *data,
+                               double *result, kmd_stream_t stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(n >= 0, "negative size");
+    require(result != nullptr, "result must not be null");
+    hipStream_t s = as_stream(stream);
+    TempBuffer dev(sizeof(double), s);
+    dispatch_t(dtype, [&](auto *tp) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      if (n == 0) {
+        KMD_HIP_CHECK(hipMemsetAsync(dev.as<double>(), 0, sizeof(double), s));
+      } else {
+        launch_sum_to_scalar<T>(n, (const T *)data, dev.as<T>(), s);
+      }
+      T host = T(0);
+      KMD_HIP_CHECK(hipMemcpyAsync(&host, dev.as<T>(), sizeof(T),
+                                   hipMemcpyDeviceToHost, s));
+      KMD_HIP_CHECK(hipStreamSynchronize(s));
+      *result = (double)host;
+    });
+  });
+}
+
+int kmd_fused_lloyd_single_step(
+    int dtype, int itype, int64_t n_samples, int64_t n_features,
+    int64_t n_clusters, int64_t n_copies, int64_t centroids_window_height,
+    int64_t work_group_size, const void *X_t, const void *sample_weight,
+    const void *centroids_t, const void *centroids_half_l2_norm,
+    void *assignments_idx, void *new_centroids_t_private_copies,
+    void *cluster_sizes_private_copies, kmd_stream_t stream) {
+  return guarded([&] {
+    check_types(dtype, itype);
+    require(n_samples >= 0 && n_features >= 1 && n_clusters >= 1,
+            "Unexpected array dimensions");
+    require(n_copies >= 1, "number of private copies must be positive");
+    require(work_group_size > 0 && centroids_window_height > 0,
+            "work_group_size and centroids_window_height must be positive");
+    require(n_clusters <= INT32_MAX && n_features <= INT32_MAX,
+            "n_clusters and n_features must fit in 32 bits");
+    dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      using I = std::remove_pointer_t<decltype(ip)>;
+      launch_fused_step_copies<T, I>(
+          n_samples, n_features, n_clusters, n_copies, centroids_window_height,
+          (const T *)X_t, (const T *)sample_weight, (const T *)centroids_t,
+          (const T *)centroids_half_l2_norm, (I *)assignments_idx,
+          (T *)new_centroids_t_private_copies, (T *)cluster_sizes_private_copies,
+          as_stream(stream));
+    });
+  });
+}
+
+int kmd_compute_number_of_private_copies(int dtype, int device,
+                                         int64_t n_samples, int64_t n_features,
+                                         int64_t n_clusters, double occupancy,
+                                         int64_t work_group_size,
+                                         int64_t *out) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(occupancy > 0.0 && occupancy < 1.0,
+            "Expecting a fraction strictly between 0 and 1");
+    require(work_group_size > 0, "work_group_size must be positive");
+    require(n_samples >= 0 && n_features >= 0 && n_clusters > 0,
+            "n_clusters must be positive");
+    require(out != nullptr, "out must not be null");
+    const int64_t cache = device_l2_bytes(device);
+    const int64_t elem = dtype == KMD_F32 ? 4 : 8;
+    const int64_t cluster_bytes = elem * n_clusters * (n_features + 1);
+    int64_t copies =
+        (int64_t)std::floor((double)cache * occupancy / (double)cluster_bytes);
+    const int64_t global = ceil_div<int64_t>(n_samples, work_group_size) *
+                           work_group_size;
+    const int64_t subgroups = global / 8;
+    if (subgroups < copies) copies = subgroups;
+    if (copies < 1) copies = 1;
+    *out = copies;
+  });
+}
+
+int kmd_kmeans_lloyd_driver(
+    int dtype, int itype, int64_t n_samples, int64_t n_features,
+    int64_t n_clusters, const void *X_t, const void *sample_weight,
+    void *init_centroids_t, void *assignments_id, void *res_centroids_t,
+    double tol, int verbose, int64_t max_iter, int64_t centroids_window_height,
+    int64_t work_group_size, double occupancy, int64_t *n_iters,
+    double *total_inertia, kmd_print_fn print_fn, void *print_user_data,
+    kmd_stream_t stream) {
+  return guarded([&] {
+    check_types(dtype, itype);
+    require(occupancy > 0.0 && occupancy < 1.0,
+            "Fraction `centroids_private_copies_max_cache_occupancy` is out "
+            "of bounds");
+    require(tol >= 0.0, "Tolerance must be non-negative");
+    require(max_iter >= 0, "max_iter must be non-negative");
+    require(n_samples >= 0 && n_features >= 1 && n_clusters >= 1,
+            "Array dimensions are not consistent");
+    require(work_group_size > 0 && centroids_window_height > 0,
+            "work_group_size and centroids_window_height must be positive");
+    require(n_clusters <= INT32_MAX && n_features <= INT32_MAX,
+            "n_clusters and n_features must fit in 32 bits");
+    require(n_iters != nullptr && total_inertia != nullptr,
+            "output scalars must not be null");
+    dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      using I = std::remove_pointer_t<decltype(ip)>;
+      driver_lloyd<T, I>(n_samples, n_features, n_clusters, (const T *)X_t,
+                         (const T *)sample_weight, (T *)init_centroids_t,
+                         (I *)assignments_id, (T *)res_centroids_t, tol,
+                         verbose != 0, max_iter, n_iters, total_inertia,
+                         print_fn, print_user_data, as_stream(stream), nullptr);
+    });
+  });
+}
+
+int kmd_iterative_merge_sort(int dtype, int64_t n, const void *input,
+                             void *output, kmd_stream_t stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(n >= 0, "negative size");
+    dispatch_t(dtype, [&](auto *tp) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      launch_merge_sort<T>(n, (const T *)input, (T *)output, as_stream(stream));
+    });
+  });
+}
+
+int kmd_kmeans_lloyd_driver_ex(
+    int dtype, int itype, int precision, int64_t n_samples,
+    int64_t n_features, int64_t n_clusters, const void *X_t,
+    const void *sample_weight, void *init_centroids_t, void *assignments_id,
+    void *res_centroids_t, double tol, int verbose, int64_t max_iter,
+    int64_t *n_iters, double *total_inertia, int64_t *n_rescored,
+    double *fused_kernel_ms, int64_t *fused_launches, kmd_print_fn print_fn,
+    void *print_user_data, kmd_stream_t stream) {
+  return guarded([&] {
+    check_types(dtype, itype);
+    require(precision == KMD_PREC_FP32 || precision == KMD_PREC_BF16X3 ||
+                precision == KMD_PREC_BF16,
+            "precision must be KMD_PREC_FP32, KMD_PREC_BF16X3 or KMD_PREC_BF16");
+    require(precision == KMD_PREC_FP32,
+            "only KMD_PREC_FP32 is available in this build");
+    require(tol >= 0.0, "Tolerance must be non-negative");
+    require(max_iter >= 0, "max_iter must be non-negative");
+    require(n_samples >= 0 && n_features >= 1 && n_clusters >= 1,
+            "Array dimensions are not consistent");
+    require(n_clusters <= INT32_MAX && n_features <= INT32_MAX,
+            "n_clusters and n_features must fit in 32 bits");
+    require(n_iters != nullptr && total_inertia != nullptr,
+            "output scalars must not be null");
+    DriverStats st;
+    dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      using I = std::remove_pointer_t<decltype(ip)>;
+      driver_lloyd<T, I>(n_samples, n_features, n_clusters, (const T *)X_t,
+                         (const T *)sample_weight, (T *)init_centroids_t,
+                         (I *)assignments_id, (T *)res_centroids_t, tol,
+                         verbose != 0, max_iter, n_iters, total_inertia,
+                         print_fn, print_user_data, as_stream(stream), &st);
+    });
+    if (n_rescored) *n_rescored = 0;
+    if (fused_kernel_ms) *fused_kernel_ms = st.fused_kernel_ms;
+    if (fused_launches) *fused_launches = st.fused_launches;
+  });
+}
+
+}  // extern "C"

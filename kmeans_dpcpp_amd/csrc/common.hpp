@@ -1,0 +1,177 @@
+// Shared host/device helpers of the gfx950 Lloyd library: error state,
+// launch helpers, dtype dispatch, float atomics, stream-ordered temporaries.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+
+#include "../../include/kmeans_dpcpp_amd.h"
+
+namespace kmd {
+
+constexpr int kWave = 64;  // CDNA wavefront width
+
+// Error raised by argument checks; mapped to KMD_EVALUE at the C boundary.
+struct ValueError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+// Error raised by HIP failures; mapped to KMD_ERUNTIME.
+struct RuntimeError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void set_last_error(const std::string &msg);
+
+#define KMD_HIP_CHECK(expr)                                                   \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess) {                                                   \
+      throw ::kmd::RuntimeError(std::string(#expr) + " failed: " +           \
+                                hipGetErrorString(_e) + " at " __FILE__ ":" + \
+                                std::to_string(__LINE__));                    \
+    }                                                                         \
+  } while (0)
+
+#define KMD_LAUNCH_CHECK() KMD_HIP_CHECK(hipGetLastError())
+
+inline void require(bool cond, const char *msg) {
+  if (!cond) throw ValueError(msg);
+}
+
+template <typename T>
+__host__ __device__ inline T ceil_div(T n, T m) {
+  return (n + m - 1) / m;
+}
+
+// Number of compute units of the current device (cached per device).
+int device_cu_count();
+// L2 cache size in bytes of `device` (the reference uses SYCL's
+// global_mem_cache_size, src/lloyd_single_step.hpp:23).
+int64_t device_l2_bytes(int device);
+
+// Stream-ordered temporary device buffer (hipMallocAsync / hipFreeAsync).
+// Freed in stream order when the object is destroyed, so work enqueued on the
+// same stream before destruction may still use it.
+class TempBuffer {
+ public:
+  TempBuffer(size_t bytes, hipStream_t s) : stream_(s), bytes_(bytes) {
+    if (bytes_ > 0) KMD_HIP_CHECK(hipMallocAsync(&ptr_, bytes_, stream_));
+  }
+  ~TempBuffer() {
+    if (ptr_) (void)hipFreeAsync(ptr_, stream_);
+  }
+  TempBuffer(const TempBuffer &) = delete;
+  TempBuffer &operator=(const TempBuffer &) = delete;
+  template <typename T>
+  T *as() const {
+    return static_cast<T *>(ptr_);
+  }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  hipStream_t stream_;
+  size_t bytes_;
+  void *ptr_ = nullptr;
+};
+
+// ---- device helpers -----------------------------------------------------
+
+// Hardware float atomics (global_atomic_add_f32/f64, ds_add_f32/f64). The
+// library is built with -munsafe-fp-atomics; every buffer it adds into is
+// coarse-grained device memory.
+template <typename T>
+__device__ inline void atomic_add(T *addr, T v) {
+  atomicAdd(addr, v);
+}
+
+// Running (value, index) argmin with the reference's rule: a candidate
+// replaces the current best iff value < best (strict), scanning centroids in
+// ascending order (src/device_functions.hpp:139-162).  For partial results
+// scanned out of order, ties go to the lower index, which is the same rule.
+template <typename T>
+struct ArgMin {
+  T val;
+  int idx;
+  __device__ inline void init() {
+    val = std::numeric_limits<T>::infinity();
+    idx = 0;
+  }
+  __device__ inline void take(T v, int i) {
+    bool upd = (v < val) || (v == val && i < idx);
+    val = upd ? v : val;
+    idx = upd ? i : idx;
+  }
+};
+
+template <typename T>
+__device__ inline T shfl_xor_t(T v, int mask) {
+  return __shfl_xor(v, mask, kWave);
+}
+
+}  // namespace kmd
+
+// ---- internal launchers (defined in the .hip files) ------------------------
+namespace kmd {
+
+template <typename T>
+void launch_broadcast_divide(int64_t n_rows, int64_t n_cols, T *x, const T *y,
+                             hipStream_t s);
+template <typename T>
+void launch_half_l2_norm(int64_t d, int64_t k, const T *C, T *out,
+                         hipStream_t s);
+template <typename T, typename I>
+void launch_reduce_centroids(int64_t n_copies, int64_t d, int64_t k,
+                             const T *sizes_copies, const T *cent_copies,
+                             T *out_sizes, T *out_cent, I *empty_list,
+                             I *n_empty, int64_t list_len, hipStream_t s);
+template <typename T>
+void launch_centroid_shifts(int64_t d, int64_t k, const T *old_c,
+                            const T *new_c, T *out, hipStream_t s);
+template <typename T, typename I>
+void launch_inertia(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
+                    const T *C, const I *labels, T *out, hipStream_t s);
+template <typename T>
+void launch_reduce_sum(int64_t n, const T *data, double *dev_result,
+                       hipStream_t s);
+template <typename T>
+void launch_sum_to_scalar(int64_t n, const T *data, T *dev_result,
+                          hipStream_t s);
+template <typename T>
+void launch_max(int64_t n, const T *data, T *out, hipStream_t s);
+template <typename T>
+void launch_merge_sort(int64_t n, const T *in, T *out, hipStream_t s);
+template <typename T>
+void launch_threshold(int64_t n, const T *data, int64_t topk, T *threshold,
+                      hipStream_t s);
+template <typename T, typename I>
+void launch_select_far(int64_t n_selected, int64_t n, const T *dist,
+                       const T *threshold, I *selected, I *n_gt, I *n_eq,
+                       hipStream_t s);
+template <typename T, typename I>
+void launch_relocate(int64_t n, int64_t d, int64_t k, int64_t n_empty,
+                     const T *X, const T *w, const I *labels,
+                     const I *empty_list, const T *sq_dist, T *centroids,
+                     T *sizes, T *per_sample_inertia, hipStream_t s);
+
+// Lloyd kernels (lloyd.hip)
+template <typename T, typename I>
+void launch_assignment(int64_t n, int64_t d, int64_t k, int64_t h,
+                       const T *X, const T *C, const T *hn, I *labels,
+                       hipStream_t s);
+template <typename T>
+void launch_distances(int64_t n, int64_t d, int64_t k, int64_t h,
+                      const T *X, const T *C, T *D, hipStream_t s);
+template <typename T, typename I>
+void launch_fused_step_copies(int64_t n, int64_t d, int64_t k,
+                              int64_t n_copies, int64_t h, const T *X,
+                              const T *w, const T *C, const T *hn, I *labels,
+                              T *copies, T *sizes_copies, hipStream_t s);
+
+}  // namespace kmd

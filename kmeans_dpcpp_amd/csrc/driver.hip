@@ -1,0 +1,259 @@
+// Host driver of a full Lloyd fit (src/kmeans_lloyd_driver.hpp:21-431).
+//
+// Semantics kept from the reference: iterate while it < max_iter and the sum
+// of squared centroid shifts, compared in the data type, exceeds tol (raw, not
+// variance-scaled); each iteration computes half-norms, the fused
+// assignment+update, the reduction of private copies with the empty-cluster
+// list, the relocation of empty clusters (on unnormalised sums, before the
+// division), the division, the shifts and their sum; the epilogue recomputes
+// labels and the weighted inertia of the final centroids and copies them to
+// res_centroids_t.  init_centroids_t is used as the other ping-pong buffer
+// and is overwritten.
+//
+// Differences (all on the host side, none in results):
+//  * one host synchronisation per iteration instead of two: the next
+//    iteration's front half (half-norms, fused step, reduction) is enqueued
+//    before reading back {shift sum of this iteration, empty count of the
+//    next}; when the fit stops, that speculative work is discarded (labels are
+//    recomputed by the epilogue, the written buffer is the scratch one);
+//  * private copies are one per block of the persistent fused kernel and are
+//    written with plain stores, so the sums are deterministic;
+//  * the relocation path reuses the labels of the fused step instead of
+//    recomputing them (the reference recomputes them from the same centroids
+//    and half-norms, src/kmeans_lloyd_driver.hpp:226-250);
+//  * the verbose line reports the inertia of the centroids that produced the
+//    labels (the reference evaluates it against the undivided sums).
+#include <cstring>
+#include <memory>
+#include <sstream>
+
+#include "common.hpp"
+#include "lloyd_fast.hpp"
+
+namespace kmd {
+
+template <typename T, typename I>
+struct LloydWorkspace {
+  int64_t n, d, k;
+  int blocks = 1;          // private copies (one per block when store_flush)
+  bool fast = false;       // MFMA fast path
+  bool store_flush = false;
+  FusedPlan plan;
+  TempBuffer hn, sizes, shifts, per_sample, copies, sizes_copies, list,
+      scalars;
+  LloydWorkspace(int64_t n_, int64_t d_, int64_t k_, int blocks_, bool fast_,
+                 bool store_, const FusedPlan &plan_, hipStream_t s)
+      : n(n_), d(d_), k(k_), blocks(blocks_), fast(fast_), store_flush(store_),
+        plan(plan_), hn(sizeof(T) * k_, s), sizes(sizeof(T) * k_, s),
+        shifts(sizeof(T) * k_, s), per_sample(sizeof(T) * (n_ > 0 ? n_ : 1), s),
+        copies(sizeof(T) * (size_t)blocks_ * d_ * k_, s),
+        sizes_copies(sizeof(T) * (size_t)blocks_ * k_, s),
+        list(sizeof(I) * (k_ + 1), s), scalars(32, s) {}
+  I *n_empty() const { return reinterpret_cast<I *>(scalars.as<char>() + 8); }
+  double *shift_sum() const { return scalars.as<double>(); }
+};
+
+template <typename T, typename I>
+static std::unique_ptr<LloydWorkspace<T, I>> make_workspace(int64_t n,
+                                                            int64_t d,
+                                                            int64_t k,
+                                                            hipStream_t s) {
+  if (fast_fused_supported<T>(d, k)) {
+    return std::make_unique<LloydWorkspace<T, I>>(
+        n, d, k, fast_fused_blocks(n, d, k), true, true, FusedPlan{}, s);
+  }
+  FusedPlan p = plan_generic_fused<T>(n, d, k);
+  if (p.priv)
+    return std::make_unique<LloydWorkspace<T, I>>(n, d, k, p.blocks, false,
+                                                  true, p, s);
+  // sums too large for LDS: global atomics into a few zeroed copies
+  const int copies = 8;
+  return std::make_unique<LloydWorkspace<T, I>>(n, d, k, copies, false, false,
+                                                p, s);
+}
+
+// Optional timing of the fused-step kernel with HIP events on the driver's
+// stream (one event pair per iteration, read back after each host sync).
+struct FusedTimer {
+  bool on = false;
+  hipEvent_t start = nullptr, stop = nullptr;
+  bool pending = false;
+  double total_ms = 0.0;
+  int64_t launches = 0;
+  explicit FusedTimer(bool enable) : on(enable) {
+    if (on) {
+      KMD_HIP_CHECK(hipEventCreate(&start));
+      KMD_HIP_CHECK(hipEventCreate(&stop));
+    }
+  }
+  ~FusedTimer() {
+    if (start) (void)hipEventDestroy(start);
+    if (stop) (void)hipEventDestroy(stop);
+  }
+  void begin(hipStream_t s) {
+    if (on) KMD_HIP_CHECK(hipEventRecord(start, s));
+  }
+  void end(hipStream_t s) {
+    if (!on) return;
+    KMD_HIP_CHECK(hipEventRecord(stop, s));
+    pending = true;
+  }
+  // call after a host synchronisation of the stream
+  void collect() {
+    if (!on || !pending) return;
+    float ms = 0.f;
+    KMD_HIP_CHECK(hipEventElapsedTime(&ms, start, stop));
+    total_ms += ms;
+    ++launches;
+    pending = false;
+  }
+};
+
+// hn, fused step and reduction for centroids `cur`, sums into `next`.
+template <typename T, typename I>
+static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
+                        const T *cur, T *next, I *labels, hipStream_t s,
+                        FusedTimer &timer) {
+  const int64_t n = ws.n, d = ws.d, k = ws.k;
+  launch_half_l2_norm<T>(d, k, cur, ws.hn.template as<T>(), s);
+  KMD_HIP_CHECK(hipMemsetAsync(ws.n_empty(), 0, sizeof(I), s));
+  if (!ws.store_flush) {
+    KMD_HIP_CHECK(hipMemsetAsync(ws.copies.template as<T>(), 0,
+                                 ws.copies.bytes(), s));
+    KMD_HIP_CHECK(hipMemsetAsync(ws.sizes_copies.template as<T>(), 0,
+                                 ws.sizes_copies.bytes(), s));
+  }
+  if (n > 0) {
+    timer.begin(s);
+    if (ws.fast) {
+      launch_fast_fused<T, I>(n, d, k, ws.blocks, true, X, w, cur,
+                              ws.hn.template as<const T>(), labels,
+                              ws.copies.template as<T>(),
+                              ws.sizes_copies.template as<T>(), s);
+    } else {
+      launch_fused_generic<T, I>(n, d, k, ws.blocks, ws.plan, ws.store_flush,
+                                 X, w, cur, ws.hn.template as<const T>(),
+                                 labels, ws.copies.template as<T>(),
+                                 ws.sizes_copies.template as<T>(), s);
+    }
+    timer.end(s);
+  } else {
+    KMD_HIP_CHECK(hipMemsetAsync(ws.copies.template as<T>(), 0,
+                                 ws.copies.bytes(), s));
+    KMD_HIP_CHECK(hipMemsetAsync(ws.sizes_copies.template as<T>(), 0,
+                                 ws.sizes_copies.bytes(), s));
+  }
+  launch_reduce_centroids<T, I>(ws.blocks, d, k,
+                                ws.sizes_copies.template as<const T>(),
+                                ws.copies.template as<const T>(),
+                                ws.sizes.template as<T>(), next,
+                                ws.list.template as<I>(), ws.n_empty(), k, s);
+}
+
+template <typename T, typename I>
+void driver_lloyd(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
+                  T *init, I *labels, T *res, double tol, bool verbose,
+                  int64_t max_iter, int64_t *n_iters_out, double *inertia_out,
+                  kmd_print_fn print_fn, void *print_user, hipStream_t s,
+                  DriverStats *stats) {
+  auto ws_ptr = make_workspace<T, I>(n, d, k, s);
+  FusedTimer timer(stats != nullptr);
+  auto &ws = *ws_ptr;
+  T *cur = init;
+  T *next = res;
+  const T tol_t = static_cast<T>(tol);
+  int64_t it = 0;
+  T shift = std::numeric_limits<T>::infinity();
+  unsigned char host_scalars[16];
+  auto read_scalars = [&]() {
+    KMD_HIP_CHECK(hipMemcpyAsync(host_scalars, ws.scalars.template as<char>(),
+                                 16, hipMemcpyDeviceToHost, s));
+    KMD_HIP_CHECK(hipStreamSynchronize(s));
+    timer.collect();
+  };
+  auto host_n_empty = [&]() -> int64_t {
+    I v;
+    std::memcpy(&v, host_scalars + 8, sizeof(I));
+    return (int64_t)v;
+  };
+  auto host_shift = [&]() -> double {
+    double v;
+    std::memcpy(&v, host_scalars, sizeof(double));
+    return v;
+  };
+
+  if (max_iter > 0) {
+    lloyd_front<T, I>(ws, X, w, cur, next, labels, s, timer);
+    read_scalars();
+  }
+  while (it < max_iter && shift > tol_t) {
+    const int64_t n_empty = host_n_empty();
+    if (verbose) {
+      launch_inertia<T, I>(n, d, k, X, w, cur, labels,
+                           ws.per_sample.template as<T>(), s);
+      double tot = 0.0;
+      TempBuffer dres(sizeof(double), s);
+      launch_reduce_sum<T>(n, ws.per_sample.template as<const T>(),
+                           dres.as<double>(), s);
+      KMD_HIP_CHECK(hipMemcpyAsync(&tot, dres.as<double>(), sizeof(double),
+                                   hipMemcpyDeviceToHost, s));
+      KMD_HIP_CHECK(hipStreamSynchronize(s));
+      std::ostringstream ss;
+      ss << "Iteration: " << it << " Inertia: " << static_cast<T>(tot);
+      if (print_fn) print_fn(ss.str().c_str(), print_user);
+    }
+    if (n_empty > 0) {
+      // squared distance to the nearest centroid (unit weights), then move
+      // the farthest samples into the empty clusters
+      launch_inertia<T, I>(n, d, k, X, nullptr, cur, labels,
+                           ws.per_sample.template as<T>(), s);
+      launch_relocate<T, I>(n, d, k, n_empty, X, w, labels,
+                            ws.list.template as<const I>(),
+                            ws.per_sample.template as<const T>(), next,
+                            ws.sizes.template as<T>(),
+                            ws.per_sample.template as<T>(), s);
+    }
+    launch_broadcast_divide<T>(d, k, next, ws.sizes.template as<const T>(), s);
+    launch_centroid_shifts<T>(d, k, cur, next, ws.shifts.template as<T>(), s);
+    launch_reduce_sum<T>(k, ws.shifts.template as<const T>(), ws.shift_sum(),
+                         s);
+    std::swap(cur, next);
+    ++it;
+    if (it < max_iter) lloyd_front<T, I>(ws, X, w, cur, next, labels, s, timer);
+    read_scalars();
+    shift = static_cast<T>(host_shift());
+  }
+
+  // epilogue: labels and weighted inertia of the final centroids
+  launch_half_l2_norm<T>(d, k, cur, ws.hn.template as<T>(), s);
+  launch_assignment<T, I>(n, d, k, 0, X, cur, ws.hn.template as<const T>(),
+                          labels, s);
+  launch_inertia<T, I>(n, d, k, X, w, cur, labels,
+                       ws.per_sample.template as<T>(), s);
+  if (cur != res)
+    KMD_HIP_CHECK(hipMemcpyAsync(res, cur, sizeof(T) * d * k,
+                                 hipMemcpyDeviceToDevice, s));
+  launch_reduce_sum<T>(n, ws.per_sample.template as<const T>(),
+                       ws.shift_sum(), s);
+  read_scalars();
+  *inertia_out = static_cast<double>(static_cast<T>(host_shift()));
+  *n_iters_out = it;
+  if (stats) {
+    stats->fused_kernel_ms = timer.total_ms;
+    stats->fused_launches = timer.launches;
+    stats->fused_path = ws.fast ? 1 : 0;
+  }
+}
+
+#define KMD_INST_DRIVER(T, I)                                                 \
+  template void driver_lloyd<T, I>(int64_t, int64_t, int64_t, const T *,     \
+                                   const T *, T *, I *, T *, double, bool,    \
+                                   int64_t, int64_t *, double *,              \
+                                   kmd_print_fn, void *, hipStream_t,         \
+                                   DriverStats *);
+KMD_INST_DRIVER(float, int32_t)
+KMD_INST_DRIVER(float, int64_t)
+KMD_INST_DRIVER(double, int32_t)
+KMD_INST_DRIVER(double, int64_t)
+
+}  // namespace kmd

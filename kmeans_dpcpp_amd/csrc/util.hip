@@ -1,0 +1,551 @@
+// Small wave-level kernels around the Lloyd step: broadcast divide, half
+// squared norms, private-copy reduction with the empty-cluster list, centroid
+// shifts, per-sample inertia, deterministic sum / max reductions, selection of
+// far samples and empty-cluster relocation.
+#include "common.hpp"
+
+namespace kmd {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ inline int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// Block-wide sum of a double (256 threads); result valid in every thread.
+__device__ inline double block_sum(double v, double *lds /*[kBlock/kWave]*/) {
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  const int w = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane_id() == 0) lds[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < (int)(blockDim.x / kWave); ++i) t += lds[i];
+  return t;
+}
+
+// ---- broadcast divide: x[r, c] /= y[c] (src/util_kernels.hpp:9-41) ---------
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    broadcast_divide_krn(int64_t total, int64_t n_cols, T *x, const T *y) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * kBlock) {
+    x[i] = x[i] / y[i % n_cols];
+  }
+}
+
+// ---- half squared norms (src/util_kernels.hpp:47-82) -----------------------
+// Column c, rows ascending, fused multiply-add accumulation, then / 2.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    half_l2_norm_krn(int64_t d, int64_t k, const T *C, T *out) {
+  int64_t c = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (c >= k) return;
+  T acc = T(0);
+  for (int64_t f = 0; f < d; ++f) {
+    T v = C[f * k + c];
+    acc = fma(v, v, acc);
+  }
+  out[c] = acc / T(2);
+}
+
+// ---- reduce private copies (src/util_kernels.hpp:84-161) -------------------
+// Blocks [0, nb-1) sum centroid copies element-wise (copy order ascending);
+// the last block sums the size copies and appends empty clusters in
+// ascending order starting at n_empty[0].
+template <typename T, typename I>
+__global__ void __launch_bounds__(1024)
+    reduce_centroids_krn(int64_t n_copies, int64_t dk, int64_t k,
+                         const T *sizes_copies, const T *cent_copies,
+                         T *out_sizes, T *out_cent, I *empty_list, I *n_empty,
+                         int64_t list_len) {
+  if (blockIdx.x + 1 < gridDim.x) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= dk) return;
+    T acc = T(0);
+    for (int64_t c = 0; c < n_copies; ++c) acc += cent_copies[c * dk + i];
+    out_cent[i] = acc;
+    return;
+  }
+  // last block: sizes + empty list, processed in chunks of blockDim clusters
+  __shared__ int wave_counts[1024 / kWave];
+  __shared__ int64_t running;
+  if (threadIdx.x == 0) running = (int64_t)n_empty[0];
+  __syncthreads();
+  const int64_t base = running;
+  const int nwaves = blockDim.x / kWave;
+  const int w = threadIdx.x / kWave;
+  for (int64_t c0 = 0; c0 < k; c0 += blockDim.x) {
+    int64_t c = c0 + threadIdx.x;
+    bool empty = false;
+    if (c < k) {
+      T acc = T(0);
+      for (int64_t cp = 0; cp < n_copies; ++cp) acc += sizes_copies[cp * k + c];
+      out_sizes[c] = acc;
+      empty = (acc == T(0));
+    }
+    uint64_t mask = __ballot(empty);
+    int before_in_wave = __popcll(mask & ((1ull << lane_id()) - 1ull));
+    if (lane_id() == 0) wave_counts[w] = __popcll(mask);
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int i = 0; i < nwaves; ++i) {
+      before += (i < w) ? wave_counts[i] : 0;
+      total += wave_counts[i];
+    }
+    if (empty) {
+      int64_t pos = running + before + before_in_wave;
+      if (pos >= 0 && pos < list_len) empty_list[pos] = (I)c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) running += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) n_empty[0] = (I)running;
+  (void)base;
+}
+
+// ---- centroid shifts (src/util_kernels.hpp:478-516) ------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    centroid_shifts_krn(int64_t d, int64_t k, const T *a, const T *b, T *out) {
+  int64_t c = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (c >= k) return;
+  T acc = T(0);
+  for (int64_t f = 0; f < d; ++f) {
+    T diff = a[f * k + c] - b[f * k + c];
+    acc = fma(diff, diff, acc);
+  }
+  out[c] = acc;
+}
+
+// ---- per-sample inertia (src/compute_inertia.hpp:11-99) --------------------
+template <typename T, typename I>
+__global__ void __launch_bounds__(kBlock)
+    inertia_krn(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
+                const T *C, const I *labels, T *out) {
+  for (int64_t s = blockIdx.x * (int64_t)kBlock + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * kBlock) {
+    int64_t c = (int64_t)labels[s];
+    T acc = T(0);
+    if (c >= 0 && c < k) {
+      for (int64_t f = 0; f < d; ++f) {
+        T diff = X[f * n + s] - C[f * k + c];
+        acc = fma(diff, diff, acc);
+      }
+    } else {
+      acc = std::numeric_limits<T>::quiet_NaN();
+    }
+    out[s] = (w != nullptr) ? acc * w[s] : acc;
+  }
+}
+
+// ---- deterministic two-stage sum (src/compute_inertia.hpp:104-135) ---------
+// Stage 1: fixed grid, each block sums a grid-stride slice in FP64.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    sum_stage1_krn(int64_t n, const T *x, double *partials) {
+  __shared__ double lds[kBlock / kWave];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    acc += (double)x[i];
+  double t = block_sum(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+template <typename OutT>
+__global__ void __launch_bounds__(kBlock)
+    sum_stage2_krn(int nb, const double *partials, OutT *out) {
+  __shared__ double lds[kBlock / kWave];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kBlock) acc += partials[i];
+  double t = block_sum(acc, lds);
+  if (threadIdx.x == 0) out[0] = (OutT)t;
+}
+
+// ---- max reduction (src/util_kernels.hpp:174-188); NaN is ignored ----------
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    max_stage_krn(int64_t n, const T *x, T *out) {
+  __shared__ T lds[kBlock / kWave];
+  T m = -std::numeric_limits<T>::infinity();
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    T v = x[i];
+    m = (v > m) ? v : m;
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    T v = __shfl_xor(m, o, kWave);
+    m = (v > m) ? v : m;
+  }
+  if (lane_id() == 0) lds[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T r = lds[0];
+    for (int i = 1; i < kBlock / kWave; ++i) r = (lds[i] > r) ? lds[i] : r;
+    out[blockIdx.x] = r;
+  }
+}
+
+// ---- select samples far from centroids (src/util_kernels.hpp:208-297) -----
+// Deterministic stable compaction in three launches: per-block counts,
+// scatter, counter update.  Each block owns a contiguous chunk.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    select_count_krn(int64_t n, int64_t chunk, const T *dist, const T *thr,
+                     int64_t *counts /*[2*nb]*/) {
+  __shared__ int64_t lds[2][kBlock / kWave];
+  const T t = thr[0];
+  int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  int64_t gt = 0, eq = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+    T v = dist[i];
+    gt += (v > t);
+    eq += (v == t);
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    gt += __shfl_xor(gt, o, kWave);
+    eq += __shfl_xor(eq, o, kWave);
+  }
+  if (lane_id() == 0) {
+    lds[0][threadIdx.x / kWave] = gt;
+    lds[1][threadIdx.x / kWave] = eq;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t a = 0, b = 0;
+    for (int i = 0; i < kBlock / kWave; ++i) {
+      a += lds[0][i];
+      b += lds[1][i];
+    }
+    counts[2 * blockIdx.x] = a;
+    counts[2 * blockIdx.x + 1] = b;
+  }
+}
+
+template <typename T, typename I>
+__global__ void __launch_bounds__(kBlock)
+    select_scatter_krn(int64_t n, int64_t chunk, int64_t n_selected,
+                       const T *dist, const T *thr, const int64_t *counts,
+                       int nb, I *selected, const I *n_gt, const I *n_eq) {
+  __shared__ int64_t off_gt, off_eq, tot_gt;
+  __shared__ int wc[2][kBlock / kWave];
+  if (threadIdx.x == 0) {
+    int64_t a = 0, b = 0, ta = 0;
+    for (int i = 0; i < nb; ++i) {
+      if (i < (int)blockIdx.x) {
+        a += counts[2 * i];
+        b += counts[2 * i + 1];
+      }
+      ta += counts[2 * i];
+    }
+    off_gt = a;
+    off_eq = b;
+    tot_gt = ta;
+  }
+  __syncthreads();
+  const T t = thr[0];
+  const int64_t base_gt = (int64_t)n_gt[0];
+  const int64_t base_eq = (int64_t)n_eq[0];
+  const int64_t eq_limit = max((int64_t)0, n_selected - tot_gt);
+  int64_t run_gt = off_gt, run_eq = off_eq;
+  const int w = threadIdx.x / kWave;
+  int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  for (int64_t i0 = lo; i0 < hi; i0 += kBlock) {
+    int64_t i = i0 + threadIdx.x;
+    bool g = false, e = false;
+    if (i < hi) {
+      T v = dist[i];
+      g = v > t;
+      e = v == t;
+    }
+    uint64_t mg = __ballot(g), me = __ballot(e);
+    uint64_t below = (1ull << lane_id()) - 1ull;
+    if (lane_id() == 0) {
+      wc[0][w] = __popcll(mg);
+      wc[1][w] = __popcll(me);
+    }
+    __syncthreads();
+    int bg = 0, be = 0, tg = 0, te = 0;
+    for (int q = 0; q < kBlock / kWave; ++q) {
+      bg += (q < w) ? wc[0][q] : 0;
+      be += (q < w) ? wc[1][q] : 0;
+      tg += wc[0][q];
+      te += wc[1][q];
+    }
+    if (g) {
+      int64_t pos = base_gt + run_gt + bg + __popcll(mg & below);
+      if (pos >= 0 && pos < n) selected[pos] = (I)i;
+    }
+    if (e) {
+      int64_t r = run_eq + be + __popcll(me & below);
+      if (r < eq_limit) {
+        int64_t pos = n - (base_eq + r);
+        if (pos >= 0 && pos < n) selected[pos] = (I)i;
+      }
+    }
+    run_gt += tg;
+    run_eq += te;
+    __syncthreads();
+  }
+}
+
+template <typename I>
+__global__ void select_finalize_krn(int64_t n_selected, const int64_t *counts,
+                                    int nb, I *n_gt, I *n_eq) {
+  int64_t a = 0, b = 0;
+  for (int i = 0; i < nb; ++i) {
+    a += counts[2 * i];
+    b += counts[2 * i + 1];
+  }
+  int64_t eq_limit = n_selected - a;
+  if (eq_limit < 0) eq_limit = 0;
+  n_gt[0] = (I)((int64_t)n_gt[0] + a);
+  n_eq[0] = (I)((int64_t)n_eq[0] + (b < eq_limit ? b : eq_limit));
+}
+
+// ---- relocation ordering + application (src/util_kernels.hpp:302-390) -----
+// order[j] = j-th farthest sample by (sq_dist desc, index asc), j < m.
+// Inputs: selected[0:g] = indices with dist > T (ascending index),
+// selected[n-1], selected[n-2], ... = indices with dist == T (ascending).
+template <typename T, typename I>
+__global__ void __launch_bounds__(1024)
+    relocation_order_krn(int64_t n, int64_t m, const T *dist,
+                         const I *selected, const I *n_gt, const I *n_eq,
+                         int64_t *order) {
+  const int64_t g = (int64_t)n_gt[0];
+  const int64_t e = (int64_t)n_eq[0] - 1;  // eq counter started at 1
+  for (int64_t i = threadIdx.x; i < g; i += blockDim.x) {
+    int64_t si = (int64_t)selected[i];
+    T di = dist[si];
+    int64_t rank = 0;
+    for (int64_t j = 0; j < g; ++j) {
+      int64_t sj = (int64_t)selected[j];
+      T dj = dist[sj];
+      rank += (dj > di) || (dj == di && sj < si);
+    }
+    if (rank < m) order[rank] = si;
+  }
+  for (int64_t r = threadIdx.x; r < e; r += blockDim.x) {
+    int64_t pos = g + r;
+    if (pos < m) order[pos] = (int64_t)selected[n - 1 - r];
+  }
+}
+
+template <typename T, typename I>
+__global__ void __launch_bounds__(kBlock)
+    relocate_apply_krn(int64_t n, int64_t d, int64_t k, int64_t m,
+                       const T *X, const T *w, const I *labels,
+                       const I *empty_list, const int64_t *order, T *C,
+                       T *sizes, T *inertia) {
+  int64_t f = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (f >= d) return;
+  for (int64_t j = 0; j < m; ++j) {
+    int64_t s = order[j];
+    int64_t p = (int64_t)labels[s];
+    int64_t e = (int64_t)empty_list[j];
+    if (s < 0 || s >= n || p < 0 || p >= k || e < 0 || e >= k) continue;
+    T ws = w[s];
+    T add = X[f * n + s] * ws;
+    C[f * k + p] -= add;
+    C[f * k + e] = add;
+    if (f == 0) {
+      sizes[p] -= ws;
+      sizes[e] = ws;
+      inertia[s] = T(0);
+    }
+  }
+}
+
+template <typename T, typename I>
+__global__ void init_counters_krn(I *n_gt, I *n_eq) {
+  n_gt[0] = I(0);
+  n_eq[0] = I(1);
+}
+
+inline int grid_for(int64_t n, int per_block, int cap) {
+  int64_t g = ceil_div<int64_t>(n, per_block);
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+}  // namespace
+
+// ---- launchers ---------------------------------------------------------------
+
+template <typename T>
+void launch_broadcast_divide(int64_t n_rows, int64_t n_cols, T *x, const T *y,
+                             hipStream_t s) {
+  int64_t total = n_rows * n_cols;
+  if (total == 0) return;
+  hipLaunchKernelGGL(broadcast_divide_krn<T>, dim3(grid_for(total, kBlock, 65536)),
+                     dim3(kBlock), 0, s, total, n_cols, x, y);
+  KMD_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_half_l2_norm(int64_t d, int64_t k, const T *C, T *out,
+                         hipStream_t s) {
+  if (k == 0) return;
+  hipLaunchKernelGGL(half_l2_norm_krn<T>, dim3(ceil_div<int64_t>(k, kBlock)),
+                     dim3(kBlock), 0, s, d, k, C, out);
+  KMD_LAUNCH_CHECK();
+}
+
+template <typename T, typename I>
+void launch_reduce_centroids(int64_t n_copies, int64_t d, int64_t k,
+                             const T *sizes_copies, const T *cent_copies,
+                             T *out_sizes, T *out_cent, I *empty_list,
+                             I *n_empty, int64_t list_len, hipStream_t s) {
+  const int64_t dk = d * k;
+  const int nb = (int)ceil_div<int64_t>(dk, 1024) + 1;
+  hipLaunchKernelGGL((reduce_centroids_krn<T, I>), dim3(nb), dim3(1024), 0, s,
+                     n_copies, dk, k, sizes_copies, cent_copies, out_sizes,
+                     out_cent, empty_list, n_empty, list_len);
+  KMD_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_centroid_shifts(int64_t d, int64_t k, const T *a, const T *b,
+                            T *out, hipStream_t s) {
+  if (k == 0) return;
+  hipLaunchKernelGGL(centroid_shifts_krn<T>, dim3(ceil_div<int64_t>(k, kBlock)),
+                     dim3(kBlock), 0, s, d, k, a, b, out);
+  KMD_LAUNCH_CHECK();
+}
+
+template <typename T, typename I>
+void launch_inertia(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
+                    const T *C, const I *labels, T *out, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL((inertia_krn<T, I>), dim3(grid_for(n, kBlock, 8192)),
+                     dim3(kBlock), 0, s, n, d, k, X, w, C, labels, out);
+  KMD_LAUNCH_CHECK();
+}
+
+static int sum_blocks(int64_t n) { return grid_for(n, kBlock * 16, 1024); }
+
+template <typename T>
+void launch_reduce_sum(int64_t n, const T *x, double *dev_result,
+                       hipStream_t s) {
+  const int nb = sum_blocks(n);
+  TempBuffer partials(sizeof(double) * nb, s);
+  hipLaunchKernelGGL(sum_stage1_krn<T>, dim3(nb), dim3(kBlock), 0, s, n, x,
+                     partials.as<double>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(sum_stage2_krn<double>, dim3(1), dim3(kBlock), 0, s, nb,
+                     partials.as<const double>(), dev_result);
+  KMD_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_sum_to_scalar(int64_t n, const T *x, T *dev_result,
+                          hipStream_t s) {
+  const int nb = sum_blocks(n);
+  TempBuffer partials(sizeof(double) * nb, s);
+  hipLaunchKernelGGL(sum_stage1_krn<T>, dim3(nb), dim3(kBlock), 0, s, n, x,
+                     partials.as<double>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(sum_stage2_krn<T>, dim3(1), dim3(kBlock), 0, s, nb,
+                     partials.as<const double>(), dev_result);
+  KMD_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_max(int64_t n, const T *x, T *out, hipStream_t s) {
+  const int nb = grid_for(n, kBlock * 16, 1024);
+  TempBuffer partials(sizeof(T) * nb, s);
+  hipLaunchKernelGGL(max_stage_krn<T>, dim3(nb), dim3(kBlock), 0, s, n, x,
+                     partials.as<T>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(max_stage_krn<T>, dim3(1), dim3(kBlock), 0, s, (int64_t)nb,
+                     partials.as<const T>(), out);
+  KMD_LAUNCH_CHECK();
+}
+
+template <typename T, typename I>
+void launch_select_far(int64_t n_selected, int64_t n, const T *dist,
+                       const T *thr, I *selected, I *n_gt, I *n_eq,
+                       hipStream_t s) {
+  const int nb = grid_for(n, kBlock * 32, 1024);
+  const int64_t chunk = ceil_div<int64_t>(n, nb);
+  TempBuffer counts(sizeof(int64_t) * 2 * nb, s);
+  hipLaunchKernelGGL(select_count_krn<T>, dim3(nb), dim3(kBlock), 0, s, n,
+                     chunk, dist, thr, counts.as<int64_t>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL((select_scatter_krn<T, I>), dim3(nb), dim3(kBlock), 0, s,
+                     n, chunk, n_selected, dist, thr,
+                     counts.as<const int64_t>(), nb, selected, n_gt, n_eq);
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(select_finalize_krn<I>, dim3(1), dim3(1), 0, s,
+                     n_selected, counts.as<const int64_t>(), nb, n_gt, n_eq);
+  KMD_LAUNCH_CHECK();
+}
+
+template <typename T, typename I>
+void launch_relocate(int64_t n, int64_t d, int64_t k, int64_t n_empty,
+                     const T *X, const T *w, const I *labels,
+                     const I *empty_list, const T *sq_dist, T *centroids,
+                     T *sizes, T *per_sample_inertia, hipStream_t s) {
+  const int64_t m = n_empty < n ? n_empty : n;
+  if (m <= 0 || d <= 0) return;
+  TempBuffer thr(sizeof(T), s);
+  TempBuffer sel(sizeof(I) * (n + 2), s);
+  TempBuffer order(sizeof(int64_t) * m, s);
+  I *n_gt = sel.as<I>() + n;
+  I *n_eq = n_gt + 1;
+  launch_threshold<T>(n, sq_dist, m, thr.as<T>(), s);
+  hipLaunchKernelGGL((init_counters_krn<T, I>), dim3(1), dim3(1), 0, s, n_gt,
+                     n_eq);
+  KMD_LAUNCH_CHECK();
+  launch_select_far<T, I>(m, n, sq_dist, thr.as<const T>(), sel.as<I>(), n_gt,
+                          n_eq, s);
+  hipLaunchKernelGGL((relocation_order_krn<T, I>), dim3(1), dim3(1024), 0, s, n,
+                     m, sq_dist, sel.as<const I>(), n_gt, n_eq,
+                     order.as<int64_t>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL((relocate_apply_krn<T, I>), dim3(ceil_div<int64_t>(d, kBlock)),
+                     dim3(kBlock), 0, s, n, d, k, m, X, w, labels, empty_list,
+                     order.as<const int64_t>(), centroids, sizes,
+                     per_sample_inertia);
+  KMD_LAUNCH_CHECK();
+}
+
+// ---- explicit instantiations -----------------------------------------------
+#define KMD_INST_T(T)                                                          \
+  template void launch_broadcast_divide<T>(int64_t, int64_t, T *, const T *,  \
+                                           hipStream_t);                      \
+  template void launch_half_l2_norm<T>(int64_t, int64_t, const T *, T *,      \
+                                       hipStream_t);                          \
+  template void launch_centroid_shifts<T>(int64_t, int64_t, const T *,        \
+                                          const T *, T *, hipStream_t);       \
+  template void launch_reduce_sum<T>(int64_t, const T *, double *,            \
+                                     hipStream_t);                            \
+  template void launch_sum_to_scalar<T>(int64_t, const T *, T *,              \
+                                        hipStream_t);                         \
+  template void launch_max<T>(int64_t, const T *, T *, hipStream_t);
+KMD_INST_T(float)
+KMD_INST_T(double)
+
+#define KMD_INST_TI(T, I)                                                     \
+  template void launch_reduce_centroids<T, I>(                                \
+      int64_t, int64_t, int64_t, const T *, const T *, T *, T *, I *, I *,    \
+      int64_t, hipStream_t);                                                  \
+  template void launch_inertia<T, I>(int64_t, int64_t, int64_t, const T *,   \
+                                     const T *, const T *, const I *, T *,    \
+                                     hipStream_t);                            \
+  template void launch_select_far<T, I>(int64_t, int64_t, const T *,         \
+                                        const T *, I *, I *, I *,             \
+                                        hipStream_t);                         \
+  template void launch_relocate<T, I>(int64_t, int64_t, int64_t, int64_t,    \
+                                      const T *, const T *, const I *,        \
+                                      const I *, const T *, T *, T *, T *,    \
+                                      hipStream_t);
+KMD_INST_TI(float, int32_t)
+KMD_INST_TI(float, int64_t)
+KMD_INST_TI(double, int32_t)
+KMD_INST_TI(double, int64_t)
+
+}  // namespace kmd

@@ -1,0 +1,135 @@
+"""GPU tests of kmeans_lloyd_driver (src/kmeans_lloyd_driver.hpp:21-431
+semantics): the reference's known answer (n_iter == 2, exact labels),
+parity with the C restatement and the FP64 oracle on the CPU-runnable
+benchmark configuration (n=50,000, d=14, k=127), empty-cluster relocation
+with one and several empty clusters, max_iter = 0, verbose output and
+stream/depends ordering."""
+import numpy as np
+import pytest
+from numpy.testing import assert_allclose, assert_array_equal
+
+from oracle import c_oracle as co
+from oracle import lloyd_oracle as lo
+from tests.conftest import CORNERS, to_dev, to_np
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def q():
+    return torch.cuda.current_stream()
+
+
+def run_driver(kd, Xt, w, init, tol, max_iter, itype=np.int32, verbose=False,
+               h=8, wg=128, occ=0.7):
+    dev = torch.device("cuda", 0)
+    X_d, w_d = to_dev(Xt), to_dev(w)
+    init_d = to_dev(init)
+    res_d = torch.empty_like(init_d)
+    lab_d = torch.empty(Xt.shape[1], dtype=torch.int32 if itype == np.int32 else torch.int64,
+                        device=dev)
+    n_iter, inertia = kd.kmeans_lloyd_driver(X_d, w_d, init_d, lab_d, res_d, tol, verbose,
+                                             max_iter, h, wg, occ, sycl_queue=q())
+    return to_np(lab_d).astype(np.int64), to_np(res_d), int(n_iter), inertia
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("it", [np.int32, np.int64])
+def test_driver_kat(kd, golden, torch_dev, dt, it):
+    X = golden[f"drv_X_{'f32' if dt == np.float32 else 'f64'}"].astype(dt)
+    Xt = np.ascontiguousarray(X.T)
+    Ct = np.ascontiguousarray(CORNERS.T.astype(dt))
+    lab, C, n_iter, inertia = run_driver(kd, Xt, np.ones(X.shape[0], dt), Ct,
+                                         float(golden["drv_tol"]), int(golden["drv_max_iter"]),
+                                         itype=it)
+    assert_array_equal(lab, np.repeat(np.arange(8), 32))
+    assert n_iter < int(golden["drv_max_iter"])
+    assert n_iter == int(golden["drv_expected_n_iter"])
+    assert inertia.shape == (1,) and inertia.dtype == dt
+    assert_allclose(C, Xt.reshape(3, 8, 32).mean(axis=-1), rtol=4 * np.finfo(dt).resolution)
+    _, _, _, inert_o = lo.kmeans_lloyd(Xt, np.ones(X.shape[0]), Ct, 1e-6, 255, dtype=np.float64)
+    assert_allclose(inertia[0], inert_o, rtol=1e-5)
+
+
+def test_driver_c0_config_parity(kd, torch_dev):
+    """n=50,000, d=14, k=127 blobs (std 1, centers in [-10,10]), random-sample
+    init, max_iter=100, tol=1e-4."""
+    Xt, init = lo.gaussian_blobs(50_000, 14, 127, seed=1000, center_box=10.0, std=1.0)
+    w = np.ones(Xt.shape[1], np.float32)
+    lab, C, n_iter, inertia = run_driver(kd, Xt, w, init, 1e-4, 100)
+    lc, Cc, itc, inc = co.lloyd(Xt, w, init, 1e-4, 100)
+    l64, C64, it64, in64 = lo.kmeans_lloyd(Xt, w, init, 1e-4, 100, dtype=np.float64)
+    assert abs(n_iter - itc) <= 1
+    assert (lab != lc).sum() <= 5
+    scale = np.abs(C64).max()
+    assert np.abs(C - C64).max() <= 1e-5 * scale
+    assert_allclose(inertia[0], in64, rtol=1e-5)
+
+
+def test_driver_relocates_empty_clusters(kd, torch_dev):
+    """Init centroids far from the data leave clusters empty in the first
+    iteration; the fit must follow the oracle's relocation semantics."""
+    Xt, init = lo.gaussian_blobs(20_000, 8, 16, seed=77, dtype=np.float64)
+    init = init.copy()
+    init[:, 3] = 1e4
+    init[:, 7] = -1e4
+    init[:, 11] = 5e3
+    w = np.linspace(0.5, 1.5, Xt.shape[1])
+    lab, C, n_iter, inertia = run_driver(kd, Xt, w, init, 1e-9, 50)
+    l64, C64, it64, in64 = lo.kmeans_lloyd(Xt, w, init, 1e-9, 50, dtype=np.float64)
+    assert n_iter == it64
+    assert_array_equal(lab, l64)
+    assert_allclose(C, C64, rtol=1e-9, atol=1e-9)
+    assert_allclose(inertia[0], in64, rtol=1e-9)
+
+
+def test_driver_max_iter_zero(kd, torch_dev):
+    Xt, init = lo.gaussian_blobs(3000, 5, 7, seed=5)
+    w = np.ones(3000, np.float32)
+    lab, C, n_iter, inertia = run_driver(kd, Xt, w, init, 0.0, 0)
+    assert n_iter == 0
+    assert_array_equal(C, init)
+    assert_array_equal(lab, co.assign(Xt, init))
+    exp = np.sum(lo.compute_inertia(Xt.astype(np.float64), w, init.astype(np.float64), lab))
+    assert_allclose(inertia[0], exp, rtol=1e-5)
+
+
+def test_driver_verbose_prints_each_iteration(kd, torch_dev, capfd):
+    Xt, init = lo.gaussian_blobs(2000, 4, 5, seed=6)
+    lab, C, n_iter, inertia = run_driver(kd, Xt, np.ones(2000, np.float32), init, 0.0, 3,
+                                         verbose=True)
+    out = capfd.readouterr().out
+    assert out.count("Iteration:") == n_iter
+    assert "Inertia:" in out
+
+
+def test_driver_other_stream_and_depends(kd, torch_dev):
+    """Work enqueued on a library stream after an event of torch's stream."""
+    Xt, init = lo.gaussian_blobs(10_000, 6, 9, seed=8)
+    s = kd.Stream()
+    X_d = to_dev(Xt)
+    C_d = to_dev(init)
+    hn = torch.empty(9, device=torch_dev)
+    ev_torch = kd.record_event(q())
+    ht, ev = kd.half_l2_norm_squared(C_d, hn, 64, s, depends=[ev_torch])
+    ids = torch.empty(10_000, dtype=torch.int32, device=torch_dev)
+    ht2, ev2 = kd.assignment(X_d, C_d, hn, ids, 8, 256, s, depends=[ev])
+    ht2.wait()
+    ht.wait()
+    assert ev2.is_complete
+    assert_array_equal(to_np(ids), co.assign(Xt, init))
+    s.synchronize()
+    s.close()
+
+
+def test_driver_ragged_and_multiwindow_shapes(kd, torch_dev):
+    for n, d, k in ((1, 3, 1), (257, 33, 65), (4099, 70, 3), (12_345, 17, 300)):
+        Xt, init = lo.gaussian_blobs(n, d, k if k <= n else n, seed=n, dtype=np.float64)
+        kk = init.shape[1]
+        w = np.ones(n)
+        lab, C, n_iter, inertia = run_driver(kd, Xt, w, init, 1e-12, 20, itype=np.int64)
+        l64, C64, it64, in64 = lo.kmeans_lloyd(Xt, w, init, 1e-12, 20, dtype=np.float64)
+        assert n_iter == it64, (n, d, kk)
+        assert_array_equal(lab, l64)
+        assert_allclose(C, C64, rtol=1e-9, atol=1e-9)
