@@ -91,14 +91,14 @@ def cpu_baseline(X_t, init_t, n, d, k, budget_s):
     cores = len(os.sched_getaffinity(0))
     threads = int(os.environ.get("OMP_NUM_THREADS", cores))
     threads = max(1, min(threads, cores))
-    n_s = min(n, 1_000_000)
+    n_s = min(n, 2_000_000)
     Xs = X_t[:, :n_s].cpu().numpy().copy()
     Cs = init_t.cpu().numpy().copy()
     w = np.ones(n_s, dtype=np.float32)
     t0 = time.perf_counter()
     c_oracle.lloyd(Xs, w, Cs, 0.0, 1, nthreads=threads)
     t1 = time.perf_counter() - t0          # one iteration + epilogue (~2 passes)
-    iters = max(1, min(50, int(budget_s / max(t1 / 2, 1e-3)) - 1))
+    iters = max(1, min(500, int(budget_s / max(t1 / 2, 1e-3)) - 1))
     t0 = time.perf_counter()
     _, _, it, _ = c_oracle.lloyd(Xs, w, Cs, 0.0, iters, nthreads=threads)
     el = time.perf_counter() - t0

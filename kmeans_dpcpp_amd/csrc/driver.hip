@@ -35,41 +35,58 @@ namespace kmd {
 template <typename T, typename I>
 struct LloydWorkspace {
   int64_t n, d, k;
-  int blocks = 1;          // private copies (one per block when store_flush)
-  bool fast = false;       // MFMA fast path
+  int blocks = 1;          // private copies / partial slots
+  bool fast = false;       // MFMA fast path (fixed-point partials)
   bool store_flush = false;
   FusedPlan plan;
   TempBuffer hn, sizes, shifts, per_sample, copies, sizes_copies, list,
-      scalars;
+      scalars, scales;
+  // copies_bytes / sizes_bytes: private copies of the generic kernel, or the
+  // fast path's per-block partials (in `copies`, sizes_bytes = 0)
   LloydWorkspace(int64_t n_, int64_t d_, int64_t k_, int blocks_, bool fast_,
-                 bool store_, const FusedPlan &plan_, hipStream_t s)
+                 bool store_, const FusedPlan &plan_, size_t copies_bytes,
+                 size_t sizes_bytes, hipStream_t s)
       : n(n_), d(d_), k(k_), blocks(blocks_), fast(fast_), store_flush(store_),
         plan(plan_), hn(sizeof(T) * k_, s), sizes(sizeof(T) * k_, s),
         shifts(sizeof(T) * k_, s), per_sample(sizeof(T) * (n_ > 0 ? n_ : 1), s),
-        copies(sizeof(T) * (size_t)blocks_ * d_ * k_, s),
-        sizes_copies(sizeof(T) * (size_t)blocks_ * k_, s),
-        list(sizeof(I) * (k_ + 1), s), scalars(32, s) {}
+        copies(copies_bytes, s), sizes_copies(sizes_bytes, s),
+        list(sizeof(I) * (k_ + 1), s), scalars(32, s),
+        scales(fast_ ? sizeof(float) * (d_ + 1) + 16 : 0, s) {}
   I *n_empty() const { return reinterpret_cast<I *>(scalars.as<char>() + 8); }
   double *shift_sum() const { return scalars.as<double>(); }
+  float *scale() const { return scales.as<float>(); }
+  int *nonfinite() const {
+    return reinterpret_cast<int *>(scales.as<char>() + sizeof(float) * (d + 1));
+  }
 };
 
 template <typename T, typename I>
 static std::unique_ptr<LloydWorkspace<T, I>> make_workspace(int64_t n,
                                                             int64_t d,
                                                             int64_t k,
+                                                            const T *X,
+                                                            const T *w,
                                                             hipStream_t s) {
-  if (fast_fused_supported<T>(d, k)) {
-    return std::make_unique<LloydWorkspace<T, I>>(
-        n, d, k, fast_fused_blocks(n, d, k), true, true, FusedPlan{}, s);
+  if (n > 0 && fast_fused_supported<T>(d, k)) {
+    const int blocks = fast_fused_blocks<I>(n, d, k);
+    auto ws = std::make_unique<LloydWorkspace<T, I>>(
+        n, d, k, blocks, true, true, FusedPlan{},
+        sizeof(int64_t) * (size_t)blocks * (d * k + k), 0, s);
+    // fixed-point scales: X and w do not change during the fit
+    if constexpr (std::is_same<T, float>::value)
+      launch_fixed_point_scales(n, d, X, w, ws->scale(), ws->nonfinite(), s);
+    return ws;
   }
   FusedPlan p = plan_generic_fused<T>(n, d, k);
   if (p.priv)
-    return std::make_unique<LloydWorkspace<T, I>>(n, d, k, p.blocks, false,
-                                                  true, p, s);
+    return std::make_unique<LloydWorkspace<T, I>>(
+        n, d, k, p.blocks, false, true, p, sizeof(T) * (size_t)p.blocks * d * k,
+        sizeof(T) * (size_t)p.blocks * k, s);
   // sums too large for LDS: global atomics into a few zeroed copies
   const int copies = 8;
-  return std::make_unique<LloydWorkspace<T, I>>(n, d, k, copies, false, false,
-                                                p, s);
+  return std::make_unique<LloydWorkspace<T, I>>(
+      n, d, k, copies, false, false, p, sizeof(T) * (size_t)copies * d * k,
+      sizeof(T) * (size_t)copies * k, s);
 }
 
 // Optional timing of the fused-step kernel with HIP events on the driver's
@@ -117,7 +134,20 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
   const int64_t n = ws.n, d = ws.d, k = ws.k;
   launch_half_l2_norm<T>(d, k, cur, ws.hn.template as<T>(), s);
   KMD_HIP_CHECK(hipMemsetAsync(ws.n_empty(), 0, sizeof(I), s));
-  if (!ws.store_flush) {
+  if (ws.fast) {
+    timer.begin(s);
+    launch_fast_fused_fixed<T, I>(n, d, k, X, w, cur,
+                                  ws.hn.template as<const T>(), labels,
+                                  ws.scale(), ws.nonfinite(),
+                                  ws.copies.template as<void>(), ws.blocks, s);
+    timer.end(s);
+    launch_reduce_fixed_partials<T, I>(
+        ws.blocks, d, k, ws.copies.template as<const void>(), ws.scale(),
+        ws.nonfinite(), ws.sizes.template as<T>(), next,
+        ws.list.template as<I>(), ws.n_empty(), k, s);
+    return;
+  }
+  if (!ws.store_flush || n == 0) {
     KMD_HIP_CHECK(hipMemsetAsync(ws.copies.template as<T>(), 0,
                                  ws.copies.bytes(), s));
     KMD_HIP_CHECK(hipMemsetAsync(ws.sizes_copies.template as<T>(), 0,
@@ -125,23 +155,11 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
   }
   if (n > 0) {
     timer.begin(s);
-    if (ws.fast) {
-      launch_fast_fused<T, I>(n, d, k, ws.blocks, true, X, w, cur,
-                              ws.hn.template as<const T>(), labels,
-                              ws.copies.template as<T>(),
-                              ws.sizes_copies.template as<T>(), s);
-    } else {
-      launch_fused_generic<T, I>(n, d, k, ws.blocks, ws.plan, ws.store_flush,
-                                 X, w, cur, ws.hn.template as<const T>(),
-                                 labels, ws.copies.template as<T>(),
-                                 ws.sizes_copies.template as<T>(), s);
-    }
+    launch_fused_generic<T, I>(n, d, k, ws.blocks, ws.plan, ws.store_flush, X,
+                               w, cur, ws.hn.template as<const T>(), labels,
+                               ws.copies.template as<T>(),
+                               ws.sizes_copies.template as<T>(), s);
     timer.end(s);
-  } else {
-    KMD_HIP_CHECK(hipMemsetAsync(ws.copies.template as<T>(), 0,
-                                 ws.copies.bytes(), s));
-    KMD_HIP_CHECK(hipMemsetAsync(ws.sizes_copies.template as<T>(), 0,
-                                 ws.sizes_copies.bytes(), s));
   }
   launch_reduce_centroids<T, I>(ws.blocks, d, k,
                                 ws.sizes_copies.template as<const T>(),
@@ -156,7 +174,7 @@ void driver_lloyd(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
                   int64_t max_iter, int64_t *n_iters_out, double *inertia_out,
                   kmd_print_fn print_fn, void *print_user, hipStream_t s,
                   DriverStats *stats) {
-  auto ws_ptr = make_workspace<T, I>(n, d, k, s);
+  auto ws_ptr = make_workspace<T, I>(n, d, k, X, w, s);
   FusedTimer timer(stats != nullptr);
   auto &ws = *ws_ptr;
   T *cur = init;

@@ -1,29 +1,42 @@
 // FP32 MFMA fast path of the fused Lloyd step and of the assignment
 // (float data, d <= 64, centroids + sums fitting in LDS).
 //
-// The cross term dot[c, s] = sum_f C[f, c] X[f, s] is a
-// (k x d) . (d x n) product computed with v_mfma_f32_32x32x2f32: the 32 rows
-// of an MFMA tile are centroids, its 32 columns are samples, its K = 2 steps
-// walk the features.  In that layout each lane's 16 accumulators belong to a
-// single sample (column lane % 32) and 16 centroids, so the argmin is 16
-// register compares plus one exchange with lane ^ 32.
+// Cross term.  dot[c, s] = sum_f C[f, c] X[f, s] is a (k x d) . (d x n)
+// product computed with v_mfma_f32_32x32x2f32: the 32 rows of an MFMA tile
+// are centroids, its 32 columns are samples, its K = 2 steps walk the
+// features.  In that layout each lane's 16 accumulators belong to a single
+// sample (column lane % 32) and 16 centroids, so the argmin is 16 register
+// compares plus one exchange with lane ^ 32.
 //
-// Per wave, a super-block of 32 * S samples is processed at once: lane l
-// holds X[f][s0 + S * (l % 32) + q] for its feature parity f = 2 ks + l / 32
-// and q < S, read once from HBM with one S-wide load per feature (the
-// column a sample occupies is a free choice of the product, so neighbouring
-// samples sit in one lane).  The whole centroid matrix (zero-padded to 32
+// Samples.  A wave processes super-blocks of 32 * S samples: lane l holds
+// X[f][s0 + S (l % 32) + q] for its feature parity f = 2 ks + l / 32 and
+// q < S, read once from HBM with one S-wide load per feature (the column a
+// sample occupies is a free choice of the product, so neighbouring samples
+// sit in one lane).  The centroid matrix (zero-padded to 32
 // columns and an even number of rows) and the half-norms (+inf padded) stay
-// in LDS for the life of the block, and every A operand read from LDS feeds
-// S MFMAs.  The update adds w x into per-block centroid sums and counts in
-// LDS (ds_add_f32), flushed once per block either as a plain store into the
-// block's own private copy (driver; deterministic) or with global atomics
-// into copy (block % n_copies) (the fused_lloyd_single_step entry point).
+// in LDS for the life of the persistent block; each A operand read from LDS
+// feeds S MFMAs.
 //
-// Arithmetic: FP32 products accumulated in FP32 by the matrix core; the
-// pseudo-inertia hn[c] - dot and the argmin rule (strict <, lowest index on
-// ties) are those of src/device_functions.hpp:139-162.  Relative to the
-// sequential FMA chain of the reference the dot products differ by a few
+// Update.  w_s x_s is accumulated into per-block sums in LDS as 64-bit
+// fixed-point integers (ds_add_u64): q = rint(w x 2^E_f), with one exponent
+// per feature chosen by a pre-pass so that every element fits an int32
+// (max|w| max|x_f| 2^E_f < 2^30, a 2^-31 relative resolution per element)
+// and the total fits an int64 (n max|w| max|x_f| 2^E_f < 2^61).
+// Integer addition is exact and associative, so the sums do not depend on
+// the order samples arrive in (no atomics-order nondeterminism) and the only
+// rounding is one per element and one at the final conversion.  LDS integer
+// atomics run at full LDS rate, unlike ds_add_f32.  If X or w holds a
+// non-finite value the pre-pass raises a flag and the kernel accumulates
+// plain FP32 with ds_add_f32 instead (so NaN/inf propagate as in the
+// reference).  Per-block sums are flushed once: as raw partials into the
+// block's own slot (driver: exact, deterministic reduction) or converted to
+// FP32 and added with global atomics into private copy (block % n_copies)
+// (the fused_lloyd_single_step entry point).
+//
+// Arithmetic of the labels: FP32 products accumulated in FP32 by the matrix
+// core; the pseudo-inertia hn[c] - dot and the argmin rule (strict <, lowest
+// index on ties) are those of src/device_functions.hpp:139-162.  Relative to
+// the sequential FMA chain of the reference the dot products differ by a few
 // ulps, so labels can differ only on sub-ulp margins.
 #include "lloyd_fast.hpp"
 
@@ -32,28 +45,19 @@ namespace kmd {
 namespace {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned long long u64;
 
-constexpr int kThreads = 256;
-constexpr int kWavesPerBlock = kThreads / kWave;
-// 32-sample MFMA column blocks per wave (S).  S = 2 keeps the d = 32 kernel
-// at 3 waves per SIMD; S = 4 halves the LDS reads of A operands.
-constexpr int kSDefault = 2;
-
-inline int mfma_s() {
-  static const int s = [] {
-    const char *e = std::getenv("KMD_MFMA_S");
-    return (e && std::atoi(e) == 4) ? 4 : kSDefault;
-  }();
-  return s;
-}
+// 32-sample MFMA column blocks per wave; each A operand read from LDS feeds
+// S MFMAs.
+constexpr int kS = 2;
 constexpr int kMaxD = 64;
 constexpr size_t kLdsPerCu = 160 * 1024;
 
 inline int pad32(int64_t k) { return (int)ceil_div<int64_t>(k, 32) * 32; }
 
 // xr[ks][q] = X[2 ks + half][s_first + q] (0 outside the matrix).  FULL:
-// d == 2 KS, so every feature row exists.  VEC: rows are 16-byte aligned and
-// s_first is a multiple of S, so one S-wide load reads the S samples.
+// d == 2 KS, so every feature row exists.  vec: rows are S-element aligned
+// and s_first is a multiple of S, so one S-wide load reads the S samples.
 template <int KS, int S, bool FULL>
 __device__ inline void load_x(const float *__restrict__ X, int64_t n, int d,
                               int64_t s_first, int half, bool vec,
@@ -65,13 +69,7 @@ __device__ inline void load_x(const float *__restrict__ X, int64_t n, int d,
     if (FULL || f < d) {
       const float *row = X + (int64_t)f * n + s_first;
       if (whole) {
-        if constexpr (S == 4) {
-          const float4 v = *reinterpret_cast<const float4 *>(row);
-          xr[ks][0] = v.x;
-          xr[ks][1] = v.y;
-          xr[ks][2] = v.z;
-          xr[ks][3] = v.w;
-        } else if constexpr (S == 2) {
+        if constexpr (S == 2) {
           const float2 v = *reinterpret_cast<const float2 *>(row);
           xr[ks][0] = v.x;
           xr[ks][1] = v.y;
@@ -91,28 +89,90 @@ __device__ inline void load_x(const float *__restrict__ X, int64_t n, int d,
   }
 }
 
-template <int KS, int S, bool FULL, typename I, bool UPDATE, bool STORE_FLUSH>
-__global__ void __launch_bounds__(kThreads)
-    mfma_lloyd_krn(int64_t n, int d, int k, int kpad, int vec,
-                   const float *__restrict__ X, const float *__restrict__ w,
-                   const float *__restrict__ C, const float *__restrict__ hn,
-                   I *__restrict__ labels, float *__restrict__ copies,
-                   float *__restrict__ sizes_copies, int64_t n_copies) {
-  extern __shared__ __align__(16) float lds[];
-  float *Cl = lds;                    // [2*KS][kpad] centroid rows
-  float *Hl = Cl + 2 * KS * kpad;     // [kpad] half-norms
-  float *Sl = Hl + kpad;              // [d][k] sums then [k] counts
+// acc[q] = C[:, tile] . X[:, columns of block q] for one 32-centroid tile.
+template <int KS, int S>
+__device__ inline void tile_mfma(const float *__restrict__ Ccol, int kpad,
+                                 int half, const float (&xr)[KS][S],
+                                 f32x16 (&acc)[S]) {
+  const float a0 = Ccol[half * kpad];
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    f32x16 z = {};
+    acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, xr[0][q], z, 0, 0, 0);
+  }
+#pragma unroll
+  for (int ks = 1; ks < KS; ++ks) {
+    const float a = Ccol[(2 * ks + half) * kpad];
+#pragma unroll
+    for (int q = 0; q < S; ++q)
+      acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, xr[ks][q], acc[q], 0, 0,
+                                                     0);
+  }
+}
+
+// Running argmin over the 16 rows this lane holds in tile `ct`:
+// m = 8 (r / 4) + 4 half + (r % 4), ascending in r.
+template <int S>
+__device__ inline void tile_argmin(const float *__restrict__ Hl, int ct,
+                                   int half, const f32x16 (&acc)[S],
+                                   float (&bestv)[S], int (&besti)[S]) {
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int m0 = ct * 32 + 8 * g + 4 * half;
+    const float4 h4 = *reinterpret_cast<const float4 *>(Hl + m0);
+    const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        const float v = hv[e] - acc[q][4 * g + e];
+        const bool upd = v < bestv[q];
+        bestv[q] = upd ? v : bestv[q];
+        besti[q] = upd ? (m0 + e) : besti[q];
+      }
+    }
+  }
+}
+
+// Launch parameters shared by the kernel variants.
+struct FastArgs {
+  int64_t n;
+  int d, k, kpad, vec;
+  const float *X, *w, *C, *hn;
+  const float *scale;       // [d + 1] fixed-point scales (2^E_f, 2^E_w)
+  const int *nonfinite;     // 1: X or w has a non-finite value
+  void *labels;
+  void *copies;             // u64 [blocks][d*k + k] or float [n_copies][d*k]
+  float *sizes_copies;      // float [n_copies][k] (atomic flush only)
+  int64_t n_copies;
+};
+
+template <int NT, int KS, int S, bool FULL, typename I, bool UPDATE,
+          bool STORE_FLUSH>
+__global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
+  constexpr int kWavesPerBlock = NT / kWave;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int64_t n = a.n;
+  const int d = a.d, k = a.k, kpad = a.kpad;
+  float *Cl = reinterpret_cast<float *>(lds_raw);  // [2*KS][kpad]
+  float *Hl = Cl + 2 * KS * kpad;                   // [kpad]
+  float *Sc = Hl + kpad;                                  // [d + 1] scales
+  u64 *S64 = reinterpret_cast<u64 *>(Sc + (kMaxD + 2));  // [d*k + k] sums
+  float *S32 = reinterpret_cast<float *>(S64);            // non-finite path
   const int tid = threadIdx.x;
   const int64_t dk = (int64_t)d * k;
+  const bool fixed = UPDATE ? (a.nonfinite[0] == 0) : false;
 
-  for (int i = tid; i < 2 * KS * kpad; i += kThreads) {
+  for (int i = tid; i < 2 * KS * kpad; i += NT) {
     const int f = i / kpad, c = i - f * kpad;
-    Cl[i] = (f < d && c < k) ? C[(int64_t)f * k + c] : 0.f;
+    Cl[i] = (f < d && c < k) ? a.C[(int64_t)f * k + c] : 0.f;
   }
-  for (int c = tid; c < kpad; c += kThreads)
-    Hl[c] = (c < k) ? hn[c] : __builtin_huge_valf();
-  if (UPDATE)
-    for (int64_t i = tid; i < dk + k; i += kThreads) Sl[i] = 0.f;
+  for (int c = tid; c < kpad; c += NT)
+    Hl[c] = (c < k) ? a.hn[c] : __builtin_huge_valf();
+  if (UPDATE) {
+    for (int f = tid; f <= d; f += NT) Sc[f] = a.scale[f];
+    for (int64_t i = tid; i < dk + k; i += NT) S64[i] = 0ull;
+  }
   __syncthreads();
 
   const int lane = tid & (kWave - 1);
@@ -122,12 +182,16 @@ __global__ void __launch_bounds__(kThreads)
   const int n_ct = kpad / 32;
   constexpr int kSuper = 32 * S;
   const int64_t n_super = ceil_div<int64_t>(n, kSuper);
+  const float *X = a.X;
+  const float *w = a.w;
+  const bool vec = a.vec != 0;
+  I *labels = reinterpret_cast<I *>(a.labels);
 
   for (int64_t sb = (int64_t)blockIdx.x * kWavesPerBlock + wave; sb < n_super;
        sb += (int64_t)gridDim.x * kWavesPerBlock) {
     const int64_t s_first = sb * kSuper + (int64_t)S * col;
     float xr[KS][S];
-    load_x<KS, S, FULL>(X, n, d, s_first, half, vec != 0, xr);
+    load_x<KS, S, FULL>(X, n, d, s_first, half, vec, xr);
 
     float bestv[S];
     int besti[S];
@@ -138,36 +202,8 @@ __global__ void __launch_bounds__(kThreads)
     }
     for (int ct = 0; ct < n_ct; ++ct) {
       f32x16 acc[S];
-#pragma unroll
-      for (int q = 0; q < S; ++q)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[q][r] = 0.f;
-      const float *Ccol = Cl + ct * 32 + col;
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        const float a = Ccol[(2 * ks + half) * kpad];
-#pragma unroll
-        for (int q = 0; q < S; ++q)
-          acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, xr[ks][q], acc[q], 0,
-                                                         0, 0);
-      }
-      // rows held by this lane: m = 8 (r / 4) + 4 half + (r % 4), ascending in r
-#pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const int m0 = ct * 32 + 8 * g + 4 * half;
-        const float4 h4 = *reinterpret_cast<const float4 *>(Hl + m0);
-        const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-#pragma unroll
-          for (int q = 0; q < S; ++q) {
-            const float v = hv[e] - acc[q][4 * g + e];
-            const bool upd = v < bestv[q];
-            bestv[q] = upd ? v : bestv[q];
-            besti[q] = upd ? (m0 + e) : besti[q];
-          }
-        }
-      }
+      tile_mfma<KS, S>(Cl + ct * 32 + col, kpad, half, xr, acc);
+      tile_argmin<S>(Hl, ct, half, acc, bestv, besti);
     }
     // merge the two halves of the column (lanes l and l ^ 32)
 #pragma unroll
@@ -189,37 +225,115 @@ __global__ void __launch_bounds__(kThreads)
         if (s_first + q < n) {
           const float ws = w[s_first + q];
           const int lab = besti[q];
+          if (fixed) {
+            // |w x| 2^E_f < 2^30: the rounded element fits an int32; the
+            // sign-extended value is added into the 64-bit sum
 #pragma unroll
-          for (int ks = 0; ks < KS; ++ks) {
-            const int f = 2 * ks + half;
-            if (FULL || f < d)
-              atomic_add(&Sl[(int64_t)f * k + lab], xr[ks][q] * ws);
+            for (int ks = 0; ks < KS; ++ks) {
+              const int f = 2 * ks + half;
+              if (FULL || f < d) {
+                const int qv = __float2int_rn((xr[ks][q] * ws) * Sc[f]);
+                atomicAdd(&S64[(int64_t)f * k + lab], (u64)(long long)qv);
+              }
+            }
+            if (half == 0)
+              atomicAdd(&S64[dk + lab],
+                        (u64)(long long)__float2int_rn(ws * Sc[d]));
+          } else {
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+              const int f = 2 * ks + half;
+              if (FULL || f < d)
+                atomic_add(&S32[(int64_t)f * k + lab], xr[ks][q] * ws);
+            }
+            if (half == 0) atomic_add(&S32[dk + lab], ws);
           }
-          if (half == 0) atomic_add(&Sl[dk + lab], ws);
         }
       }
     }
   }
   if (UPDATE) {
     __syncthreads();
-    const int64_t copy = STORE_FLUSH ? blockIdx.x : (blockIdx.x % n_copies);
-    float *cp = copies + copy * dk;
-    float *sp = sizes_copies + copy * k;
     if (STORE_FLUSH) {
-      for (int64_t i = tid; i < dk; i += kThreads) cp[i] = Sl[i];
-      for (int64_t c = tid; c < k; c += kThreads) sp[c] = Sl[dk + c];
-    } else {
-      for (int64_t i = tid; i < dk; i += kThreads) {
-        const float v = Sl[i];
-        if (v != 0.f) atomic_add(&cp[i], v);
+      // raw partials (int64 fixed point, or FP32 on the non-finite path)
+      // into this block's slot
+      if (fixed) {
+        u64 *cp = reinterpret_cast<u64 *>(a.copies) + blockIdx.x * (dk + k);
+        for (int64_t i = tid; i < dk + k; i += NT) cp[i] = S64[i];
+      } else {
+        float *cp = reinterpret_cast<float *>(a.copies) + blockIdx.x * (dk + k);
+        for (int64_t i = tid; i < dk + k; i += NT) cp[i] = S32[i];
       }
-      for (int64_t c = tid; c < k; c += kThreads) {
-        const float v = Sl[dk + c];
-        if (v != 0.f) atomic_add(&sp[c], v);
+    } else {
+      const int64_t copy = blockIdx.x % a.n_copies;
+      float *cp = reinterpret_cast<float *>(a.copies) + copy * dk;
+      float *sp = a.sizes_copies + copy * k;
+      for (int64_t i = tid; i < dk + k; i += NT) {
+        float v;
+        if (fixed) {
+          const int f = (int)(i / k);  // f == d for the counts
+          v = (float)((double)(long long)S64[i] / (double)Sc[f]);
+        } else {
+          v = S32[i];
+        }
+        if (v != 0.f) {
+          if (i < dk)
+            atomic_add(&cp[i], v);
+          else
+            atomic_add(&sp[i - dk], v);
+        }
       }
     }
   }
 }
+
+// ---- fixed-point scales ------------------------------------------------------
+
+// maxbits[f] = max over samples of the bit pattern of |X[f, s]| (for
+// non-negative floats the bit pattern orders like the value; any NaN is
+// above +inf); maxbits[d] the same for the weights.
+__global__ void __launch_bounds__(256)
+    absmax_bits_krn(int64_t n, int d, const float *__restrict__ X,
+                    const float *__restrict__ w, unsigned *maxbits) {
+  const int f = blockIdx.y;  // 0..d (d = weights)
+  const float *row = (f < d) ? X + (int64_t)f * n : w;
+  unsigned m = 0u;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256)
+    m = max(m, __float_as_uint(row[i]) & 0x7fffffffu);
+  for (int o = kWave / 2; o > 0; o >>= 1)
+    m = max(m, (unsigned)__shfl_xor((int)m, o, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(&maxbits[f], m);
+}
+
+// scale[f] = 2^E_f with max|w| max|x_f| 2^E_f < 2^30 (each rounded element
+// fits an int32) and n max|w| max|x_f| 2^E_f < 2^61 (the sum fits an
+// int64); scale[d] likewise for the counts.  nonfinite[0] = 1 if any value
+// is infinite or NaN.
+__global__ void scales_krn(int64_t n, int d, const unsigned *maxbits,
+                           float *scale, int *nonfinite) {
+  const unsigned wb = maxbits[d];
+  int bad = wb >= 0x7f800000u;
+  const double wmax = (double)__uint_as_float(wb);
+  for (int f = 0; f <= d; ++f) {
+    const unsigned b = maxbits[f];
+    bad |= b >= 0x7f800000u;
+    const double m = wmax * (f < d ? (double)__uint_as_float(b) : 1.0);
+    int e = 0;
+    if (m > 0.0 && m < 1e300) {
+      int ex_elem, ex_sum;
+      frexp(m, &ex_elem);                       // m < 2^ex_elem
+      frexp(m * (double)(n > 0 ? n : 1), &ex_sum);
+      const int e1 = 30 - ex_elem, e2 = 61 - ex_sum;
+      e = e1 < e2 ? e1 : e2;
+      e = e > 120 ? 120 : (e < -120 ? -120 : e);
+    }
+    scale[f] = ldexpf(1.f, e);
+  }
+  nonfinite[0] = bad;
+}
+
+// ---- launch planning -----------------------------------------------------
 
 inline int ks_bucket(int64_t d) {
   const int ks = (int)ceil_div<int64_t>(d, 2);
@@ -236,69 +350,22 @@ inline size_t lds_bytes(int64_t d, int64_t k, bool update) {
   const int kpad = pad32(k);
   const int KS = ks_bucket(d);
   size_t b = sizeof(float) * ((size_t)2 * KS * kpad + kpad);
-  if (update) b += sizeof(float) * (size_t)(d + 1) * k;
+  if (update)
+    b += sizeof(float) * (kMaxD + 2) + sizeof(u64) * (size_t)(d + 1) * k;
   return b;
 }
 
-inline int blocks_per_cu(size_t lds) {
-  int b = (int)(kLdsPerCu / (lds + 512));
-  if (b > 4) b = 4;
-  return b;
-}
+using KernFn = void (*)(FastArgs);
 
-int plan_blocks(int64_t n, size_t lds, int S) {
-  const int64_t n_super = ceil_div<int64_t>(n, 32 * S);
-  int64_t blocks = (int64_t)device_cu_count() * blocks_per_cu(lds);
-  const int64_t need = ceil_div<int64_t>(n_super, kWavesPerBlock);
-  if (blocks > need) blocks = need;
-  if (blocks < 1) blocks = 1;
-  return (int)blocks;
-}
-
-template <int KS, int S, bool FULL, typename I, bool UPDATE, bool STORE>
-void launch_ks(int64_t n, int64_t d, int64_t k, int blocks, size_t lds,
-               bool vec, const float *X, const float *w, const float *C,
-               const float *hn, I *labels, float *copies, float *sizes_copies,
-               int64_t n_copies, hipStream_t s) {
-  auto kern = mfma_lloyd_krn<KS, S, FULL, I, UPDATE, STORE>;
-  static bool attr_set = false;  // per instantiation
-  if (!attr_set) {
-    KMD_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)kLdsPerCu));
-    attr_set = true;
-  }
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kThreads), lds, s, n, (int)d,
-                     (int)k, pad32(k), (int)vec, X, w, C, hn, labels, copies,
-                     sizes_copies, n_copies);
-  KMD_LAUNCH_CHECK();
-}
-
-template <typename I, bool UPDATE, bool STORE>
-void dispatch_ks(int64_t n, int64_t d, int64_t k, int blocks, size_t lds,
-                 bool vec, const float *X, const float *w, const float *C,
-                 const float *hn, I *labels, float *copies,
-                 float *sizes_copies, int64_t n_copies, hipStream_t s) {
-#define KMD_KS_LAUNCH(KS, SS)                                             \
-  if (d == 2 * KS)                                                        \
-    launch_ks<KS, SS, true, I, UPDATE, STORE>(n, d, k, blocks, lds, vec,  \
-                                              X, w, C, hn, labels,        \
-                                              copies, sizes_copies,       \
-                                              n_copies, s);               \
-  else                                                                    \
-    launch_ks<KS, SS, false, I, UPDATE, STORE>(n, d, k, blocks, lds, vec, \
-                                               X, w, C, hn, labels,       \
-                                               copies, sizes_copies,      \
-                                               n_copies, s);
+template <int NT, typename I, bool UPDATE, bool STORE>
+KernFn pick_nt(int64_t d) {
+  const int ks = ks_bucket(d);
+  const bool full = (d == 2 * ks);
 #define KMD_KS_CASE(KS)                                                   \
   case KS:                                                                \
-    if (mfma_s() == 4) {                                                  \
-      KMD_KS_LAUNCH(KS, 4)                                                \
-    } else {                                                              \
-      KMD_KS_LAUNCH(KS, 2)                                                \
-    }                                                                     \
-    return;
-  switch (ks_bucket(d)) {
+    return full ? &mfma_lloyd_krn<NT, KS, kS, true, I, UPDATE, STORE>     \
+                : &mfma_lloyd_krn<NT, KS, kS, false, I, UPDATE, STORE>;
+  switch (ks) {
     KMD_KS_CASE(2)
     KMD_KS_CASE(4)
     KMD_KS_CASE(8)
@@ -307,28 +374,146 @@ void dispatch_ks(int64_t n, int64_t d, int64_t k, int blocks, size_t lds,
     KMD_KS_CASE(24)
     KMD_KS_CASE(32)
     default:
-      throw RuntimeError("unsupported feature count for the MFMA path");
+      return nullptr;
   }
 #undef KMD_KS_CASE
-#undef KMD_KS_LAUNCH
+}
+
+// Threads per block of the MFMA kernels: 8 waves share one copy of the
+// centroids and of the per-block sums in LDS.
+constexpr int kThreads = 512;
+
+// A resolved launch: kernel variant, block size, dynamic LDS and the
+// persistent grid (occupancy-limited, never more blocks than work).
+struct MfmaLaunch {
+  KernFn fn = nullptr;
+  int threads = kThreads;
+  int blocks = 0;
+  size_t lds = 0;
+};
+
+template <typename I, bool UPDATE, bool STORE>
+MfmaLaunch plan_launch(int64_t n, int64_t d, int64_t k) {
+  MfmaLaunch L;
+  if (d < 1 || d > kMaxD || k < 1 || k > (1 << 20)) return L;
+  L.lds = lds_bytes(d, k, UPDATE);
+  if (L.lds > kLdsPerCu) return L;
+  L.threads = kThreads;
+  L.fn = pick_nt<kThreads, I, UPDATE, STORE>(d);
+  if (L.fn == nullptr) return L;
+  static thread_local KernFn attr_done[128] = {};
+  bool seen = false;
+  for (KernFn f : attr_done) seen |= (f == L.fn);
+  if (!seen) {
+    KMD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(L.fn),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)kLdsPerCu));
+    for (KernFn &f : attr_done)
+      if (f == nullptr) {
+        f = L.fn;
+        break;
+      }
+  }
+  int per_cu = 0;
+  KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void *>(L.fn), L.threads, L.lds));
+  if (per_cu < 1) {
+    L.fn = nullptr;
+    return L;
+  }
+  const int64_t waves_per_block = L.threads / kWave;
+  const int64_t n_super = ceil_div<int64_t>(n > 0 ? n : 1, 32 * kS);
+  int64_t blocks = (int64_t)device_cu_count() * per_cu;
+  const int64_t need = ceil_div<int64_t>(n_super, waves_per_block);
+  if (blocks > need) blocks = need;
+  if (blocks < 1) blocks = 1;
+  L.blocks = (int)blocks;
+  return L;
+}
+
+inline void run(const MfmaLaunch &L, const FastArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(L.fn, dim3(L.blocks), dim3(L.threads), L.lds, s, a);
+  KMD_LAUNCH_CHECK();
 }
 
 inline bool vec_ok(const float *X, int64_t n) {
-  const int64_t S = mfma_s();
+  const int64_t S = kS;
   return (n % S == 0) && ((reinterpret_cast<uintptr_t>(X) & (4 * S - 1)) == 0);
+}
+
+inline FastArgs make_args(int64_t n, int64_t d, int64_t k, const float *X,
+                          const float *w, const float *C, const float *hn,
+                          void *labels) {
+  FastArgs a{};
+  a.n = n;
+  a.d = (int)d;
+  a.k = (int)k;
+  a.kpad = pad32(k);
+  a.vec = vec_ok(X, n) ? 1 : 0;
+  a.X = X;
+  a.w = w;
+  a.C = C;
+  a.hn = hn;
+  a.labels = labels;
+  return a;
 }
 
 }  // namespace
 
+void launch_fixed_point_scales(int64_t n, int64_t d, const float *X,
+                               const float *w, float *scale, int *nonfinite,
+                               hipStream_t s) {
+  TempBuffer bits(sizeof(unsigned) * (d + 1), s);
+  KMD_HIP_CHECK(hipMemsetAsync(bits.as<unsigned>(), 0, bits.bytes(), s));
+  int gx = (int)ceil_div<int64_t>(n > 0 ? n : 1, 256 * 8);
+  const int cap = 4096 / (int)(d + 1) + 1;
+  if (gx > cap) gx = cap;
+  if (gx < 1) gx = 1;
+  if (n > 0) {
+    hipLaunchKernelGGL(absmax_bits_krn, dim3(gx, (unsigned)(d + 1)), dim3(256),
+                       0, s, n, (int)d, X, w, bits.as<unsigned>());
+    KMD_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(scales_krn, dim3(1), dim3(1), 0, s, n, (int)d,
+                     bits.as<const unsigned>(), scale, nonfinite);
+  KMD_LAUNCH_CHECK();
+}
+
 template <typename T>
 bool fast_fused_supported(int64_t d, int64_t k) {
   if (!std::is_same<T, float>::value) return false;
-  if (d < 1 || d > kMaxD || k < 1) return false;
-  return blocks_per_cu(lds_bytes(d, k, true)) >= 1;
+  return plan_launch<int32_t, true, true>(1, d, k).fn != nullptr &&
+         plan_launch<int64_t, true, true>(1, d, k).fn != nullptr &&
+         plan_launch<int32_t, true, false>(1, d, k).fn != nullptr &&
+         plan_launch<int64_t, true, false>(1, d, k).fn != nullptr;
 }
 
+template <typename I>
 int fast_fused_blocks(int64_t n, int64_t d, int64_t k) {
-  return plan_blocks(n, lds_bytes(d, k, true), mfma_s());
+  return plan_launch<I, true, true>(n, d, k).blocks;
+}
+
+template <typename T, typename I>
+void launch_fast_fused_fixed(int64_t n, int64_t d, int64_t k, const T *X,
+                             const T *w, const T *C, const T *hn, I *labels,
+                             const float *scale, const int *nonfinite,
+                             void *partials, int64_t n_partials,
+                             hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (n <= 0) return;
+    const MfmaLaunch L = plan_launch<I, true, true>(n, d, k);
+    if (L.fn == nullptr) throw RuntimeError("MFMA fused step: no launch plan");
+    if (n_partials < L.blocks)
+      throw RuntimeError("fused step needs one partial slot per block");
+    FastArgs a = make_args(n, d, k, X, w, C, hn, labels);
+    a.scale = scale;
+    a.nonfinite = nonfinite;
+    a.copies = partials;
+    a.n_copies = n_partials;
+    run(L, a, s);
+  } else {
+    throw RuntimeError("MFMA fused step is FP32 only");
+  }
 }
 
 template <typename T, typename I>
@@ -338,17 +523,21 @@ void launch_fast_fused(int64_t n, int64_t d, int64_t k, int64_t n_copies,
                        hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     if (n <= 0) return;
-    const size_t lds = lds_bytes(d, k, true);
-    const int blocks = plan_blocks(n, lds, mfma_s());
-    if (store_flush && n_copies < blocks)
-      throw RuntimeError("store-flush needs one private copy per block");
     if (store_flush)
-      dispatch_ks<I, true, true>(n, d, k, blocks, lds, vec_ok(X, n), X, w, C,
-                                 hn, labels, copies, sizes_copies, n_copies, s);
-    else
-      dispatch_ks<I, true, false>(n, d, k, blocks, lds, vec_ok(X, n), X, w, C,
-                                  hn, labels, copies, sizes_copies, n_copies,
-                                  s);
+      throw RuntimeError("store-flush partials go through the fixed-point path");
+    const MfmaLaunch L = plan_launch<I, true, false>(n, d, k);
+    if (L.fn == nullptr) throw RuntimeError("MFMA fused step: no launch plan");
+    TempBuffer sc(sizeof(float) * (d + 1) + 16, s);
+    int *flag =
+        reinterpret_cast<int *>(sc.as<char>() + sizeof(float) * (d + 1));
+    launch_fixed_point_scales(n, d, X, w, sc.as<float>(), flag, s);
+    FastArgs a = make_args(n, d, k, X, w, C, hn, labels);
+    a.scale = sc.as<const float>();
+    a.nonfinite = flag;
+    a.copies = copies;
+    a.sizes_copies = sizes_copies;
+    a.n_copies = n_copies;
+    run(L, a, s);
   } else {
     throw RuntimeError("MFMA fused step is FP32 only");
   }
@@ -359,16 +548,18 @@ void launch_fast_assign(int64_t n, int64_t d, int64_t k, const T *X,
                         const T *C, const T *hn, I *labels, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     if (n <= 0) return;
-    const size_t lds = lds_bytes(d, k, false);
-    const int blocks = plan_blocks(n, lds, mfma_s());
-    dispatch_ks<I, false, false>(n, d, k, blocks, lds, vec_ok(X, n), X,
-                                 nullptr, C, hn, labels, nullptr, nullptr, 1,
-                                 s);
+    const MfmaLaunch L = plan_launch<I, false, false>(n, d, k);
+    if (L.fn == nullptr) throw RuntimeError("MFMA assignment: no launch plan");
+    FastArgs a = make_args(n, d, k, X, nullptr, C, hn, labels);
+    a.n_copies = 1;
+    run(L, a, s);
   } else {
     throw RuntimeError("MFMA assignment is FP32 only");
   }
 }
 
+template int fast_fused_blocks<int32_t>(int64_t, int64_t, int64_t);
+template int fast_fused_blocks<int64_t>(int64_t, int64_t, int64_t);
 template bool fast_fused_supported<float>(int64_t, int64_t);
 template bool fast_fused_supported<double>(int64_t, int64_t);
 #define KMD_INST_FAST(T, I)                                                   \
@@ -376,6 +567,9 @@ template bool fast_fused_supported<double>(int64_t, int64_t);
                                         bool, const T *, const T *,           \
                                         const T *, const T *, I *, T *, T *,  \
                                         hipStream_t);                         \
+  template void launch_fast_fused_fixed<T, I>(                                \
+      int64_t, int64_t, int64_t, const T *, const T *, const T *, const T *,  \
+      I *, const float *, const int *, void *, int64_t, hipStream_t);        \
   template void launch_fast_assign<T, I>(int64_t, int64_t, int64_t,           \
                                          const T *, const T *, const T *,     \
                                          I *, hipStream_t);

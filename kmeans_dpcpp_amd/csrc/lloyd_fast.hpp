@@ -34,6 +34,7 @@ bool fast_fused_supported(int64_t d, int64_t k);
 
 // Number of blocks the MFMA fused kernel launches for n samples (so callers
 // that use store_flush can size one private copy per block).
+template <typename I>
 int fast_fused_blocks(int64_t n, int64_t d, int64_t k);
 
 template <typename T, typename I>
@@ -45,5 +46,32 @@ void launch_fast_fused(int64_t n, int64_t d, int64_t k, int64_t n_copies,
 template <typename T, typename I>
 void launch_fast_assign(int64_t n, int64_t d, int64_t k, const T *X,
                         const T *C, const T *hn, I *labels, hipStream_t s);
+
+// Fixed-point scales of the MFMA update: scale[f] = 2^E_f (f < d) and
+// scale[d] = 2^E_w such that max|w| max|x_f| 2^E_f < 2^30 (one element fits
+// an int32) and n max|w| max|x_f| 2^E_f < 2^61 (the sum fits an int64);
+// *nonfinite = 1 when X or w holds inf/NaN (the kernel then sums in FP32).
+void launch_fixed_point_scales(int64_t n, int64_t d, const float *X,
+                               const float *w, float *scale, int *nonfinite,
+                               hipStream_t s);
+
+// MFMA fused step writing one raw partial per block into
+// partials[block][d*k + k] (int64 fixed point, or FP32 when *nonfinite).
+template <typename T, typename I>
+void launch_fast_fused_fixed(int64_t n, int64_t d, int64_t k, const T *X,
+                             const T *w, const T *C, const T *hn, I *labels,
+                             const float *scale, const int *nonfinite,
+                             void *partials, int64_t n_partials,
+                             hipStream_t s);
+
+// Sums the per-block partials of launch_fast_fused_fixed (exactly, in int64,
+// when the fixed-point path ran) into out_cent (d x k) and out_sizes (k),
+// then appends empty clusters to empty_list at n_empty[0]++.
+template <typename T, typename I>
+void launch_reduce_fixed_partials(int64_t n_blocks, int64_t d, int64_t k,
+                                  const void *partials, const float *scale,
+                                  const int *nonfinite, T *out_sizes,
+                                  T *out_cent, I *empty_list, I *n_empty,
+                                  int64_t list_len, hipStream_t s);
 
 }  // namespace kmd

@@ -3,6 +3,7 @@
 // shifts, per-sample inertia, deterministic sum / max reductions, selection of
 // far samples and empty-cluster relocation.
 #include "common.hpp"
+#include "lloyd_fast.hpp"
 
 namespace kmd {
 
@@ -50,42 +51,111 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---- reduce private copies (src/util_kernels.hpp:84-161) -------------------
-// Blocks [0, nb-1) sum centroid copies element-wise (copy order ascending);
-// the last block sums the size copies and appends empty clusters in
-// ascending order starting at n_empty[0].
+// out[i] = sum over copies of src[copy * stride + i].  A 1024-thread block
+// owns 64 consecutive elements; its 16 wavefronts each sum the copies
+// congruent to their index mod 16 (ascending), then the 16 partial sums are
+// added in ascending order: a fixed order, so the result is deterministic.
+constexpr int kRedElems = 64;
+constexpr int kRedGroups = 16;
+
+template <typename T>
+__global__ void __launch_bounds__(kRedElems * kRedGroups)
+    sum_copies_krn(int64_t n_copies, int64_t m, int64_t stride,
+                   const T *__restrict__ src, T *__restrict__ out) {
+  __shared__ T part[kRedGroups][kRedElems];
+  const int e = threadIdx.x % kRedElems, g = threadIdx.x / kRedElems;
+  const int64_t i = blockIdx.x * (int64_t)kRedElems + e;
+  T acc = T(0);
+  if (i < m) {
+    int64_t c = g;
+    for (; c + 3 * kRedGroups < n_copies; c += 4 * kRedGroups) {
+      const T a0 = src[c * stride + i];
+      const T a1 = src[(c + kRedGroups) * stride + i];
+      const T a2 = src[(c + 2 * kRedGroups) * stride + i];
+      const T a3 = src[(c + 3 * kRedGroups) * stride + i];
+      acc += a0;
+      acc += a1;
+      acc += a2;
+      acc += a3;
+    }
+    for (; c < n_copies; c += kRedGroups) acc += src[c * stride + i];
+  }
+  part[g][e] = acc;
+  __syncthreads();
+  if (g == 0 && i < m) {
+    T t = T(0);
+    for (int q = 0; q < kRedGroups; ++q) t += part[q][e];
+    out[i] = t;
+  }
+}
+
+// Per-block partials of the MFMA fused step, [n_blocks][d*k + k]: int64
+// fixed-point sums (exact, so any summation order gives the same result),
+// converted once with the feature's scale; or FP32 sums in a fixed order
+// when the inputs held non-finite values.
+template <typename T>
+__global__ void __launch_bounds__(kRedElems * kRedGroups)
+    sum_partials_fixed_krn(int64_t n_blocks, int64_t d, int64_t k,
+                           const void *__restrict__ partials,
+                           const float *__restrict__ scale,
+                           const int *__restrict__ nonfinite,
+                           T *__restrict__ out_sizes, T *__restrict__ out_cent) {
+  __shared__ long long ipart[kRedGroups][kRedElems];
+  __shared__ float fpart[kRedGroups][kRedElems];
+  const int e = threadIdx.x % kRedElems, g = threadIdx.x / kRedElems;
+  const int64_t m = d * k + k;
+  const int64_t i = blockIdx.x * (int64_t)kRedElems + e;
+  const bool fixed = nonfinite[0] == 0;
+  long long iacc = 0;
+  float facc = 0.f;
+  if (i < m) {
+    if (fixed) {
+      const long long *p = reinterpret_cast<const long long *>(partials);
+      for (int64_t c = g; c < n_blocks; c += kRedGroups) iacc += p[c * m + i];
+    } else {
+      const float *p = reinterpret_cast<const float *>(partials);
+      for (int64_t c = g; c < n_blocks; c += kRedGroups) facc += p[c * m + i];
+    }
+  }
+  ipart[g][e] = iacc;
+  fpart[g][e] = facc;
+  __syncthreads();
+  if (g == 0 && i < m) {
+    T v;
+    if (fixed) {
+      long long t = 0;
+      for (int q = 0; q < kRedGroups; ++q) t += ipart[q][e];
+      const int64_t f = i / k;  // == d for the counts
+      v = (T)((double)t / (double)scale[f]);
+    } else {
+      float t = 0.f;
+      for (int q = 0; q < kRedGroups; ++q) t += fpart[q][e];
+      v = (T)t;
+    }
+    if (i < d * k)
+      out_cent[i] = v;
+    else
+      out_sizes[i - d * k] = v;
+  }
+}
+
+// Appends every cluster with size 0 to empty_list at n_empty[0]++, in
+// ascending cluster order (one block; prefix sums over ballots).
 template <typename T, typename I>
 __global__ void __launch_bounds__(1024)
-    reduce_centroids_krn(int64_t n_copies, int64_t dk, int64_t k,
-                         const T *sizes_copies, const T *cent_copies,
-                         T *out_sizes, T *out_cent, I *empty_list, I *n_empty,
-                         int64_t list_len) {
-  if (blockIdx.x + 1 < gridDim.x) {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= dk) return;
-    T acc = T(0);
-    for (int64_t c = 0; c < n_copies; ++c) acc += cent_copies[c * dk + i];
-    out_cent[i] = acc;
-    return;
-  }
-  // last block: sizes + empty list, processed in chunks of blockDim clusters
+    empty_list_krn(int64_t k, const T *__restrict__ sizes, I *empty_list,
+                   I *n_empty, int64_t list_len) {
   __shared__ int wave_counts[1024 / kWave];
   __shared__ int64_t running;
   if (threadIdx.x == 0) running = (int64_t)n_empty[0];
   __syncthreads();
-  const int64_t base = running;
   const int nwaves = blockDim.x / kWave;
   const int w = threadIdx.x / kWave;
   for (int64_t c0 = 0; c0 < k; c0 += blockDim.x) {
-    int64_t c = c0 + threadIdx.x;
-    bool empty = false;
-    if (c < k) {
-      T acc = T(0);
-      for (int64_t cp = 0; cp < n_copies; ++cp) acc += sizes_copies[cp * k + c];
-      out_sizes[c] = acc;
-      empty = (acc == T(0));
-    }
-    uint64_t mask = __ballot(empty);
-    int before_in_wave = __popcll(mask & ((1ull << lane_id()) - 1ull));
+    const int64_t c = c0 + threadIdx.x;
+    const bool empty = (c < k) && (sizes[c] == T(0));
+    const uint64_t mask = __ballot(empty);
+    const int before_in_wave = __popcll(mask & ((1ull << lane_id()) - 1ull));
     if (lane_id() == 0) wave_counts[w] = __popcll(mask);
     __syncthreads();
     int before = 0, total = 0;
@@ -94,7 +164,7 @@ __global__ void __launch_bounds__(1024)
       total += wave_counts[i];
     }
     if (empty) {
-      int64_t pos = running + before + before_in_wave;
+      const int64_t pos = running + before + before_in_wave;
       if (pos >= 0 && pos < list_len) empty_list[pos] = (I)c;
     }
     __syncthreads();
@@ -102,7 +172,6 @@ __global__ void __launch_bounds__(1024)
     __syncthreads();
   }
   if (threadIdx.x == 0) n_empty[0] = (I)running;
-  (void)base;
 }
 
 // ---- centroid shifts (src/util_kernels.hpp:478-516) ------------------------
@@ -400,10 +469,36 @@ void launch_reduce_centroids(int64_t n_copies, int64_t d, int64_t k,
                              T *out_sizes, T *out_cent, I *empty_list,
                              I *n_empty, int64_t list_len, hipStream_t s) {
   const int64_t dk = d * k;
-  const int nb = (int)ceil_div<int64_t>(dk, 1024) + 1;
-  hipLaunchKernelGGL((reduce_centroids_krn<T, I>), dim3(nb), dim3(1024), 0, s,
-                     n_copies, dk, k, sizes_copies, cent_copies, out_sizes,
-                     out_cent, empty_list, n_empty, list_len);
+  if (dk > 0) {
+    hipLaunchKernelGGL(sum_copies_krn<T>, dim3(ceil_div<int64_t>(dk, kRedElems)),
+                       dim3(kRedElems * kRedGroups), 0, s, n_copies, dk, dk,
+                       cent_copies, out_cent);
+    KMD_LAUNCH_CHECK();
+  }
+  if (k > 0) {
+    hipLaunchKernelGGL(sum_copies_krn<T>, dim3(ceil_div<int64_t>(k, kRedElems)),
+                       dim3(kRedElems * kRedGroups), 0, s, n_copies, k, k,
+                       sizes_copies, out_sizes);
+    KMD_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL((empty_list_krn<T, I>), dim3(1), dim3(1024), 0, s, k,
+                     (const T *)out_sizes, empty_list, n_empty, list_len);
+  KMD_LAUNCH_CHECK();
+}
+
+template <typename T, typename I>
+void launch_reduce_fixed_partials(int64_t n_blocks, int64_t d, int64_t k,
+                                  const void *partials, const float *scale,
+                                  const int *nonfinite, T *out_sizes,
+                                  T *out_cent, I *empty_list, I *n_empty,
+                                  int64_t list_len, hipStream_t s) {
+  const int64_t m = d * k + k;
+  hipLaunchKernelGGL(sum_partials_fixed_krn<T>, dim3(ceil_div<int64_t>(m, kRedElems)),
+                     dim3(kRedElems * kRedGroups), 0, s, n_blocks, d, k,
+                     partials, scale, nonfinite, out_sizes, out_cent);
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL((empty_list_krn<T, I>), dim3(1), dim3(1024), 0, s, k,
+                     (const T *)out_sizes, empty_list, n_empty, list_len);
   KMD_LAUNCH_CHECK();
 }
 
@@ -542,7 +637,10 @@ KMD_INST_T(double)
   template void launch_relocate<T, I>(int64_t, int64_t, int64_t, int64_t,    \
                                       const T *, const T *, const I *,        \
                                       const I *, const T *, T *, T *, T *,    \
-                                      hipStream_t);
+                                      hipStream_t);                           \
+  template void launch_reduce_fixed_partials<T, I>(                           \
+      int64_t, int64_t, int64_t, const void *, const float *, const int *,   \
+      T *, T *, I *, I *, int64_t, hipStream_t);
 KMD_INST_TI(float, int32_t)
 KMD_INST_TI(float, int64_t)
 KMD_INST_TI(double, int32_t)

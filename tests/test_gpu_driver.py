@@ -133,3 +133,45 @@ def test_driver_ragged_and_multiwindow_shapes(kd, torch_dev):
         assert n_iter == it64, (n, d, kk)
         assert_array_equal(lab, l64)
         assert_allclose(C, C64, rtol=1e-9, atol=1e-9)
+
+
+def separated_blobs(n, d, k, seed):
+    """Blobs far apart relative to their spread, init near the true centers:
+    no sample is within rounding of a tie, so FP32 and FP64 fits follow the
+    same trajectory and can be compared after several iterations."""
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(-10, 10, size=(k, d))
+    blob = rng.integers(0, k, size=n)
+    X = centers[blob] + 0.3 * rng.standard_normal((n, d))
+    init = centers + 0.3 * rng.standard_normal((k, d))
+    return (np.ascontiguousarray(X.T.astype(np.float32)),
+            np.ascontiguousarray(init.T.astype(np.float32)))
+
+
+def test_driver_fp32_is_deterministic(kd, torch_dev):
+    """Centroid sums of the FP32 path are exact fixed-point integers, so two
+    runs give bit-identical centroids, labels and inertia; centroids stay
+    within 1e-5 relative of the FP64 oracle."""
+    Xt, init = separated_blobs(200_003, 32, 64, seed=21)
+    w = np.random.default_rng(4).uniform(0.5, 2.0, 200_003).astype(np.float32)
+    r1 = run_driver(kd, Xt, w, init, 0.0, 7)
+    r2 = run_driver(kd, Xt, w, init, 0.0, 7)
+    assert_array_equal(r1[0], r2[0])
+    assert_array_equal(r1[1].view(np.uint32), r2[1].view(np.uint32))
+    assert r1[2] == r2[2]
+    assert r1[3][0] == r2[3][0]
+    l64, C64, it64, in64 = lo.kmeans_lloyd(Xt, w, init, 0.0, 7, dtype=np.float64)
+    assert r1[2] == it64          # converges exactly (shift 0) like the oracle
+    assert_array_equal(r1[0], l64)
+    assert np.abs(r1[1] - C64).max() <= 1e-5 * np.abs(C64).max()
+    assert_allclose(r1[3][0], in64, rtol=1e-5)
+
+
+def test_driver_nonfinite_input_propagates(kd, torch_dev):
+    """A NaN sample makes the fixed-point pre-pass fall back to FP32 sums;
+    the NaN reaches its cluster's centroid as in the reference."""
+    Xt, init = lo.gaussian_blobs(4096, 8, 4, seed=2)
+    Xt = Xt.copy()
+    Xt[3, 100] = np.nan
+    lab, C, n_iter, inertia = run_driver(kd, Xt, np.ones(4096, np.float32), init, 0.0, 1)
+    assert np.isnan(C).any()
