@@ -203,6 +203,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "inertia": inertia,
+        "path": stats.get("path"),
+        "rescored_per_iter": stats.get("n_rescored", 0) / max(1, n_iter),
     }
     print(json.dumps(line), flush=True)
 

@@ -238,15 +238,18 @@ int kmd_iterative_merge_sort(int dtype, int64_t n, const void *input,
  * sums and the inertia reduction run in FP64.  Optional outputs (host
  * memory, may be NULL): *n_rescored = samples re-scored in FP32 summed over
  * iterations; *fused_kernel_ms = summed HIP-event duration of the fused-step
- * kernel launches, *fused_launches = their number. */
+ * kernel launches (in the mixed modes: BF16 kernel + FP32 re-score),
+ * *fused_launches = their number; *fused_path = 0 generic kernel, 1 FP32
+ * MFMA, 2 BF16 + re-score, 3 BF16x3 + re-score (a mixed mode falls back to
+ * FP32 MFMA for shapes it does not serve or non-finite inputs). */
 int kmd_kmeans_lloyd_driver_ex(
     int dtype, int itype, int precision, int64_t n_samples,
     int64_t n_features, int64_t n_clusters, const void *X_t,
     const void *sample_weight, void *init_centroids_t, void *assignments_id,
     void *res_centroids_t, double tol, int verbose, int64_t max_iter,
     int64_t *n_iters, double *total_inertia, int64_t *n_rescored,
-    double *fused_kernel_ms, int64_t *fused_launches, kmd_print_fn print_fn,
-    void *print_user_data, kmd_stream_t stream);
+    double *fused_kernel_ms, int64_t *fused_launches, int *fused_path,
+    kmd_print_fn print_fn, void *print_user_data, kmd_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -45,7 +45,7 @@ void driver_lloyd(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
                   T *init, I *labels, T *res, double tol, bool verbose,
                   int64_t max_iter, int64_t *n_iters_out, double *inertia_out,
                   kmd_print_fn print_fn, void *print_user, hipStream_t s,
-                  DriverStats *stats);
+                  int precision, DriverStats *stats);
 
 }  // namespace kmd
 
@@ -538,7 +538,8 @@ int kmd_kmeans_lloyd_driver(
                          (const T *)sample_weight, (T *)init_centroids_t,
                          (I *)assignments_id, (T *)res_centroids_t, tol,
                          verbose != 0, max_iter, n_iters, total_inertia,
-                         print_fn, print_user_data, as_stream(stream), nullptr);
+                         print_fn, print_user_data, as_stream(stream),
+                         KMD_PREC_FP32, nullptr);
     });
   });
 }
@@ -561,15 +562,13 @@ int kmd_kmeans_lloyd_driver_ex(
     const void *sample_weight, void *init_centroids_t, void *assignments_id,
     void *res_centroids_t, double tol, int verbose, int64_t max_iter,
     int64_t *n_iters, double *total_inertia, int64_t *n_rescored,
-    double *fused_kernel_ms, int64_t *fused_launches, kmd_print_fn print_fn,
-    void *print_user_data, kmd_stream_t stream) {
+    double *fused_kernel_ms, int64_t *fused_launches, int *fused_path,
+    kmd_print_fn print_fn, void *print_user_data, kmd_stream_t stream) {
   return guarded([&] {
     check_types(dtype, itype);
     require(precision == KMD_PREC_FP32 || precision == KMD_PREC_BF16X3 ||
                 precision == KMD_PREC_BF16,
             "precision must be KMD_PREC_FP32, KMD_PREC_BF16X3 or KMD_PREC_BF16");
-    require(precision == KMD_PREC_FP32,
-            "only KMD_PREC_FP32 is available in this build");
     require(tol >= 0.0, "Tolerance must be non-negative");
     require(max_iter >= 0, "max_iter must be non-negative");
     require(n_samples >= 0 && n_features >= 1 && n_clusters >= 1,
@@ -586,9 +585,11 @@ int kmd_kmeans_lloyd_driver_ex(
                          (const T *)sample_weight, (T *)init_centroids_t,
                          (I *)assignments_id, (T *)res_centroids_t, tol,
                          verbose != 0, max_iter, n_iters, total_inertia,
-                         print_fn, print_user_data, as_stream(stream), &st);
+                         print_fn, print_user_data, as_stream(stream),
+                         precision, &st);
     });
-    if (n_rescored) *n_rescored = 0;
+    if (n_rescored) *n_rescored = st.n_rescored;
+    if (fused_path) *fused_path = st.fused_path;
     if (fused_kernel_ms) *fused_kernel_ms = st.fused_kernel_ms;
     if (fused_launches) *fused_launches = st.fused_launches;
   });

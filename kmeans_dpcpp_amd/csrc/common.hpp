@@ -69,6 +69,22 @@ class TempBuffer {
   }
   TempBuffer(const TempBuffer &) = delete;
   TempBuffer &operator=(const TempBuffer &) = delete;
+  TempBuffer(TempBuffer &&o) noexcept
+      : stream_(o.stream_), bytes_(o.bytes_), ptr_(o.ptr_) {
+    o.ptr_ = nullptr;
+    o.bytes_ = 0;
+  }
+  TempBuffer &operator=(TempBuffer &&o) noexcept {
+    if (this != &o) {
+      if (ptr_) (void)hipFreeAsync(ptr_, stream_);
+      stream_ = o.stream_;
+      bytes_ = o.bytes_;
+      ptr_ = o.ptr_;
+      o.ptr_ = nullptr;
+      o.bytes_ = 0;
+    }
+    return *this;
+  }
   template <typename T>
   T *as() const {
     return static_cast<T *>(ptr_);

@@ -38,9 +38,14 @@ struct LloydWorkspace {
   int blocks = 1;          // private copies / partial slots
   bool fast = false;       // MFMA fast path (fixed-point partials)
   bool store_flush = false;
+  int mixed_terms = 0;     // BF16 split terms (1 or 3) of the mixed path, 0: FP32
+  int blocks_b = 0;        // partial slots of the FP32 re-score (mixed path)
   FusedPlan plan;
   TempBuffer hn, sizes, shifts, per_sample, copies, sizes_copies, list,
       scalars, scales;
+  TempBuffer aop{0, nullptr}, bound{0, nullptr}, queue{0, nullptr},
+      queue_counts{0, nullptr};
+  int64_t seg_cap = 0;     // capacity of each block's re-score queue segment
   // copies_bytes / sizes_bytes: private copies of the generic kernel, or the
   // fast path's per-block partials (in `copies`, sizes_bytes = 0)
   LloydWorkspace(int64_t n_, int64_t d_, int64_t k_, int blocks_, bool fast_,
@@ -53,6 +58,9 @@ struct LloydWorkspace {
         list(sizeof(I) * (k_ + 1), s), scalars(32, s),
         scales(fast_ ? sizeof(float) * (d_ + 1) + 16 : 0, s) {}
   I *n_empty() const { return reinterpret_cast<I *>(scalars.as<char>() + 8); }
+  int *queue_count() const {
+    return reinterpret_cast<int *>(scalars.as<char>() + 16);
+  }
   double *shift_sum() const { return scalars.as<double>(); }
   float *scale() const { return scales.as<float>(); }
   int *nonfinite() const {
@@ -60,13 +68,54 @@ struct LloydWorkspace {
   }
 };
 
+// Mixed-precision workspace: BF16 kernel partials, then FP32 re-score
+// partials; returns nullptr when the shape is not served or the inputs hold
+// non-finite values (the FP32 path handles those).
+template <typename T, typename I>
+static std::unique_ptr<LloydWorkspace<T, I>> make_mixed_workspace(
+    int64_t n, int64_t d, int64_t k, int terms, const T *X, const T *w,
+    hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (n <= 0 || !fast_fused_supported<T>(d, k)) return nullptr;
+    const int ba = mixed_blocks<I>(n, d, k, terms);
+    const int bb = fast_rescore_blocks<I>(d, k, ba);
+    if (ba < 1 || bb < 1) return nullptr;
+    auto ws = std::make_unique<LloydWorkspace<T, I>>(
+        n, d, k, ba, true, true, FusedPlan{},
+        sizeof(int64_t) * (size_t)(ba + bb) * (d * k + k), 0, s);
+    ws->mixed_terms = terms;
+    ws->blocks_b = bb;
+    ws->aop = TempBuffer(mixed_aop_bytes(d, k, terms), s);
+    ws->bound = TempBuffer(2 * sizeof(float), s);
+    ws->seg_cap = mixed_seg_cap(n, d, ba);
+    ws->queue = TempBuffer(sizeof(int) * (size_t)ba * ws->seg_cap, s);
+    ws->queue_counts = TempBuffer(sizeof(int) * (size_t)ba, s);
+    launch_fixed_point_scales(n, d, X, w, ws->scale(), ws->nonfinite(), s);
+    int bad = 0;
+    KMD_HIP_CHECK(hipMemcpyAsync(&bad, ws->nonfinite(), sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+    KMD_HIP_CHECK(hipStreamSynchronize(s));
+    if (bad) return nullptr;
+    return ws;
+  } else {
+    (void)n, (void)d, (void)k, (void)terms, (void)X, (void)w, (void)s;
+    return nullptr;
+  }
+}
+
 template <typename T, typename I>
 static std::unique_ptr<LloydWorkspace<T, I>> make_workspace(int64_t n,
                                                             int64_t d,
                                                             int64_t k,
                                                             const T *X,
                                                             const T *w,
+                                                            int precision,
                                                             hipStream_t s) {
+  if (precision != KMD_PREC_FP32) {
+    auto ws = make_mixed_workspace<T, I>(
+        n, d, k, precision == KMD_PREC_BF16X3 ? 3 : 1, X, w, s);
+    if (ws) return ws;
+  }
   if (n > 0 && fast_fused_supported<T>(d, k)) {
     const int blocks = fast_fused_blocks<I>(n, d, k);
     auto ws = std::make_unique<LloydWorkspace<T, I>>(
@@ -134,6 +183,32 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
   const int64_t n = ws.n, d = ws.d, k = ws.k;
   launch_half_l2_norm<T>(d, k, cur, ws.hn.template as<T>(), s);
   KMD_HIP_CHECK(hipMemsetAsync(ws.n_empty(), 0, sizeof(I), s));
+  if constexpr (std::is_same<T, float>::value) {
+    if (ws.mixed_terms > 0) {
+      launch_mixed_prep(d, k, ws.mixed_terms, cur, ws.hn.template as<const T>(),
+                        ws.aop.template as<void>(), ws.bound.template as<float>(),
+                        s);
+      timer.begin(s);
+      launch_mixed_step<I>(n, d, k, ws.mixed_terms, ws.blocks, X, w,
+                           ws.aop.template as<const void>(),
+                           ws.bound.template as<const float>(), ws.scale(),
+                           labels, ws.copies.template as<void>(),
+                           ws.queue.template as<int>(),
+                           ws.queue_counts.template as<int>(), s);
+      launch_fast_rescore<I>(n, d, k, X, w, cur, ws.hn.template as<const T>(),
+                             labels, ws.scale(), ws.nonfinite(),
+                             ws.copies.template as<void>(), ws.blocks,
+                             ws.queue.template as<const int>(),
+                             ws.queue_counts.template as<const int>(),
+                             ws.blocks, ws.seg_cap, ws.queue_count(), s);
+      timer.end(s);
+      launch_reduce_fixed_partials<T, I>(
+          ws.blocks + ws.blocks_b, d, k, ws.copies.template as<const void>(),
+          ws.scale(), ws.nonfinite(), ws.sizes.template as<T>(), next,
+          ws.list.template as<I>(), ws.n_empty(), k, s);
+      return;
+    }
+  }
   if (ws.fast) {
     timer.begin(s);
     launch_fast_fused_fixed<T, I>(n, d, k, X, w, cur,
@@ -173,8 +248,8 @@ void driver_lloyd(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
                   T *init, I *labels, T *res, double tol, bool verbose,
                   int64_t max_iter, int64_t *n_iters_out, double *inertia_out,
                   kmd_print_fn print_fn, void *print_user, hipStream_t s,
-                  DriverStats *stats) {
-  auto ws_ptr = make_workspace<T, I>(n, d, k, X, w, s);
+                  int precision, DriverStats *stats) {
+  auto ws_ptr = make_workspace<T, I>(n, d, k, X, w, precision, s);
   FusedTimer timer(stats != nullptr);
   auto &ws = *ws_ptr;
   T *cur = init;
@@ -182,10 +257,11 @@ void driver_lloyd(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
   const T tol_t = static_cast<T>(tol);
   int64_t it = 0;
   T shift = std::numeric_limits<T>::infinity();
-  unsigned char host_scalars[16];
+  unsigned char host_scalars[24];
+  int64_t rescored = 0;
   auto read_scalars = [&]() {
     KMD_HIP_CHECK(hipMemcpyAsync(host_scalars, ws.scalars.template as<char>(),
-                                 16, hipMemcpyDeviceToHost, s));
+                                 24, hipMemcpyDeviceToHost, s));
     KMD_HIP_CHECK(hipStreamSynchronize(s));
     timer.collect();
   };
@@ -206,6 +282,11 @@ void driver_lloyd(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
   }
   while (it < max_iter && shift > tol_t) {
     const int64_t n_empty = host_n_empty();
+    if (ws.mixed_terms > 0) {
+      int qc;
+      std::memcpy(&qc, host_scalars + 16, sizeof(int));
+      rescored += qc;
+    }
     if (verbose) {
       launch_inertia<T, I>(n, d, k, X, w, cur, labels,
                            ws.per_sample.template as<T>(), s);
@@ -259,7 +340,9 @@ void driver_lloyd(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
   if (stats) {
     stats->fused_kernel_ms = timer.total_ms;
     stats->fused_launches = timer.launches;
-    stats->fused_path = ws.fast ? 1 : 0;
+    stats->fused_path = ws.mixed_terms > 0 ? (ws.mixed_terms == 3 ? 3 : 2)
+                                           : (ws.fast ? 1 : 0);
+    stats->n_rescored = rescored;
   }
 }
 
@@ -267,7 +350,7 @@ void driver_lloyd(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
   template void driver_lloyd<T, I>(int64_t, int64_t, int64_t, const T *,     \
                                    const T *, T *, I *, T *, double, bool,    \
                                    int64_t, int64_t *, double *,              \
-                                   kmd_print_fn, void *, hipStream_t,         \
+                                   kmd_print_fn, void *, hipStream_t, int,    \
                                    DriverStats *);
 KMD_INST_DRIVER(float, int32_t)
 KMD_INST_DRIVER(float, int64_t)

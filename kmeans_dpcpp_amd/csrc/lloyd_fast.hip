@@ -145,10 +145,22 @@ struct FastArgs {
   void *copies;             // u64 [blocks][d*k + k] or float [n_copies][d*k]
   float *sizes_copies;      // float [n_copies][k] (atomic flush only)
   int64_t n_copies;
+  int64_t slot0;            // first partial slot of this launch (store flush)
+  const int *queue;         // gather mode: [segment][seg_cap] sample indices
+  const int *queue_counts;  // gather mode: [segment] entries used
+  int n_segments;           // gather mode: <= kMaxSegments
+  int64_t seg_cap;
+  int *n_rescored;          // gather mode: total queued samples (block 0)
 };
 
+constexpr int kMaxSegments = 1024;
+
+// GATHER: process the samples listed in the segments of a.queue (segment g
+// holds a.queue_counts[g] entries from g * seg_cap) instead of 0..n-1: the
+// re-scoring of samples the BF16 path left undecided.  Item p of the
+// concatenated queue lives in the segment g with off[g] <= p < off[g + 1].
 template <int NT, int KS, int S, bool FULL, typename I, bool UPDATE,
-          bool STORE_FLUSH>
+          bool STORE_FLUSH, bool GATHER = false>
 __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
   constexpr int kWavesPerBlock = NT / kWave;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -162,6 +174,29 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
   const int tid = threadIdx.x;
   const int64_t dk = (int64_t)d * k;
   const bool fixed = UPDATE ? (a.nonfinite[0] == 0) : false;
+  int *seg_off = reinterpret_cast<int *>(S64 + dk + k);  // gather: [nseg+1]
+  if constexpr (GATHER) {
+    // exclusive prefix sum of the segment counts (one wave, 64 per step)
+    if (tid < kWave) {
+      int carry = 0;
+      for (int g0 = 0; g0 < a.n_segments; g0 += kWave) {
+        const int g = g0 + tid;
+        const int c = (g < a.n_segments) ? a.queue_counts[g] : 0;
+        int incl = c;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+          const int t = __shfl_up(incl, o, kWave);
+          if (tid >= o) incl += t;
+        }
+        if (g < a.n_segments) seg_off[g] = carry + incl - c;
+        carry += __shfl(incl, kWave - 1, kWave);
+      }
+      if (tid == 0) {
+        seg_off[a.n_segments] = carry;
+        if (blockIdx.x == 0) a.n_rescored[0] = carry;
+      }
+    }
+  }
 
   for (int i = tid; i < 2 * KS * kpad; i += NT) {
     const int f = i / kpad, c = i - f * kpad;
@@ -181,7 +216,8 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
   const int half = lane >> 5;
   const int n_ct = kpad / 32;
   constexpr int kSuper = 32 * S;
-  const int64_t n_super = ceil_div<int64_t>(n, kSuper);
+  const int64_t n_items = GATHER ? (int64_t)seg_off[a.n_segments] : n;
+  const int64_t n_super = ceil_div<int64_t>(n_items, kSuper);
   const float *X = a.X;
   const float *w = a.w;
   const bool vec = a.vec != 0;
@@ -189,9 +225,40 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
 
   for (int64_t sb = (int64_t)blockIdx.x * kWavesPerBlock + wave; sb < n_super;
        sb += (int64_t)gridDim.x * kWavesPerBlock) {
-    const int64_t s_first = sb * kSuper + (int64_t)S * col;
+    const int64_t pos_first = sb * kSuper + (int64_t)S * col;
+    int64_t sidx[S];
+    bool valid[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      valid[q] = pos_first + q < n_items;
+      sidx[q] = pos_first + q;
+      if (GATHER && valid[q]) {
+        // last segment whose offset is <= the item's position
+        const int p = (int)(pos_first + q);
+        int lo = 0, hi = a.n_segments - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (seg_off[mid] <= p)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        sidx[q] = a.queue[(int64_t)lo * a.seg_cap + (p - seg_off[lo])];
+      }
+    }
     float xr[KS][S];
-    load_x<KS, S, FULL>(X, n, d, s_first, half, vec, xr);
+    if (GATHER) {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int f = 2 * ks + half;
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+          xr[ks][q] = ((FULL || f < d) && valid[q]) ? X[(int64_t)f * n + sidx[q]]
+                                                     : 0.f;
+      }
+    } else {
+      load_x<KS, S, FULL>(X, n, d, pos_first, half, vec, xr);
+    }
 
     float bestv[S];
     int besti[S];
@@ -217,13 +284,13 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
     if (half == 0) {
 #pragma unroll
       for (int q = 0; q < S; ++q)
-        if (s_first + q < n) labels[s_first + q] = (I)besti[q];
+        if (valid[q]) labels[sidx[q]] = (I)besti[q];
     }
     if (UPDATE) {
 #pragma unroll
       for (int q = 0; q < S; ++q) {
-        if (s_first + q < n) {
-          const float ws = w[s_first + q];
+        if (valid[q]) {
+          const float ws = w[sidx[q]];
           const int lab = besti[q];
           if (fixed) {
             // |w x| 2^E_f < 2^30: the rounded element fits an int32; the
@@ -258,10 +325,12 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
       // raw partials (int64 fixed point, or FP32 on the non-finite path)
       // into this block's slot
       if (fixed) {
-        u64 *cp = reinterpret_cast<u64 *>(a.copies) + blockIdx.x * (dk + k);
+        u64 *cp = reinterpret_cast<u64 *>(a.copies) +
+                  (a.slot0 + blockIdx.x) * (dk + k);
         for (int64_t i = tid; i < dk + k; i += NT) cp[i] = S64[i];
       } else {
-        float *cp = reinterpret_cast<float *>(a.copies) + blockIdx.x * (dk + k);
+        float *cp = reinterpret_cast<float *>(a.copies) +
+                    (a.slot0 + blockIdx.x) * (dk + k);
         for (int64_t i = tid; i < dk + k; i += NT) cp[i] = S32[i];
       }
     } else {
@@ -558,6 +627,103 @@ void launch_fast_assign(int64_t n, int64_t d, int64_t k, const T *X,
   }
 }
 
+// ---- FP32 re-score of queued samples (mixed-precision path) ---------------
+
+namespace {
+template <typename I>
+KernFn pick_rescore(int64_t d) {
+  const int ks = ks_bucket(d);
+  const bool full = (d == 2 * ks);
+#define KMD_KS_CASE(KS)                                                     \
+  case KS:                                                                  \
+    return full ? &mfma_lloyd_krn<kThreads, KS, kS, true, I, true, true, true> \
+                : &mfma_lloyd_krn<kThreads, KS, kS, false, I, true, true, true>;
+  switch (ks) {
+    KMD_KS_CASE(2)
+    KMD_KS_CASE(4)
+    KMD_KS_CASE(8)
+    KMD_KS_CASE(12)
+    KMD_KS_CASE(16)
+    KMD_KS_CASE(24)
+    KMD_KS_CASE(32)
+    default:
+      return nullptr;
+  }
+#undef KMD_KS_CASE
+}
+
+template <typename I>
+MfmaLaunch plan_rescore(int64_t d, int64_t k) {
+  MfmaLaunch L;
+  if (d < 1 || d > kMaxD || k < 1) return L;
+  L.lds = lds_bytes(d, k, true) + sizeof(int) * (kMaxSegments + 1);
+  if (L.lds > kLdsPerCu) return L;
+  L.fn = pick_rescore<I>(d);
+  if (L.fn == nullptr) return L;
+  KMD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(L.fn),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)kLdsPerCu));
+  int per_cu = 0;
+  KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void *>(L.fn), L.threads, L.lds));
+  if (per_cu < 1) {
+    L.fn = nullptr;
+    return L;
+  }
+  const int64_t blocks = (int64_t)device_cu_count() * per_cu;
+  L.blocks = (int)(blocks < 1 ? 1 : blocks);
+  return L;
+}
+}  // namespace
+
+template <typename I>
+int fast_rescore_blocks(int64_t d, int64_t k, int n_segments) {
+  if (n_segments < 1 || n_segments > kMaxSegments) return 0;
+  return plan_rescore<I>(d, k).blocks;
+}
+
+template <typename I>
+void launch_fast_rescore(int64_t n, int64_t d, int64_t k, const float *X,
+                         const float *w, const float *C, const float *hn,
+                         I *labels, const float *scale, const int *nonfinite,
+                         void *partials, int64_t slot0, const int *queue,
+                         const int *queue_counts, int n_segments,
+                         int64_t seg_cap, int *n_rescored, hipStream_t s) {
+  const MfmaLaunch L = plan_rescore<I>(d, k);
+  if (L.fn == nullptr) throw RuntimeError("FP32 re-score: no launch plan");
+  if (n_segments < 1 || n_segments > kMaxSegments)
+    throw RuntimeError("FP32 re-score: segment count out of range");
+  FastArgs a = make_args(n, d, k, X, w, C, hn, labels);
+  a.vec = 0;
+  a.scale = scale;
+  a.nonfinite = nonfinite;
+  a.copies = partials;
+  a.n_copies = L.blocks;
+  a.slot0 = slot0;
+  a.queue = queue;
+  a.queue_counts = queue_counts;
+  a.n_segments = n_segments;
+  a.seg_cap = seg_cap;
+  a.n_rescored = n_rescored;
+  run(L, a, s);
+}
+
+template int fast_rescore_blocks<int32_t>(int64_t, int64_t, int);
+template int fast_rescore_blocks<int64_t>(int64_t, int64_t, int);
+template void launch_fast_rescore<int32_t>(int64_t, int64_t, int64_t,
+                                           const float *, const float *,
+                                           const float *, const float *,
+                                           int32_t *, const float *,
+                                           const int *, void *, int64_t,
+                                           const int *, const int *, int,
+                                           int64_t, int *, hipStream_t);
+template void launch_fast_rescore<int64_t>(int64_t, int64_t, int64_t,
+                                           const float *, const float *,
+                                           const float *, const float *,
+                                           int64_t *, const float *,
+                                           const int *, void *, int64_t,
+                                           const int *, const int *, int,
+                                           int64_t, int *, hipStream_t);
 template int fast_fused_blocks<int32_t>(int64_t, int64_t, int64_t);
 template int fast_fused_blocks<int64_t>(int64_t, int64_t, int64_t);
 template bool fast_fused_supported<float>(int64_t, int64_t);

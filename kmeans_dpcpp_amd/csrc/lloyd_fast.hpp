@@ -8,7 +8,8 @@ namespace kmd {
 struct DriverStats {
   double fused_kernel_ms = 0.0;  // summed over launches
   int64_t fused_launches = 0;
-  int fused_path = 0;            // 1 = MFMA fast path, 0 = generic
+  int fused_path = 0;   // 0 generic, 1 FP32 MFMA, 2 BF16 (+FP32 re-score), 3 BF16x3
+  int64_t n_rescored = 0;       // samples re-scored in FP32 (mixed path)
 };
 
 // Launch plan of a persistent fused-step kernel: grid size, dynamic LDS and
@@ -73,5 +74,46 @@ void launch_reduce_fixed_partials(int64_t n_blocks, int64_t d, int64_t k,
                                   const int *nonfinite, T *out_sizes,
                                   T *out_cent, I *empty_list, I *n_empty,
                                   int64_t list_len, hipStream_t s);
+
+}  // namespace kmd
+
+namespace kmd {
+
+// ---- mixed-precision path (lloyd_mixed.hip) --------------------------------
+// Bits of the packed centroid index (ceil(log2(k padded to 32))).
+int mixed_idx_bits(int64_t k);
+// Bytes of the BF16 A-operand array for T split terms (1 or 3).
+size_t mixed_aop_bytes(int64_t d, int64_t k, int T);
+// Blocks of the BF16 kernel for this shape; 0 when the shape is not served.
+template <typename I>
+int mixed_blocks(int64_t n, int64_t d, int64_t k, int T);
+// Builds the BF16 A operands (centroids + split half-norms) and the error
+// bound inputs bound[0] = max ||c||, bound[1] = max hn.
+void launch_mixed_prep(int64_t d, int64_t k, int T, const float *C,
+                       const float *hn, void *aop, float *bound,
+                       hipStream_t s);
+// Capacity of each block's segment of the re-score queue.
+int64_t mixed_seg_cap(int64_t n, int64_t d, int blocks);
+// BF16 assignment + fixed-point update; block b appends its undecided samples
+// to queue[b * seg_cap + 0 : queue_counts[b]].  Writes `blocks` partial slots.
+template <typename I>
+void launch_mixed_step(int64_t n, int64_t d, int64_t k, int T, int blocks,
+                       const float *X, const float *w, const void *aop,
+                       const float *bound, const float *scale, I *labels,
+                       void *partials, int *queue, int *queue_counts,
+                       hipStream_t s);
+// FP32 MFMA re-score of the queued samples (the FP32 path's arithmetic):
+// reads the n_segments segments of the queue, writes
+// fast_rescore_blocks<I>(d, k, n_segments) partial slots from slot0 and the
+// number of re-scored samples to *n_rescored.
+template <typename I>
+int fast_rescore_blocks(int64_t d, int64_t k, int n_segments);
+template <typename I>
+void launch_fast_rescore(int64_t n, int64_t d, int64_t k, const float *X,
+                         const float *w, const float *C, const float *hn,
+                         I *labels, const float *scale, const int *nonfinite,
+                         void *partials, int64_t slot0, const int *queue,
+                         const int *queue_counts, int n_segments,
+                         int64_t seg_cap, int *n_rescored, hipStream_t s);
 
 }  // namespace kmd

@@ -1,0 +1,556 @@
+// Mixed-precision Lloyd step: BF16 MFMA cross term with a certified margin
+// test and exact FP32 re-scoring of undecided samples.
+//
+// Contraction.  With v_mfma_f32_32x32x16_bf16 (rows = 32 centroids, columns
+// = 32 samples, K = 16) each sample/centroid pair gets
+//   D = sum_f bf16(x_f) bf16(c_f) - (h1 + h2 + h3)                   (T = 1)
+//   D = sum_f (xh ch + xh cl + xl ch) - (h1 + h2 + h3)               (T = 3)
+// accumulated in FP32, where xh = bf16(x), xl = bf16(x - xh) (likewise for
+// c) and h1 + h2 + h3 is the FP32 half-norm split exactly into three BF16
+// terms, so hn is folded into the matrix product (the samples' side of those
+// K columns is -1).  The argmin of hn - dot is the argmax of D.
+//
+// Decision.  Each lane keeps the best and second-best D of its sample (the
+// centroid index packed into the low mantissa bits of the value: one
+// v_and_or and two v_med3 per candidate).  Let E1 bound the error
+// of D against exact arithmetic (bf16 rounding of the operands, FP32
+// accumulation, index packing) and E2 the error of the FP32 path's
+// pseudo-inertia (FP32 MFMA accumulation and the final subtraction), both
+// from ||x||, max ||c|| and max hn.  If best - second > 2 (E1 + E2) the FP32
+// path has the same, unique, argmin; the sample is labelled here and its
+// w x added to the block's int64 fixed-point sums.  Otherwise its index is
+// appended to a queue and the FP32 MFMA kernel re-scores it with exactly the
+// FP32 path's arithmetic (same instruction sequence per sample/centroid
+// pair), so every label equals the FP32 path's label.  Both kernels add into
+// exact int64 partial sums, so which kernel handled a sample does not change
+// the sums.
+#include "lloyd_fast.hpp"
+
+namespace kmd {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned long long u64;
+
+// threads per block: 12 waves (3 per SIMD) where the registers allow it
+constexpr int nt_of(int D) { return D <= 32 ? 768 : 512; }
+constexpr int kS = 2;                  // 32-sample column blocks per wave
+constexpr size_t kLdsPerCu = 160 * 1024;
+constexpr int kScPad = 66;             // scale slots in LDS (d + 1 <= 65)
+
+template <int D, int T>
+struct MixedShape {
+  static constexpr int KD = D / 16;                 // k-steps per segment
+  static constexpr int KK = (T * D + 3 + 15) / 16;  // k-steps incl. hn
+  static constexpr int XH = D / 2;                  // features per lane
+};
+
+struct MixedArgs {
+  int64_t n;
+  int d, k, kpad, idx_bits;
+  const float *X, *w;
+  const bf16x8 *aop;     // [n_ct][KK][64] A operands
+  const float *bound;    // [2]: max ||c||, max hn
+  const float *scale;    // [d + 1] fixed-point scales
+  void *labels;
+  u64 *partials;         // [slot][d*k + k]
+  int *queue;            // [block][seg_cap] samples left for the FP32 re-score
+  int *queue_counts;     // [block] entries used in each segment
+  int64_t seg_cap;
+  float pinf;            // +inf, passed at run time so that med3(b, v, pinf)
+                         // stays one v_med3 (no max pattern, no canonicalise)
+};
+
+__device__ inline __bf16 to_bf16(float x) { return (__bf16)x; }
+
+// v_max3_f32 on values known to be finite (written out so that no NaN
+// canonicalisation is inserted in front of it)
+__device__ inline float max3f(float a, float b, float c) {
+  float r;
+  asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// A operands: element j of lane l for (tile ct, k-step kk) is row
+// m = l % 32 (centroid ct*32 + m) at K position p = 16 kk + 8 (l / 32) + j.
+template <int D, int T>
+__global__ void __launch_bounds__(256)
+    prep_aop_krn(int d, int k, int kpad, const float *__restrict__ C,
+                 const float *__restrict__ hn, bf16x8 *__restrict__ aop) {
+  constexpr int KK = MixedShape<D, T>::KK;
+  const int64_t total = (int64_t)(kpad / 32) * KK * 64;
+  const int64_t gid = blockIdx.x * 256ll + threadIdx.x;
+  if (gid >= total) return;
+  const int lane = (int)(gid % 64);
+  const int kk = (int)((gid / 64) % KK);
+  const int ct = (int)(gid / (64 * KK));
+  const int m = lane % 32, half = lane / 32;
+  const int c = ct * 32 + m;
+  bf16x8 v;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int p = 16 * kk + 8 * half + j;
+    float val = 0.f;
+    if (p < T * D) {
+      const int seg = p / D, f = p % D;
+      const float cv = (f < d && c < k) ? C[(int64_t)f * k + c] : 0.f;
+      const float hi = (float)to_bf16(cv);
+      val = (T == 3 && seg == 1) ? (float)to_bf16(cv - hi) : hi;
+    } else if (p < T * D + 3) {
+      const int t = p - T * D;
+      if (c < k) {
+        const float h = hn[c];
+        const float h1 = (float)to_bf16(h);
+        const float r1 = h - h1;
+        const float h2 = (float)to_bf16(r1);
+        const float h3 = (float)to_bf16(r1 - h2);
+        val = (t == 0) ? h1 : (t == 1 ? h2 : h3);
+      } else {
+        val = (t == 0) ? 1e30f : 0.f;  // padding column: never the best
+      }
+    }
+    v[j] = to_bf16(val);
+  }
+  aop[gid] = v;
+}
+
+// bound[0] = max_c ||c|| = sqrt(2 max hn), bound[1] = max_c hn (one block).
+__global__ void __launch_bounds__(256)
+    bound_krn(int k, const float *__restrict__ hn, float *bound) {
+  __shared__ float red[256 / kWave];
+  float m = 0.f;
+  for (int c = threadIdx.x; c < k; c += 256) m = fmaxf(m, hn[c]);
+  for (int o = kWave / 2; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.f;
+    for (int i = 0; i < 256 / kWave; ++i) t = fmaxf(t, red[i]);
+    bound[0] = sqrtf(2.f * t);
+    bound[1] = t;
+  }
+}
+
+// FULL: d == D (no feature beyond d in the lane layout).  NT threads per
+// block: one block per CU (the LDS sums fill most of it), so NT sets the
+// waves per SIMD that hide each other's load and matrix-core latency.
+template <int D, int T, bool FULL, typename I, int NT>
+__global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
+  using Sh = MixedShape<D, T>;
+  constexpr int KD = Sh::KD, KK = Sh::KK, XH = Sh::XH;
+  constexpr int kWavesPerBlock = NT / kWave;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int64_t n = a.n;
+  const int d = a.d, k = a.k, n_ct = a.kpad / 32;
+  const int64_t dk = (int64_t)d * k;
+  bf16x8 *Al = reinterpret_cast<bf16x8 *>(lds_raw);           // [n_ct*KK*64]
+  float *Sc = reinterpret_cast<float *>(Al + (int64_t)n_ct * KK * 64);
+  u64 *S64 = reinterpret_cast<u64 *>(Sc + kScPad);             // [d*k + k]
+  int &q_used = *reinterpret_cast<int *>(S64 + dk + k);  // queue entries used
+  const int tid = threadIdx.x;
+  for (int64_t i = tid; i < (int64_t)n_ct * KK * 64; i += NT) Al[i] = a.aop[i];
+  for (int f = tid; f <= d; f += NT) Sc[f] = a.scale[f];
+  for (int64_t i = tid; i < dk + k; i += NT) S64[i] = 0ull;
+  if (tid == 0) q_used = 0;
+  int *qseg = a.queue + (int64_t)blockIdx.x * a.seg_cap;
+  __syncthreads();
+
+  const int lane = tid & (kWave - 1);
+  const int wave = tid / kWave;
+  const int col = lane & 31;
+  const int half = lane >> 5;
+  const unsigned idx_mask = (1u << a.idx_bits) - 1u;
+  // error bound E1 + E2 = ||x|| cmax alpha + hnmax beta + pack (|b1| + |b2|)
+  const float cmax = a.bound[0], hnmax = a.bound[1];
+  // the split of hn into bf16 terms and the bound need a finite, moderate hn
+  const bool finite_c = hnmax < 1e36f;
+  const float pinf = a.pinf;
+  constexpr float u = 1.0f / 256.0f;  // bf16 unit roundoff
+  constexpr float acc_err = (float)(T * D + 3 + D + 8) * 5.9604645e-8f;
+  const float alpha = ((T == 1) ? (2.f * u + u * u) : (3.2f * u * u)) + acc_err;
+  const float beta = acc_err;
+  const float pack_err = ldexpf(1.0f, a.idx_bits - 23) * 1.01f;
+  const float *X = a.X;
+  const float *w = a.w;
+  I *labels = reinterpret_cast<I *>(a.labels);
+  const bool vec = ((reinterpret_cast<uintptr_t>(X) & 7u) == 0) && (n % 2 == 0);
+
+  constexpr int kSuper = 32 * kS;
+  const int64_t n_super = ceil_div<int64_t>(n, kSuper);
+  // lane features f(i) = 16 (i / 8) + 8 half + i % 8
+  auto load = [&](int64_t s0, float (&dst)[XH][kS]) {
+    const bool whole = vec && (s0 + kS <= n);
+#pragma unroll
+    for (int i = 0; i < XH; ++i) {
+      const int f = 16 * (i / 8) + 8 * half + (i % 8);
+      if (FULL || f < d) {
+        const float *row = X + (int64_t)f * n + s0;
+        if (whole) {
+          const float2 v = *reinterpret_cast<const float2 *>(row);
+          dst[i][0] = v.x;
+          dst[i][1] = v.y;
+        } else {
+#pragma unroll
+          for (int q = 0; q < kS; ++q) dst[i][q] = (s0 + q < n) ? row[q] : 0.f;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < kS; ++q) dst[i][q] = 0.f;
+      }
+    }
+  };
+  const int64_t sb_stride = (int64_t)gridDim.x * kWavesPerBlock;
+  for (int64_t sb = (int64_t)blockIdx.x * kWavesPerBlock + wave; sb < n_super;
+       sb += sb_stride) {
+    const int64_t s0 = sb * kSuper + (int64_t)kS * col;
+    // x is held only until it is converted (registers go to the other
+    // waves); the update reads it again, from L2
+    float xr[XH][kS];
+    load(s0, xr);
+    float xnorm[kS];
+#pragma unroll
+    for (int q = 0; q < kS; ++q) {
+      float ss = 0.f;
+#pragma unroll
+      for (int i = 0; i < XH; ++i) ss = fmaf(xr[i][q], xr[i][q], ss);
+      ss += __shfl_xor(ss, 32, kWave);
+      xnorm[q] = sqrtf(ss) * 1.0001f;
+    }
+    // B operands (samples): bf16 high parts, low parts (T = 3) and the
+    // -1 entries that pick up the half-norm columns
+    bf16x8 bhi[KD][kS];
+    bf16x8 blo[(T == 3) ? KD : 1][kS];
+#pragma unroll
+    for (int kd = 0; kd < KD; ++kd)
+#pragma unroll
+      for (int q = 0; q < kS; ++q)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const float x = xr[8 * kd + j][q];
+          const __bf16 h = to_bf16(x);
+          bhi[kd][q][j] = h;
+          if constexpr (T == 3) blo[kd][q][j] = to_bf16(x - (float)h);
+        }
+    bf16x8 bh;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      bh[j] = to_bf16((half == 0 && j < 3) ? -1.f : 0.f);
+
+    float b1[kS], b2[kS];
+#pragma unroll
+    for (int q = 0; q < kS; ++q) {
+      b1[q] = -__builtin_huge_valf();
+      b2[q] = -__builtin_huge_valf();
+    }
+    // products of tile ct into acc (K steps: x high parts, then for T = 3
+    // the two cross terms, then the half-norm columns)
+    auto tile_mma = [&](int ct, f32x16(&acc)[kS]) {
+      const bf16x8 *At = Al + (int64_t)ct * KK * 64 + lane;
+#pragma unroll
+      for (int kk = 0; kk < KK; ++kk) {
+        const bf16x8 av = At[kk * 64];
+#pragma unroll
+        for (int q = 0; q < kS; ++q) {
+          bf16x8 bv;
+          if (kk < KD)
+            bv = bhi[kk][q];
+          else if (T == 3 && kk < 2 * KD)
+            bv = bhi[(kk - KD) % KD][q];
+          else if (T == 3 && kk < 3 * KD)
+            bv = blo[(kk - 2 * KD) % KD][q];
+          else
+            bv = bh;
+          if (kk == 0) {
+            const f32x16 z = {};
+            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, z, 0, 0, 0);
+          } else {
+            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[q], 0,
+                                                              0, 0);
+          }
+        }
+      }
+    };
+    // top-2 update with the 16 rows of tile ct this lane holds:
+    // m = 8 (r / 4) + 4 half + (r % 4)
+    // Two candidates u, v at a time: top-2 of {b1, b2, u, v} (b1 >= b2) is
+    // b1' = max3(b1, u, v), b2' = max(b2, med3(b1, u, v)).  med3(x, y, +inf)
+    // is max(x, y) without the NaN canonicalisation of a max; the values are
+    // finite here or the sample is re-scored.
+    auto tile_track = [&](int ct, const f32x16(&acc)[kS]) {
+#pragma unroll
+      for (int r = 0; r < 16; r += 2) {
+        const unsigned i0 = (unsigned)(ct * 32 + 8 * (r / 4) + 4 * half + (r % 4));
+#pragma unroll
+        for (int q = 0; q < kS; ++q) {
+          const float u =
+              __uint_as_float((__float_as_uint(acc[q][r]) & ~idx_mask) | i0);
+          const float v = __uint_as_float(
+              (__float_as_uint(acc[q][r + 1]) & ~idx_mask) | (i0 + 1u));
+          const float m = __builtin_amdgcn_fmed3f(b1[q], u, v);
+          b1[q] = max3f(b1[q], u, v);
+          b2[q] = __builtin_amdgcn_fmed3f(b2[q], m, pinf);
+        }
+      }
+    };
+    if constexpr (D <= 16) {
+      // the products of the next tile run on the matrix core while the
+      // vector unit ranks the current one
+      f32x16 acc0[kS], acc1[kS];
+      tile_mma(0, acc0);
+      for (int ct = 0; ct < n_ct; ct += 2) {
+        const bool has1 = ct + 1 < n_ct;
+        if (has1) tile_mma(ct + 1, acc1);
+        tile_track(ct, acc0);
+        if (has1) {
+          if (ct + 2 < n_ct) tile_mma(ct + 2, acc0);
+          tile_track(ct + 1, acc1);
+        }
+      }
+    } else {
+      // wider samples leave no registers for a second accumulator set
+      for (int ct = 0; ct < n_ct; ++ct) {
+        f32x16 acc[kS];
+        tile_mma(ct, acc);
+        tile_track(ct, acc);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kS; ++q) {
+      const float o1 = __shfl_xor(b1[q], 32, kWave);
+      const float o2 = __shfl_xor(b2[q], 32, kWave);
+      const float n2 = fmaxf(fminf(b1[q], o1), fmaxf(b2[q], o2));
+      b1[q] = fmaxf(b1[q], o1);
+      b2[q] = n2;
+    }
+    bool amb[kS];
+    int lab[kS];
+#pragma unroll
+    for (int q = 0; q < kS; ++q) {
+      const float e = xnorm[q] * cmax * alpha + hnmax * beta +
+                      pack_err * (fabsf(b1[q]) + fabsf(b2[q]));
+      const float gap = b1[q] - b2[q];
+      // far from overflow |D| <= ||x|| ||c|| (1 + 2^-8)^2 + hn stays finite
+      amb[q] = !(gap > 2.f * e) || !(xnorm[q] * cmax < 1e36f) || !finite_c;
+      lab[q] = (int)(__float_as_uint(b1[q]) & idx_mask);
+    }
+    // labels of decided samples; queue the others for the FP32 re-score
+#pragma unroll
+    for (int q = 0; q < kS; ++q) {
+      const int64_t s = s0 + q;
+      const bool valid = s < n;
+      if (half == 0 && valid && !amb[q]) labels[s] = (I)lab[q];
+      const bool enq = (half == 0) && valid && amb[q];
+      const uint64_t mask = __ballot(enq);
+      if (mask != 0ull) {
+        const int cnt = __popcll(mask);
+        const int leader = __ffsll((long long)mask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&q_used, cnt);
+        base = __shfl(base, leader, kWave);
+        if (enq) {
+          const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+          qseg[base + rank] = (int)s;
+        }
+      }
+    }
+    // int64 fixed-point update, sums centroid-major in LDS: S64[l (d+1) + f]
+    // (f = d: the count), so each lane's 16 features are immediate offsets
+    // from one address.  Undecided and out-of-range samples add zeros
+    // (weight 0), which keeps the wave convergent.
+    load(s0, xr);
+#pragma unroll
+    for (int q = 0; q < kS; ++q) {
+      const int64_t s = s0 + q;
+      const float ws = (s < n && !amb[q]) ? w[s] : 0.f;
+      const int l = min(lab[q], k - 1);
+      u64 *row = S64 + (int64_t)l * (d + 1);
+#pragma unroll
+      for (int i = 0; i < XH; ++i) {
+        const int f = 16 * (i / 8) + 8 * half + (i % 8);
+        if (FULL || f < d) {
+          const int qv = __float2int_rn((xr[i][q] * ws) * Sc[f]);
+          atomicAdd(row + f, (u64)(long long)qv);
+        }
+      }
+      if (half == 0)
+        atomicAdd(row + d, (u64)(long long)__float2int_rn(ws * Sc[d]));
+    }
+  }
+  __syncthreads();
+  // partial slot layout: [f][c] sums, then [c] counts
+  u64 *cp = a.partials + (int64_t)blockIdx.x * (dk + k);
+  for (int64_t i = tid; i < dk + k; i += NT) {
+    const int64_t f = i / k, c = i - f * k;
+    cp[i] = S64[c * (d + 1) + f];
+  }
+  if (tid == 0) a.queue_counts[blockIdx.x] = q_used;
+}
+
+inline int d_bucket(int64_t d) {
+  if (d <= 16) return 16;
+  if (d <= 32) return 32;
+  if (d <= 48) return 48;
+  return 64;
+}
+
+inline int kk_of(int D, int T) { return (T * D + 3 + 15) / 16; }
+
+inline size_t mixed_lds_bytes(int64_t d, int64_t k, int T) {
+  const int D = d_bucket(d);
+  const int64_t kpad = ceil_div<int64_t>(k, 32) * 32;
+  return (size_t)(kpad / 32) * kk_of(D, T) * 64 * 16 + sizeof(float) * kScPad +
+         sizeof(u64) * (size_t)(d + 1) * k + 16;
+}
+
+using MixedFn = void (*)(MixedArgs);
+
+template <typename I>
+MixedFn pick_mixed(int64_t d, int T) {
+  const int D = d_bucket(d);
+#define KMD_MIXED_CASE(DD)                                                  \
+  if (D == DD) {                                                            \
+    constexpr int NT = nt_of(DD);                                           \
+    if (d == DD)                                                            \
+      return T == 3 ? &mixed_lloyd_krn<DD, 3, true, I, NT>                  \
+                    : &mixed_lloyd_krn<DD, 1, true, I, NT>;                 \
+    return T == 3 ? &mixed_lloyd_krn<DD, 3, false, I, NT>                   \
+                  : &mixed_lloyd_krn<DD, 1, false, I, NT>;                  \
+  }
+  KMD_MIXED_CASE(16)
+  KMD_MIXED_CASE(32)
+  KMD_MIXED_CASE(48)
+  KMD_MIXED_CASE(64)
+#undef KMD_MIXED_CASE
+  return nullptr;
+}
+
+}  // namespace
+
+int mixed_idx_bits(int64_t k) {
+  const int64_t kpad = ceil_div<int64_t>(k, 32) * 32;
+  int b = 1;
+  while ((int64_t(1) << b) < kpad) ++b;
+  return b;
+}
+
+size_t mixed_aop_bytes(int64_t d, int64_t k, int T) {
+  const int D = d_bucket(d);
+  const int64_t kpad = ceil_div<int64_t>(k, 32) * 32;
+  return (size_t)(kpad / 32) * kk_of(D, T) * 64 * 16;
+}
+
+int64_t mixed_seg_cap(int64_t n, int64_t d, int blocks) {
+  const int nt = nt_of(d_bucket(d));
+  const int64_t n_super = ceil_div<int64_t>(n > 0 ? n : 1, 32 * kS);
+  const int64_t waves = (int64_t)blocks * (nt / kWave);
+  return ceil_div<int64_t>(n_super, waves) * (nt / kWave) * 32 * kS;
+}
+
+template <typename I>
+int mixed_blocks(int64_t n, int64_t d, int64_t k, int T) {
+  if (d < 1 || d > 64 || k < 1 || n >= (int64_t(1) << 31)) return 0;
+  if (mixed_idx_bits(k) > 12) return 0;  // index packing: <= 4096 centroids
+  const size_t lds = mixed_lds_bytes(d, k, T);
+  if (lds > kLdsPerCu) return 0;
+  MixedFn fn = pick_mixed<I>(d, T);
+  if (fn == nullptr) return 0;
+  static thread_local MixedFn attr_done[64] = {};
+  bool seen = false;
+  for (MixedFn f : attr_done) seen |= (f == fn);
+  if (!seen) {
+    KMD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)kLdsPerCu));
+    for (MixedFn &f : attr_done)
+      if (f == nullptr) {
+        f = fn;
+        break;
+      }
+  }
+  int per_cu = 0;
+  KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void *>(fn), nt_of(d_bucket(d)), lds));
+  if (per_cu < 1) return 0;
+  const int64_t need =
+      ceil_div<int64_t>(ceil_div<int64_t>(n > 0 ? n : 1, 32 * kS),
+                        nt_of(d_bucket(d)) / kWave);
+  int64_t blocks = (int64_t)device_cu_count() * per_cu;
+  if (blocks > need) blocks = need;
+  if (blocks > 1024) blocks = 1024;  // queue segments the re-score reads
+  return (int)(blocks < 1 ? 1 : blocks);
+}
+
+void launch_mixed_prep(int64_t d, int64_t k, int T, const float *C,
+                       const float *hn, void *aop, float *bound,
+                       hipStream_t s) {
+  const int D = d_bucket(d);
+  const int kpad = (int)(ceil_div<int64_t>(k, 32) * 32);
+  const int64_t total = (int64_t)(kpad / 32) * kk_of(D, T) * 64;
+  const dim3 grid((unsigned)ceil_div<int64_t>(total, 256));
+#define KMD_PREP_CASE(DD)                                                     \
+  if (D == DD) {                                                              \
+    if (T == 3)                                                               \
+      hipLaunchKernelGGL((prep_aop_krn<DD, 3>), grid, dim3(256), 0, s, (int)d, \
+                         (int)k, kpad, C, hn, (bf16x8 *)aop);                 \
+    else                                                                      \
+      hipLaunchKernelGGL((prep_aop_krn<DD, 1>), grid, dim3(256), 0, s, (int)d, \
+                         (int)k, kpad, C, hn, (bf16x8 *)aop);                 \
+  }
+  KMD_PREP_CASE(16)
+  KMD_PREP_CASE(32)
+  KMD_PREP_CASE(48)
+  KMD_PREP_CASE(64)
+#undef KMD_PREP_CASE
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bound_krn, dim3(1), dim3(256), 0, s, (int)k, hn, bound);
+  KMD_LAUNCH_CHECK();
+}
+
+template <typename I>
+void launch_mixed_step(int64_t n, int64_t d, int64_t k, int T, int blocks,
+                       const float *X, const float *w, const void *aop,
+                       const float *bound, const float *scale, I *labels,
+                       void *partials, int *queue, int *queue_counts,
+                       hipStream_t s) {
+  MixedFn fn = pick_mixed<I>(d, T);
+  if (fn == nullptr || blocks < 1)
+    throw RuntimeError("mixed-precision step: unsupported shape");
+  MixedArgs a{};
+  a.n = n;
+  a.d = (int)d;
+  a.k = (int)k;
+  a.kpad = (int)(ceil_div<int64_t>(k, 32) * 32);
+  a.idx_bits = mixed_idx_bits(k);
+  a.X = X;
+  a.w = w;
+  a.aop = reinterpret_cast<const bf16x8 *>(aop);
+  a.bound = bound;
+  a.scale = scale;
+  a.labels = labels;
+  a.partials = reinterpret_cast<u64 *>(partials);
+  a.queue = queue;
+  a.queue_counts = queue_counts;
+  a.seg_cap = mixed_seg_cap(n, d, blocks);
+  a.pinf = std::numeric_limits<float>::infinity();
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(nt_of(d_bucket(d))),
+                     mixed_lds_bytes(d, k, T), s,
+                     a);
+  KMD_LAUNCH_CHECK();
+}
+
+template int mixed_blocks<int32_t>(int64_t, int64_t, int64_t, int);
+template int mixed_blocks<int64_t>(int64_t, int64_t, int64_t, int);
+template void launch_mixed_step<int32_t>(int64_t, int64_t, int64_t, int, int,
+                                         const float *, const float *,
+                                         const void *, const float *,
+                                         const float *, int32_t *, void *,
+                                         int *, int *, hipStream_t);
+template void launch_mixed_step<int64_t>(int64_t, int64_t, int64_t, int, int,
+                                         const float *, const float *,
+                                         const void *, const float *,
+                                         const float *, int64_t *, void *,
+                                         int *, int *, hipStream_t);
+
+}  // namespace kmd

@@ -717,6 +717,7 @@ py::tuple py_kmeans_lloyd_driver_ex(py::object X_, py::object w_,
   wait_depends(s, depends);
   int64_t n_iters = 0, n_rescored = 0, launches = 0;
   double inertia = 0.0, kernel_ms = 0.0;
+  int path = 0;
   int rc;
   {
     py::gil_scoped_release nogil;
@@ -724,13 +725,16 @@ py::tuple py_kmeans_lloyd_driver_ex(py::object X_, py::object w_,
         X.dtype(), lab.itype(), prec, n_samples, n_features, n_clusters,
         X.data, w.data, init.data, lab.data, res.data, tol, verbose ? 1 : 0,
         (int64_t)max_iter, &n_iters, &inertia, &n_rescored, &kernel_ms,
-        &launches, verbose ? &print_to_python : nullptr, nullptr, s);
+        &launches, &path, verbose ? &print_to_python : nullptr, nullptr, s);
   }
   check_rc(rc);
   py::dict stats;
   stats["n_rescored"] = n_rescored;
   stats["fused_kernel_ms"] = kernel_ms;
   stats["fused_launches"] = launches;
+  static const char *kPaths[] = {"generic", "fp32-mfma", "bf16+rescore",
+                                 "bf16x3+rescore"};
+  stats["path"] = kPaths[(path >= 0 && path < 4) ? path : 0];
   return py::make_tuple((size_t)n_iters, inertia, stats);
 }
 

@@ -175,3 +175,51 @@ def test_driver_nonfinite_input_propagates(kd, torch_dev):
     Xt[3, 100] = np.nan
     lab, C, n_iter, inertia = run_driver(kd, Xt, np.ones(4096, np.float32), init, 0.0, 1)
     assert np.isnan(C).any()
+
+
+def run_driver_ex(kd, Xt, w, init, tol, max_iter, precision):
+    dev = torch.device("cuda", 0)
+    X_d, w_d = to_dev(Xt), to_dev(w)
+    init_d = to_dev(init)
+    res_d = torch.empty_like(init_d)
+    lab_d = torch.empty(Xt.shape[1], dtype=torch.int32, device=dev)
+    n_iter, inertia, stats = kd.kmeans_lloyd_driver_ex(X_d, w_d, init_d, lab_d, res_d, tol,
+                                                       False, max_iter, precision, q())
+    return to_np(lab_d).astype(np.int64), to_np(res_d), int(n_iter), inertia, stats
+
+
+@pytest.mark.parametrize("shape,seed,std", [((50_000, 14, 127), 1000, 1.0),
+                                            ((300_001, 32, 256), 7, 2.0),
+                                            ((100_000, 64, 100), 3, 1.5),
+                                            ((65_537, 5, 33), 9, 3.0)])
+def test_mixed_precision_matches_fp32_bit_exactly(kd, torch_dev, shape, seed, std):
+    """BF16 (+ certified-margin FP32 re-score) and BF16x3 fits give the FP32
+    path's labels; with exact fixed-point sums the centroids are then
+    bit-identical too, iteration for iteration."""
+    n, d, k = shape
+    Xt, init = lo.gaussian_blobs(n, d, k, seed=seed, std=std)
+    w = np.ones(n, np.float32)
+    ref = run_driver_ex(kd, Xt, w, init, 1e-4, 25, "fp32")
+    assert ref[4]["path"] == "fp32-mfma"
+    rescored = {}
+    for prec, path in (("bf16", "bf16+rescore"), ("bf16x3", "bf16x3+rescore")):
+        got = run_driver_ex(kd, Xt, w, init, 1e-4, 25, prec)
+        assert got[4]["path"] == path
+        assert got[2] == ref[2]
+        assert_array_equal(got[0], ref[0])
+        assert_array_equal(got[1].view(np.uint32), ref[1].view(np.uint32))
+        assert got[3] == ref[3]
+        rescored[prec] = got[4]["n_rescored"]
+    assert rescored["bf16x3"] <= rescored["bf16"]
+    # and the fit is within 1e-5 (relative to the centroid scale) of FP64
+    l64, C64, it64, in64 = lo.kmeans_lloyd(Xt, w, init, 1e-4, 25, dtype=np.float64)
+    if it64 == ref[2] and (l64 == ref[0]).mean() > 0.9999:
+        assert np.abs(ref[1] - C64).max() <= 1e-5 * np.abs(C64).max()
+
+
+def test_mixed_precision_falls_back_on_nonfinite(kd, torch_dev):
+    Xt, init = lo.gaussian_blobs(4096, 8, 4, seed=2)
+    Xt = Xt.copy()
+    Xt[3, 100] = np.inf
+    got = run_driver_ex(kd, Xt, np.ones(4096, np.float32), init, 0.0, 1, "bf16")
+    assert got[4]["path"] == "fp32-mfma"
