@@ -43,6 +43,10 @@ def parse():
     p.add_argument("--k", type=int, default=K)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--copy-probe-mib", type=int, default=0,
+                   help="first run one device-to-device copy of this many MiB "
+                        "(a kernel of known HBM traffic: calibrates rocprofv3 "
+                        "FETCH_SIZE / WRITE_SIZE)")
     return p.parse_args()
 
 
@@ -130,6 +134,13 @@ def main():
     w = torch.ones(n, dtype=torch.float32, device=dev)
     labels = torch.empty(n, dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream()
+    if args.copy_probe_mib > 0:
+        src = torch.full((args.copy_probe_mib << 18,), 1.0, dtype=torch.float32,
+                         device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # warmup: W iterations (untimed)
     c_a = init_t.clone()

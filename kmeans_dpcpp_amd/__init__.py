@@ -6,7 +6,7 @@ a ROCm device); the reference's `sycl_queue` argument takes a stream-like
 (None, an int hipStream_t, a torch.cuda.Stream or :class:`Stream`).
 
 Importing this package loads the native module built by
-``python -m kmeans_dpcpp_amd.build``; there is no fallback implementation.
+``python kmeans_dpcpp_amd/build.py``; there is no fallback implementation.
 """
 from __future__ import annotations
 
@@ -19,7 +19,7 @@ try:
 except ImportError as _e:  # pragma: no cover - exercised only without a build
     raise ImportError(
         "kmeans_dpcpp_amd native module is not built; run "
-        "`python -m kmeans_dpcpp_amd.build` (needs hipcc for gfx950): "
+        "`python kmeans_dpcpp_amd/build.py` (needs hipcc for gfx950): "
         f"{_e}") from _e
 
 from ._kmeans_dpcpp_amd import (  # noqa: E402

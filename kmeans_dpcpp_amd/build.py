@@ -4,7 +4,7 @@ Outputs (next to this file):
   libkmeans_dpcpp_amd.so          HIP kernels + host driver + C ABI
   _kmeans_dpcpp_amd<EXT_SUFFIX>   pybind11 boundary (plain C++, links the above)
 
-Usage: python -m kmeans_dpcpp_amd.build   (or kmeans_dpcpp_amd.build.build())
+Usage: python kmeans_dpcpp_amd/build.py   (or __graft_entry__.build())
 """
 from __future__ import annotations
 
