@@ -240,8 +240,10 @@ int kmd_iterative_merge_sort(int dtype, int64_t n, const void *input,
  * iterations; *fused_kernel_ms = summed HIP-event duration of the fused-step
  * kernel launches (in the mixed modes: BF16 kernel + FP32 re-score),
  * *fused_launches = their number; *fused_path = 0 generic kernel, 1 FP32
- * MFMA, 2 BF16 + re-score, 3 BF16x3 + re-score (a mixed mode falls back to
- * FP32 MFMA for shapes it does not serve or non-finite inputs). */
+ * MFMA, 2 BF16 + re-score, 3 BF16x3 + re-score, 4/5/6 the same three on the
+ * tiled path (d <= 128 with centroids or sums beyond LDS: centroid tiles
+ * streamed through LDS, update as a separate exact pass); a mixed mode falls
+ * back to FP32 for shapes it does not serve or non-finite inputs. */
 int kmd_kmeans_lloyd_driver_ex(
     int dtype, int itype, int precision, int64_t n_samples,
     int64_t n_features, int64_t n_clusters, const void *X_t,

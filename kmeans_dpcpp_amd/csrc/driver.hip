@@ -23,6 +23,7 @@
 //    and half-norms, src/kmeans_lloyd_driver.hpp:226-250);
 //  * the verbose line reports the inertia of the centroids that produced the
 //    labels (the reference evaluates it against the undivided sums).
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <sstream>
@@ -39,12 +40,14 @@ struct LloydWorkspace {
   bool fast = false;       // MFMA fast path (fixed-point partials)
   bool store_flush = false;
   int mixed_terms = 0;     // BF16 split terms (1 or 3) of the mixed path, 0: FP32
+  bool tiled = false;      // tiled path: centroids streamed, separate update
   int blocks_b = 0;        // partial slots of the FP32 re-score (mixed path)
   FusedPlan plan;
   TempBuffer hn, sizes, shifts, per_sample, copies, sizes_copies, list,
       scalars, scales;
   TempBuffer aop{0, nullptr}, bound{0, nullptr}, queue{0, nullptr},
       queue_counts{0, nullptr};
+  TempBuffer xr{0, nullptr}, upd{0, nullptr};  // tiled: X sample-major, update scratch
   int64_t seg_cap = 0;     // capacity of each block's re-score queue segment
   // copies_bytes / sizes_bytes: private copies of the generic kernel, or the
   // fast path's per-block partials (in `copies`, sizes_bytes = 0)
@@ -103,6 +106,55 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_mixed_workspace(
   }
 }
 
+// Tiled workspace (float, finite inputs): X_r, one int64 sum slot, update
+// scratch; with precision != FP32 also the BF16 operands and queue.  Returns
+// nullptr for non-finite inputs (the generic path propagates them).
+template <typename T, typename I>
+static std::unique_ptr<LloydWorkspace<T, I>> make_tiled_workspace(
+    int64_t n, int64_t d, int64_t k, int precision, const T *X, const T *w,
+    hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (n <= 0 || !tiled_supported(d, k)) return nullptr;
+    auto ws = std::make_unique<LloydWorkspace<T, I>>(
+        n, d, k, 1, true, true, FusedPlan{}, sizeof(int64_t) * (size_t)(d * k + k),
+        0, s);
+    launch_fixed_point_scales(n, d, X, w, ws->scale(), ws->nonfinite(), s);
+    int bad = 0;
+    KMD_HIP_CHECK(hipMemcpyAsync(&bad, ws->nonfinite(), sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+    KMD_HIP_CHECK(hipStreamSynchronize(s));
+    if (bad) return nullptr;
+    ws->tiled = true;
+    ws->xr = TempBuffer(sizeof(float) * (size_t)n * d, s);
+    launch_transpose(n, d, X, ws->xr.template as<float>(), s);
+    ws->upd = TempBuffer(tiled_update_scratch_bytes(n, k), s);
+    if (precision != KMD_PREC_FP32) {
+      const int terms = precision == KMD_PREC_BF16X3 ? 3 : 1;
+      const int ba = tiled_mixed_blocks<I>(n, d, k, terms);
+      if (ba >= 1) {
+        ws->mixed_terms = terms;
+        ws->blocks_b = ba;
+        ws->seg_cap = tiled_mixed_seg_cap(n, d, terms, ba);
+        ws->aop = TempBuffer(tiled_aop_bytes(d, k, terms), s);
+        ws->bound = TempBuffer(2 * sizeof(float), s);
+        ws->queue = TempBuffer(sizeof(int) * (size_t)ba * ws->seg_cap, s);
+        ws->queue_counts = TempBuffer(sizeof(int) * (size_t)ba, s);
+      }
+    }
+    return ws;
+  } else {
+    (void)n, (void)d, (void)k, (void)precision, (void)X, (void)w, (void)s;
+    return nullptr;
+  }
+}
+
+// Test hook: KMD_FORCE_TILED=1 selects the tiled path for every shape it
+// serves (the LDS-resident kernels are otherwise preferred).
+static bool force_tiled() {
+  const char *e = std::getenv("KMD_FORCE_TILED");
+  return e != nullptr && e[0] == '1';
+}
+
 template <typename T, typename I>
 static std::unique_ptr<LloydWorkspace<T, I>> make_workspace(int64_t n,
                                                             int64_t d,
@@ -111,9 +163,14 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_workspace(int64_t n,
                                                             const T *w,
                                                             int precision,
                                                             hipStream_t s) {
-  if (precision != KMD_PREC_FP32) {
+  const bool tiled_first = force_tiled();
+  if (precision != KMD_PREC_FP32 && !tiled_first) {
     auto ws = make_mixed_workspace<T, I>(
         n, d, k, precision == KMD_PREC_BF16X3 ? 3 : 1, X, w, s);
+    if (ws) return ws;
+  }
+  if (tiled_first || !fast_fused_supported<T>(d, k)) {
+    auto ws = make_tiled_workspace<T, I>(n, d, k, precision, X, w, s);
     if (ws) return ws;
   }
   if (n > 0 && fast_fused_supported<T>(d, k)) {
@@ -184,6 +241,37 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
   launch_half_l2_norm<T>(d, k, cur, ws.hn.template as<T>(), s);
   KMD_HIP_CHECK(hipMemsetAsync(ws.n_empty(), 0, sizeof(I), s));
   if constexpr (std::is_same<T, float>::value) {
+    if (ws.tiled) {
+      const float *xr = ws.xr.template as<const float>();
+      timer.begin(s);
+      if (ws.mixed_terms > 0) {
+        launch_tiled_mixed_prep(d, k, ws.mixed_terms, cur,
+                                ws.hn.template as<const T>(),
+                                ws.aop.template as<void>(),
+                                ws.bound.template as<float>(), s);
+        launch_tiled_mixed_assign<I>(n, d, k, ws.mixed_terms, ws.blocks_b, X,
+                                     ws.aop.template as<const void>(),
+                                     ws.bound.template as<const float>(), labels,
+                                     ws.queue.template as<int>(),
+                                     ws.queue_counts.template as<int>(), s);
+        launch_tiled_rescore<I>(n, d, k, xr, cur, ws.hn.template as<const T>(),
+                                labels, ws.queue.template as<const int>(),
+                                ws.queue_counts.template as<const int>(),
+                                ws.blocks_b, ws.seg_cap, ws.queue_count(), s);
+      } else {
+        launch_tiled_assign<I>(n, d, k, X, cur, ws.hn.template as<const T>(),
+                               labels, s);
+      }
+      launch_tiled_update<I>(n, d, k, xr, w, labels, ws.scale(),
+                             ws.upd.template as<void>(),
+                             ws.copies.template as<void>(), s);
+      timer.end(s);
+      launch_reduce_fixed_partials<T, I>(
+          1, d, k, ws.copies.template as<const void>(), ws.scale(),
+          ws.nonfinite(), ws.sizes.template as<T>(), next,
+          ws.list.template as<I>(), ws.n_empty(), k, s);
+      return;
+    }
     if (ws.mixed_terms > 0) {
       launch_mixed_prep(d, k, ws.mixed_terms, cur, ws.hn.template as<const T>(),
                         ws.aop.template as<void>(), ws.bound.template as<float>(),
@@ -340,8 +428,10 @@ void driver_lloyd(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
   if (stats) {
     stats->fused_kernel_ms = timer.total_ms;
     stats->fused_launches = timer.launches;
-    stats->fused_path = ws.mixed_terms > 0 ? (ws.mixed_terms == 3 ? 3 : 2)
-                                           : (ws.fast ? 1 : 0);
+    int path = ws.mixed_terms > 0 ? (ws.mixed_terms == 3 ? 3 : 2)
+                                  : (ws.fast ? 1 : 0);
+    if (ws.tiled) path += 3;  // 4 tiled FP32, 5 tiled BF16, 6 tiled BF16x3
+    stats->fused_path = path;
     stats->n_rescored = rescored;
   }
 }

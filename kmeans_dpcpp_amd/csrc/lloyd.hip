@@ -206,6 +206,12 @@ void launch_assignment(int64_t n, int64_t d, int64_t k, int64_t h,
     launch_fast_assign<T, I>(n, d, k, X, C, hn, labels, s);
     return;
   }
+  if constexpr (std::is_same<T, float>::value) {
+    if (tiled_supported(d, k)) {
+      launch_tiled_assign<I>(n, d, k, X, C, hn, labels, s);
+      return;
+    }
+  }
   const int64_t blocks = ceil_div<int64_t>(n, kBlock);
   hipLaunchKernelGGL((lloyd_tile_krn<T, I, MODE_ASSIGN>), dim3(blocks),
                      dim3(kBlock), window_lds_bytes<T>(), s, n, (int)d, (int)k,

@@ -116,4 +116,43 @@ void launch_fast_rescore(int64_t n, int64_t d, int64_t k, const float *X,
                          const int *queue_counts, int n_segments,
                          int64_t seg_cap, int *n_rescored, hipStream_t s);
 
+// ---- tiled path (lloyd_tiled.hip): d <= 128, centroids streamed via LDS ----
+bool tiled_supported(int64_t d, int64_t k);
+// X_r (n, d) = transpose of X_t (d, n)
+void launch_transpose(int64_t n, int64_t d, const float *X_t, float *X_r,
+                      hipStream_t s);
+// FP32 MFMA assignment (labels only)
+template <typename I>
+void launch_tiled_assign(int64_t n, int64_t d, int64_t k, const float *X_t,
+                         const float *C, const float *hn, I *labels,
+                         hipStream_t s);
+// FP32 re-score of the samples in the queue segments (same arithmetic as
+// launch_tiled_assign); samples read from X_r; *n_rescored = queue length
+template <typename I>
+void launch_tiled_rescore(int64_t n, int64_t d, int64_t k, const float *X_r,
+                          const float *C, const float *hn, I *labels,
+                          const int *queue, const int *queue_counts,
+                          int n_segments, int64_t seg_cap, int *n_rescored,
+                          hipStream_t s);
+// BF16 (T = 1) / BF16x3 (T = 3) assignment with the certified margin test:
+// grid size (0 when not served), queue segment capacity, A-operand bytes
+template <typename I>
+int tiled_mixed_blocks(int64_t n, int64_t d, int64_t k, int T);
+int64_t tiled_mixed_seg_cap(int64_t n, int64_t d, int T, int blocks);
+size_t tiled_aop_bytes(int64_t d, int64_t k, int T);
+void launch_tiled_mixed_prep(int64_t d, int64_t k, int T, const float *C,
+                             const float *hn, void *aop, float *bound,
+                             hipStream_t s);
+template <typename I>
+void launch_tiled_mixed_assign(int64_t n, int64_t d, int64_t k, int T,
+                               int blocks, const float *X_t, const void *aop,
+                               const float *bound, I *labels, int *queue,
+                               int *queue_counts, hipStream_t s);
+// Exact int64 fixed-point sums [d*k + k] of w x per label from X_r
+size_t tiled_update_scratch_bytes(int64_t n, int64_t k);
+template <typename I>
+void launch_tiled_update(int64_t n, int64_t d, int64_t k, const float *X_r,
+                         const float *w, const I *labels, const float *scale,
+                         void *scratch, void *sums, hipStream_t s);
+
 }  // namespace kmd

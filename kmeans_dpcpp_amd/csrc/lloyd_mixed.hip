@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(256)
         const float h3 = (float)to_bf16(r1 - h2);
         val = (t == 0) ? h1 : (t == 1 ? h2 : h3);
       } else {
-        val = (t == 0) ? 1e30f : 0.f;  // padding column: never the best
+        val = (t == 0) ? 3.0e38f : 0.f;  // padding centroid: never the best
       }
     }
     v[j] = to_bf16(val);
@@ -332,8 +332,9 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
                       pack_err * (fabsf(b1[q]) + fabsf(b2[q]));
       const float gap = b1[q] - b2[q];
       // far from overflow |D| <= ||x|| ||c|| (1 + 2^-8)^2 + hn stays finite
-      amb[q] = !(gap > 2.f * e) || !(xnorm[q] * cmax < 1e36f) || !finite_c;
       lab[q] = (int)(__float_as_uint(b1[q]) & idx_mask);
+      amb[q] = !(gap > 2.f * e) || !(xnorm[q] * cmax < 1e36f) || !finite_c ||
+               lab[q] >= k;
     }
     // labels of decided samples; queue the others for the FP32 re-score
 #pragma unroll

@@ -1,0 +1,989 @@
+// Tiled Lloyd path: shapes the LDS-resident kernels cannot hold (d up to 128,
+// k up to 32768), e.g. n = 2M, d = 128, k = 1024.
+//
+// Assignment.  Samples stay in registers for a whole pass over the
+// centroids; the centroids stream through LDS in chunks of 32-centroid tiles
+// that the 8 waves of a block share (every block walks the same chunks, so
+// they are L2 hits after the first block).  Blocks step through rounds of
+// 8 super-blocks in lock step, so every barrier is reached by all waves.
+//   FP32: v_mfma_f32_32x32x2f32, the FP32 path's arithmetic (products
+//         accumulated in FP32 over ascending k-steps, argmin of hn - dot with
+//         strict < and lowest index on ties).  A gather mode re-scores the
+//         samples listed in queue segments, reading them from a sample-major
+//         copy of X.
+//   BF16: v_mfma_f32_32x32x16_bf16 with the half norm folded into the
+//         product and a certified best / second-best margin test (the scheme
+//         of lloyd_mixed.hip); undecided samples go to per-block queue
+//         segments for the FP32 gather re-score, so labels equal the FP32
+//         path's.
+//
+// Update.  With d * k beyond LDS, per-block sums do not fit, so the update is
+// its own pass: labels are counted, the cluster totals scanned, sample
+// indices scattered into cluster order, and each (cluster, chunk of 256
+// members) work item sums w x over its members from the sample-major copy
+// X_r (one contiguous row per sample) as int64 fixed point (the scales of
+// launch_fixed_point_scales), then adds into the cluster's int64 sums with
+// one global atomic per feature.  Integer addition is exact and
+// order-independent, so the sums equal those of the fused kernels bit for
+// bit.
+#include <algorithm>
+
+#include "lloyd_fast.hpp"
+
+namespace kmd {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned long long u64;
+
+constexpr int kTNT = 512;                 // threads per block (8 waves)
+constexpr int kTW = kTNT / kWave;         // waves per block
+constexpr int kChunkBytes = 64 * 1024;    // LDS bytes of one centroid chunk
+constexpr int kMaxSegments = 1024;        // queue segments the re-score reads
+constexpr int kMaxTiledD = 128;
+constexpr int kMembers = 256;             // members per update work item
+constexpr size_t kLdsPerCu = 160 * 1024;
+
+// ---- FP32 assignment --------------------------------------------------------
+
+// FP32 k-steps (2 features each), bucketed.
+inline int ks_of(int64_t d) {
+  const int64_t ks = (d + 1) / 2;
+  if (ks <= 16) return 16;
+  if (ks <= 32) return 32;
+  return 64;
+}
+
+template <int KS>
+constexpr int tile_floats() {  // A values + the tile's 32 half norms
+  return 2 * KS * 32 + 32;
+}
+template <int KS>
+constexpr int fp32_tiles_per_chunk() {
+  return (kChunkBytes / (4 * tile_floats<KS>())) < 1
+             ? 1
+             : kChunkBytes / (4 * tile_floats<KS>());
+}
+// samples (32-column MFMA blocks) per wave: the registers hold KS * S values
+template <int KS>
+constexpr int fp32_s() {
+  return KS <= 32 ? 2 : 1;
+}
+
+struct TiledArgs {
+  int64_t n;
+  int d, k, n_ct;
+  const float *X;  // X_t (d, n); gather mode: X_r (n, d)
+  const float *C, *hn;
+  void *labels;
+  const int *queue, *queue_counts;  // gather mode
+  int n_segments;
+  int64_t seg_cap;
+  int *n_rescored;
+};
+
+// Tile layout in LDS: element (ks, half, col) = C[2 ks + half][tile*32 + col]
+// at ((ks / 4) * 2 + half) * 128 + col * 4 + ks % 4, so each lane reads four
+// k-steps with one 16-byte load; then the 32 half norms (+inf padded).
+template <int KS, typename I, bool GATHER>
+__global__ void __launch_bounds__(kTNT) tiled_fp32_krn(TiledArgs a) {
+  constexpr int S = fp32_s<KS>();
+  constexpr int TF = tile_floats<KS>();
+  constexpr int CPC = fp32_tiles_per_chunk<KS>();
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  float *Cl = reinterpret_cast<float *>(lds_raw);         // [CPC][TF]
+  int *seg_off = reinterpret_cast<int *>(Cl + CPC * TF);  // [n_segments + 1]
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1), wave = tid / kWave;
+  const int col = lane & 31, half = lane >> 5;
+  const int64_t n = a.n;
+  const int d = a.d, k = a.k, n_ct = a.n_ct;
+  I *labels = reinterpret_cast<I *>(a.labels);
+
+  if constexpr (GATHER) {
+    if (tid < kWave) {
+      int carry = 0;
+      for (int g0 = 0; g0 < a.n_segments; g0 += kWave) {
+        const int g = g0 + tid;
+        const int c = (g < a.n_segments) ? a.queue_counts[g] : 0;
+        int incl = c;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+          const int t = __shfl_up(incl, o, kWave);
+          if (tid >= o) incl += t;
+        }
+        if (g < a.n_segments) seg_off[g] = carry + incl - c;
+        carry += __shfl(incl, kWave - 1, kWave);
+      }
+      if (tid == 0) {
+        seg_off[a.n_segments] = carry;
+        if (blockIdx.x == 0) a.n_rescored[0] = carry;
+      }
+    }
+    __syncthreads();
+  }
+  const int64_t n_items = GATHER ? (int64_t)seg_off[a.n_segments] : n;
+  constexpr int kSuper = 32 * S;
+  const int64_t n_super = ceil_div<int64_t>(n_items, kSuper);
+  const int64_t per_round = (int64_t)gridDim.x * kTW;
+  const int64_t rounds = ceil_div<int64_t>(n_super, per_round);
+  const bool vec = !GATHER && S == 2 &&
+                   ((reinterpret_cast<uintptr_t>(a.X) & 7u) == 0) && (n % 2 == 0);
+
+  for (int64_t r = 0; r < rounds; ++r) {
+    const int64_t sb = r * per_round + (int64_t)blockIdx.x * kTW + wave;
+    const bool active = sb < n_super;
+    const int64_t p0 = sb * kSuper + (int64_t)S * col;
+    int64_t sidx[S];
+    bool valid[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      valid[q] = active && (p0 + q < n_items);
+      sidx[q] = p0 + q;
+      if (GATHER && valid[q]) {
+        const int p = (int)(p0 + q);
+        int lo = 0, hi = a.n_segments - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (seg_off[mid] <= p)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        sidx[q] = a.queue[(int64_t)lo * a.seg_cap + (p - seg_off[lo])];
+      }
+    }
+    float xr[KS][S];
+    if (GATHER) {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int f = 2 * ks + half;
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+          xr[ks][q] = (valid[q] && f < d) ? a.X[sidx[q] * d + f] : 0.f;
+      }
+    } else {
+      const bool whole = vec && active && (p0 + S <= n);
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int f = 2 * ks + half;
+        const float *row = a.X + (int64_t)f * n + p0;
+        if (f < d && whole) {
+          if constexpr (S == 2) {
+            const float2 v = *reinterpret_cast<const float2 *>(row);
+            xr[ks][0] = v.x;
+            xr[ks][1] = v.y;
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < S; ++q)
+            xr[ks][q] = (f < d && valid[q]) ? row[q] : 0.f;
+        }
+      }
+    }
+    float bestv[S];
+    int besti[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      bestv[q] = __builtin_huge_valf();
+      besti[q] = 0;
+    }
+    for (int c0 = 0; c0 < n_ct; c0 += CPC) {
+      const int cnt = min(CPC, n_ct - c0);
+      __syncthreads();
+      for (int i = tid; i < cnt * TF; i += kTNT) {
+        const int t = i / TF, e = i - t * TF;
+        const int ct = c0 + t;
+        if (e < 2 * KS * 32) {
+          // source order: feature-major (consecutive threads read
+          // consecutive centroids of one feature row)
+          const int f = e >> 5, cc = e & 31;
+          const int c = ct * 32 + cc;
+          const int ks = f >> 1, hf = f & 1;
+          Cl[t * TF + ((ks >> 2) * 2 + hf) * 128 + cc * 4 + (ks & 3)] =
+              (f < d && c < k) ? a.C[(int64_t)f * k + c] : 0.f;
+        } else {
+          const int c = ct * 32 + (e - 2 * KS * 32);
+          Cl[t * TF + e] = (c < k) ? a.hn[c] : __builtin_huge_valf();
+        }
+      }
+      __syncthreads();
+      if (active) {
+        for (int t = 0; t < cnt; ++t) {
+          const float *At = Cl + t * TF;
+          f32x16 acc[S];
+#pragma unroll
+          for (int k4 = 0; k4 < KS / 4; ++k4) {
+            const f32x4 av =
+                *reinterpret_cast<const f32x4 *>(At + (k4 * 2 + half) * 128 + col * 4);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+#pragma unroll
+              for (int q = 0; q < S; ++q) {
+                if (k4 == 0 && j == 0) {
+                  const f32x16 z = {};
+                  acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[0][q], z,
+                                                                0, 0, 0);
+                } else {
+                  acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                      av[j], xr[4 * k4 + j][q], acc[q], 0, 0, 0);
+                }
+              }
+            }
+          }
+          // rows of this lane: m = 8 (r / 4) + 4 half + (r % 4), ascending
+          const float *Ht = At + 2 * KS * 32;
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            const int m0 = 8 * g + 4 * half;
+            const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + m0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+#pragma unroll
+              for (int q = 0; q < S; ++q) {
+                const float v = h4[e] - acc[q][4 * g + e];
+                const bool upd = v < bestv[q];
+                bestv[q] = upd ? v : bestv[q];
+                besti[q] = upd ? ((c0 + t) * 32 + m0 + e) : besti[q];
+              }
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      const float ov = __shfl_xor(bestv[q], 32, kWave);
+      const int oi = __shfl_xor(besti[q], 32, kWave);
+      const bool take = (ov < bestv[q]) || (ov == bestv[q] && oi < besti[q]);
+      bestv[q] = take ? ov : bestv[q];
+      besti[q] = take ? oi : besti[q];
+      if (half == 0 && valid[q]) labels[sidx[q]] = (I)besti[q];
+    }
+  }
+}
+
+// ---- BF16 assignment with certified margin ---------------------------------
+
+inline int bf16_d_of(int64_t d) {
+  if (d <= 32) return 32;
+  if (d <= 64) return 64;
+  return 128;
+}
+inline int kk_of(int D, int T) { return (T * D + 3 + 15) / 16; }
+constexpr int bf16_s(int D, int T) { return D * T < 128 ? 2 : 1; }
+
+struct TiledMixedArgs {
+  int64_t n;
+  int d, k, n_ct, idx_bits, fp32_len;
+  const float *X;        // X_t (d, n)
+  const bf16x8 *aop;     // [n_ct][KK][64]
+  const float *bound;    // [2]: max ||c||, max hn
+  void *labels;
+  int *queue;            // [block][seg_cap]
+  int *queue_counts;     // [block]
+  int64_t seg_cap;
+  float pinf;
+};
+
+__device__ inline float max3f(float a, float b, float c) {
+  float r;
+  asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+template <int D, int T, typename I>
+__global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
+  constexpr int KD = D / 16, KK = (T * D + 3 + 15) / 16, XH = D / 2;
+  constexpr int S = bf16_s(D, T);
+  constexpr int CPC = (kChunkBytes / (KK * 64 * 16)) < 1
+                          ? 1
+                          : kChunkBytes / (KK * 64 * 16);
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  bf16x8 *Al = reinterpret_cast<bf16x8 *>(lds_raw);           // [CPC][KK][64]
+  int &q_used = *reinterpret_cast<int *>(Al + CPC * KK * 64);
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1), wave = tid / kWave;
+  const int col = lane & 31, half = lane >> 5;
+  const int64_t n = a.n;
+  const int d = a.d, n_ct = a.n_ct;
+  I *labels = reinterpret_cast<I *>(a.labels);
+  int *qseg = a.queue + (int64_t)blockIdx.x * a.seg_cap;
+  if (tid == 0) q_used = 0;
+
+  const unsigned idx_mask = (1u << a.idx_bits) - 1u;
+  const float cmax = a.bound[0], hnmax = a.bound[1];
+  const bool finite_c = hnmax < 1e36f;
+  const float pinf = a.pinf;
+  constexpr float u = 1.0f / 256.0f;
+  const float acc_err = (float)(T * D + 3 + a.fp32_len + 8) * 5.9604645e-8f;
+  const float alpha = ((T == 1) ? (2.f * u + u * u) : (3.2f * u * u)) + acc_err;
+  const float beta = acc_err;
+  const float pack_err = ldexpf(1.0f, a.idx_bits - 23) * 1.01f;
+  const bool vec = S == 2 && ((reinterpret_cast<uintptr_t>(a.X) & 7u) == 0) &&
+                   (n % 2 == 0);
+
+  constexpr int kSuper = 32 * S;
+  const int64_t n_super = ceil_div<int64_t>(n, kSuper);
+  const int64_t per_round = (int64_t)gridDim.x * kTW;
+  const int64_t rounds = ceil_div<int64_t>(n_super, per_round);
+  for (int64_t r = 0; r < rounds; ++r) {
+    const int64_t sb = r * per_round + (int64_t)blockIdx.x * kTW + wave;
+    const bool active = sb < n_super;
+    const int64_t s0 = sb * kSuper + (int64_t)S * col;
+    // lane features f(i) = 16 (i / 8) + 8 half + i % 8
+    float xnorm[S];
+    bf16x8 bhi[KD][S];
+    bf16x8 blo[(T == 3) ? KD : 1][S];
+    {
+      float ss[S];
+#pragma unroll
+      for (int q = 0; q < S; ++q) ss[q] = 0.f;
+      const bool whole = vec && active && (s0 + S <= n);
+#pragma unroll
+      for (int i = 0; i < XH; ++i) {
+        const int f = 16 * (i / 8) + 8 * half + (i % 8);
+        float xv[S];
+        const float *row = a.X + (int64_t)f * n + s0;
+        if (f < d && whole) {
+          if constexpr (S == 2) {
+            const float2 v = *reinterpret_cast<const float2 *>(row);
+            xv[0] = v.x;
+            xv[1] = v.y;
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < S; ++q)
+            xv[q] = (f < d && active && s0 + q < n) ? row[q] : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+          ss[q] = fmaf(xv[q], xv[q], ss[q]);
+          const __bf16 h = (__bf16)xv[q];
+          bhi[i / 8][q][i % 8] = h;
+          if constexpr (T == 3) blo[i / 8][q][i % 8] = (__bf16)(xv[q] - (float)h);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        const float t = ss[q] + __shfl_xor(ss[q], 32, kWave);
+        xnorm[q] = sqrtf(t) * 1.0001f;
+      }
+    }
+    bf16x8 bh;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) bh[j] = (__bf16)((half == 0 && j < 3) ? -1.f : 0.f);
+
+    float b1[S], b2[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      b1[q] = -__builtin_huge_valf();
+      b2[q] = -__builtin_huge_valf();
+    }
+    for (int c0 = 0; c0 < n_ct; c0 += CPC) {
+      const int cnt = min(CPC, n_ct - c0);
+      __syncthreads();
+      const bf16x8 *src = a.aop + (int64_t)c0 * KK * 64;
+      for (int i = tid; i < cnt * KK * 64; i += kTNT) Al[i] = src[i];
+      __syncthreads();
+      if (active) {
+        for (int t = 0; t < cnt; ++t) {
+          const bf16x8 *At = Al + t * KK * 64 + lane;
+          f32x16 acc[S];
+#pragma unroll
+          for (int kk = 0; kk < KK; ++kk) {
+            const bf16x8 av = At[kk * 64];
+#pragma unroll
+            for (int q = 0; q < S; ++q) {
+              bf16x8 bv;
+              if (kk < KD)
+                bv = bhi[kk][q];
+              else if (T == 3 && kk < 2 * KD)
+                bv = bhi[(kk - KD) % KD][q];
+              else if (T == 3 && kk < 3 * KD)
+                bv = blo[(kk - 2 * KD) % KD][q];
+              else
+                bv = bh;
+              if (kk == 0) {
+                const f32x16 z = {};
+                acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, z, 0, 0, 0);
+              } else {
+                acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[q], 0,
+                                                                  0, 0);
+              }
+            }
+          }
+          const int ct = c0 + t;
+#pragma unroll
+          for (int rr = 0; rr < 16; rr += 2) {
+            const unsigned i0 =
+                (unsigned)(ct * 32 + 8 * (rr / 4) + 4 * half + (rr % 4));
+#pragma unroll
+            for (int q = 0; q < S; ++q) {
+              const float uu =
+                  __uint_as_float((__float_as_uint(acc[q][rr]) & ~idx_mask) | i0);
+              const float vv = __uint_as_float(
+                  (__float_as_uint(acc[q][rr + 1]) & ~idx_mask) | (i0 + 1u));
+              const float m = __builtin_amdgcn_fmed3f(b1[q], uu, vv);
+              b1[q] = max3f(b1[q], uu, vv);
+              b2[q] = __builtin_amdgcn_fmed3f(b2[q], m, pinf);
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      const float o1 = __shfl_xor(b1[q], 32, kWave);
+      const float o2 = __shfl_xor(b2[q], 32, kWave);
+      const float n2 = fmaxf(fminf(b1[q], o1), fmaxf(b2[q], o2));
+      b1[q] = fmaxf(b1[q], o1);
+      b2[q] = n2;
+      const float e = xnorm[q] * cmax * alpha + hnmax * beta +
+                      pack_err * (fabsf(b1[q]) + fabsf(b2[q]));
+      const unsigned lab = __float_as_uint(b1[q]) & idx_mask;
+      const bool amb = !(b1[q] - b2[q] > 2.f * e) || !(xnorm[q] * cmax < 1e36f) ||
+                       !finite_c || lab >= (unsigned)a.k;
+      const int64_t s = s0 + q;
+      const bool valid = active && s < n;
+      if (half == 0 && valid && !amb) labels[s] = (I)lab;
+      const bool enq = (half == 0) && valid && amb;
+      const uint64_t mask = __ballot(enq);
+      if (mask != 0ull) {
+        const int c = __popcll(mask);
+        const int leader = __ffsll((long long)mask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&q_used, c);
+        base = __shfl(base, leader, kWave);
+        if (enq) {
+          const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+          qseg[base + rank] = (int)s;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) a.queue_counts[blockIdx.x] = q_used;
+}
+
+// A operands of the BF16 kernel for a runtime bucket D: element j of lane l
+// for (tile ct, k-step kk) is centroid ct*32 + l%32 at K position
+// p = 16 kk + 8 (l / 32) + j: p < T*D the split centroid features, then the
+// three bf16 terms of hn (padding centroids: 3e38, far below every real
+// score that passes the kernel's finiteness guards, so they never win).
+__global__ void __launch_bounds__(256)
+    tiled_prep_krn(int d, int k, int D, int T, int KK, int n_ct,
+                   const float *__restrict__ C, const float *__restrict__ hn,
+                   bf16x8 *__restrict__ aop) {
+  const int64_t total = (int64_t)n_ct * KK * 64;
+  const int64_t gid = blockIdx.x * 256ll + threadIdx.x;
+  if (gid >= total) return;
+  const int lane = (int)(gid % 64);
+  const int kk = (int)((gid / 64) % KK);
+  const int ct = (int)(gid / (64 * KK));
+  const int c = ct * 32 + lane % 32;
+  bf16x8 v;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int p = 16 * kk + 8 * (lane / 32) + j;
+    float val = 0.f;
+    if (p < T * D) {
+      const int seg = p / D, f = p % D;
+      const float cv = (f < d && c < k) ? C[(int64_t)f * k + c] : 0.f;
+      const float hi = (float)(__bf16)cv;
+      val = (T == 3 && seg == 1) ? (float)(__bf16)(cv - hi) : hi;
+    } else if (p < T * D + 3) {
+      const int t = p - T * D;
+      if (c < k) {
+        const float h = hn[c];
+        const float h1 = (float)(__bf16)h;
+        const float r1 = h - h1;
+        const float h2 = (float)(__bf16)r1;
+        const float h3 = (float)(__bf16)(r1 - h2);
+        val = (t == 0) ? h1 : (t == 1 ? h2 : h3);
+      } else {
+        val = (t == 0) ? 3.0e38f : 0.f;
+      }
+    }
+    v[j] = (__bf16)val;
+  }
+  aop[gid] = v;
+}
+
+__global__ void __launch_bounds__(256)
+    tiled_bound_krn(int k, const float *__restrict__ hn, float *bound) {
+  __shared__ float red[256 / kWave];
+  float m = 0.f;
+  for (int c = threadIdx.x; c < k; c += 256) m = fmaxf(m, hn[c]);
+  for (int o = kWave / 2; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.f;
+    for (int i = 0; i < 256 / kWave; ++i) t = fmaxf(t, red[i]);
+    bound[0] = sqrtf(2.f * t);
+    bound[1] = t;
+  }
+}
+
+// ---- transpose and update ----------------------------------------------------
+
+__global__ void __launch_bounds__(256)
+    transpose_krn(int64_t n, int d, const float *__restrict__ X_t,
+                  float *__restrict__ X_r) {
+  __shared__ float tile[32][33];
+  const int64_t s0 = (int64_t)blockIdx.x * 32;
+  const int f0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows per pass
+  for (int r = ty; r < 32; r += 8) {
+    const int f = f0 + r;
+    const int64_t s = s0 + tx;
+    tile[r][tx] = (f < d && s < n) ? X_t[(int64_t)f * n + s] : 0.f;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t s = s0 + r;
+    const int f = f0 + tx;
+    if (s < n && f < d) X_r[s * d + f] = tile[tx][r];
+  }
+}
+
+// totals[c] += number of samples with label c in block b's chunk (one
+// global atomic per block and non-empty cluster)
+template <typename I>
+__global__ void __launch_bounds__(256)
+    label_hist_krn(int64_t n, int k, int64_t chunk, const I *__restrict__ labels,
+                   int *__restrict__ totals) {
+  extern __shared__ int h[];
+  for (int c = threadIdx.x; c < k; c += 256) h[c] = 0;
+  __syncthreads();
+  const int64_t beg = (int64_t)blockIdx.x * chunk;
+  const int64_t end = min(n, beg + chunk);
+  for (int64_t s = beg + threadIdx.x; s < end; s += 256)
+    atomicAdd(&h[(int)labels[s]], 1);
+  __syncthreads();
+  for (int c = threadIdx.x; c < k; c += 256)
+    if (h[c] != 0) atomicAdd(&totals[c], h[c]);
+}
+
+// start[c] = exclusive scan of totals (start[k] = n); items[c] = exclusive
+// scan of ceil(totals[c] / members) (items[k] = number of work items).  One
+// block.
+__global__ void __launch_bounds__(1024)
+    cluster_scan_krn(int k, int members, const int *__restrict__ totals,
+                     int *__restrict__ start, int *__restrict__ items) {
+  __shared__ int wsum_a[1024 / kWave], wsum_b[1024 / kWave];
+  __shared__ int carry_a, carry_b;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  if (tid == 0) {
+    carry_a = 0;
+    carry_b = 0;
+  }
+  __syncthreads();
+  for (int c0 = 0; c0 < k; c0 += 1024) {
+    const int c = c0 + tid;
+    const int va = (c < k) ? totals[c] : 0;
+    const int vb = (c < k) ? (va + members - 1) / members : 0;
+    int ia = va, ib = vb;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int ta = __shfl_up(ia, o, kWave), tb = __shfl_up(ib, o, kWave);
+      if (lane >= o) {
+        ia += ta;
+        ib += tb;
+      }
+    }
+    if (lane == kWave - 1) {
+      wsum_a[wave] = ia;
+      wsum_b[wave] = ib;
+    }
+    __syncthreads();
+    int pa = 0, pb = 0;
+    for (int w2 = 0; w2 < wave; ++w2) {
+      pa += wsum_a[w2];
+      pb += wsum_b[w2];
+    }
+    if (c < k) {
+      start[c] = carry_a + pa + ia - va;
+      items[c] = carry_b + pb + ib - vb;
+    }
+    __syncthreads();
+    if (tid == 1023) {
+      carry_a += pa + ia;
+      carry_b += pb + ib;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    start[k] = carry_a;
+    items[k] = carry_b;
+  }
+}
+
+// perm[start[c] + base + rank] = s for the samples of block b's chunk with
+// label c, where base is the block's share of cluster c claimed from
+// cursor[c] (the order inside a cluster is irrelevant: the sums are exact)
+template <typename I>
+__global__ void __launch_bounds__(256)
+    scatter_krn(int64_t n, int k, int64_t chunk, const I *__restrict__ labels,
+                const int *__restrict__ start, int *__restrict__ cursor,
+                int *__restrict__ perm) {
+  extern __shared__ int h[];
+  for (int c = threadIdx.x; c < k; c += 256) h[c] = 0;
+  __syncthreads();
+  const int64_t beg = (int64_t)blockIdx.x * chunk;
+  const int64_t end = min(n, beg + chunk);
+  for (int64_t s = beg + threadIdx.x; s < end; s += 256)
+    atomicAdd(&h[(int)labels[s]], 1);
+  __syncthreads();
+  for (int c = threadIdx.x; c < k; c += 256)
+    h[c] = (h[c] != 0) ? start[c] + atomicAdd(&cursor[c], h[c]) : 0;
+  __syncthreads();
+  for (int64_t s = beg + threadIdx.x; s < end; s += 256) {
+    const int pos = atomicAdd(&h[(int)labels[s]], 1);
+    perm[pos] = (int)s;
+  }
+}
+
+__global__ void __launch_bounds__(256) fill_i32_krn(int64_t m, int *p) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * 256)
+    p[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) fill_u64_krn(int64_t m, u64 *p) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * 256)
+    p[i] = 0ull;
+}
+
+// One work item = up to kMembers members of one cluster; thread f sums
+// rint((x_f w) 2^E_f) (f < d) or rint(w 2^E_w) (f == d) over them and adds
+// the int64 total into sums[f * k + c].
+__global__ void __launch_bounds__(320)
+    segment_sum_krn(int d, int k, const float *__restrict__ X_r,
+                    const float *__restrict__ w, const float *__restrict__ scale,
+                    const int *__restrict__ perm, const int *__restrict__ start,
+                    const int *__restrict__ items, u64 *__restrict__ sums) {
+  __shared__ int sidx[kMembers];
+  __shared__ float sw[kMembers];
+  const int item = blockIdx.x;
+  if (item >= items[k]) return;
+  int lo = 0, hi = k - 1;  // the last cluster whose first item is <= item
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid] <= item)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int c = lo;
+  const int m0 = start[c] + (item - items[c]) * kMembers;
+  const int cnt = min(kMembers, start[c + 1] - m0);
+  for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+    const int s = perm[m0 + j];
+    sidx[j] = s;
+    sw[j] = w[s];
+  }
+  __syncthreads();
+  for (int f = threadIdx.x; f <= d; f += blockDim.x) {
+    const float sc = scale[f];
+    long long acc = 0;
+    if (f < d) {
+      int j = 0;
+      for (; j + 4 <= cnt; j += 4) {
+        float xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xv[u] = X_r[(int64_t)sidx[j + u] * d + f];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += __float2int_rn((xv[u] * sw[j + u]) * sc);
+      }
+      for (; j < cnt; ++j)
+        acc += __float2int_rn((X_r[(int64_t)sidx[j] * d + f] * sw[j]) * sc);
+    } else {
+      for (int j = 0; j < cnt; ++j) acc += __float2int_rn(sw[j] * sc);
+    }
+    if (acc != 0) atomicAdd(&sums[(int64_t)f * k + c], (u64)acc);
+  }
+}
+
+// ---- plans --------------------------------------------------------------------
+
+using TiledFn = void (*)(TiledArgs);
+using TiledMixedFn = void (*)(TiledMixedArgs);
+
+template <typename I, bool GATHER>
+TiledFn pick_fp32(int64_t d) {
+  switch (ks_of(d)) {
+    case 16: return &tiled_fp32_krn<16, I, GATHER>;
+    case 32: return &tiled_fp32_krn<32, I, GATHER>;
+    default: return &tiled_fp32_krn<64, I, GATHER>;
+  }
+}
+
+size_t fp32_lds_bytes(int64_t d, bool gather) {
+  const int KS = ks_of(d);
+  const int TF = 2 * KS * 32 + 32;
+  int cpc = kChunkBytes / (4 * TF);
+  if (cpc < 1) cpc = 1;
+  return (size_t)cpc * TF * 4 + (gather ? sizeof(int) * (kMaxSegments + 1) : 0);
+}
+
+template <typename I>
+TiledMixedFn pick_mixed_tiled(int64_t d, int T) {
+  const int D = bf16_d_of(d);
+#define KMD_TM_CASE(DD)                                                     \
+  if (D == DD)                                                              \
+    return T == 3 ? &tiled_mixed_krn<DD, 3, I> : &tiled_mixed_krn<DD, 1, I>;
+  KMD_TM_CASE(32)
+  KMD_TM_CASE(64)
+  KMD_TM_CASE(128)
+#undef KMD_TM_CASE
+  return nullptr;
+}
+
+size_t mixed_tiled_lds_bytes(int64_t d, int T) {
+  const int KK = kk_of(bf16_d_of(d), T);
+  int cpc = kChunkBytes / (KK * 64 * 16);
+  if (cpc < 1) cpc = 1;
+  return (size_t)cpc * KK * 64 * 16 + 16;
+}
+
+template <typename F>
+int occupancy_grid(F fn, size_t lds) {
+  static thread_local const void *attr_done[64] = {};
+  const void *p = reinterpret_cast<const void *>(fn);
+  bool seen = false;
+  for (const void *f : attr_done) seen |= (f == p);
+  if (!seen) {
+    KMD_HIP_CHECK(hipFuncSetAttribute(p, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)kLdsPerCu));
+    for (const void *&f : attr_done)
+      if (f == nullptr) {
+        f = p;
+        break;
+      }
+  }
+  int per_cu = 0;
+  KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p, kTNT, lds));
+  if (per_cu < 1) return 0;
+  return device_cu_count() * per_cu;
+}
+
+inline int64_t update_blocks(int64_t n) {
+  // chunks of >= 16K samples, at most 1024 blocks (each block claims its
+  // share of every cluster with one global atomic)
+  int64_t nb = ceil_div<int64_t>(n, 16384);
+  if (nb > 1024) nb = 1024;
+  return nb < 1 ? 1 : nb;
+}
+
+}  // namespace
+
+bool tiled_supported(int64_t d, int64_t k) {
+  // k: the update counts clusters in LDS
+  return d >= 1 && d <= kMaxTiledD && k >= 1 && k <= 32768;
+}
+
+void launch_transpose(int64_t n, int64_t d, const float *X_t, float *X_r,
+                      hipStream_t s) {
+  if (n <= 0) return;
+  const dim3 grid((unsigned)ceil_div<int64_t>(n, 32), (unsigned)ceil_div<int64_t>(d, 32));
+  hipLaunchKernelGGL(transpose_krn, grid, dim3(256), 0, s, n, (int)d, X_t, X_r);
+  KMD_LAUNCH_CHECK();
+}
+
+template <typename I>
+void launch_tiled_assign(int64_t n, int64_t d, int64_t k, const float *X_t,
+                         const float *C, const float *hn, I *labels,
+                         hipStream_t s) {
+  if (n <= 0) return;
+  require(tiled_supported(d, k), "tiled assignment: unsupported shape");
+  TiledFn fn = pick_fp32<I, false>(d);
+  const size_t lds = fp32_lds_bytes(d, false);
+  const int grid_max = occupancy_grid(fn, lds);
+  if (grid_max < 1) throw RuntimeError("tiled assignment: no launch plan");
+  const int S = ks_of(d) <= 32 ? 2 : 1;
+  const int64_t need = ceil_div<int64_t>(ceil_div<int64_t>(n, 32 * S), kTW);
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(grid_max, need));
+  TiledArgs a{};
+  a.n = n;
+  a.d = (int)d;
+  a.k = (int)k;
+  a.n_ct = (int)ceil_div<int64_t>(k, 32);
+  a.X = X_t;
+  a.C = C;
+  a.hn = hn;
+  a.labels = labels;
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kTNT), lds, s, a);
+  KMD_LAUNCH_CHECK();
+}
+
+template <typename I>
+void launch_tiled_rescore(int64_t n, int64_t d, int64_t k, const float *X_r,
+                          const float *C, const float *hn, I *labels,
+                          const int *queue, const int *queue_counts,
+                          int n_segments, int64_t seg_cap, int *n_rescored,
+                          hipStream_t s) {
+  require(n_segments >= 1 && n_segments <= kMaxSegments,
+          "tiled re-score: segment count out of range");
+  TiledFn fn = pick_fp32<I, true>(d);
+  const size_t lds = fp32_lds_bytes(d, true);
+  const int grid = occupancy_grid(fn, lds);
+  if (grid < 1) throw RuntimeError("tiled re-score: no launch plan");
+  TiledArgs a{};
+  a.n = n;
+  a.d = (int)d;
+  a.k = (int)k;
+  a.n_ct = (int)ceil_div<int64_t>(k, 32);
+  a.X = X_r;
+  a.C = C;
+  a.hn = hn;
+  a.labels = labels;
+  a.queue = queue;
+  a.queue_counts = queue_counts;
+  a.n_segments = n_segments;
+  a.seg_cap = seg_cap;
+  a.n_rescored = n_rescored;
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(kTNT), lds, s, a);
+  KMD_LAUNCH_CHECK();
+}
+
+template <typename I>
+int tiled_mixed_blocks(int64_t n, int64_t d, int64_t k, int T) {
+  if (!tiled_supported(d, k) || n <= 0 || n >= (int64_t(1) << 31)) return 0;
+  if (mixed_idx_bits(k) > 12) return 0;  // packed index: <= 4096 centroids
+  TiledMixedFn fn = pick_mixed_tiled<I>(d, T);
+  if (fn == nullptr) return 0;
+  const int grid = occupancy_grid(fn, mixed_tiled_lds_bytes(d, T));
+  if (grid < 1) return 0;
+  const int S = bf16_s(bf16_d_of(d), T);
+  const int64_t need = ceil_div<int64_t>(ceil_div<int64_t>(n, 32 * S), kTW);
+  int64_t blocks = std::min<int64_t>(grid, need);
+  if (blocks > kMaxSegments) blocks = kMaxSegments;
+  return (int)std::max<int64_t>(1, blocks);
+}
+
+int64_t tiled_mixed_seg_cap(int64_t n, int64_t d, int T, int blocks) {
+  const int S = bf16_s(bf16_d_of(d), T);
+  const int64_t n_super = ceil_div<int64_t>(n > 0 ? n : 1, 32 * S);
+  const int64_t rounds = ceil_div<int64_t>(n_super, (int64_t)blocks * kTW);
+  return rounds * kTW * 32 * S;
+}
+
+size_t tiled_aop_bytes(int64_t d, int64_t k, int T) {
+  return (size_t)ceil_div<int64_t>(k, 32) * kk_of(bf16_d_of(d), T) * 64 * 16;
+}
+
+void launch_tiled_mixed_prep(int64_t d, int64_t k, int T, const float *C,
+                             const float *hn, void *aop, float *bound,
+                             hipStream_t s) {
+  const int D = bf16_d_of(d);
+  const int KK = kk_of(D, T);
+  const int n_ct = (int)ceil_div<int64_t>(k, 32);
+  const int64_t total = (int64_t)n_ct * KK * 64;
+  hipLaunchKernelGGL(tiled_prep_krn, dim3((unsigned)ceil_div<int64_t>(total, 256)),
+                     dim3(256), 0, s, (int)d, (int)k, D, T, KK, n_ct, C, hn,
+                     reinterpret_cast<bf16x8 *>(aop));
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(tiled_bound_krn, dim3(1), dim3(256), 0, s, (int)k, hn, bound);
+  KMD_LAUNCH_CHECK();
+}
+
+template <typename I>
+void launch_tiled_mixed_assign(int64_t n, int64_t d, int64_t k, int T,
+                               int blocks, const float *X_t, const void *aop,
+                               const float *bound, I *labels, int *queue,
+                               int *queue_counts, hipStream_t s) {
+  TiledMixedFn fn = pick_mixed_tiled<I>(d, T);
+  if (fn == nullptr || blocks < 1)
+    throw RuntimeError("tiled BF16 assignment: unsupported shape");
+  TiledMixedArgs a{};
+  a.n = n;
+  a.d = (int)d;
+  a.k = (int)k;
+  a.n_ct = (int)ceil_div<int64_t>(k, 32);
+  a.idx_bits = mixed_idx_bits(k);
+  a.fp32_len = 2 * ks_of(d);
+  a.X = X_t;
+  a.aop = reinterpret_cast<const bf16x8 *>(aop);
+  a.bound = bound;
+  a.labels = labels;
+  a.queue = queue;
+  a.queue_counts = queue_counts;
+  a.seg_cap = tiled_mixed_seg_cap(n, d, T, blocks);
+  a.pinf = std::numeric_limits<float>::infinity();
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kTNT), mixed_tiled_lds_bytes(d, T), s,
+                     a);
+  KMD_LAUNCH_CHECK();
+}
+
+size_t tiled_update_scratch_bytes(int64_t n, int64_t k) {
+  return sizeof(int) * (size_t)(4 * (k + 1) + (n > 0 ? n : 1)) + 256;
+}
+
+template <typename I>
+void launch_tiled_update(int64_t n, int64_t d, int64_t k, const float *X_r,
+                         const float *w, const I *labels, const float *scale,
+                         void *scratch, void *sums, hipStream_t s) {
+  const int64_t nb = update_blocks(n);
+  const int64_t chunk = ceil_div<int64_t>(n > 0 ? n : 1, nb);
+  int *totals = reinterpret_cast<int *>(scratch);  // [k + 1]
+  int *cursor = totals + (k + 1);                  // [k + 1]
+  int *start = cursor + (k + 1);                   // [k + 1]
+  int *items = start + (k + 1);                    // [k + 1]
+  int *perm = items + (k + 1);                     // [n]
+  u64 *S = reinterpret_cast<u64 *>(sums);
+  const int64_t m = (d + 1) * k;
+  hipLaunchKernelGGL(fill_u64_krn,
+                     dim3((unsigned)std::min<int64_t>(1024, ceil_div<int64_t>(m, 256))),
+                     dim3(256), 0, s, m, S);
+  KMD_LAUNCH_CHECK();
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_i32_krn, dim3(8), dim3(256), 0, s, 2 * (k + 1), totals);
+  KMD_LAUNCH_CHECK();
+  const size_t hist_lds = sizeof(int) * (size_t)k;
+  require(hist_lds <= kLdsPerCu, "tiled update: too many clusters for LDS counts");
+  hipLaunchKernelGGL((label_hist_krn<I>), dim3((unsigned)nb), dim3(256), hist_lds, s,
+                     n, (int)k, chunk, labels, totals);
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(cluster_scan_krn, dim3(1), dim3(1024), 0, s, (int)k, kMembers,
+                     totals, start, items);
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL((scatter_krn<I>), dim3((unsigned)nb), dim3(256), hist_lds, s, n,
+                     (int)k, chunk, labels, start, cursor, perm);
+  KMD_LAUNCH_CHECK();
+  // at most n / kMembers + k work items; blocks beyond items[k] return
+  const int64_t max_items = n / kMembers + k;
+  const int threads =
+      (int)std::min<int64_t>(320, ceil_div<int64_t>(d + 1, kWave) * kWave);
+  hipLaunchKernelGGL(segment_sum_krn, dim3((unsigned)max_items), dim3(threads), 0, s,
+                     (int)d, (int)k, X_r, w, scale, perm, start, items, S);
+  KMD_LAUNCH_CHECK();
+}
+
+#define KMD_INST_TILED(I)                                                      \
+  template void launch_tiled_assign<I>(int64_t, int64_t, int64_t,              \
+                                       const float *, const float *,           \
+                                       const float *, I *, hipStream_t);       \
+  template void launch_tiled_rescore<I>(int64_t, int64_t, int64_t,             \
+                                        const float *, const float *,          \
+                                        const float *, I *, const int *,       \
+                                        const int *, int, int64_t, int *,      \
+                                        hipStream_t);                          \
+  template int tiled_mixed_blocks<I>(int64_t, int64_t, int64_t, int);          \
+  template void launch_tiled_mixed_assign<I>(int64_t, int64_t, int64_t, int,   \
+                                             int, const float *, const void *, \
+                                             const float *, I *, int *, int *, \
+                                             hipStream_t);                     \
+  template void launch_tiled_update<I>(int64_t, int64_t, int64_t,              \
+                                       const float *, const float *,           \
+                                       const I *, const float *, void *,       \
+                                       void *, hipStream_t);
+KMD_INST_TILED(int32_t)
+KMD_INST_TILED(int64_t)
+
+}  // namespace kmd

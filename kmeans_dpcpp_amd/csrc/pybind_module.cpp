@@ -732,9 +732,11 @@ py::tuple py_kmeans_lloyd_driver_ex(py::object X_, py::object w_,
   stats["n_rescored"] = n_rescored;
   stats["fused_kernel_ms"] = kernel_ms;
   stats["fused_launches"] = launches;
-  static const char *kPaths[] = {"generic", "fp32-mfma", "bf16+rescore",
-                                 "bf16x3+rescore"};
-  stats["path"] = kPaths[(path >= 0 && path < 4) ? path : 0];
+  static const char *kPaths[] = {"generic",        "fp32-mfma",
+                                 "bf16+rescore",   "bf16x3+rescore",
+                                 "tiled-fp32",     "tiled-bf16+rescore",
+                                 "tiled-bf16x3+rescore"};
+  stats["path"] = kPaths[(path >= 0 && path < 7) ? path : 0];
   return py::make_tuple((size_t)n_iters, inertia, stats);
 }
 

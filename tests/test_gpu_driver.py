@@ -223,3 +223,79 @@ def test_mixed_precision_falls_back_on_nonfinite(kd, torch_dev):
     Xt[3, 100] = np.inf
     got = run_driver_ex(kd, Xt, np.ones(4096, np.float32), init, 0.0, 1, "bf16")
     assert got[4]["path"] == "fp32-mfma"
+
+
+# ---- tiled path (centroid tiles streamed through LDS, separate exact update)
+
+def same_fit(a, b):
+    assert a[2] == b[2]
+    assert_array_equal(a[0], b[0])
+    assert_array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+    assert a[3] == b[3]
+
+
+@pytest.mark.parametrize("shape", [(30_000, 24, 40), (20_001, 64, 100), (8_193, 7, 33)])
+def test_tiled_fp32_equals_lds_resident_fp32(kd, torch_dev, monkeypatch, shape):
+    """Forced onto the tiled path, a shape the LDS-resident kernel serves
+    gives the same labels, centroids (bit for bit), iteration count and
+    inertia: the same FP32 MFMA arithmetic, and int64 sums either way."""
+    n, d, k = shape
+    Xt, init = lo.gaussian_blobs(n, d, k, seed=11, std=1.5)
+    w = np.random.default_rng(5).uniform(0.5, 2.0, n).astype(np.float32)
+    ref = run_driver_ex(kd, Xt, w, init, 1e-4, 20, "fp32")
+    assert ref[4]["path"] == "fp32-mfma"
+    monkeypatch.setenv("KMD_FORCE_TILED", "1")
+    got = run_driver_ex(kd, Xt, w, init, 1e-4, 20, "fp32")
+    assert got[4]["path"] == "tiled-fp32"
+    same_fit(got, ref)
+
+
+def test_tiled_fp32_wide_shape_vs_oracle(kd, torch_dev):
+    """d = 100, k = 300 (beyond the LDS-resident kernels): separated blobs
+    with init near the centers, so no sample is within rounding of a tie;
+    labels and iterations equal the FP64 oracle's, centroids within 1e-5."""
+    Xt, init = separated_blobs(40_000, 100, 300, seed=31)
+    w = np.ones(40_000, np.float32)
+    got = run_driver_ex(kd, Xt, w, init, 0.0, 6, "fp32")
+    assert got[4]["path"] == "tiled-fp32"
+    l64, C64, it64, in64 = lo.kmeans_lloyd(Xt, w, init, 0.0, 6, dtype=np.float64)
+    assert got[2] == it64
+    assert_array_equal(got[0], l64)
+    assert np.abs(got[1] - C64).max() <= 1e-5 * np.abs(C64).max()
+    assert_allclose(got[3], in64, rtol=1e-5)
+
+
+@pytest.mark.parametrize("shape,seed,std", [((60_000, 100, 300), 41, 2.0),
+                                            ((50_001, 128, 1024), 43, 1.0),
+                                            ((30_000, 48, 2000), 45, 1.5)])
+def test_tiled_mixed_matches_tiled_fp32_bit_exactly(kd, torch_dev, shape, seed, std):
+    """BF16 and BF16x3 on the tiled path, with the certified margin and the
+    FP32 gather re-score, give the tiled FP32 fit bit for bit."""
+    n, d, k = shape
+    Xt, init = lo.gaussian_blobs(n, d, k, seed=seed, std=std)
+    w = np.ones(n, np.float32)
+    ref = run_driver_ex(kd, Xt, w, init, 1e-4, 8, "fp32")
+    assert ref[4]["path"] == "tiled-fp32"
+    for prec in ("bf16", "bf16x3"):
+        got = run_driver_ex(kd, Xt, w, init, 1e-4, 8, prec)
+        assert got[4]["path"] == f"tiled-{prec}+rescore"
+        same_fit(got, ref)
+
+
+def test_assignment_wide_shape_matches_c_restatement(kd, torch_dev):
+    """kmd_assignment at d = 128, k = 1024 (tiled FP32 kernel) against the C
+    restatement: labels equal except where the oracle's margin is below the
+    FP32 rounding bound."""
+    rng = np.random.default_rng(8)
+    Xt = rng.standard_normal((128, 5000)).astype(np.float32)
+    C = rng.standard_normal((128, 1024)).astype(np.float32)
+    hn = lo.half_l2_norm_squared(C)
+    lab_d = torch.empty(5000, dtype=torch.int32, device="cuda")
+    _, ev = kd.assignment(to_dev(Xt), to_dev(C), to_dev(hn), lab_d, 8, 128, q())
+    ev.wait()
+    got = to_np(lab_d)
+    ref = co.assign(Xt, C, hn)
+    margin = lo.pseudo_margin(Xt.astype(np.float64), C.astype(np.float64))
+    bad = got != ref
+    assert (margin[bad] < 1e-3).all(), margin[bad]
+    assert bad.mean() < 1e-3
