@@ -253,6 +253,36 @@ int kmd_kmeans_lloyd_driver_ex(
     double *fused_kernel_ms, int64_t *fused_launches, int *fused_path,
     kmd_print_fn print_fn, void *print_user_data, kmd_stream_t stream);
 
+/* One fit of a batch (float32 data, int32 labels; all pointers device
+ * memory on the current device, as for kmd_kmeans_lloyd_driver). */
+typedef struct kmd_fit {
+  int64_t n_samples, n_features, n_clusters;
+  const void *X_t;            /* (n_features, n_samples) */
+  const void *sample_weight;  /* (n_samples) */
+  void *init_centroids_t;     /* (n_features, n_clusters), overwritten */
+  void *assignments_id;       /* (n_samples) int32 */
+  void *res_centroids_t;      /* (n_features, n_clusters) */
+  double tol;
+  int64_t max_iter;
+  int64_t n_iters;            /* out */
+  double inertia;             /* out */
+} kmd_fit;
+
+/* Many independent Lloyd fits at once (no counterpart in the reference,
+ * which fits one problem per kmeans_lloyd_driver call).  Each fit gets
+ * exactly the results kmd_kmeans_lloyd_driver(KMD_F32, KMD_I32, ...) gives
+ * it alone: labels, centroids, iteration count and inertia.  Fits with
+ * d <= 64, k <= 256, k <= n and finite inputs run together: one persistent
+ * launch per step of an iteration over a work list of (fit, tile) items,
+ * per-fit convergence masks, the iteration captured once in a hipGraph and
+ * replayed, `graph_replays` iterations between host checks of the masks
+ * (<= 0: 4).  Other fits run through the single-fit driver.  precision:
+ * KMD_PREC_FP32 only.  *graph_launches (may be NULL) = replays issued.
+ * Blocking. */
+int kmd_kmeans_lloyd_batched(int64_t n_fits, kmd_fit *fits, int precision,
+                             int graph_replays, int64_t *graph_launches,
+                             kmd_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,7 @@ from ._kmeans_dpcpp_amd import (  # noqa: E402
     fused_lloyd_single_step,
     half_l2_norm_squared,
     iterative_merge_sort,
+    kmeans_lloyd_batched,
     kmeans_lloyd_driver,
     kmeans_lloyd_driver_ex,
     record_event,

@@ -1,0 +1,877 @@
+// Many independent Lloyd fits, one launch per step of an iteration for all
+// of them, the iteration captured once in a hipGraph and replayed.
+//
+// Work lists.  The host flattens the fits into (fit, 512-sample tile) items
+// for the assignment (one list per feature bucket d <= 16 / 32 / 64, one
+// persistent launch each) and (fit, 4096-sample chunk) items for the update.
+// A block takes items grid-stride, loads the item's fit's centroid tiles and
+// half norms into LDS and labels the tile's samples with the FP32 MFMA
+// arithmetic of the single-fit kernels (v_mfma_f32_32x32x2f32 over ascending
+// k-steps, argmin of hn - dot, strict <, lowest index on ties).  Items of a
+// fit whose mask is off are skipped, so converged fits cost nothing.
+//
+// Update.  Per fit, exactly as the tiled single-fit path: labels counted,
+// cluster totals scanned, samples scattered into cluster order and summed
+// as int64 fixed point (the fit's own scales) from a sample-major copy of
+// its X.  The sums therefore equal the single-fit driver's bit for bit.
+//
+// Finalize (one block per fit): sums -> centroids and sizes, ascending
+// empty-cluster list, relocation of empty clusters to the farthest samples
+// (sq-dist desc, index asc), division, centroid shifts, the shift sum in the
+// same FP64 order as the single-fit reduction, then it++ and the fit's mask
+// (it < max_iter && shift > tol).  With the same labels, sums and reduction
+// orders, every fit ends with the single-fit driver's labels, centroids,
+// iteration count and inertia.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "lloyd_fast.hpp"
+
+namespace kmd {
+
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned long long u64;
+
+constexpr int kNT = 512;                 // assignment block (8 waves)
+constexpr int kW = kNT / kWave;
+constexpr int kTileSamples = kW * 64;     // samples per assignment item
+constexpr int kChunk = 4096;              // samples per update item
+constexpr int kMembers = 256;             // members per segment-sum item
+constexpr int kMaxK = 256;
+constexpr int kMaxD = 64;
+constexpr int kFB = 256;                  // finalize / per-fit block
+
+struct BFit {
+  int64_t n;
+  int d, k;
+  int64_t xr_off;   // X_r pool offset (floats)
+  int64_t s_off;    // per-sample pools (perm, dist)
+  int64_t sum_off;  // int64 sums pool ((d + 1) * k)
+  int kb;           // per-centroid pools of k + 1 entries
+  int sc_off;       // scale pool (d + 1)
+  const float *X, *w;
+  float *C[2];      // C[0] = init (overwritten), C[1] = result
+  int *labels;
+  int64_t max_iter;
+  float tol;
+};
+
+struct BState {
+  int it, active, cur, n_empty;
+  int seg_items, seg_base;
+  double shift;
+  double inertia;
+};
+
+struct AItem {  // assignment tile
+  int fit;
+  int first;
+};
+struct CItem {  // update chunk
+  int fit;
+  int first, count;
+};
+
+__device__ inline double block_sum256(double v, double *lds) {
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  const int w = threadIdx.x / kWave;
+  __syncthreads();
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < (int)(blockDim.x / kWave); ++i) t += lds[i];
+  return t;
+}
+
+// ---- half norms -------------------------------------------------------------
+__global__ void __launch_bounds__(kFB)
+    bt_hn_krn(const BFit *__restrict__ fits, const BState *__restrict__ st,
+              float *__restrict__ hn, int all) {
+  const int f = blockIdx.x;
+  if (!all && !st[f].active) return;
+  const BFit F = fits[f];
+  const float *C = F.C[st[f].cur];
+  for (int c = threadIdx.x; c < F.k; c += kFB) {
+    float acc = 0.f;
+    for (int j = 0; j < F.d; ++j) {
+      const float v = C[(int64_t)j * F.k + c];
+      acc = fmaf(v, v, acc);
+    }
+    hn[F.kb + c] = acc / 2.f;
+  }
+}
+
+// ---- assignment -----------------------------------------------------------------
+template <int KS>
+constexpr int tile_floats() {
+  return 2 * KS * 32 + 32;
+}
+
+template <int KS>
+__global__ void __launch_bounds__(kNT)
+    bt_assign_krn(const BFit *__restrict__ fits, const BState *__restrict__ st,
+                  const AItem *__restrict__ items, int n_items,
+                  const float *__restrict__ hn_pool, int all) {
+  constexpr int TF = tile_floats<KS>();
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  float *Cl = reinterpret_cast<float *>(lds_raw);  // [kMaxK / 32][TF]
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1), wave = tid / kWave;
+  const int col = lane & 31, half = lane >> 5;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const AItem item = items[it];
+    if (!all && !st[item.fit].active) continue;  // block-uniform
+    const BFit F = fits[item.fit];
+    const float *C = F.C[st[item.fit].cur];
+    const float *hn = hn_pool + F.kb;
+    const int d = F.d, k = F.k, n_ct = (k + 31) / 32;
+    __syncthreads();
+    for (int i = tid; i < n_ct * TF; i += kNT) {
+      const int t = i / TF, e = i - t * TF;
+      if (e < 2 * KS * 32) {
+        const int f = e >> 5, cc = e & 31;
+        const int c = t * 32 + cc;
+        const int ks = f >> 1, hf = f & 1;
+        Cl[t * TF + ((ks >> 2) * 2 + hf) * 128 + cc * 4 + (ks & 3)] =
+            (f < d && c < k) ? C[(int64_t)f * k + c] : 0.f;
+      } else {
+        const int c = t * 32 + (e - 2 * KS * 32);
+        Cl[t * TF + e] = (c < k) ? hn[c] : __builtin_huge_valf();
+      }
+    }
+    __syncthreads();
+    const int64_t n = F.n;
+    const int64_t p0 = (int64_t)item.first + wave * 64 + 2 * col;
+    float xr[KS][2];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int f = 2 * ks + half;
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+        xr[ks][q] = (f < d && p0 + q < n) ? F.X[(int64_t)f * n + p0 + q] : 0.f;
+    }
+    float bestv[2] = {__builtin_huge_valf(), __builtin_huge_valf()};
+    int besti[2] = {0, 0};
+    for (int t = 0; t < n_ct; ++t) {
+      const float *At = Cl + t * TF;
+      f32x16 acc[2];
+#pragma unroll
+      for (int k4 = 0; k4 < KS / 4; ++k4) {
+        const f32x4 av =
+            *reinterpret_cast<const f32x4 *>(At + (k4 * 2 + half) * 128 + col * 4);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            if (k4 == 0 && j == 0) {
+              const f32x16 z = {};
+              acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[0][q], z, 0,
+                                                            0, 0);
+            } else {
+              acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                  av[j], xr[4 * k4 + j][q], acc[q], 0, 0, 0);
+            }
+          }
+      }
+      const float *Ht = At + 2 * KS * 32;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int m0 = 8 * g + 4 * half;
+        const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + m0);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const float v = h4[e] - acc[q][4 * g + e];
+            const bool upd = v < bestv[q];
+            bestv[q] = upd ? v : bestv[q];
+            besti[q] = upd ? (t * 32 + m0 + e) : besti[q];
+          }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const float ov = __shfl_xor(bestv[q], 32, kWave);
+      const int oi = __shfl_xor(besti[q], 32, kWave);
+      const bool take = (ov < bestv[q]) || (ov == bestv[q] && oi < besti[q]);
+      besti[q] = take ? oi : besti[q];
+      if (half == 0 && p0 + q < n) F.labels[p0 + q] = besti[q];
+    }
+  }
+}
+
+// ---- update -------------------------------------------------------------------
+__global__ void __launch_bounds__(kFB)
+    bt_zero_krn(const BFit *__restrict__ fits, const BState *__restrict__ st,
+                u64 *__restrict__ sums, int *__restrict__ totals,
+                int *__restrict__ cursor) {
+  const int f = blockIdx.x;
+  if (!st[f].active) return;
+  const BFit F = fits[f];
+  const int64_t m = (int64_t)(F.d + 1) * F.k;
+  for (int64_t i = threadIdx.x; i < m; i += kFB) sums[F.sum_off + i] = 0ull;
+  for (int c = threadIdx.x; c <= F.k; c += kFB) {
+    totals[F.kb + c] = 0;
+    cursor[F.kb + c] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    bt_hist_krn(const BFit *__restrict__ fits, const BState *__restrict__ st,
+                const CItem *__restrict__ items, int *__restrict__ totals) {
+  __shared__ int h[kMaxK];
+  const CItem item = items[blockIdx.x];
+  if (!st[item.fit].active) return;
+  const BFit F = fits[item.fit];
+  for (int c = threadIdx.x; c < F.k; c += 256) h[c] = 0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < item.count; j += 256)
+    atomicAdd(&h[F.labels[item.first + j]], 1);
+  __syncthreads();
+  for (int c = threadIdx.x; c < F.k; c += 256)
+    if (h[c] != 0) atomicAdd(&totals[F.kb + c], h[c]);
+}
+
+// per fit: start[c] (exclusive scan of totals, start[k] = n) and the first
+// segment-sum item of each cluster (items[k] = the fit's item count)
+__global__ void __launch_bounds__(kFB)
+    bt_cluster_scan_krn(const BFit *__restrict__ fits, BState *__restrict__ st,
+                        const int *__restrict__ totals, int *__restrict__ start,
+                        int *__restrict__ citems) {
+  __shared__ int wa[kFB / kWave], wb[kFB / kWave];
+  const int f = blockIdx.x;
+  if (!st[f].active) {
+    if (threadIdx.x == 0) st[f].seg_items = 0;
+    return;
+  }
+  const BFit F = fits[f];
+  const int c = threadIdx.x;  // k <= 256
+  const int lane = c & (kWave - 1), wave = c / kWave;
+  const int va = (c < F.k) ? totals[F.kb + c] : 0;
+  const int vb = (va + kMembers - 1) / kMembers;
+  int ia = va, ib = vb;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int ta = __shfl_up(ia, o, kWave), tb = __shfl_up(ib, o, kWave);
+    if (lane >= o) {
+      ia += ta;
+      ib += tb;
+    }
+  }
+  if (lane == kWave - 1) {
+    wa[wave] = ia;
+    wb[wave] = ib;
+  }
+  __syncthreads();
+  int pa = 0, pb = 0, ta = 0, tb = 0;
+  for (int w2 = 0; w2 < kFB / kWave; ++w2) {
+    if (w2 < wave) {
+      pa += wa[w2];
+      pb += wb[w2];
+    }
+    ta += wa[w2];
+    tb += wb[w2];
+  }
+  if (c < F.k) {
+    start[F.kb + c] = pa + ia - va;
+    citems[F.kb + c] = pb + ib - vb;
+  }
+  if (c == 0) {
+    start[F.kb + F.k] = ta;
+    citems[F.kb + F.k] = tb;
+    st[f].seg_items = tb;
+  }
+}
+
+// exclusive scan of the fits' segment-item counts (one block)
+__global__ void __launch_bounds__(1024)
+    bt_fit_scan_krn(int n_fits, BState *__restrict__ st, int *__restrict__ total) {
+  __shared__ int ws[1024 / kWave];
+  __shared__ int carry;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (int f0 = 0; f0 < n_fits; f0 += 1024) {
+    const int f = f0 + tid;
+    const int v = (f < n_fits) ? st[f].seg_items : 0;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int t = __shfl_up(incl, o, kWave);
+      if (lane >= o) incl += t;
+    }
+    if (lane == kWave - 1) ws[wave] = incl;
+    __syncthreads();
+    int p = 0, tot = 0;
+    for (int w2 = 0; w2 < 1024 / kWave; ++w2) {
+      if (w2 < wave) p += ws[w2];
+      tot += ws[w2];
+    }
+    if (f < n_fits) st[f].seg_base = carry + p + incl - v;
+    __syncthreads();
+    if (tid == 0) carry += tot;
+    __syncthreads();
+  }
+  if (tid == 0) total[0] = carry;
+}
+
+__global__ void __launch_bounds__(256)
+    bt_scatter_krn(const BFit *__restrict__ fits, const BState *__restrict__ st,
+                   const CItem *__restrict__ items, const int *__restrict__ start,
+                   int *__restrict__ cursor, int *__restrict__ perm) {
+  __shared__ int h[kMaxK];
+  const CItem item = items[blockIdx.x];
+  if (!st[item.fit].active) return;
+  const BFit F = fits[item.fit];
+  for (int c = threadIdx.x; c < F.k; c += 256) h[c] = 0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < item.count; j += 256)
+    atomicAdd(&h[F.labels[item.first + j]], 1);
+  __syncthreads();
+  for (int c = threadIdx.x; c < F.k; c += 256)
+    h[c] = (h[c] != 0) ? start[F.kb + c] + atomicAdd(&cursor[F.kb + c], h[c]) : 0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < item.count; j += 256) {
+    const int s = item.first + j;
+    const int pos = atomicAdd(&h[F.labels[s]], 1);
+    perm[F.s_off + pos] = s;
+  }
+}
+
+// one block = up to kMembers members of one cluster of one fit
+__global__ void __launch_bounds__(128)
+    bt_segsum_krn(int n_fits, const BFit *__restrict__ fits,
+                  const BState *__restrict__ st, const int *__restrict__ total,
+                  const float *__restrict__ xr_pool, const float *__restrict__ scale_pool,
+                  const int *__restrict__ perm, const int *__restrict__ start,
+                  const int *__restrict__ citems, u64 *__restrict__ sums) {
+  __shared__ int sidx[kMembers];
+  __shared__ float sw[kMembers];
+  const int g = blockIdx.x;
+  if (g >= total[0]) return;
+  int lo = 0, hi = n_fits - 1;  // the last fit whose first item is <= g
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (st[mid].seg_base <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const BFit F = fits[lo];
+  const int local = g - st[lo].seg_base;
+  const int *ci = citems + F.kb;
+  int a = 0, b = F.k - 1;  // the last cluster whose first item is <= local
+  while (a < b) {
+    const int mid = (a + b + 1) >> 1;
+    if (ci[mid] <= local)
+      a = mid;
+    else
+      b = mid - 1;
+  }
+  const int c = a;
+  const int m0 = start[F.kb + c] + (local - ci[c]) * kMembers;
+  const int cnt = min(kMembers, start[F.kb + c + 1] - m0);
+  for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+    const int s = perm[F.s_off + m0 + j];
+    sidx[j] = s;
+    sw[j] = F.w[s];
+  }
+  __syncthreads();
+  const float *Xr = xr_pool + F.xr_off;
+  const int d = F.d;
+  for (int f = threadIdx.x; f <= d; f += blockDim.x) {
+    const float sc = scale_pool[F.sc_off + f];
+    long long acc = 0;
+    if (f < d) {
+      for (int j = 0; j < cnt; ++j)
+        acc += __float2int_rn((Xr[(int64_t)sidx[j] * d + f] * sw[j]) * sc);
+    } else {
+      for (int j = 0; j < cnt; ++j) acc += __float2int_rn(sw[j] * sc);
+    }
+    if (acc != 0) atomicAdd(&sums[F.sum_off + (int64_t)f * F.k + c], (u64)acc);
+  }
+}
+
+// ---- finalize (one block per fit) ----------------------------------------------
+__global__ void __launch_bounds__(kFB)
+    bt_finalize_krn(const BFit *__restrict__ fits, BState *__restrict__ st,
+                    const u64 *__restrict__ sums, const float *__restrict__ scale_pool,
+                    float *__restrict__ sizes_pool, int *__restrict__ list_pool,
+                    float *__restrict__ dist_pool) {
+  __shared__ double red[kFB / kWave];
+  __shared__ int wcount[kFB / kWave];
+  __shared__ int n_empty_s;
+  __shared__ float bv[kFB / kWave];
+  __shared__ int bi[kFB / kWave];
+  __shared__ int order[kMaxK];
+  const int f = blockIdx.x;
+  if (!st[f].active) return;
+  const BFit F = fits[f];
+  const int d = F.d, k = F.k, tid = threadIdx.x;
+  const int64_t n = F.n;
+  const int cur = st[f].cur;
+  const float *Cc = F.C[cur];
+  float *Cn = F.C[cur ^ 1];
+  float *sizes = sizes_pool + F.kb;
+  int *list = list_pool + F.kb;
+  const float *sc = scale_pool + F.sc_off;
+  const int64_t dk = (int64_t)d * k;
+  // sums -> centroid sums and sizes (the single-slot partial reduction)
+  for (int64_t i = tid; i < dk + k; i += kFB) {
+    const int64_t fe = i / k;
+    const float v = (float)((double)(long long)sums[F.sum_off + i] / (double)sc[fe]);
+    if (i < dk)
+      Cn[i] = v;
+    else
+      sizes[i - dk] = v;
+  }
+  __syncthreads();
+  // ascending empty-cluster list (k <= 256: one pass)
+  {
+    const bool empty = (tid < k) && (sizes[tid] == 0.f);
+    const uint64_t mask = __ballot(empty);
+    const int lane = tid & (kWave - 1), w = tid / kWave;
+    const int before_in_wave = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wcount[w] = __popcll(mask);
+    __syncthreads();
+    int before = 0, tot = 0;
+    for (int i = 0; i < kFB / kWave; ++i) {
+      before += (i < w) ? wcount[i] : 0;
+      tot += wcount[i];
+    }
+    if (empty) list[before + before_in_wave] = tid;
+    if (tid == 0) n_empty_s = tot;
+    __syncthreads();
+  }
+  const int n_empty = n_empty_s;
+  if (n_empty > 0) {
+    // squared distance to the current centroid (unit weights)
+    float *dist = dist_pool + F.s_off;
+    for (int64_t s = tid; s < n; s += kFB) {
+      const int c = F.labels[s];
+      float acc = 0.f;
+      for (int j = 0; j < d; ++j) {
+        const float diff = F.X[(int64_t)j * n + s] - Cc[(int64_t)j * k + c];
+        acc = fmaf(diff, diff, acc);
+      }
+      dist[s] = acc;
+    }
+    __syncthreads();
+    const int m = (int)min((int64_t)n_empty, n);
+    for (int j = 0; j < m; ++j) {
+      // farthest remaining sample: distance desc, index asc
+      float best = -1.f;
+      int bidx = 0x7fffffff;
+      for (int64_t s = tid; s < n; s += kFB) {
+        const float v = dist[s];
+        if (v > best || (v == best && (int)s < bidx)) {
+          best = v;
+          bidx = (int)s;
+        }
+      }
+      for (int o = kWave / 2; o > 0; o >>= 1) {
+        const float ov = __shfl_xor(best, o, kWave);
+        const int oi = __shfl_xor(bidx, o, kWave);
+        if (ov > best || (ov == best && oi < bidx)) {
+          best = ov;
+          bidx = oi;
+        }
+      }
+      if ((tid & (kWave - 1)) == 0) {
+        bv[tid / kWave] = best;
+        bi[tid / kWave] = bidx;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        float b = bv[0];
+        int x = bi[0];
+        for (int i = 1; i < kFB / kWave; ++i)
+          if (bv[i] > b || (bv[i] == b && bi[i] < x)) {
+            b = bv[i];
+            x = bi[i];
+          }
+        order[j] = x;
+        dist[x] = -2.f;  // taken
+      }
+      __syncthreads();
+    }
+    // move w x of the chosen samples (in order) into the empty clusters
+    for (int fe = tid; fe < d; fe += kFB) {
+      for (int j = 0; j < m; ++j) {
+        const int s = order[j];
+        const int p = F.labels[s];
+        const int e = list[j];
+        const float ws = F.w[s];
+        const float add = F.X[(int64_t)fe * n + s] * ws;
+        Cn[(int64_t)fe * k + p] -= add;
+        Cn[(int64_t)fe * k + e] = add;
+        if (fe == 0) {
+          sizes[p] -= ws;
+          sizes[e] = ws;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // divide, then shifts and their sum in the single-fit order
+  for (int64_t i = tid; i < dk; i += kFB) Cn[i] = Cn[i] / sizes[i % k];
+  __syncthreads();
+  double acc = 0.0;
+  for (int c = tid; c < k; c += kFB) {
+    float s2 = 0.f;
+    for (int j = 0; j < d; ++j) {
+      const float diff = Cc[(int64_t)j * k + c] - Cn[(int64_t)j * k + c];
+      s2 = fmaf(diff, diff, s2);
+    }
+    acc += (double)s2;
+  }
+  const double tot = block_sum256(acc, red);
+  if (tid == 0) {
+    BState S = st[f];
+    S.n_empty = n_empty;
+    S.shift = tot;
+    S.cur = cur ^ 1;
+    S.it += 1;
+    const float shift_t = (float)tot;
+    S.active = (S.it < F.max_iter) && (shift_t > F.tol);
+    st[f] = S;
+  }
+}
+
+// ---- epilogue --------------------------------------------------------------------
+// per-sample weighted inertia of the final labels
+__global__ void __launch_bounds__(256)
+    bt_inertia_krn(const BFit *__restrict__ fits, const BState *__restrict__ st,
+                   const CItem *__restrict__ items, float *__restrict__ out_pool) {
+  const CItem item = items[blockIdx.x];
+  const BFit F = fits[item.fit];
+  const float *C = F.C[st[item.fit].cur];
+  for (int j = threadIdx.x; j < item.count; j += 256) {
+    const int64_t s = item.first + j;
+    const int c = F.labels[s];
+    float acc = 0.f;
+    for (int fe = 0; fe < F.d; ++fe) {
+      const float diff = F.X[(int64_t)fe * F.n + s] - C[(int64_t)fe * F.k + c];
+      acc = fmaf(diff, diff, acc);
+    }
+    out_pool[F.s_off + s] = acc * F.w[s];
+  }
+}
+
+// stage 1 of the single-fit two-stage sum: block b of fit f sums the
+// grid-stride slice i = b*256 + t, step nb_f * 256
+__global__ void __launch_bounds__(256)
+    bt_sum1_krn(const BFit *__restrict__ fits, const int2 *__restrict__ map,
+                const int *__restrict__ nb, const float *__restrict__ pool,
+                double *__restrict__ partials) {
+  __shared__ double lds[256 / kWave];
+  const int2 fb = map[blockIdx.x];  // (fit, block within the fit)
+  const BFit F = fits[fb.x];
+  const int nbf = nb[fb.x];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)fb.y * 256 + threadIdx.x; i < F.n;
+       i += (int64_t)nbf * 256)
+    acc += (double)pool[F.s_off + i];
+  const double t = block_sum256(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(256)
+    bt_sum2_krn(const int *__restrict__ nb, const int *__restrict__ first,
+                const double *__restrict__ partials, BState *__restrict__ st) {
+  __shared__ double lds[256 / kWave];
+  const int f = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nb[f]; i += 256) acc += partials[first[f] + i];
+  const double t = block_sum256(acc, lds);
+  if (threadIdx.x == 0) st[f].inertia = (double)(float)t;
+}
+
+__global__ void __launch_bounds__(256)
+    bt_copy_result_krn(const BFit *__restrict__ fits, const BState *__restrict__ st) {
+  const int f = blockIdx.x;
+  const BFit F = fits[f];
+  if (st[f].cur == 1) return;  // the result buffer already holds it
+  const int64_t m = (int64_t)F.d * F.k;
+  for (int64_t i = threadIdx.x; i < m; i += 256) F.C[1][i] = F.C[0][i];
+}
+
+__global__ void bt_init_state_krn(int n_fits, const BFit *__restrict__ fits,
+                                  BState *__restrict__ st) {
+  const int f = blockIdx.x * 256 + threadIdx.x;
+  if (f >= n_fits) return;
+  BState S{};
+  S.active = fits[f].max_iter > 0 ? 1 : 0;
+  st[f] = S;
+}
+
+// ---- host --------------------------------------------------------------------
+inline int ks_bucket(int d) { return d <= 16 ? 8 : (d <= 32 ? 16 : 32); }
+
+size_t assign_lds(int KS) { return (size_t)(kMaxK / 32) * (2 * KS * 32 + 32) * 4; }
+
+template <int KS>
+int assign_grid() {
+  static int grid = 0;
+  if (grid == 0) {
+    const void *fn = reinterpret_cast<const void *>(&bt_assign_krn<KS>);
+    KMD_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)assign_lds(KS)));
+    int per_cu = 0;
+    KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNT,
+                                                                assign_lds(KS)));
+    grid = device_cu_count() * std::max(1, per_cu);
+  }
+  return grid;
+}
+
+template <int KS>
+void launch_assign_bucket(const BFit *fits, const BState *st, const AItem *items,
+                          int n_items, const float *hn, int all, hipStream_t s) {
+  if (n_items <= 0) return;
+  const int grid = std::min(assign_grid<KS>(), n_items);
+  hipLaunchKernelGGL(bt_assign_krn<KS>, dim3(grid), dim3(kNT), assign_lds(KS), s,
+                     fits, st, items, n_items, hn, all);
+  KMD_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+bool batched_fit_supported(int64_t n, int64_t d, int64_t k) {
+  return n >= 1 && n < (int64_t(1) << 30) && d >= 1 && d <= kMaxD && k >= 1 &&
+         k <= kMaxK && k <= n;
+}
+
+// Runs every fit of `fits` (all served: batched_fit_supported) to completion.
+void batched_lloyd(const std::vector<BatchedFit> &in, int graph_replays,
+                   hipStream_t user_stream, BatchedStats *stats) {
+  const int nf = (int)in.size();
+  if (nf == 0) return;
+  hipStream_t s = nullptr;
+  KMD_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } guard{s};
+  hipEvent_t ev = nullptr;
+  KMD_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  KMD_HIP_CHECK(hipEventRecord(ev, user_stream));
+  KMD_HIP_CHECK(hipStreamWaitEvent(s, ev, 0));
+
+  // flatten
+  std::vector<BFit> fits(nf);
+  int64_t xr_tot = 0, s_tot = 0, sum_tot = 0;
+  int kb_tot = 0, sc_tot = 0;
+  int64_t max_iter = 0;
+  std::vector<AItem> aitems[3];
+  std::vector<CItem> citems;
+  int64_t seg_bound = 0;
+  for (int f = 0; f < nf; ++f) {
+    const BatchedFit &b = in[f];
+    require(batched_fit_supported(b.n, b.d, b.k), "batched fit: unsupported shape");
+    BFit &F = fits[f];
+    F.n = b.n;
+    F.d = (int)b.d;
+    F.k = (int)b.k;
+    F.xr_off = xr_tot;
+    xr_tot += b.n * b.d;
+    F.s_off = s_tot;
+    s_tot += b.n;
+    F.sum_off = sum_tot;
+    sum_tot += (b.d + 1) * b.k;
+    F.kb = kb_tot;
+    kb_tot += (int)b.k + 1;
+    F.sc_off = sc_tot;
+    sc_tot += (int)b.d + 1;
+    F.X = b.X;
+    F.w = b.w;
+    F.C[0] = b.init;
+    F.C[1] = b.res;
+    F.labels = b.labels;
+    F.max_iter = b.max_iter;
+    F.tol = (float)b.tol;
+    max_iter = std::max(max_iter, b.max_iter);
+    const int bucket = ks_bucket(F.d) == 8 ? 0 : (ks_bucket(F.d) == 16 ? 1 : 2);
+    for (int64_t first = 0; first < b.n; first += kTileSamples)
+      aitems[bucket].push_back(AItem{f, (int)first});
+    for (int64_t first = 0; first < b.n; first += kChunk)
+      citems.push_back(CItem{f, (int)first, (int)std::min<int64_t>(kChunk, b.n - first)});
+    seg_bound += b.n / kMembers + b.k;
+  }
+  // epilogue reduction map (the single-fit two-stage sum's block count)
+  std::vector<int> nb(nf), first(nf);
+  std::vector<int2> map;
+  for (int f = 0; f < nf; ++f) {
+    int64_t g = (fits[f].n + 4095) / 4096;
+    g = std::max<int64_t>(1, std::min<int64_t>(1024, g));
+    nb[f] = (int)g;
+    first[f] = (int)map.size();
+    for (int b2 = 0; b2 < g; ++b2) map.push_back(make_int2(f, b2));
+  }
+
+  auto bytes = [](size_t n, size_t sz) { return std::max<size_t>(1, n) * sz; };
+  TempBuffer d_fits(bytes(nf, sizeof(BFit)), s), d_st(bytes(nf, sizeof(BState)), s);
+  TempBuffer d_ai0(bytes(aitems[0].size(), sizeof(AItem)), s),
+      d_ai1(bytes(aitems[1].size(), sizeof(AItem)), s),
+      d_ai2(bytes(aitems[2].size(), sizeof(AItem)), s);
+  TempBuffer d_ci(bytes(citems.size(), sizeof(CItem)), s);
+  TempBuffer d_map(bytes(map.size(), sizeof(int2)), s), d_nb(bytes(nf, sizeof(int)), s),
+      d_first(bytes(nf, sizeof(int)), s), d_part(bytes(map.size(), sizeof(double)), s);
+  TempBuffer xr(bytes(xr_tot, sizeof(float)), s), perm(bytes(s_tot, sizeof(int)), s),
+      dist(bytes(s_tot, sizeof(float)), s);
+  TempBuffer sums(bytes(sum_tot, sizeof(u64)), s), scale(bytes(sc_tot, sizeof(float)), s);
+  TempBuffer hn(bytes(kb_tot, sizeof(float)), s), sizes(bytes(kb_tot, sizeof(float)), s),
+      list(bytes(kb_tot, sizeof(int)), s), totals(bytes(kb_tot, sizeof(int)), s),
+      cursor(bytes(kb_tot, sizeof(int)), s), start(bytes(kb_tot, sizeof(int)), s),
+      cit(bytes(kb_tot, sizeof(int)), s), seg_total(sizeof(int) * 4, s),
+      flags(sizeof(int) * (size_t)nf + 64, s);
+  KMD_HIP_CHECK(hipMemcpyAsync(d_fits.as<void>(), fits.data(), sizeof(BFit) * nf,
+                               hipMemcpyHostToDevice, s));
+  const AItem *ai[3] = {d_ai0.as<AItem>(), d_ai1.as<AItem>(), d_ai2.as<AItem>()};
+  TempBuffer *aib[3] = {&d_ai0, &d_ai1, &d_ai2};
+  for (int b = 0; b < 3; ++b)
+    if (!aitems[b].empty())
+      KMD_HIP_CHECK(hipMemcpyAsync(aib[b]->as<void>(), aitems[b].data(),
+                                   sizeof(AItem) * aitems[b].size(),
+                                   hipMemcpyHostToDevice, s));
+  KMD_HIP_CHECK(hipMemcpyAsync(d_ci.as<void>(), citems.data(),
+                               sizeof(CItem) * citems.size(), hipMemcpyHostToDevice, s));
+  KMD_HIP_CHECK(hipMemcpyAsync(d_map.as<void>(), map.data(), sizeof(int2) * map.size(),
+                               hipMemcpyHostToDevice, s));
+  KMD_HIP_CHECK(hipMemcpyAsync(d_nb.as<void>(), nb.data(), sizeof(int) * nf,
+                               hipMemcpyHostToDevice, s));
+  KMD_HIP_CHECK(hipMemcpyAsync(d_first.as<void>(), first.data(), sizeof(int) * nf,
+                               hipMemcpyHostToDevice, s));
+
+  // per fit: sample-major copy and fixed-point scales (the single-fit
+  // driver's pre-pass, so the integers are the same)
+  int *nonfinite = flags.as<int>();
+  for (int f = 0; f < nf; ++f) {
+    launch_transpose(fits[f].n, fits[f].d, fits[f].X, xr.as<float>() + fits[f].xr_off, s);
+    launch_fixed_point_scales(fits[f].n, fits[f].d, fits[f].X, fits[f].w,
+                              scale.as<float>() + fits[f].sc_off, nonfinite + f, s);
+  }
+  std::vector<int> host_flags(nf);
+  KMD_HIP_CHECK(hipMemcpyAsync(host_flags.data(), nonfinite, sizeof(int) * nf,
+                               hipMemcpyDeviceToHost, s));
+  KMD_HIP_CHECK(hipStreamSynchronize(s));
+  for (int f = 0; f < nf; ++f)
+    require(host_flags[f] == 0, "batched fit: X or sample_weight holds inf/NaN");
+
+  BFit *dF = d_fits.as<BFit>();
+  BState *dS = d_st.as<BState>();
+  hipLaunchKernelGGL(bt_init_state_krn, dim3((nf + 255) / 256), dim3(256), 0, s, nf,
+                     dF, dS);
+  KMD_LAUNCH_CHECK();
+
+  const int nci = (int)citems.size();
+  auto iteration = [&](hipStream_t q) {
+    hipLaunchKernelGGL(bt_hn_krn, dim3(nf), dim3(kFB), 0, q, dF, dS, hn.as<float>(), 0);
+    KMD_LAUNCH_CHECK();
+    launch_assign_bucket<8>(dF, dS, ai[0], (int)aitems[0].size(), hn.as<float>(), 0, q);
+    launch_assign_bucket<16>(dF, dS, ai[1], (int)aitems[1].size(), hn.as<float>(), 0, q);
+    launch_assign_bucket<32>(dF, dS, ai[2], (int)aitems[2].size(), hn.as<float>(), 0, q);
+    hipLaunchKernelGGL(bt_zero_krn, dim3(nf), dim3(kFB), 0, q, dF, dS, sums.as<u64>(),
+                       totals.as<int>(), cursor.as<int>());
+    KMD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bt_hist_krn, dim3(nci), dim3(256), 0, q, dF, dS,
+                       d_ci.as<CItem>(), totals.as<int>());
+    KMD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bt_cluster_scan_krn, dim3(nf), dim3(kFB), 0, q, dF, dS,
+                       totals.as<int>(), start.as<int>(), cit.as<int>());
+    KMD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bt_fit_scan_krn, dim3(1), dim3(1024), 0, q, nf, dS,
+                       seg_total.as<int>());
+    KMD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bt_scatter_krn, dim3(nci), dim3(256), 0, q, dF, dS,
+                       d_ci.as<CItem>(), start.as<int>(), cursor.as<int>(),
+                       perm.as<int>());
+    KMD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bt_segsum_krn, dim3((unsigned)std::max<int64_t>(1, seg_bound)),
+                       dim3(128), 0, q, nf, dF, dS, seg_total.as<int>(),
+                       xr.as<float>(), scale.as<float>(), perm.as<int>(),
+                       start.as<int>(), cit.as<int>(), sums.as<u64>());
+    KMD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bt_finalize_krn, dim3(nf), dim3(kFB), 0, q, dF, dS,
+                       sums.as<u64>(), scale.as<float>(), sizes.as<float>(),
+                       list.as<int>(), dist.as<float>());
+    KMD_LAUNCH_CHECK();
+  };
+
+  // launch plans (function attributes, occupancy) before the capture
+  (void)assign_grid<8>();
+  (void)assign_grid<16>();
+  (void)assign_grid<32>();
+  // capture one iteration, replay it until every fit's mask is off
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  KMD_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  iteration(s);
+  KMD_HIP_CHECK(hipStreamEndCapture(s, &graph));
+  struct GraphGuard {
+    hipGraph_t g;
+    hipGraphExec_t e;
+    ~GraphGuard() {
+      if (e) (void)hipGraphExecDestroy(e);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } gguard{graph, nullptr};
+  KMD_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  gguard.e = exec;
+
+  std::vector<BState> host_st(nf);
+  const int R = std::max(1, graph_replays);
+  int64_t replays = 0;
+  while (replays < max_iter) {
+    const int64_t r = std::min<int64_t>(R, max_iter - replays);
+    for (int64_t i = 0; i < r; ++i) KMD_HIP_CHECK(hipGraphLaunch(exec, s));
+    replays += r;
+    KMD_HIP_CHECK(hipMemcpyAsync(host_st.data(), dS, sizeof(BState) * nf,
+                                 hipMemcpyDeviceToHost, s));
+    KMD_HIP_CHECK(hipStreamSynchronize(s));
+    bool any = false;
+    for (const BState &S : host_st) any |= S.active != 0;
+    if (!any) break;
+  }
+
+  // epilogue: labels and weighted inertia of the final centroids
+  hipLaunchKernelGGL(bt_hn_krn, dim3(nf), dim3(kFB), 0, s, dF, dS, hn.as<float>(), 1);
+  KMD_LAUNCH_CHECK();
+  launch_assign_bucket<8>(dF, dS, ai[0], (int)aitems[0].size(), hn.as<float>(), 1, s);
+  launch_assign_bucket<16>(dF, dS, ai[1], (int)aitems[1].size(), hn.as<float>(), 1, s);
+  launch_assign_bucket<32>(dF, dS, ai[2], (int)aitems[2].size(), hn.as<float>(), 1, s);
+  hipLaunchKernelGGL(bt_inertia_krn, dim3(nci), dim3(256), 0, s, dF, dS,
+                     d_ci.as<CItem>(), dist.as<float>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_sum1_krn, dim3((unsigned)map.size()), dim3(256), 0, s, dF,
+                     d_map.as<int2>(), d_nb.as<int>(), dist.as<float>(),
+                     d_part.as<double>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_sum2_krn, dim3(nf), dim3(256), 0, s, d_nb.as<int>(),
+                     d_first.as<int>(), d_part.as<double>(), dS);
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_copy_result_krn, dim3(nf), dim3(256), 0, s, dF, dS);
+  KMD_LAUNCH_CHECK();
+  KMD_HIP_CHECK(hipMemcpyAsync(host_st.data(), dS, sizeof(BState) * nf,
+                               hipMemcpyDeviceToHost, s));
+  KMD_HIP_CHECK(hipStreamSynchronize(s));
+  for (int f = 0; f < nf; ++f) {
+    in[f].out_iters[0] = host_st[f].it;
+    in[f].out_inertia[0] = host_st[f].inertia;
+  }
+  if (stats) {
+    stats->graph_replays = replays;
+  }
+  // the caller's stream sees the results after this call returns (the
+  // batch stream was synchronised); release the event
+  (void)hipEventDestroy(ev);
+}
+
+}  // namespace kmd

@@ -595,4 +595,56 @@ int kmd_kmeans_lloyd_driver_ex(
   });
 }
 
+int kmd_kmeans_lloyd_batched(int64_t n_fits, kmd_fit *fits, int precision,
+                             int graph_replays, int64_t *graph_launches,
+                             kmd_stream_t stream) {
+  return guarded([&] {
+    require(n_fits >= 0, "n_fits must be non-negative");
+    require(n_fits == 0 || fits != nullptr, "fits must not be null");
+    require(precision == KMD_PREC_FP32,
+            "batched fits support precision KMD_PREC_FP32 only");
+    std::vector<BatchedFit> batch;
+    std::vector<int64_t> idx;
+    for (int64_t i = 0; i < n_fits; ++i) {
+      kmd_fit &f = fits[i];
+      require(f.n_samples >= 0 && f.n_features >= 1 && f.n_clusters >= 1,
+              "Array dimensions are not consistent");
+      require(f.tol >= 0.0, "Tolerance must be non-negative");
+      require(f.max_iter >= 0, "max_iter must be non-negative");
+      if (batched_fit_supported(f.n_samples, f.n_features, f.n_clusters)) {
+        BatchedFit b;
+        b.n = f.n_samples;
+        b.d = f.n_features;
+        b.k = f.n_clusters;
+        b.X = (const float *)f.X_t;
+        b.w = (const float *)f.sample_weight;
+        b.init = (float *)f.init_centroids_t;
+        b.res = (float *)f.res_centroids_t;
+        b.labels = (int *)f.assignments_id;
+        b.tol = f.tol;
+        b.max_iter = f.max_iter;
+        b.out_iters = &f.n_iters;
+        b.out_inertia = &f.inertia;
+        batch.push_back(b);
+      } else {
+        idx.push_back(i);
+      }
+    }
+    BatchedStats st;
+    const hipStream_t s = as_stream(stream);
+    if (!batch.empty())
+      batched_lloyd(batch, graph_replays > 0 ? graph_replays : 4, s, &st);
+    for (int64_t i : idx) {
+      kmd_fit &f = fits[i];
+      driver_lloyd<float, int32_t>(
+          f.n_samples, f.n_features, f.n_clusters, (const float *)f.X_t,
+          (const float *)f.sample_weight, (float *)f.init_centroids_t,
+          (int32_t *)f.assignments_id, (float *)f.res_centroids_t, f.tol, false,
+          f.max_iter, &f.n_iters, &f.inertia, nullptr, nullptr, s, KMD_PREC_FP32,
+          nullptr);
+    }
+    if (graph_launches) *graph_launches = st.graph_replays;
+  });
+}
+
 }  // extern "C"

@@ -1,5 +1,7 @@
 // Launch plans and the MFMA fast path of the Lloyd step (lloyd_fast.hip).
 #pragma once
+#include <vector>
+
 #include "common.hpp"
 
 namespace kmd {
@@ -154,5 +156,25 @@ template <typename I>
 void launch_tiled_update(int64_t n, int64_t d, int64_t k, const float *X_r,
                          const float *w, const I *labels, const float *scale,
                          void *scratch, void *sums, hipStream_t s);
+
+// ---- batched fits (batched.hip) -------------------------------------------
+struct BatchedFit {
+  int64_t n, d, k;
+  const float *X, *w;     // X_t (d, n), weights (n)
+  float *init, *res;      // (d, k); init is overwritten as in the driver
+  int *labels;            // (n)
+  double tol;
+  int64_t max_iter;
+  int64_t *out_iters;     // host
+  double *out_inertia;    // host
+};
+struct BatchedStats {
+  int64_t graph_replays = 0;
+};
+// Shapes the batched kernels serve: float data, d <= 64, k <= 256, k <= n.
+bool batched_fit_supported(int64_t n, int64_t d, int64_t k);
+// Runs all fits; results equal driver_lloyd<float, int32_t> fit by fit.
+void batched_lloyd(const std::vector<BatchedFit> &fits, int graph_replays,
+                   hipStream_t s, BatchedStats *stats);
 
 }  // namespace kmd

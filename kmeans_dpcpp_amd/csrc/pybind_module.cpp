@@ -742,6 +742,77 @@ py::tuple py_kmeans_lloyd_driver_ex(py::object X_, py::object w_,
 
 }  // namespace
 
+py::tuple py_kmeans_lloyd_batched(py::list fits, const std::string &precision,
+                                  py::object q, int graph_replays,
+                                  py::list depends) {
+  if (precision != "fp32")
+    throw py::value_error("batched fits support precision 'fp32' only");
+  kmd_stream_t s = resolve_stream(q);
+  std::vector<kmd_fit> v;
+  std::vector<Arr> keep;
+  keep.reserve(5 * fits.size());
+  for (py::handle h : fits) {
+    py::tuple t = py::reinterpret_borrow<py::tuple>(h);
+    if (t.size() != 7)
+      throw py::value_error(
+          "each fit is (X_t, sample_weight, init_centroid_t, assignments_id, "
+          "res_centroids_t, tol, max_iter)");
+    Arr X = to_arr(t[0], s, "X_t"), w = to_arr(t[1], s, "sample_weight");
+    Arr init = to_arr(t[2], s, "init_centroid_t");
+    Arr lab = to_arr(t[3], s, "assignments_id");
+    Arr res = to_arr(t[4], s, "res_centroids_t");
+    if (X.ndim != 2 || w.ndim != 1 || init.ndim != 2 || res.ndim != 2 ||
+        lab.ndim != 1)
+      throw py::value_error("Unsupported array dimensionalities");
+    if (!all_contig({&X, &w, &init, &lab, &res}))
+      throw py::value_error("All input arrays must be C-contiguous");
+    if (!same_device({&X, &w, &init, &lab, &res}))
+      throw py::value_error(kQueueMsg);
+    const int64_t d = X.dim(0), n = X.dim(1), k = init.dim(1);
+    if (d != init.dim(0) || d != res.dim(0) || k != res.dim(1) ||
+        n != w.dim(0) || n != lab.dim(0))
+      throw py::value_error("Array dimensions are not consistent");
+    if (!X.is_float() || X.dtype() != KMD_F32 || !X.same_type(w) ||
+        !X.same_type(init) || !X.same_type(res) || !lab.is_int() ||
+        lab.itype() != KMD_I32)
+      throw py::value_error("batched fits take float32 data and int32 labels");
+    kmd_fit f{};
+    f.n_samples = n;
+    f.n_features = d;
+    f.n_clusters = k;
+    f.X_t = X.data;
+    f.sample_weight = w.data;
+    f.init_centroids_t = init.data;
+    f.assignments_id = lab.data;
+    f.res_centroids_t = res.data;
+    f.tol = t[5].cast<double>();
+    f.max_iter = t[6].cast<int64_t>();
+    if (f.tol < 0.0) throw py::value_error("Tolerance must be non-negative");
+    if (f.max_iter < 0) throw py::value_error("max_iter must be non-negative");
+    v.push_back(f);
+    use_device(X);
+    keep.push_back(std::move(X));
+    keep.push_back(std::move(w));
+    keep.push_back(std::move(init));
+    keep.push_back(std::move(lab));
+    keep.push_back(std::move(res));
+  }
+  wait_depends(s, depends);
+  int64_t launches = 0;
+  int rc;
+  {
+    py::gil_scoped_release nogil;
+    rc = kmd_kmeans_lloyd_batched((int64_t)v.size(), v.data(), KMD_PREC_FP32,
+                                  graph_replays, &launches, s);
+  }
+  check_rc(rc);
+  py::list out;
+  for (const kmd_fit &f : v) out.append(py::make_tuple((size_t)f.n_iters, f.inertia));
+  py::dict stats;
+  stats["graph_launches"] = launches;
+  return py::make_tuple(out, stats);
+}
+
 PYBIND11_MODULE(_kmeans_dpcpp_amd, m) {
   m.doc() =
       "MI355X (gfx950) HIP implementation of the Lloyd k-means entry points "
@@ -876,6 +947,14 @@ PYBIND11_MODULE(_kmeans_dpcpp_amd, m) {
   m.def("iterative_merge_sort", &py_iterative_merge_sort,
         "Stable ascending sort of a float vector (NaN last) into output",
         py::arg("data"), py::arg("output"), py::arg("sycl_queue"),
+        py::arg("depends") = py::list());
+  m.def("kmeans_lloyd_batched", &py_kmeans_lloyd_batched,
+        "Many independent Lloyd fits at once: fits is a list of (X_t, "
+        "sample_weight, init_centroid_t, assignments_id, res_centroids_t, tol, "
+        "max_iter).  Each fit's results equal kmeans_lloyd_driver's for it "
+        "alone.  Returns ([(n_iters, inertia), ...], stats).",
+        py::arg("fits"), py::arg("precision") = "fp32",
+        py::arg("sycl_queue") = py::none(), py::arg("graph_replays") = 4,
         py::arg("depends") = py::list());
   m.def("kmeans_lloyd_driver_ex", &py_kmeans_lloyd_driver_ex,
         "Lloyd fit with a selectable cross-term precision ('fp32', 'bf16x3', "
