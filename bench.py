@@ -43,6 +43,12 @@ def parse():
     p.add_argument("--k", type=int, default=K)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--workload", default="c1", choices=["c1", "c3"],
+                   help="c1: one plain fit n=10M d=32 k=256 (the metric's "
+                        "configuration, default); c3: 512 independent fits in "
+                        "one batched call (fits/s), with the same fits run one "
+                        "by one through the single-fit driver beside it")
+    p.add_argument("--fits", type=int, default=512)
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="first run one device-to-device copy of this many MiB "
                         "(a kernel of known HBM traffic: calibrates rocprofv3 "
@@ -68,6 +74,108 @@ def make_data(n, d, k, seed, device):
     init_idx = torch.randperm(n, generator=g, device=device)[:k]
     init_t = X_t[:, init_idx].contiguous()
     return X_t, init_t
+
+
+def make_c3_batch(n_fits, seed, device):
+    """512 independent fits: n log-uniform in [1e3, 2e5], d in {16, 32, 64},
+    k in {8, 32, 64, 128, 256} with k < n / 10; Gaussian blobs (centers
+    uniform in [-20, 20]^d, per-blob std uniform in [0.5, 2]); seeded
+    random-sample init; tol 1e-4, max_iter 100."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    fits = []
+    for _ in range(n_fits):
+        n = int(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
+        d = int(rng.choice([16, 32, 64]))
+        k = int(rng.choice([8, 32, 64, 128, 256]))
+        while k >= n / 10:
+            k = int(rng.choice([8, 32, 64, 128, 256]))
+        centers = torch.rand((k, d), generator=g, device=device) * 40.0 - 20.0
+        std = torch.rand((k,), generator=g, device=device) * 1.5 + 0.5
+        blob = torch.randint(0, k, (n,), generator=g, device=device)
+        z = torch.randn((n, d), generator=g, device=device)
+        X_t = (centers[blob] + z * std[blob, None]).T.contiguous()
+        init_idx = torch.randperm(n, generator=g, device=device)[:k]
+        fits.append((X_t, torch.ones(n, device=device), X_t[:, init_idx].contiguous()))
+    return fits
+
+
+def bench_c3(args):
+    import torch
+    import kmeans_dpcpp_amd as kd
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream()
+    data = make_c3_batch(args.fits, 3001, dev)
+    n_total = sum(x.shape[1] for x, _, _ in data)
+
+    def batch_args():
+        out = []
+        for X_t, w, init in data:
+            out.append((X_t, w, init.clone(),
+                        torch.empty(X_t.shape[1], dtype=torch.int32, device=dev),
+                        torch.empty_like(init), 1e-4, 100))
+        return out
+
+    kd.kmeans_lloyd_batched(batch_args()[:8], "fp32", stream)      # warm-up
+    fits = batch_args()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res, stats = kd.kmeans_lloyd_batched(fits, "fp32", stream, 4)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    iters = [r[0] for r in res]
+    # the same fits one by one through the single-fit driver
+    seq = batch_args()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    seq_out = []
+    for (X_t, w, init, lab, out, tol, mi) in seq:
+        it_s, inr_s = kd.kmeans_lloyd_driver(X_t, w, init, lab, out, tol, False, mi,
+                                             8, 128, 0.7, stream)
+        seq_out.append((it_s, float(inr_s[0])))
+    torch.cuda.synchronize()
+    el_seq = time.perf_counter() - t1
+    differ = []
+    for i, ((X_t, _, _, lab, out, _, _), (it_s, inr_s), (it_b, inr_b), bf) in \
+            enumerate(zip(seq, seq_out, res, fits)):
+        # bitwise: an empty, unrelocated cluster is NaN in both
+        if it_s != it_b or not torch.equal(lab, bf[3]) or \
+                not torch.equal(out.view(torch.int32), bf[4].view(torch.int32)):
+            differ.append({"fit": i, "n": int(X_t.shape[1]), "d": int(X_t.shape[0]),
+                           "k": int(out.shape[1]), "iters": [int(it_s), int(it_b)],
+                           "label_mismatch": int((lab != bf[3]).sum()),
+                           "max_centroid_diff": float((out - bf[4]).abs().max())})
+    same = not differ
+    line = {
+        "metric": METRIC,
+        "value": round(len(fits) / el, 3),
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": round(1e3 * el, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(fits)} independent Lloyd fits in one batched "
+                               f"call (n log-uniform 1e3..2e5, d in {{16,32,64}}, "
+                               f"k in {{8..256}}, tol 1e-4, max_iter 100)",
+                   "total_samples": n_total},
+        "graph_launches": stats["graph_launches"],
+        "iterations": {"sum": int(sum(iters)), "max": int(max(iters)),
+                       "mean": round(sum(iters) / len(iters), 2)},
+        "sequential_single_fit_fits_per_s": round(len(fits) / el_seq, 3),
+        "speedup_vs_sequential": round(el_seq / el, 2),
+        "identical_to_single_fit": bool(same),
+        "fits_differing": differ[:5],
+        "n_fits_differing": len(differ),
+    }
+    print(json.dumps(line), flush=True)
 
 
 def traffic_from_profile(path):
@@ -125,6 +233,9 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the Lloyd "
               f"path is single-GPU in this build", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c3":
+        bench_c3(args)
+        return
     import torch
     import kmeans_dpcpp_amd as kd
 

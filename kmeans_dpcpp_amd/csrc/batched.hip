@@ -399,17 +399,67 @@ __global__ void __launch_bounds__(128)
   }
 }
 
+// ---- empty clusters and relocation ----------------------------------------------
+// Ascending list of the fit's empty clusters (a zero int64 count is exactly a
+// zero size after conversion) and n_empty.
+__global__ void __launch_bounds__(kFB)
+    bt_empty_krn(const BFit *__restrict__ fits, BState *__restrict__ st,
+                 const u64 *__restrict__ sums, int *__restrict__ list_pool) {
+  __shared__ int wcount[kFB / kWave];
+  const int f = blockIdx.x;
+  if (!st[f].active) return;
+  const BFit F = fits[f];
+  const int tid = threadIdx.x;  // k <= 256 = kFB
+  const bool empty =
+      (tid < F.k) && (sums[F.sum_off + (int64_t)F.d * F.k + tid] == 0ull);
+  const uint64_t mask = __ballot(empty);
+  const int lane = tid & (kWave - 1), w = tid / kWave;
+  const int before_in_wave = __popcll(mask & ((1ull << lane) - 1ull));
+  if (lane == 0) wcount[w] = __popcll(mask);
+  __syncthreads();
+  int before = 0, tot = 0;
+  for (int i = 0; i < kFB / kWave; ++i) {
+    before += (i < w) ? wcount[i] : 0;
+    tot += wcount[i];
+  }
+  if (empty) list_pool[F.kb + before + before_in_wave] = tid;
+  if (tid == 0) st[f].n_empty = tot;
+}
+
+// squared distance (unit weights) of every sample to its current centroid,
+// for fits with an empty cluster (the single-fit inertia arithmetic)
+__global__ void __launch_bounds__(256)
+    bt_dist_krn(const BFit *__restrict__ fits, const BState *__restrict__ st,
+                const CItem *__restrict__ items, float *__restrict__ dist_pool) {
+  const CItem item = items[blockIdx.x];
+  if (!st[item.fit].active || st[item.fit].n_empty == 0) return;
+  const BFit F = fits[item.fit];
+  const float *C = F.C[st[item.fit].cur];
+  for (int j = threadIdx.x; j < item.count; j += 256) {
+    const int64_t s = item.first + j;
+    const int c = F.labels[s];
+    float acc = 0.f;
+    for (int fe = 0; fe < F.d; ++fe) {
+      const float diff = F.X[(int64_t)fe * F.n + s] - C[(int64_t)fe * F.k + c];
+      acc = fmaf(diff, diff, acc);
+    }
+    dist_pool[F.s_off + s] = acc;
+  }
+}
+
 // ---- finalize (one block per fit) ----------------------------------------------
 __global__ void __launch_bounds__(kFB)
     bt_finalize_krn(const BFit *__restrict__ fits, BState *__restrict__ st,
                     const u64 *__restrict__ sums, const float *__restrict__ scale_pool,
-                    float *__restrict__ sizes_pool, int *__restrict__ list_pool,
-                    float *__restrict__ dist_pool) {
+                    float *__restrict__ sizes_pool, const int *__restrict__ list_pool,
+                    const float *__restrict__ dist_pool) {
   __shared__ double red[kFB / kWave];
-  __shared__ int wcount[kFB / kWave];
-  __shared__ int n_empty_s;
-  __shared__ float bv[kFB / kWave];
-  __shared__ int bi[kFB / kWave];
+  __shared__ int hist[256];
+  __shared__ unsigned sel_prefix;
+  __shared__ int sel_left, n_sel, n_eq_taken;
+  __shared__ int wcnt[kFB / kWave];
+  __shared__ unsigned skey[kMaxK];
+  __shared__ int sidx[kMaxK];
   __shared__ int order[kMaxK];
   const int f = blockIdx.x;
   if (!st[f].active) return;
@@ -420,7 +470,7 @@ __global__ void __launch_bounds__(kFB)
   const float *Cc = F.C[cur];
   float *Cn = F.C[cur ^ 1];
   float *sizes = sizes_pool + F.kb;
-  int *list = list_pool + F.kb;
+  const int *list = list_pool + F.kb;
   const float *sc = scale_pool + F.sc_off;
   const int64_t dk = (int64_t)d * k;
   // sums -> centroid sums and sizes (the single-slot partial reduction)
@@ -433,75 +483,79 @@ __global__ void __launch_bounds__(kFB)
       sizes[i - dk] = v;
   }
   __syncthreads();
-  // ascending empty-cluster list (k <= 256: one pass)
-  {
-    const bool empty = (tid < k) && (sizes[tid] == 0.f);
-    const uint64_t mask = __ballot(empty);
-    const int lane = tid & (kWave - 1), w = tid / kWave;
-    const int before_in_wave = __popcll(mask & ((1ull << lane) - 1ull));
-    if (lane == 0) wcount[w] = __popcll(mask);
-    __syncthreads();
-    int before = 0, tot = 0;
-    for (int i = 0; i < kFB / kWave; ++i) {
-      before += (i < w) ? wcount[i] : 0;
-      tot += wcount[i];
-    }
-    if (empty) list[before + before_in_wave] = tid;
-    if (tid == 0) n_empty_s = tot;
-    __syncthreads();
-  }
-  const int n_empty = n_empty_s;
+  const int n_empty = st[f].n_empty;
   if (n_empty > 0) {
-    // squared distance to the current centroid (unit weights)
-    float *dist = dist_pool + F.s_off;
-    for (int64_t s = tid; s < n; s += kFB) {
-      const int c = F.labels[s];
-      float acc = 0.f;
-      for (int j = 0; j < d; ++j) {
-        const float diff = F.X[(int64_t)j * n + s] - Cc[(int64_t)j * k + c];
-        acc = fmaf(diff, diff, acc);
-      }
-      dist[s] = acc;
-    }
-    __syncthreads();
+    // the m farthest samples by (distance desc, index asc): radix-select
+    // the m-th largest distance (non-negative floats order as their bits),
+    // gather the samples above it and the lowest-index ones equal to it,
+    // then rank the m of them
+    const float *dist = dist_pool + F.s_off;
     const int m = (int)min((int64_t)n_empty, n);
-    for (int j = 0; j < m; ++j) {
-      // farthest remaining sample: distance desc, index asc
-      float best = -1.f;
-      int bidx = 0x7fffffff;
+    if (tid == 0) {
+      sel_prefix = 0u;
+      sel_left = m;
+    }
+    for (int shift = 24; shift >= 0; shift -= 8) {
+      hist[tid] = 0;
+      __syncthreads();
+      const unsigned pre = sel_prefix;
+      const unsigned hi_mask = (shift == 24) ? 0u : (0xffffffffu << (shift + 8));
       for (int64_t s = tid; s < n; s += kFB) {
-        const float v = dist[s];
-        if (v > best || (v == best && (int)s < bidx)) {
-          best = v;
-          bidx = (int)s;
-        }
-      }
-      for (int o = kWave / 2; o > 0; o >>= 1) {
-        const float ov = __shfl_xor(best, o, kWave);
-        const int oi = __shfl_xor(bidx, o, kWave);
-        if (ov > best || (ov == best && oi < bidx)) {
-          best = ov;
-          bidx = oi;
-        }
-      }
-      if ((tid & (kWave - 1)) == 0) {
-        bv[tid / kWave] = best;
-        bi[tid / kWave] = bidx;
+        const unsigned key = __float_as_uint(dist[s]);
+        if ((key & hi_mask) == pre) atomicAdd(&hist[(key >> shift) & 255u], 1);
       }
       __syncthreads();
       if (tid == 0) {
-        float b = bv[0];
-        int x = bi[0];
-        for (int i = 1; i < kFB / kWave; ++i)
-          if (bv[i] > b || (bv[i] == b && bi[i] < x)) {
-            b = bv[i];
-            x = bi[i];
-          }
-        order[j] = x;
-        dist[x] = -2.f;  // taken
+        int left = sel_left, b = 255;
+        for (; b > 0; --b) {
+          if (hist[b] >= left) break;
+          left -= hist[b];
+        }
+        sel_prefix = pre | ((unsigned)b << shift);
+        sel_left = left;  // how many of the keys equal to the prefix are taken
       }
       __syncthreads();
     }
+    const unsigned thr = sel_prefix;
+    const int need_eq = sel_left;
+    if (tid == 0) {
+      n_sel = 0;
+      n_eq_taken = 0;
+    }
+    __syncthreads();
+    // gather in index order (chunks of kFB samples, ballot ranks)
+    const int lane = tid & (kWave - 1), w = tid / kWave;
+    for (int64_t c0 = 0; c0 < n; c0 += kFB) {
+      const int64_t s = c0 + tid;
+      const unsigned key = (s < n) ? __float_as_uint(dist[s]) : 0u;
+      const bool gt = (s < n) && key > thr;
+      const bool eq = (s < n) && key == thr;
+      const uint64_t meq = __ballot(eq);
+      if (lane == 0) wcnt[w] = __popcll(meq);
+      __syncthreads();
+      int before = 0, tot = 0;
+      for (int i = 0; i < kFB / kWave; ++i) {
+        before += (i < w) ? wcnt[i] : 0;
+        tot += wcnt[i];
+      }
+      const int eq_rank = n_eq_taken + before + __popcll(meq & ((1ull << lane) - 1ull));
+      const bool take = gt || (eq && eq_rank < need_eq);
+      if (take) {
+        const int pos = atomicAdd(&n_sel, 1);
+        skey[pos] = key;
+        sidx[pos] = (int)s;
+      }
+      __syncthreads();
+      if (tid == 0) n_eq_taken += tot;
+      __syncthreads();
+    }
+    for (int i = tid; i < m; i += kFB) {
+      int rank = 0;
+      for (int j = 0; j < m; ++j)
+        rank += (skey[j] > skey[i]) || (skey[j] == skey[i] && sidx[j] < sidx[i]);
+      order[rank] = sidx[i];
+    }
+    __syncthreads();
     // move w x of the chosen samples (in order) into the empty clusters
     for (int fe = tid; fe < d; fe += kFB) {
       for (int j = 0; j < m; ++j) {
@@ -535,7 +589,6 @@ __global__ void __launch_bounds__(kFB)
   const double tot = block_sum256(acc, red);
   if (tid == 0) {
     BState S = st[f];
-    S.n_empty = n_empty;
     S.shift = tot;
     S.cur = cur ^ 1;
     S.it += 1;
@@ -798,6 +851,12 @@ void batched_lloyd(const std::vector<BatchedFit> &in, int graph_replays,
                        dim3(128), 0, q, nf, dF, dS, seg_total.as<int>(),
                        xr.as<float>(), scale.as<float>(), perm.as<int>(),
                        start.as<int>(), cit.as<int>(), sums.as<u64>());
+    KMD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bt_empty_krn, dim3(nf), dim3(kFB), 0, q, dF, dS,
+                       sums.as<u64>(), list.as<int>());
+    KMD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bt_dist_krn, dim3(nci), dim3(256), 0, q, dF, dS,
+                       d_ci.as<CItem>(), dist.as<float>());
     KMD_LAUNCH_CHECK();
     hipLaunchKernelGGL(bt_finalize_krn, dim3(nf), dim3(kFB), 0, q, dF, dS,
                        sums.as<u64>(), scale.as<float>(), sizes.as<float>(),

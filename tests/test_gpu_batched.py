@@ -84,7 +84,10 @@ def test_batched_relocates_like_the_driver(kd, torch_dev):
     for i in range(6):
         Xt, w, init = make_fit(rng, 5000 + 37 * i, 16 + 8 * i, 16, seed=500 + i)
         init = init.copy()
-        init[:, 1:4] = init[:, [0]]  # three duplicates of centroid 0
+        if i % 2 == 0:
+            init[:, 1:4] = init[:, [0]]  # three duplicates of centroid 0
+        else:
+            init[:, :] = init[:, [0]]    # all equal: k - 1 empty clusters
         probs.append((Xt, w, init, 0.0, 5))
     refs = [single(kd, Xt, w, init, tol, mi) for Xt, w, init, tol, mi in probs]
     got, _ = batched(kd, probs, 2)
