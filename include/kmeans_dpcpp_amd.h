@@ -283,6 +283,30 @@ int kmd_kmeans_lloyd_batched(int64_t n_fits, kmd_fit *fits, int precision,
                              int graph_replays, int64_t *graph_launches,
                              kmd_stream_t stream);
 
+/* Continuous batching (no counterpart in the reference; for a serving loop
+ * that admits fit requests while others run).  An engine holds a running
+ * set of fits on one GPU.  kmd_engine_add joins a fit (the shapes of
+ * kmd_kmeans_lloyd_batched's batched kernels; its arrays and the kmd_fit
+ * struct must stay valid until the fit is reported finished);
+ * kmd_engine_step advances every active fit by `iterations` Lloyd
+ * iterations (use_graph: replay a captured hipGraph while the set is
+ * unchanged) and finishes the fits that stopped, writing n_iters and
+ * inertia into their kmd_fit; kmd_engine_take_finished returns their ids
+ * (up to `capacity`; the rest stay pending).  Results equal
+ * kmd_kmeans_lloyd_driver's for each fit alone, whatever else is in the
+ * set.  kmd_engine_footprint = device bytes the engine holds per fit. */
+typedef struct kmd_engine *kmd_engine_t;
+int kmd_engine_create(kmd_engine_t *engine, kmd_stream_t stream);
+int kmd_engine_destroy(kmd_engine_t engine);
+int kmd_engine_footprint(int64_t n_samples, int64_t n_features,
+                         int64_t n_clusters, int64_t *bytes);
+int kmd_engine_add(kmd_engine_t engine, kmd_fit *fit, int64_t *id);
+int kmd_engine_step(kmd_engine_t engine, int iterations, int use_graph,
+                    int64_t *n_active);
+int kmd_engine_take_finished(kmd_engine_t engine, int64_t *ids,
+                             int64_t capacity, int64_t *count);
+int kmd_engine_resident_bytes(kmd_engine_t engine, int64_t *bytes);
+
 #ifdef __cplusplus
 }
 #endif

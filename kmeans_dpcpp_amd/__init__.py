@@ -23,6 +23,7 @@ except ImportError as _e:  # pragma: no cover - exercised only without a build
         f"{_e}") from _e
 
 from ._kmeans_dpcpp_amd import (  # noqa: E402
+    Engine,
     Event,
     assignment,
     broadcast_divide,

@@ -48,26 +48,32 @@ constexpr int kMaxK = 256;
 constexpr int kMaxD = 64;
 constexpr int kFB = 256;                  // finalize / per-fit block
 
-struct BFit {
-  int64_t n;
-  int d, k;
-  int64_t xr_off;   // X_r pool offset (floats)
-  int64_t s_off;    // per-sample pools (perm, dist)
-  int64_t sum_off;  // int64 sums pool ((d + 1) * k)
-  int kb;           // per-centroid pools of k + 1 entries
-  int sc_off;       // scale pool (d + 1)
-  const float *X, *w;
-  float *C[2];      // C[0] = init (overwritten), C[1] = result
-  int *labels;
-  int64_t max_iter;
-  float tol;
-};
-
 struct BState {
   int it, active, cur, n_empty;
   int seg_items, seg_base;
   double shift;
   double inertia;
+};
+
+// One fit: the caller's arrays and the fit's own device workspace (so fits
+// can join and leave a running set without moving anyone's state).
+struct BFit {
+  int64_t n;
+  int d, k;
+  const float *X, *w;
+  float *C[2];      // C[0] = init (overwritten), C[1] = result
+  int *labels;
+  int64_t max_iter;
+  float tol;
+  BState *st;
+  float *xr;        // (n, d) sample-major copy of X
+  int *perm;        // (n) samples in cluster order
+  float *dist;      // (n) relocation distances / per-sample inertia
+  u64 *sums;        // ((d + 1) k) int64 fixed-point sums
+  float *scale;     // (d + 1) fixed-point scales
+  float *hn, *sizes;                        // (k + 1)
+  int *list, *totals, *cursor, *start, *citems;  // (k + 1)
+  int *nonfinite;   // (1)
 };
 
 struct AItem {  // assignment tile
@@ -92,19 +98,18 @@ __device__ inline double block_sum256(double v, double *lds) {
 
 // ---- half norms -------------------------------------------------------------
 __global__ void __launch_bounds__(kFB)
-    bt_hn_krn(const BFit *__restrict__ fits, const BState *__restrict__ st,
-              float *__restrict__ hn, int all) {
+    bt_hn_krn(const BFit *__restrict__ fits, int all) {
   const int f = blockIdx.x;
-  if (!all && !st[f].active) return;
+  if (!all && !fits[f].st[0].active) return;
   const BFit F = fits[f];
-  const float *C = F.C[st[f].cur];
+  const float *C = F.C[fits[f].st[0].cur];
   for (int c = threadIdx.x; c < F.k; c += kFB) {
     float acc = 0.f;
     for (int j = 0; j < F.d; ++j) {
       const float v = C[(int64_t)j * F.k + c];
       acc = fmaf(v, v, acc);
     }
-    hn[F.kb + c] = acc / 2.f;
+    F.hn[c] = acc / 2.f;
   }
 }
 
@@ -116,9 +121,8 @@ constexpr int tile_floats() {
 
 template <int KS>
 __global__ void __launch_bounds__(kNT)
-    bt_assign_krn(const BFit *__restrict__ fits, const BState *__restrict__ st,
-                  const AItem *__restrict__ items, int n_items,
-                  const float *__restrict__ hn_pool, int all) {
+    bt_assign_krn(const BFit *__restrict__ fits, const AItem *__restrict__ items, int n_items,
+                  int all) {
   constexpr int TF = tile_floats<KS>();
   extern __shared__ __align__(16) unsigned char lds_raw[];
   float *Cl = reinterpret_cast<float *>(lds_raw);  // [kMaxK / 32][TF]
@@ -127,10 +131,10 @@ __global__ void __launch_bounds__(kNT)
   const int col = lane & 31, half = lane >> 5;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     const AItem item = items[it];
-    if (!all && !st[item.fit].active) continue;  // block-uniform
+    if (!all && !fits[item.fit].st[0].active) continue;  // block-uniform
     const BFit F = fits[item.fit];
-    const float *C = F.C[st[item.fit].cur];
-    const float *hn = hn_pool + F.kb;
+    const float *C = F.C[fits[item.fit].st[0].cur];
+    const float *hn = F.hn;
     const int d = F.d, k = F.k, n_ct = (k + 31) / 32;
     __syncthreads();
     for (int i = tid; i < n_ct * TF; i += kNT) {
@@ -209,26 +213,23 @@ __global__ void __launch_bounds__(kNT)
 
 // ---- update -------------------------------------------------------------------
 __global__ void __launch_bounds__(kFB)
-    bt_zero_krn(const BFit *__restrict__ fits, const BState *__restrict__ st,
-                u64 *__restrict__ sums, int *__restrict__ totals,
-                int *__restrict__ cursor) {
+    bt_zero_krn(const BFit *__restrict__ fits) {
   const int f = blockIdx.x;
-  if (!st[f].active) return;
+  if (!fits[f].st[0].active) return;
   const BFit F = fits[f];
   const int64_t m = (int64_t)(F.d + 1) * F.k;
-  for (int64_t i = threadIdx.x; i < m; i += kFB) sums[F.sum_off + i] = 0ull;
+  for (int64_t i = threadIdx.x; i < m; i += kFB) F.sums[i] = 0ull;
   for (int c = threadIdx.x; c <= F.k; c += kFB) {
-    totals[F.kb + c] = 0;
-    cursor[F.kb + c] = 0;
+    F.totals[c] = 0;
+    F.cursor[c] = 0;
   }
 }
 
 __global__ void __launch_bounds__(256)
-    bt_hist_krn(const BFit *__restrict__ fits, const BState *__restrict__ st,
-                const CItem *__restrict__ items, int *__restrict__ totals) {
+    bt_hist_krn(const BFit *__restrict__ fits, const CItem *__restrict__ items) {
   __shared__ int h[kMaxK];
   const CItem item = items[blockIdx.x];
-  if (!st[item.fit].active) return;
+  if (!fits[item.fit].st[0].active) return;
   const BFit F = fits[item.fit];
   for (int c = threadIdx.x; c < F.k; c += 256) h[c] = 0;
   __syncthreads();
@@ -236,25 +237,23 @@ __global__ void __launch_bounds__(256)
     atomicAdd(&h[F.labels[item.first + j]], 1);
   __syncthreads();
   for (int c = threadIdx.x; c < F.k; c += 256)
-    if (h[c] != 0) atomicAdd(&totals[F.kb + c], h[c]);
+    if (h[c] != 0) atomicAdd(&F.totals[c], h[c]);
 }
 
 // per fit: start[c] (exclusive scan of totals, start[k] = n) and the first
 // segment-sum item of each cluster (items[k] = the fit's item count)
 __global__ void __launch_bounds__(kFB)
-    bt_cluster_scan_krn(const BFit *__restrict__ fits, BState *__restrict__ st,
-                        const int *__restrict__ totals, int *__restrict__ start,
-                        int *__restrict__ citems) {
+    bt_cluster_scan_krn(const BFit *__restrict__ fits) {
   __shared__ int wa[kFB / kWave], wb[kFB / kWave];
   const int f = blockIdx.x;
-  if (!st[f].active) {
-    if (threadIdx.x == 0) st[f].seg_items = 0;
+  if (!fits[f].st[0].active) {
+    if (threadIdx.x == 0) fits[f].st[0].seg_items = 0;
     return;
   }
   const BFit F = fits[f];
   const int c = threadIdx.x;  // k <= 256
   const int lane = c & (kWave - 1), wave = c / kWave;
-  const int va = (c < F.k) ? totals[F.kb + c] : 0;
+  const int va = (c < F.k) ? F.totals[c] : 0;
   const int vb = (va + kMembers - 1) / kMembers;
   int ia = va, ib = vb;
 #pragma unroll
@@ -280,19 +279,19 @@ __global__ void __launch_bounds__(kFB)
     tb += wb[w2];
   }
   if (c < F.k) {
-    start[F.kb + c] = pa + ia - va;
-    citems[F.kb + c] = pb + ib - vb;
+    F.start[c] = pa + ia - va;
+    F.citems[c] = pb + ib - vb;
   }
   if (c == 0) {
-    start[F.kb + F.k] = ta;
-    citems[F.kb + F.k] = tb;
-    st[f].seg_items = tb;
+    F.start[F.k] = ta;
+    F.citems[F.k] = tb;
+    fits[f].st[0].seg_items = tb;
   }
 }
 
 // exclusive scan of the fits' segment-item counts (one block)
 __global__ void __launch_bounds__(1024)
-    bt_fit_scan_krn(int n_fits, BState *__restrict__ st, int *__restrict__ total) {
+    bt_fit_scan_krn(int n_fits, const BFit *__restrict__ fits, int *__restrict__ total) {
   __shared__ int ws[1024 / kWave];
   __shared__ int carry;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
@@ -300,7 +299,7 @@ __global__ void __launch_bounds__(1024)
   __syncthreads();
   for (int f0 = 0; f0 < n_fits; f0 += 1024) {
     const int f = f0 + tid;
-    const int v = (f < n_fits) ? st[f].seg_items : 0;
+    const int v = (f < n_fits) ? fits[f].st[0].seg_items : 0;
     int incl = v;
 #pragma unroll
     for (int o = 1; o < kWave; o <<= 1) {
@@ -314,7 +313,7 @@ __global__ void __launch_bounds__(1024)
       if (w2 < wave) p += ws[w2];
       tot += ws[w2];
     }
-    if (f < n_fits) st[f].seg_base = carry + p + incl - v;
+    if (f < n_fits) fits[f].st[0].seg_base = carry + p + incl - v;
     __syncthreads();
     if (tid == 0) carry += tot;
     __syncthreads();
@@ -323,12 +322,10 @@ __global__ void __launch_bounds__(1024)
 }
 
 __global__ void __launch_bounds__(256)
-    bt_scatter_krn(const BFit *__restrict__ fits, const BState *__restrict__ st,
-                   const CItem *__restrict__ items, const int *__restrict__ start,
-                   int *__restrict__ cursor, int *__restrict__ perm) {
+    bt_scatter_krn(const BFit *__restrict__ fits, const CItem *__restrict__ items) {
   __shared__ int h[kMaxK];
   const CItem item = items[blockIdx.x];
-  if (!st[item.fit].active) return;
+  if (!fits[item.fit].st[0].active) return;
   const BFit F = fits[item.fit];
   for (int c = threadIdx.x; c < F.k; c += 256) h[c] = 0;
   __syncthreads();
@@ -336,22 +333,18 @@ __global__ void __launch_bounds__(256)
     atomicAdd(&h[F.labels[item.first + j]], 1);
   __syncthreads();
   for (int c = threadIdx.x; c < F.k; c += 256)
-    h[c] = (h[c] != 0) ? start[F.kb + c] + atomicAdd(&cursor[F.kb + c], h[c]) : 0;
+    h[c] = (h[c] != 0) ? F.start[c] + atomicAdd(&F.cursor[c], h[c]) : 0;
   __syncthreads();
   for (int j = threadIdx.x; j < item.count; j += 256) {
     const int s = item.first + j;
     const int pos = atomicAdd(&h[F.labels[s]], 1);
-    perm[F.s_off + pos] = s;
+    F.perm[pos] = s;
   }
 }
 
 // one block = up to kMembers members of one cluster of one fit
 __global__ void __launch_bounds__(128)
-    bt_segsum_krn(int n_fits, const BFit *__restrict__ fits,
-                  const BState *__restrict__ st, const int *__restrict__ total,
-                  const float *__restrict__ xr_pool, const float *__restrict__ scale_pool,
-                  const int *__restrict__ perm, const int *__restrict__ start,
-                  const int *__restrict__ citems, u64 *__restrict__ sums) {
+    bt_segsum_krn(int n_fits, const BFit *__restrict__ fits, const int *__restrict__ total) {
   __shared__ int sidx[kMembers];
   __shared__ float sw[kMembers];
   const int g = blockIdx.x;
@@ -359,14 +352,14 @@ __global__ void __launch_bounds__(128)
   int lo = 0, hi = n_fits - 1;  // the last fit whose first item is <= g
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (st[mid].seg_base <= g)
+    if (fits[mid].st[0].seg_base <= g)
       lo = mid;
     else
       hi = mid - 1;
   }
   const BFit F = fits[lo];
-  const int local = g - st[lo].seg_base;
-  const int *ci = citems + F.kb;
+  const int local = g - fits[lo].st[0].seg_base;
+  const int *ci = F.citems;
   int a = 0, b = F.k - 1;  // the last cluster whose first item is <= local
   while (a < b) {
     const int mid = (a + b + 1) >> 1;
@@ -376,18 +369,18 @@ __global__ void __launch_bounds__(128)
       b = mid - 1;
   }
   const int c = a;
-  const int m0 = start[F.kb + c] + (local - ci[c]) * kMembers;
-  const int cnt = min(kMembers, start[F.kb + c + 1] - m0);
+  const int m0 = F.start[c] + (local - ci[c]) * kMembers;
+  const int cnt = min(kMembers, F.start[c + 1] - m0);
   for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
-    const int s = perm[F.s_off + m0 + j];
+    const int s = F.perm[m0 + j];
     sidx[j] = s;
     sw[j] = F.w[s];
   }
   __syncthreads();
-  const float *Xr = xr_pool + F.xr_off;
+  const float *Xr = F.xr;
   const int d = F.d;
   for (int f = threadIdx.x; f <= d; f += blockDim.x) {
-    const float sc = scale_pool[F.sc_off + f];
+    const float sc = F.scale[f];
     long long acc = 0;
     if (f < d) {
       for (int j = 0; j < cnt; ++j)
@@ -395,7 +388,7 @@ __global__ void __launch_bounds__(128)
     } else {
       for (int j = 0; j < cnt; ++j) acc += __float2int_rn(sw[j] * sc);
     }
-    if (acc != 0) atomicAdd(&sums[F.sum_off + (int64_t)f * F.k + c], (u64)acc);
+    if (acc != 0) atomicAdd(&F.sums[(int64_t)f * F.k + c], (u64)acc);
   }
 }
 
@@ -403,15 +396,14 @@ __global__ void __launch_bounds__(128)
 // Ascending list of the fit's empty clusters (a zero int64 count is exactly a
 // zero size after conversion) and n_empty.
 __global__ void __launch_bounds__(kFB)
-    bt_empty_krn(const BFit *__restrict__ fits, BState *__restrict__ st,
-                 const u64 *__restrict__ sums, int *__restrict__ list_pool) {
+    bt_empty_krn(const BFit *__restrict__ fits) {
   __shared__ int wcount[kFB / kWave];
   const int f = blockIdx.x;
-  if (!st[f].active) return;
+  if (!fits[f].st[0].active) return;
   const BFit F = fits[f];
   const int tid = threadIdx.x;  // k <= 256 = kFB
   const bool empty =
-      (tid < F.k) && (sums[F.sum_off + (int64_t)F.d * F.k + tid] == 0ull);
+      (tid < F.k) && (F.sums[(int64_t)F.d * F.k + tid] == 0ull);
   const uint64_t mask = __ballot(empty);
   const int lane = tid & (kWave - 1), w = tid / kWave;
   const int before_in_wave = __popcll(mask & ((1ull << lane) - 1ull));
@@ -422,19 +414,18 @@ __global__ void __launch_bounds__(kFB)
     before += (i < w) ? wcount[i] : 0;
     tot += wcount[i];
   }
-  if (empty) list_pool[F.kb + before + before_in_wave] = tid;
-  if (tid == 0) st[f].n_empty = tot;
+  if (empty) F.list[before + before_in_wave] = tid;
+  if (tid == 0) fits[f].st[0].n_empty = tot;
 }
 
 // squared distance (unit weights) of every sample to its current centroid,
 // for fits with an empty cluster (the single-fit inertia arithmetic)
 __global__ void __launch_bounds__(256)
-    bt_dist_krn(const BFit *__restrict__ fits, const BState *__restrict__ st,
-                const CItem *__restrict__ items, float *__restrict__ dist_pool) {
+    bt_dist_krn(const BFit *__restrict__ fits, const CItem *__restrict__ items) {
   const CItem item = items[blockIdx.x];
-  if (!st[item.fit].active || st[item.fit].n_empty == 0) return;
+  if (!fits[item.fit].st[0].active || fits[item.fit].st[0].n_empty == 0) return;
   const BFit F = fits[item.fit];
-  const float *C = F.C[st[item.fit].cur];
+  const float *C = F.C[fits[item.fit].st[0].cur];
   for (int j = threadIdx.x; j < item.count; j += 256) {
     const int64_t s = item.first + j;
     const int c = F.labels[s];
@@ -443,16 +434,13 @@ __global__ void __launch_bounds__(256)
       const float diff = F.X[(int64_t)fe * F.n + s] - C[(int64_t)fe * F.k + c];
       acc = fmaf(diff, diff, acc);
     }
-    dist_pool[F.s_off + s] = acc;
+    F.dist[s] = acc;
   }
 }
 
 // ---- finalize (one block per fit) ----------------------------------------------
 __global__ void __launch_bounds__(kFB)
-    bt_finalize_krn(const BFit *__restrict__ fits, BState *__restrict__ st,
-                    const u64 *__restrict__ sums, const float *__restrict__ scale_pool,
-                    float *__restrict__ sizes_pool, const int *__restrict__ list_pool,
-                    const float *__restrict__ dist_pool) {
+    bt_finalize_krn(const BFit *__restrict__ fits) {
   __shared__ double red[kFB / kWave];
   __shared__ int hist[256];
   __shared__ unsigned sel_prefix;
@@ -462,34 +450,34 @@ __global__ void __launch_bounds__(kFB)
   __shared__ int sidx[kMaxK];
   __shared__ int order[kMaxK];
   const int f = blockIdx.x;
-  if (!st[f].active) return;
+  if (!fits[f].st[0].active) return;
   const BFit F = fits[f];
   const int d = F.d, k = F.k, tid = threadIdx.x;
   const int64_t n = F.n;
-  const int cur = st[f].cur;
+  const int cur = fits[f].st[0].cur;
   const float *Cc = F.C[cur];
   float *Cn = F.C[cur ^ 1];
-  float *sizes = sizes_pool + F.kb;
-  const int *list = list_pool + F.kb;
-  const float *sc = scale_pool + F.sc_off;
+  float *sizes = F.sizes;
+  const int *list = F.list;
+  const float *sc = F.scale;
   const int64_t dk = (int64_t)d * k;
   // sums -> centroid sums and sizes (the single-slot partial reduction)
   for (int64_t i = tid; i < dk + k; i += kFB) {
     const int64_t fe = i / k;
-    const float v = (float)((double)(long long)sums[F.sum_off + i] / (double)sc[fe]);
+    const float v = (float)((double)(long long)F.sums[i] / (double)sc[fe]);
     if (i < dk)
       Cn[i] = v;
     else
       sizes[i - dk] = v;
   }
   __syncthreads();
-  const int n_empty = st[f].n_empty;
+  const int n_empty = fits[f].st[0].n_empty;
   if (n_empty > 0) {
     // the m farthest samples by (distance desc, index asc): radix-select
     // the m-th largest distance (non-negative floats order as their bits),
     // gather the samples above it and the lowest-index ones equal to it,
     // then rank the m of them
-    const float *dist = dist_pool + F.s_off;
+    const float *dist = F.dist;
     const int m = (int)min((int64_t)n_empty, n);
     if (tid == 0) {
       sel_prefix = 0u;
@@ -588,24 +576,23 @@ __global__ void __launch_bounds__(kFB)
   }
   const double tot = block_sum256(acc, red);
   if (tid == 0) {
-    BState S = st[f];
+    BState S = fits[f].st[0];
     S.shift = tot;
     S.cur = cur ^ 1;
     S.it += 1;
     const float shift_t = (float)tot;
     S.active = (S.it < F.max_iter) && (shift_t > F.tol);
-    st[f] = S;
+    fits[f].st[0] = S;
   }
 }
 
 // ---- epilogue --------------------------------------------------------------------
 // per-sample weighted inertia of the final labels
 __global__ void __launch_bounds__(256)
-    bt_inertia_krn(const BFit *__restrict__ fits, const BState *__restrict__ st,
-                   const CItem *__restrict__ items, float *__restrict__ out_pool) {
+    bt_inertia_krn(const BFit *__restrict__ fits, const CItem *__restrict__ items) {
   const CItem item = items[blockIdx.x];
   const BFit F = fits[item.fit];
-  const float *C = F.C[st[item.fit].cur];
+  const float *C = F.C[fits[item.fit].st[0].cur];
   for (int j = threadIdx.x; j < item.count; j += 256) {
     const int64_t s = item.first + j;
     const int c = F.labels[s];
@@ -614,7 +601,7 @@ __global__ void __launch_bounds__(256)
       const float diff = F.X[(int64_t)fe * F.n + s] - C[(int64_t)fe * F.k + c];
       acc = fmaf(diff, diff, acc);
     }
-    out_pool[F.s_off + s] = acc * F.w[s];
+    F.dist[s] = acc * F.w[s];
   }
 }
 
@@ -622,8 +609,7 @@ __global__ void __launch_bounds__(256)
 // grid-stride slice i = b*256 + t, step nb_f * 256
 __global__ void __launch_bounds__(256)
     bt_sum1_krn(const BFit *__restrict__ fits, const int2 *__restrict__ map,
-                const int *__restrict__ nb, const float *__restrict__ pool,
-                double *__restrict__ partials) {
+                const int *__restrict__ nb, double *__restrict__ partials) {
   __shared__ double lds[256 / kWave];
   const int2 fb = map[blockIdx.x];  // (fit, block within the fit)
   const BFit F = fits[fb.x];
@@ -631,41 +617,33 @@ __global__ void __launch_bounds__(256)
   double acc = 0.0;
   for (int64_t i = (int64_t)fb.y * 256 + threadIdx.x; i < F.n;
        i += (int64_t)nbf * 256)
-    acc += (double)pool[F.s_off + i];
+    acc += (double)F.dist[i];
   const double t = block_sum256(acc, lds);
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
 __global__ void __launch_bounds__(256)
-    bt_sum2_krn(const int *__restrict__ nb, const int *__restrict__ first,
-                const double *__restrict__ partials, BState *__restrict__ st) {
+    bt_sum2_krn(const BFit *__restrict__ fits, const int *__restrict__ nb,
+                const int *__restrict__ first, const double *__restrict__ partials) {
   __shared__ double lds[256 / kWave];
   const int f = blockIdx.x;
   double acc = 0.0;
   for (int i = threadIdx.x; i < nb[f]; i += 256) acc += partials[first[f] + i];
   const double t = block_sum256(acc, lds);
-  if (threadIdx.x == 0) st[f].inertia = (double)(float)t;
+  if (threadIdx.x == 0) fits[f].st[0].inertia = (double)(float)t;
 }
 
 __global__ void __launch_bounds__(256)
-    bt_copy_result_krn(const BFit *__restrict__ fits, const BState *__restrict__ st) {
+    bt_copy_result_krn(const BFit *__restrict__ fits) {
   const int f = blockIdx.x;
   const BFit F = fits[f];
-  if (st[f].cur == 1) return;  // the result buffer already holds it
+  if (fits[f].st[0].cur == 1) return;  // the result buffer already holds it
   const int64_t m = (int64_t)F.d * F.k;
   for (int64_t i = threadIdx.x; i < m; i += 256) F.C[1][i] = F.C[0][i];
 }
 
-__global__ void bt_init_state_krn(int n_fits, const BFit *__restrict__ fits,
-                                  BState *__restrict__ st) {
-  const int f = blockIdx.x * 256 + threadIdx.x;
-  if (f >= n_fits) return;
-  BState S{};
-  S.active = fits[f].max_iter > 0 ? 1 : 0;
-  st[f] = S;
-}
 
-// ---- host --------------------------------------------------------------------
+// ---- host ---------------------------------------------------------------------
 inline int ks_bucket(int d) { return d <= 16 ? 8 : (d <= 32 ? 16 : 32); }
 
 size_t assign_lds(int KS) { return (size_t)(kMaxK / 32) * (2 * KS * 32 + 32) * 4; }
@@ -686,14 +664,51 @@ int assign_grid() {
 }
 
 template <int KS>
-void launch_assign_bucket(const BFit *fits, const BState *st, const AItem *items,
-                          int n_items, const float *hn, int all, hipStream_t s) {
+void launch_assign_bucket(const BFit *fits, const AItem *items, int n_items,
+                          int all, hipStream_t s) {
   if (n_items <= 0) return;
   const int grid = std::min(assign_grid<KS>(), n_items);
   hipLaunchKernelGGL(bt_assign_krn<KS>, dim3(grid), dim3(kNT), assign_lds(KS), s,
-                     fits, st, items, n_items, hn, all);
+                     fits, items, n_items, all);
   KMD_LAUNCH_CHECK();
 }
+
+__global__ void bt_set_state_krn(BState *st, int active) {
+  BState S{};
+  S.active = active;
+  st[0] = S;
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Byte layout of one fit's workspace arena.
+struct Arena {
+  size_t st, xr, perm, dist, sums, scale, hn, sizes, list, totals, cursor, start,
+      citems, nonfinite, total;
+  Arena(int64_t n, int64_t d, int64_t k) {
+    size_t o = 0;
+    auto take = [&](size_t b) {
+      const size_t at = o;
+      o += align256(b);
+      return at;
+    };
+    st = take(sizeof(BState));
+    xr = take(sizeof(float) * (size_t)n * d);
+    perm = take(sizeof(int) * (size_t)n);
+    dist = take(sizeof(float) * (size_t)n);
+    sums = take(sizeof(u64) * (size_t)(d + 1) * k);
+    scale = take(sizeof(float) * (size_t)(d + 1));
+    hn = take(sizeof(float) * (size_t)(k + 1));
+    sizes = take(sizeof(float) * (size_t)(k + 1));
+    list = take(sizeof(int) * (size_t)(k + 1));
+    totals = take(sizeof(int) * (size_t)(k + 1));
+    cursor = take(sizeof(int) * (size_t)(k + 1));
+    start = take(sizeof(int) * (size_t)(k + 1));
+    citems = take(sizeof(int) * (size_t)(k + 1));
+    nonfinite = take(sizeof(int));
+    total = o;
+  }
+};
 
 }  // namespace
 
@@ -702,235 +717,379 @@ bool batched_fit_supported(int64_t n, int64_t d, int64_t k) {
          k <= kMaxK && k <= n;
 }
 
-// Runs every fit of `fits` (all served: batched_fit_supported) to completion.
-void batched_lloyd(const std::vector<BatchedFit> &in, int graph_replays,
-                   hipStream_t user_stream, BatchedStats *stats) {
-  const int nf = (int)in.size();
-  if (nf == 0) return;
-  hipStream_t s = nullptr;
-  KMD_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  struct StreamGuard {
-    hipStream_t s;
-    ~StreamGuard() { (void)hipStreamDestroy(s); }
-  } guard{s};
-  hipEvent_t ev = nullptr;
-  KMD_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-  KMD_HIP_CHECK(hipEventRecord(ev, user_stream));
-  KMD_HIP_CHECK(hipStreamWaitEvent(s, ev, 0));
+int64_t batch_fit_footprint(int64_t n, int64_t d, int64_t k) {
+  return (int64_t)Arena(n, d, k).total;
+}
 
-  // flatten
-  std::vector<BFit> fits(nf);
-  int64_t xr_tot = 0, s_tot = 0, sum_tot = 0;
-  int kb_tot = 0, sc_tot = 0;
-  int64_t max_iter = 0;
-  std::vector<AItem> aitems[3];
-  std::vector<CItem> citems;
-  int64_t seg_bound = 0;
-  for (int f = 0; f < nf; ++f) {
-    const BatchedFit &b = in[f];
-    require(batched_fit_supported(b.n, b.d, b.k), "batched fit: unsupported shape");
-    BFit &F = fits[f];
-    F.n = b.n;
-    F.d = (int)b.d;
-    F.k = (int)b.k;
-    F.xr_off = xr_tot;
-    xr_tot += b.n * b.d;
-    F.s_off = s_tot;
-    s_tot += b.n;
-    F.sum_off = sum_tot;
-    sum_tot += (b.d + 1) * b.k;
-    F.kb = kb_tot;
-    kb_tot += (int)b.k + 1;
-    F.sc_off = sc_tot;
-    sc_tot += (int)b.d + 1;
-    F.X = b.X;
-    F.w = b.w;
-    F.C[0] = b.init;
-    F.C[1] = b.res;
-    F.labels = b.labels;
-    F.max_iter = b.max_iter;
-    F.tol = (float)b.tol;
-    max_iter = std::max(max_iter, b.max_iter);
-    const int bucket = ks_bucket(F.d) == 8 ? 0 : (ks_bucket(F.d) == 16 ? 1 : 2);
-    for (int64_t first = 0; first < b.n; first += kTileSamples)
-      aitems[bucket].push_back(AItem{f, (int)first});
-    for (int64_t first = 0; first < b.n; first += kChunk)
-      citems.push_back(CItem{f, (int)first, (int)std::min<int64_t>(kChunk, b.n - first)});
-    seg_bound += b.n / kMembers + b.k;
-  }
-  // epilogue reduction map (the single-fit two-stage sum's block count)
-  std::vector<int> nb(nf), first(nf);
-  std::vector<int2> map;
-  for (int f = 0; f < nf; ++f) {
-    int64_t g = (fits[f].n + 4095) / 4096;
-    g = std::max<int64_t>(1, std::min<int64_t>(1024, g));
-    nb[f] = (int)g;
-    first[f] = (int)map.size();
-    for (int b2 = 0; b2 < g; ++b2) map.push_back(make_int2(f, b2));
-  }
+// A running set of fits: fits join with add(), step() advances every active
+// fit by whole iterations (one launch per step of an iteration for all of
+// them; replayed from a captured hipGraph while the set is unchanged) and
+// runs the epilogue of fits that stopped; finished fits leave the set (and
+// free their workspace) at the next step after take_finished().
+struct BatchEngine {
+  struct Slot;
+  explicit BatchEngine(hipStream_t user_stream);
+  ~BatchEngine();
+  BatchEngine(const BatchEngine &) = delete;
+  BatchEngine &operator=(const BatchEngine &) = delete;
+  int64_t add(const BatchedFit &f, bool check_now);
+  int64_t step(int iterations, bool use_graph);
+  std::vector<int64_t> take_finished();
+  int64_t resident_bytes() const;
+  void rebuild();
+  void iteration(hipStream_t q);
+  void epilogue(const std::vector<Slot *> &done);
+  void check_inputs();
+  void drop_graph();
 
-  auto bytes = [](size_t n, size_t sz) { return std::max<size_t>(1, n) * sz; };
-  TempBuffer d_fits(bytes(nf, sizeof(BFit)), s), d_st(bytes(nf, sizeof(BState)), s);
-  TempBuffer d_ai0(bytes(aitems[0].size(), sizeof(AItem)), s),
-      d_ai1(bytes(aitems[1].size(), sizeof(AItem)), s),
-      d_ai2(bytes(aitems[2].size(), sizeof(AItem)), s);
-  TempBuffer d_ci(bytes(citems.size(), sizeof(CItem)), s);
-  TempBuffer d_map(bytes(map.size(), sizeof(int2)), s), d_nb(bytes(nf, sizeof(int)), s),
-      d_first(bytes(nf, sizeof(int)), s), d_part(bytes(map.size(), sizeof(double)), s);
-  TempBuffer xr(bytes(xr_tot, sizeof(float)), s), perm(bytes(s_tot, sizeof(int)), s),
-      dist(bytes(s_tot, sizeof(float)), s);
-  TempBuffer sums(bytes(sum_tot, sizeof(u64)), s), scale(bytes(sc_tot, sizeof(float)), s);
-  TempBuffer hn(bytes(kb_tot, sizeof(float)), s), sizes(bytes(kb_tot, sizeof(float)), s),
-      list(bytes(kb_tot, sizeof(int)), s), totals(bytes(kb_tot, sizeof(int)), s),
-      cursor(bytes(kb_tot, sizeof(int)), s), start(bytes(kb_tot, sizeof(int)), s),
-      cit(bytes(kb_tot, sizeof(int)), s), seg_total(sizeof(int) * 4, s),
-      flags(sizeof(int) * (size_t)nf + 64, s);
-  KMD_HIP_CHECK(hipMemcpyAsync(d_fits.as<void>(), fits.data(), sizeof(BFit) * nf,
-                               hipMemcpyHostToDevice, s));
-  const AItem *ai[3] = {d_ai0.as<AItem>(), d_ai1.as<AItem>(), d_ai2.as<AItem>()};
-  TempBuffer *aib[3] = {&d_ai0, &d_ai1, &d_ai2};
-  for (int b = 0; b < 3; ++b)
-    if (!aitems[b].empty())
-      KMD_HIP_CHECK(hipMemcpyAsync(aib[b]->as<void>(), aitems[b].data(),
-                                   sizeof(AItem) * aitems[b].size(),
-                                   hipMemcpyHostToDevice, s));
-  KMD_HIP_CHECK(hipMemcpyAsync(d_ci.as<void>(), citems.data(),
-                               sizeof(CItem) * citems.size(), hipMemcpyHostToDevice, s));
-  KMD_HIP_CHECK(hipMemcpyAsync(d_map.as<void>(), map.data(), sizeof(int2) * map.size(),
-                               hipMemcpyHostToDevice, s));
-  KMD_HIP_CHECK(hipMemcpyAsync(d_nb.as<void>(), nb.data(), sizeof(int) * nf,
-                               hipMemcpyHostToDevice, s));
-  KMD_HIP_CHECK(hipMemcpyAsync(d_first.as<void>(), first.data(), sizeof(int) * nf,
-                               hipMemcpyHostToDevice, s));
+  hipStream_t user_ = nullptr, s_ = nullptr;
+  hipEvent_t ev_ = nullptr;
+  std::vector<std::unique_ptr<Slot>> slots_;
+  std::vector<Slot *> unchecked_;
+  std::vector<BFit> host_fits_;
+  std::vector<AItem> ai_[3];
+  std::vector<CItem> ci_;
+  std::unique_ptr<TempBuffer> fits_d_, ai_d_[3], ci_d_, seg_total_;
+  int64_t seg_bound_ = 1, next_id_ = 0, launches_ = 0;
+  bool dirty_ = true;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+  std::vector<int64_t> finished_;
+};
 
-  // per fit: sample-major copy and fixed-point scales (the single-fit
-  // driver's pre-pass, so the integers are the same)
-  int *nonfinite = flags.as<int>();
-  for (int f = 0; f < nf; ++f) {
-    launch_transpose(fits[f].n, fits[f].d, fits[f].X, xr.as<float>() + fits[f].xr_off, s);
-    launch_fixed_point_scales(fits[f].n, fits[f].d, fits[f].X, fits[f].w,
-                              scale.as<float>() + fits[f].sc_off, nonfinite + f, s);
-  }
-  std::vector<int> host_flags(nf);
-  KMD_HIP_CHECK(hipMemcpyAsync(host_flags.data(), nonfinite, sizeof(int) * nf,
-                               hipMemcpyDeviceToHost, s));
-  KMD_HIP_CHECK(hipStreamSynchronize(s));
-  for (int f = 0; f < nf; ++f)
-    require(host_flags[f] == 0, "batched fit: X or sample_weight holds inf/NaN");
+struct BatchEngine::Slot {
+  BatchedFit desc;
+  int64_t id;
+  TempBuffer arena{0, nullptr};
+  BFit dev{};
+  bool done = false;
+};
 
-  BFit *dF = d_fits.as<BFit>();
-  BState *dS = d_st.as<BState>();
-  hipLaunchKernelGGL(bt_init_state_krn, dim3((nf + 255) / 256), dim3(256), 0, s, nf,
-                     dF, dS);
-  KMD_LAUNCH_CHECK();
-
-  const int nci = (int)citems.size();
-  auto iteration = [&](hipStream_t q) {
-    hipLaunchKernelGGL(bt_hn_krn, dim3(nf), dim3(kFB), 0, q, dF, dS, hn.as<float>(), 0);
-    KMD_LAUNCH_CHECK();
-    launch_assign_bucket<8>(dF, dS, ai[0], (int)aitems[0].size(), hn.as<float>(), 0, q);
-    launch_assign_bucket<16>(dF, dS, ai[1], (int)aitems[1].size(), hn.as<float>(), 0, q);
-    launch_assign_bucket<32>(dF, dS, ai[2], (int)aitems[2].size(), hn.as<float>(), 0, q);
-    hipLaunchKernelGGL(bt_zero_krn, dim3(nf), dim3(kFB), 0, q, dF, dS, sums.as<u64>(),
-                       totals.as<int>(), cursor.as<int>());
-    KMD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bt_hist_krn, dim3(nci), dim3(256), 0, q, dF, dS,
-                       d_ci.as<CItem>(), totals.as<int>());
-    KMD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bt_cluster_scan_krn, dim3(nf), dim3(kFB), 0, q, dF, dS,
-                       totals.as<int>(), start.as<int>(), cit.as<int>());
-    KMD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bt_fit_scan_krn, dim3(1), dim3(1024), 0, q, nf, dS,
-                       seg_total.as<int>());
-    KMD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bt_scatter_krn, dim3(nci), dim3(256), 0, q, dF, dS,
-                       d_ci.as<CItem>(), start.as<int>(), cursor.as<int>(),
-                       perm.as<int>());
-    KMD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bt_segsum_krn, dim3((unsigned)std::max<int64_t>(1, seg_bound)),
-                       dim3(128), 0, q, nf, dF, dS, seg_total.as<int>(),
-                       xr.as<float>(), scale.as<float>(), perm.as<int>(),
-                       start.as<int>(), cit.as<int>(), sums.as<u64>());
-    KMD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bt_empty_krn, dim3(nf), dim3(kFB), 0, q, dF, dS,
-                       sums.as<u64>(), list.as<int>());
-    KMD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bt_dist_krn, dim3(nci), dim3(256), 0, q, dF, dS,
-                       d_ci.as<CItem>(), dist.as<float>());
-    KMD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bt_finalize_krn, dim3(nf), dim3(kFB), 0, q, dF, dS,
-                       sums.as<u64>(), scale.as<float>(), sizes.as<float>(),
-                       list.as<int>(), dist.as<float>());
-    KMD_LAUNCH_CHECK();
-  };
-
-  // launch plans (function attributes, occupancy) before the capture
+BatchEngine::BatchEngine(hipStream_t user_stream) : user_(user_stream) {
+  KMD_HIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+  KMD_HIP_CHECK(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
   (void)assign_grid<8>();
   (void)assign_grid<16>();
   (void)assign_grid<32>();
-  // capture one iteration, replay it until every fit's mask is off
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  KMD_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  iteration(s);
-  KMD_HIP_CHECK(hipStreamEndCapture(s, &graph));
-  struct GraphGuard {
-    hipGraph_t g;
-    hipGraphExec_t e;
-    ~GraphGuard() {
-      if (e) (void)hipGraphExecDestroy(e);
-      if (g) (void)hipGraphDestroy(g);
-    }
-  } gguard{graph, nullptr};
-  KMD_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  gguard.e = exec;
+}
 
-  std::vector<BState> host_st(nf);
+BatchEngine::~BatchEngine() {
+  (void)hipStreamSynchronize(s_);
+  drop_graph();
+  // every stream-ordered buffer is freed before its stream goes away
+  unchecked_.clear();
+  slots_.clear();
+  fits_d_.reset();
+  for (auto &b : ai_d_) b.reset();
+  ci_d_.reset();
+  seg_total_.reset();
+  (void)hipStreamSynchronize(s_);
+  (void)hipEventDestroy(ev_);
+  (void)hipStreamDestroy(s_);
+}
+
+void BatchEngine::drop_graph() {
+  if (exec_) (void)hipGraphExecDestroy(exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  exec_ = nullptr;
+  graph_ = nullptr;
+}
+
+// check_now: verify the fit's inputs are finite before returning (one host
+// synchronisation); otherwise that happens at the next step for all new fits.
+int64_t BatchEngine::add(const BatchedFit &f, bool check_now) {
+  require(batched_fit_supported(f.n, f.d, f.k), "batched fit: unsupported shape");
+  auto slot = std::make_unique<Slot>();
+  slot->desc = f;
+  slot->id = next_id_++;
+  const Arena A(f.n, f.d, f.k);
+  // the caller's arrays are ready once its stream reaches this point
+  KMD_HIP_CHECK(hipEventRecord(ev_, user_));
+  KMD_HIP_CHECK(hipStreamWaitEvent(s_, ev_, 0));
+  slot->arena = TempBuffer(A.total, s_);
+  char *b = slot->arena.as<char>();
+  BFit &F = slot->dev;
+  F.n = f.n;
+  F.d = (int)f.d;
+  F.k = (int)f.k;
+  F.X = f.X;
+  F.w = f.w;
+  F.C[0] = f.init;
+  F.C[1] = f.res;
+  F.labels = f.labels;
+  F.max_iter = f.max_iter;
+  F.tol = (float)f.tol;
+  F.st = reinterpret_cast<BState *>(b + A.st);
+  F.xr = reinterpret_cast<float *>(b + A.xr);
+  F.perm = reinterpret_cast<int *>(b + A.perm);
+  F.dist = reinterpret_cast<float *>(b + A.dist);
+  F.sums = reinterpret_cast<u64 *>(b + A.sums);
+  F.scale = reinterpret_cast<float *>(b + A.scale);
+  F.hn = reinterpret_cast<float *>(b + A.hn);
+  F.sizes = reinterpret_cast<float *>(b + A.sizes);
+  F.list = reinterpret_cast<int *>(b + A.list);
+  F.totals = reinterpret_cast<int *>(b + A.totals);
+  F.cursor = reinterpret_cast<int *>(b + A.cursor);
+  F.start = reinterpret_cast<int *>(b + A.start);
+  F.citems = reinterpret_cast<int *>(b + A.citems);
+  F.nonfinite = reinterpret_cast<int *>(b + A.nonfinite);
+  hipLaunchKernelGGL(bt_set_state_krn, dim3(1), dim3(1), 0, s_, F.st,
+                     f.max_iter > 0 ? 1 : 0);
+  KMD_LAUNCH_CHECK();
+  // sample-major copy and the single-fit driver's fixed-point scales
+  launch_transpose(f.n, f.d, f.X, F.xr, s_);
+  launch_fixed_point_scales(f.n, f.d, f.X, f.w, F.scale, F.nonfinite, s_);
+  unchecked_.push_back(slot.get());
+  slots_.push_back(std::move(slot));
+  dirty_ = true;
+  const int64_t id = slots_.back()->id;
+  if (check_now) check_inputs();
+  return id;
+}
+
+void BatchEngine::check_inputs() {
+  if (unchecked_.empty()) return;
+  std::vector<int> bad(unchecked_.size());
+  for (size_t i = 0; i < unchecked_.size(); ++i)
+    KMD_HIP_CHECK(hipMemcpyAsync(&bad[i], unchecked_[i]->dev.nonfinite, sizeof(int),
+                                 hipMemcpyDeviceToHost, s_));
+  KMD_HIP_CHECK(hipStreamSynchronize(s_));
+  std::vector<Slot *> failed;
+  for (size_t i = 0; i < unchecked_.size(); ++i)
+    if (bad[i]) failed.push_back(unchecked_[i]);
+  unchecked_.clear();
+  if (!failed.empty()) {
+    for (Slot *f : failed)
+      slots_.erase(std::remove_if(slots_.begin(), slots_.end(),
+                                  [&](const std::unique_ptr<Slot> &p) {
+                                    return p.get() == f;
+                                  }),
+                   slots_.end());
+    dirty_ = true;
+    throw ValueError("batched fit: X or sample_weight holds inf/NaN");
+  }
+}
+
+void BatchEngine::rebuild() {
+  // drop finished fits, then flatten the rest into work lists
+  slots_.erase(std::remove_if(slots_.begin(), slots_.end(),
+                              [](const std::unique_ptr<Slot> &p) { return p->done; }),
+               slots_.end());
+  drop_graph();
+  host_fits_.clear();
+  for (auto &v : ai_) v.clear();
+  ci_.clear();
+  seg_bound_ = 1;
+  for (size_t f = 0; f < slots_.size(); ++f) {
+    const BFit &F = slots_[f]->dev;
+    host_fits_.push_back(F);
+    const int bucket = ks_bucket(F.d) == 8 ? 0 : (ks_bucket(F.d) == 16 ? 1 : 2);
+    for (int64_t first = 0; first < F.n; first += kTileSamples)
+      ai_[bucket].push_back(AItem{(int)f, (int)first});
+    for (int64_t first = 0; first < F.n; first += kChunk)
+      ci_.push_back(CItem{(int)f, (int)first, (int)std::min<int64_t>(kChunk, F.n - first)});
+    seg_bound_ += F.n / kMembers + F.k;
+  }
+  auto upload = [&](std::unique_ptr<TempBuffer> &buf, const void *src, size_t bytes) {
+    buf = std::make_unique<TempBuffer>(std::max<size_t>(bytes, 64), s_);
+    if (bytes) KMD_HIP_CHECK(hipMemcpyAsync(buf->as<void>(), src, bytes,
+                                            hipMemcpyHostToDevice, s_));
+  };
+  upload(fits_d_, host_fits_.data(), sizeof(BFit) * host_fits_.size());
+  for (int b = 0; b < 3; ++b) upload(ai_d_[b], ai_[b].data(), sizeof(AItem) * ai_[b].size());
+  upload(ci_d_, ci_.data(), sizeof(CItem) * ci_.size());
+  if (!seg_total_) seg_total_ = std::make_unique<TempBuffer>(256, s_);
+  KMD_HIP_CHECK(hipStreamSynchronize(s_));  // host vectors may change after this
+  dirty_ = false;
+}
+
+void BatchEngine::iteration(hipStream_t q) {
+  const BFit *dF = fits_d_->as<const BFit>();
+  const int nf = (int)host_fits_.size();
+  const int nci = (int)ci_.size();
+  if (nf == 0) return;
+  hipLaunchKernelGGL(bt_hn_krn, dim3(nf), dim3(kFB), 0, q, dF, 0);
+  KMD_LAUNCH_CHECK();
+  launch_assign_bucket<8>(dF, ai_d_[0]->as<AItem>(), (int)ai_[0].size(), 0, q);
+  launch_assign_bucket<16>(dF, ai_d_[1]->as<AItem>(), (int)ai_[1].size(), 0, q);
+  launch_assign_bucket<32>(dF, ai_d_[2]->as<AItem>(), (int)ai_[2].size(), 0, q);
+  hipLaunchKernelGGL(bt_zero_krn, dim3(nf), dim3(kFB), 0, q, dF);
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_hist_krn, dim3(nci), dim3(256), 0, q, dF, ci_d_->as<CItem>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_cluster_scan_krn, dim3(nf), dim3(kFB), 0, q, dF);
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_fit_scan_krn, dim3(1), dim3(1024), 0, q, nf, dF,
+                     seg_total_->as<int>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_scatter_krn, dim3(nci), dim3(256), 0, q, dF,
+                     ci_d_->as<CItem>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_segsum_krn, dim3((unsigned)seg_bound_), dim3(128), 0, q, nf, dF,
+                     seg_total_->as<int>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_empty_krn, dim3(nf), dim3(kFB), 0, q, dF);
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_dist_krn, dim3(nci), dim3(256), 0, q, dF, ci_d_->as<CItem>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_finalize_krn, dim3(nf), dim3(kFB), 0, q, dF);
+  KMD_LAUNCH_CHECK();
+}
+
+// Labels and weighted inertia of the final centroids for `done` fits, the
+// result copied into res when it ended in init (the single-fit epilogue).
+void BatchEngine::epilogue(const std::vector<Slot *> &done) {
+  if (done.empty()) return;
+  std::vector<BFit> fits;
+  std::vector<AItem> ai[3];
+  std::vector<CItem> ci;
+  std::vector<int> nb, first;
+  std::vector<int2> map;
+  for (size_t f = 0; f < done.size(); ++f) {
+    const BFit &F = done[f]->dev;
+    fits.push_back(F);
+    const int bucket = ks_bucket(F.d) == 8 ? 0 : (ks_bucket(F.d) == 16 ? 1 : 2);
+    for (int64_t p = 0; p < F.n; p += kTileSamples) ai[bucket].push_back(AItem{(int)f, (int)p});
+    for (int64_t p = 0; p < F.n; p += kChunk)
+      ci.push_back(CItem{(int)f, (int)p, (int)std::min<int64_t>(kChunk, F.n - p)});
+    int64_t g = (F.n + 4095) / 4096;
+    g = std::max<int64_t>(1, std::min<int64_t>(1024, g));
+    nb.push_back((int)g);
+    first.push_back((int)map.size());
+    for (int b = 0; b < g; ++b) map.push_back(make_int2((int)f, b));
+  }
+  const int nf = (int)fits.size();
+  auto up = [&](const void *src, size_t bytes) {
+    auto buf = std::make_unique<TempBuffer>(std::max<size_t>(bytes, 64), s_);
+    if (bytes) KMD_HIP_CHECK(hipMemcpyAsync(buf->as<void>(), src, bytes,
+                                            hipMemcpyHostToDevice, s_));
+    return buf;
+  };
+  auto dF = up(fits.data(), sizeof(BFit) * fits.size());
+  std::unique_ptr<TempBuffer> dai[3];
+  for (int b = 0; b < 3; ++b) dai[b] = up(ai[b].data(), sizeof(AItem) * ai[b].size());
+  auto dci = up(ci.data(), sizeof(CItem) * ci.size());
+  auto dmap = up(map.data(), sizeof(int2) * map.size());
+  auto dnb = up(nb.data(), sizeof(int) * nb.size());
+  auto dfirst = up(first.data(), sizeof(int) * first.size());
+  TempBuffer part(sizeof(double) * std::max<size_t>(1, map.size()), s_);
+  const BFit *F = dF->as<const BFit>();
+  hipLaunchKernelGGL(bt_hn_krn, dim3(nf), dim3(kFB), 0, s_, F, 1);
+  KMD_LAUNCH_CHECK();
+  launch_assign_bucket<8>(F, dai[0]->as<AItem>(), (int)ai[0].size(), 1, s_);
+  launch_assign_bucket<16>(F, dai[1]->as<AItem>(), (int)ai[1].size(), 1, s_);
+  launch_assign_bucket<32>(F, dai[2]->as<AItem>(), (int)ai[2].size(), 1, s_);
+  hipLaunchKernelGGL(bt_inertia_krn, dim3((unsigned)ci.size()), dim3(256), 0, s_, F,
+                     dci->as<CItem>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_sum1_krn, dim3((unsigned)map.size()), dim3(256), 0, s_, F,
+                     dmap->as<int2>(), dnb->as<int>(), part.as<double>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_sum2_krn, dim3(nf), dim3(256), 0, s_, F, dnb->as<int>(),
+                     dfirst->as<int>(), part.as<double>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_copy_result_krn, dim3(nf), dim3(256), 0, s_, F);
+  KMD_LAUNCH_CHECK();
+  std::vector<BState> st(nf);
+  for (int f = 0; f < nf; ++f)
+    KMD_HIP_CHECK(hipMemcpyAsync(&st[f], fits[f].st, sizeof(BState),
+                                 hipMemcpyDeviceToHost, s_));
+  KMD_HIP_CHECK(hipStreamSynchronize(s_));
+  for (int f = 0; f < nf; ++f) {
+    Slot *sl = done[f];
+    if (sl->desc.out_iters) *sl->desc.out_iters = st[f].it;
+    if (sl->desc.out_inertia) *sl->desc.out_inertia = st[f].inertia;
+    sl->done = true;
+    finished_.push_back(sl->id);
+  }
+}
+
+int64_t BatchEngine::step(int iterations, bool use_graph) {
+  check_inputs();
+  if (dirty_) rebuild();
+  if (host_fits_.empty()) return 0;
+  if (use_graph && !exec_) {
+    KMD_HIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+    iteration(s_);
+    KMD_HIP_CHECK(hipStreamEndCapture(s_, &graph_));
+    KMD_HIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+  }
+  for (int i = 0; i < iterations; ++i) {
+    if (use_graph)
+      KMD_HIP_CHECK(hipGraphLaunch(exec_, s_));
+    else
+      iteration(s_);
+  }
+  launches_ += iterations;
+  // read the masks; fits that stopped get their epilogue
+  std::vector<BState> st(host_fits_.size());
+  for (size_t f = 0; f < host_fits_.size(); ++f)
+    KMD_HIP_CHECK(hipMemcpyAsync(&st[f], host_fits_[f].st, sizeof(BState),
+                                 hipMemcpyDeviceToHost, s_));
+  KMD_HIP_CHECK(hipStreamSynchronize(s_));
+  std::vector<Slot *> stopping;
+  int64_t active = 0;
+  for (size_t f = 0; f < slots_.size(); ++f) {
+    if (slots_[f]->done) continue;
+    if (st[f].active)
+      ++active;
+    else
+      stopping.push_back(slots_[f].get());
+  }
+  epilogue(stopping);
+  return active;
+}
+
+std::vector<int64_t> BatchEngine::take_finished() {
+  std::vector<int64_t> out;
+  out.swap(finished_);
+  if (!out.empty()) dirty_ = true;  // free their workspaces at the next step
+  return out;
+}
+
+int64_t BatchEngine::resident_bytes() const {
+  int64_t b = 0;
+  for (const auto &s : slots_)
+    if (!s->done) b += (int64_t)s->arena.bytes();
+  return b;
+}
+
+BatchEngine *batch_engine_create(hipStream_t user_stream) {
+  return new BatchEngine(user_stream);
+}
+void batch_engine_destroy(BatchEngine *e) { delete e; }
+int64_t batch_engine_add(BatchEngine *e, const BatchedFit &f) {
+  return e->add(f, true);
+}
+int64_t batch_engine_step(BatchEngine *e, int iterations, bool use_graph) {
+  return e->step(iterations, use_graph);
+}
+std::vector<int64_t> batch_engine_take_finished(BatchEngine *e) {
+  return e->take_finished();
+}
+int64_t batch_engine_resident_bytes(const BatchEngine *e) { return e->resident_bytes(); }
+int64_t batch_engine_launches(const BatchEngine *e) { return e->launches_; }
+
+void batched_lloyd(const std::vector<BatchedFit> &in, int graph_replays,
+                   hipStream_t user_stream, BatchedStats *stats) {
+  if (in.empty()) return;
+  BatchEngine eng(user_stream);
+  int64_t max_iter = 0;
+  for (const BatchedFit &f : in) {
+    eng.add(f, false);
+    max_iter = std::max(max_iter, f.max_iter);
+  }
   const int R = std::max(1, graph_replays);
   int64_t replays = 0;
-  while (replays < max_iter) {
-    const int64_t r = std::min<int64_t>(R, max_iter - replays);
-    for (int64_t i = 0; i < r; ++i) KMD_HIP_CHECK(hipGraphLaunch(exec, s));
+  // a fit with max_iter 0 (or converged) finishes at the first check
+  for (;;) {
+    const int64_t r = std::min<int64_t>(R, std::max<int64_t>(0, max_iter - replays));
+    const int64_t active = eng.step((int)r, true);
     replays += r;
-    KMD_HIP_CHECK(hipMemcpyAsync(host_st.data(), dS, sizeof(BState) * nf,
-                                 hipMemcpyDeviceToHost, s));
-    KMD_HIP_CHECK(hipStreamSynchronize(s));
-    bool any = false;
-    for (const BState &S : host_st) any |= S.active != 0;
-    if (!any) break;
+    if (active == 0) break;
   }
-
-  // epilogue: labels and weighted inertia of the final centroids
-  hipLaunchKernelGGL(bt_hn_krn, dim3(nf), dim3(kFB), 0, s, dF, dS, hn.as<float>(), 1);
-  KMD_LAUNCH_CHECK();
-  launch_assign_bucket<8>(dF, dS, ai[0], (int)aitems[0].size(), hn.as<float>(), 1, s);
-  launch_assign_bucket<16>(dF, dS, ai[1], (int)aitems[1].size(), hn.as<float>(), 1, s);
-  launch_assign_bucket<32>(dF, dS, ai[2], (int)aitems[2].size(), hn.as<float>(), 1, s);
-  hipLaunchKernelGGL(bt_inertia_krn, dim3(nci), dim3(256), 0, s, dF, dS,
-                     d_ci.as<CItem>(), dist.as<float>());
-  KMD_LAUNCH_CHECK();
-  hipLaunchKernelGGL(bt_sum1_krn, dim3((unsigned)map.size()), dim3(256), 0, s, dF,
-                     d_map.as<int2>(), d_nb.as<int>(), dist.as<float>(),
-                     d_part.as<double>());
-  KMD_LAUNCH_CHECK();
-  hipLaunchKernelGGL(bt_sum2_krn, dim3(nf), dim3(256), 0, s, d_nb.as<int>(),
-                     d_first.as<int>(), d_part.as<double>(), dS);
-  KMD_LAUNCH_CHECK();
-  hipLaunchKernelGGL(bt_copy_result_krn, dim3(nf), dim3(256), 0, s, dF, dS);
-  KMD_LAUNCH_CHECK();
-  KMD_HIP_CHECK(hipMemcpyAsync(host_st.data(), dS, sizeof(BState) * nf,
-                               hipMemcpyDeviceToHost, s));
-  KMD_HIP_CHECK(hipStreamSynchronize(s));
-  for (int f = 0; f < nf; ++f) {
-    in[f].out_iters[0] = host_st[f].it;
-    in[f].out_inertia[0] = host_st[f].inertia;
-  }
-  if (stats) {
-    stats->graph_replays = replays;
-  }
-  // the caller's stream sees the results after this call returns (the
-  // batch stream was synchronised); release the event
-  (void)hipEventDestroy(ev);
+  if (stats) stats->graph_replays = replays;
+  (void)hipStreamSynchronize(user_stream);
 }
 
 }  // namespace kmd

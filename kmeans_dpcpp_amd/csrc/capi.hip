@@ -3,6 +3,7 @@
 // enqueues the kernels on the caller's stream.  Exceptions never cross the C
 // boundary: ValueError -> KMD_EVALUE, anything else -> KMD_ERUNTIME, with the
 // message kept in a thread-local buffer for kmd_last_error().
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <string>
@@ -644,6 +645,101 @@ int kmd_kmeans_lloyd_batched(int64_t n_fits, kmd_fit *fits, int precision,
           nullptr);
     }
     if (graph_launches) *graph_launches = st.graph_replays;
+  });
+}
+
+struct kmd_engine {
+  kmd::BatchEngine *eng = nullptr;
+  std::vector<int64_t> pending;
+};
+
+int kmd_engine_create(kmd_engine_t *engine, kmd_stream_t stream) {
+  return guarded([&] {
+    require(engine != nullptr, "engine must not be null");
+    auto *e = new kmd_engine();
+    try {
+      e->eng = batch_engine_create(as_stream(stream));
+    } catch (...) {
+      delete e;
+      throw;
+    }
+    *engine = e;
+  });
+}
+
+int kmd_engine_destroy(kmd_engine_t engine) {
+  return guarded([&] {
+    if (engine == nullptr) return;
+    batch_engine_destroy(engine->eng);
+    delete engine;
+  });
+}
+
+int kmd_engine_footprint(int64_t n_samples, int64_t n_features,
+                         int64_t n_clusters, int64_t *bytes) {
+  return guarded([&] {
+    require(bytes != nullptr, "bytes must not be null");
+    require(n_samples >= 1 && n_features >= 1 && n_clusters >= 1,
+            "Array dimensions are not consistent");
+    *bytes = batch_fit_footprint(n_samples, n_features, n_clusters);
+  });
+}
+
+int kmd_engine_add(kmd_engine_t engine, kmd_fit *fit, int64_t *id) {
+  return guarded([&] {
+    require(engine != nullptr && fit != nullptr && id != nullptr,
+            "engine, fit and id must not be null");
+    require(fit->tol >= 0.0, "Tolerance must be non-negative");
+    require(fit->max_iter >= 0, "max_iter must be non-negative");
+    require(batched_fit_supported(fit->n_samples, fit->n_features, fit->n_clusters),
+            "engine fits need 1 <= n_features <= 64 and 1 <= n_clusters <= "
+            "min(256, n_samples)");
+    BatchedFit b;
+    b.n = fit->n_samples;
+    b.d = fit->n_features;
+    b.k = fit->n_clusters;
+    b.X = (const float *)fit->X_t;
+    b.w = (const float *)fit->sample_weight;
+    b.init = (float *)fit->init_centroids_t;
+    b.res = (float *)fit->res_centroids_t;
+    b.labels = (int *)fit->assignments_id;
+    b.tol = fit->tol;
+    b.max_iter = fit->max_iter;
+    b.out_iters = &fit->n_iters;
+    b.out_inertia = &fit->inertia;
+    *id = batch_engine_add(engine->eng, b);
+  });
+}
+
+int kmd_engine_step(kmd_engine_t engine, int iterations, int use_graph,
+                    int64_t *n_active) {
+  return guarded([&] {
+    require(engine != nullptr, "engine must not be null");
+    require(iterations >= 0, "iterations must be non-negative");
+    const int64_t a = batch_engine_step(engine->eng, iterations, use_graph != 0);
+    for (int64_t id : batch_engine_take_finished(engine->eng))
+      engine->pending.push_back(id);
+    if (n_active) *n_active = a;
+  });
+}
+
+int kmd_engine_take_finished(kmd_engine_t engine, int64_t *ids,
+                             int64_t capacity, int64_t *count) {
+  return guarded([&] {
+    require(engine != nullptr && count != nullptr, "engine and count must not be null");
+    require(capacity >= 0 && (capacity == 0 || ids != nullptr),
+            "ids must hold capacity entries");
+    const int64_t m = std::min<int64_t>(capacity, (int64_t)engine->pending.size());
+    for (int64_t i = 0; i < m; ++i) ids[i] = engine->pending[i];
+    engine->pending.erase(engine->pending.begin(), engine->pending.begin() + m);
+    *count = m;
+  });
+}
+
+int kmd_engine_resident_bytes(kmd_engine_t engine, int64_t *bytes) {
+  return guarded([&] {
+    require(engine != nullptr && bytes != nullptr, "engine and bytes must not be null");
+    *bytes = batch_engine_resident_bytes(engine->eng);
   });
 }
 

@@ -176,5 +176,21 @@ bool batched_fit_supported(int64_t n, int64_t d, int64_t k);
 // Runs all fits; results equal driver_lloyd<float, int32_t> fit by fit.
 void batched_lloyd(const std::vector<BatchedFit> &fits, int graph_replays,
                    hipStream_t s, BatchedStats *stats);
+// Device bytes the batch engine holds for one fit while it runs.
+int64_t batch_fit_footprint(int64_t n, int64_t d, int64_t k);
+// A running set of fits (continuous batching): add() joins a fit (its
+// arrays must stay valid until it finishes; inputs with inf/NaN raise a
+// ValueError and the fit is not added), step() advances all active fits
+// by `iterations` iterations and finishes the ones that stopped (results in
+// their out pointers), take_finished() lists them; their workspaces are
+// released at the next step.
+struct BatchEngine;
+BatchEngine *batch_engine_create(hipStream_t user_stream);
+void batch_engine_destroy(BatchEngine *e);
+int64_t batch_engine_add(BatchEngine *e, const BatchedFit &f);
+int64_t batch_engine_step(BatchEngine *e, int iterations, bool use_graph);
+std::vector<int64_t> batch_engine_take_finished(BatchEngine *e);
+int64_t batch_engine_resident_bytes(const BatchEngine *e);
+int64_t batch_engine_launches(const BatchEngine *e);
 
 }  // namespace kmd

@@ -18,6 +18,7 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -813,6 +814,114 @@ py::tuple py_kmeans_lloyd_batched(py::list fits, const std::string &precision,
   return py::make_tuple(out, stats);
 }
 
+// Continuous-batching engine: fits join a running set and advance together.
+class Engine {
+ public:
+  explicit Engine(py::object q) : q_(q) {
+    s_ = resolve_stream(q);
+    check_rc(kmd_engine_create(&e_, s_));
+  }
+  ~Engine() {
+    if (e_) kmd_engine_destroy(e_);
+  }
+  int64_t add(py::object X_, py::object w_, py::object init_, py::object lab_,
+              py::object res_, double tol, int64_t max_iter) {
+    auto ent = std::make_unique<Entry>();
+    ent->keep.push_back(to_arr(X_, s_, "X_t"));
+    ent->keep.push_back(to_arr(w_, s_, "sample_weight"));
+    ent->keep.push_back(to_arr(init_, s_, "init_centroid_t"));
+    ent->keep.push_back(to_arr(lab_, s_, "assignments_id"));
+    ent->keep.push_back(to_arr(res_, s_, "res_centroids_t"));
+    Arr &X = ent->keep[0], &w = ent->keep[1], &init = ent->keep[2],
+        &lab = ent->keep[3], &res = ent->keep[4];
+    if (X.ndim != 2 || w.ndim != 1 || init.ndim != 2 || res.ndim != 2 ||
+        lab.ndim != 1)
+      throw py::value_error("Unsupported array dimensionalities");
+    if (!all_contig({&X, &w, &init, &lab, &res}))
+      throw py::value_error("All input arrays must be C-contiguous");
+    if (!same_device({&X, &w, &init, &lab, &res}))
+      throw py::value_error(kQueueMsg);
+    const int64_t d = X.dim(0), n = X.dim(1), k = init.dim(1);
+    if (d != init.dim(0) || d != res.dim(0) || k != res.dim(1) ||
+        n != w.dim(0) || n != lab.dim(0))
+      throw py::value_error("Array dimensions are not consistent");
+    if (!X.is_float() || X.dtype() != KMD_F32 || !X.same_type(w) ||
+        !X.same_type(init) || !X.same_type(res) || !lab.is_int() ||
+        lab.itype() != KMD_I32)
+      throw py::value_error("engine fits take float32 data and int32 labels");
+    if (tol < 0.0) throw py::value_error("Tolerance must be non-negative");
+    if (max_iter < 0) throw py::value_error("max_iter must be non-negative");
+    kmd_fit &f = ent->fit;
+    f.n_samples = n;
+    f.n_features = d;
+    f.n_clusters = k;
+    f.X_t = X.data;
+    f.sample_weight = w.data;
+    f.init_centroids_t = init.data;
+    f.assignments_id = lab.data;
+    f.res_centroids_t = res.data;
+    f.tol = tol;
+    f.max_iter = max_iter;
+    use_device(X);
+    int64_t id = -1;
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = kmd_engine_add(e_, &ent->fit, &id);
+    }
+    check_rc(rc);
+    entries_[id] = std::move(ent);
+    return id;
+  }
+  int64_t step(int iterations, bool use_graph) {
+    int64_t active = 0;
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = kmd_engine_step(e_, iterations, use_graph ? 1 : 0, &active);
+    }
+    check_rc(rc);
+    return active;
+  }
+  py::list take_finished() {
+    py::list out;
+    for (;;) {
+      int64_t ids[64], cnt = 0;
+      check_rc(kmd_engine_take_finished(e_, ids, 64, &cnt));
+      for (int64_t i = 0; i < cnt; ++i) {
+        auto it = entries_.find(ids[i]);
+        if (it == entries_.end()) continue;
+        out.append(py::make_tuple(ids[i], (size_t)it->second->fit.n_iters,
+                                  it->second->fit.inertia));
+        entries_.erase(it);
+      }
+      if (cnt < 64) break;
+    }
+    return out;
+  }
+  int64_t resident_bytes() {
+    int64_t b = 0;
+    check_rc(kmd_engine_resident_bytes(e_, &b));
+    return b;
+  }
+  static int64_t footprint(int64_t n, int64_t d, int64_t k) {
+    int64_t b = 0;
+    check_rc(kmd_engine_footprint(n, d, k, &b));
+    return b;
+  }
+  size_t size() const { return entries_.size(); }
+
+ private:
+  struct Entry {
+    kmd_fit fit{};
+    std::vector<Arr> keep;
+  };
+  py::object q_;
+  kmd_stream_t s_ = nullptr;
+  kmd_engine_t e_ = nullptr;
+  std::map<int64_t, std::unique_ptr<Entry>> entries_;
+};
+
 PYBIND11_MODULE(_kmeans_dpcpp_amd, m) {
   m.doc() =
       "MI355X (gfx950) HIP implementation of the Lloyd k-means entry points "
@@ -948,6 +1057,23 @@ PYBIND11_MODULE(_kmeans_dpcpp_amd, m) {
         "Stable ascending sort of a float vector (NaN last) into output",
         py::arg("data"), py::arg("output"), py::arg("sycl_queue"),
         py::arg("depends") = py::list());
+  py::class_<Engine>(m, "Engine",
+                     "Continuous batching of Lloyd fits on one GPU: add() joins a "
+                     "fit to the running set, step() advances every active fit, "
+                     "take_finished() returns [(id, n_iters, inertia)] of fits that "
+                     "stopped.  Each fit's results equal kmeans_lloyd_driver's for it "
+                     "alone.")
+      .def(py::init<py::object>(), py::arg("sycl_queue") = py::none())
+      .def("add", &Engine::add, py::arg("X_t"), py::arg("sample_weight"),
+           py::arg("init_centroid_t"), py::arg("assignments_id"),
+           py::arg("res_centroids_t"), py::arg("tol"), py::arg("max_iter"))
+      .def("step", &Engine::step, py::arg("iterations") = 1,
+           py::arg("use_graph") = false)
+      .def("take_finished", &Engine::take_finished)
+      .def("resident_bytes", &Engine::resident_bytes)
+      .def("__len__", &Engine::size)
+      .def_static("footprint", &Engine::footprint, py::arg("n_samples"),
+                  py::arg("n_features"), py::arg("n_clusters"));
   m.def("kmeans_lloyd_batched", &py_kmeans_lloyd_batched,
         "Many independent Lloyd fits at once: fits is a list of (X_t, "
         "sample_weight, init_centroid_t, assignments_id, res_centroids_t, tol, "
