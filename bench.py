@@ -43,11 +43,16 @@ def parse():
     p.add_argument("--k", type=int, default=K)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
-    p.add_argument("--workload", default="c1", choices=["c1", "c3"],
+    p.add_argument("--workload", default="c1", choices=["c1", "c3", "c4"],
                    help="c1: one plain fit n=10M d=32 k=256 (the metric's "
                         "configuration, default); c3: 512 independent fits in "
                         "one batched call (fits/s), with the same fits run one "
-                        "by one through the single-fit driver beside it")
+                        "by one through the single-fit driver beside it; c4: "
+                        "a Poisson stream of fit requests served with "
+                        "continuous batching (fits/s, p50/p99 latency) beside "
+                        "the plain per-request FP32 driver")
+    p.add_argument("--rate", type=float, default=200.0, help="c4 arrivals per s")
+    p.add_argument("--requests", type=int, default=400, help="c4 requests")
     p.add_argument("--fits", type=int, default=512)
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="first run one device-to-device copy of this many MiB "
@@ -178,6 +183,116 @@ def bench_c3(args):
     print(json.dumps(line), flush=True)
 
 
+def make_c4_requests(n_req, rate, seed, device):
+    """Poisson arrivals; 95% of requests drawn like the batched fits above,
+    5% large (n=2,000,000, d=96, k=512); per-request seeds."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    reqs = []
+    for i in range(n_req):
+        if rng.uniform() < 0.05:
+            n, d, k = 2_000_000, 96, 512
+        else:
+            n = int(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
+            d = int(rng.choice([16, 32, 64]))
+            k = int(rng.choice([8, 32, 64, 128, 256]))
+            while k >= n / 10:
+                k = int(rng.choice([8, 32, 64, 128, 256]))
+        g = torch.Generator(device=device)
+        g.manual_seed(seed * 100_003 + i)
+        centers = torch.rand((k, d), generator=g, device=device) * 40.0 - 20.0
+        std = torch.rand((k,), generator=g, device=device) * 1.5 + 0.5
+        blob = torch.randint(0, k, (n,), generator=g, device=device)
+        X_t = torch.empty((d, n), device=device)
+        for s0 in range(0, n, 1 << 20):
+            e = min(n, s0 + (1 << 20))
+            z = torch.randn((e - s0, d), generator=g, device=device)
+            X_t[:, s0:e] = (centers[blob[s0:e]] + z * std[blob[s0:e], None]).T
+        init = X_t[:, torch.randperm(n, generator=g, device=device)[:k]].contiguous()
+        reqs.append((float(t[i]), X_t, torch.ones(n, device=device), init))
+    return reqs
+
+
+def bench_c4(args):
+    import threading
+    import torch
+    import kmeans_dpcpp_amd as kd
+    from kmeans_dpcpp_amd import serving
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream()
+    reqs = make_c4_requests(args.requests, args.rate, 4001, dev)
+    torch.cuda.synchronize()
+    tol, max_iter = 1e-4, 100
+
+    # served: continuous batching; the large fits use BF16x3 + exact re-score
+    # (labels and centroids identical to FP32) on their own stream
+    budget = int(0.6 * torch.cuda.get_device_properties(0).total_memory)
+    out = []
+    with serving.Server(budget, iterations_per_tick=1, large_precision="bf16x3") as srv:
+        t0 = time.perf_counter()
+        for (ta, X_t, w, init) in reqs:
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            r = srv.submit(X_t, w, init, tol=tol, max_iter=max_iter)
+            out.append(r)
+        for r in out:
+            r.result(timeout=600)
+        el = max(r.t_done for r in out) - t0
+    lat = serving.latency_summary(out)
+    # plain per-request FP32 form: one fit at a time, first come first served
+    plain_lat, same = [], True
+    t0 = time.perf_counter()
+    for (ta, X_t, w, init), r in zip(reqs, out):
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        lab = torch.empty(X_t.shape[1], dtype=torch.int32, device=dev)
+        res = torch.empty_like(init)
+        it, inertia = kd.kmeans_lloyd_driver(X_t, w, init.clone(), lab, res, tol, False,
+                                             max_iter, 8, 128, 0.7, stream)
+        torch.cuda.synchronize()
+        plain_lat.append(time.perf_counter() - t0 - ta)
+        same &= (it == r.n_iter) and bool(torch.equal(lab, r.labels)) and bool(
+            torch.equal(res.view(torch.int32), r.centroids.view(torch.int32)))
+    el_plain = time.perf_counter() - t0
+    plain_lat.sort()
+
+    def pct(v, p):
+        return v[min(len(v) - 1, int(round(p / 100 * (len(v) - 1))))]
+    line = {
+        "metric": METRIC,
+        "value": round(len(out) / el, 3),
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(1e3 * el, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"served stream: {len(out)} Poisson arrivals at "
+                               f"{args.rate:g} req/s, 95% batched-fit sizes, 5% "
+                               f"n=2M d=96 k=512; tol 1e-4, max_iter 100; "
+                               f"continuous batching, footprint admission"},
+        "latency_s": {"p50": round(lat["p50"], 4), "p99": round(lat["p99"], 4),
+                      "mean": round(lat["mean"], 4)},
+        "plain_per_request_fp32": {
+            "fits_per_s": round(len(out) / el_plain, 3),
+            "latency_s": {"p50": round(pct(plain_lat, 50), 4),
+                          "p99": round(pct(plain_lat, 99), 4)}},
+        "speedup_fits_per_s": round(el_plain / el, 2),
+        "engine_ticks": srv.ticks,
+        "peak_resident_gb": round(srv.peak_resident / 1e9, 2),
+        "identical_to_plain": bool(same),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def traffic_from_profile(path):
     """HBM bytes per fused-step launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
@@ -235,6 +350,9 @@ def main():
         sys.exit(2)
     if args.workload == "c3":
         bench_c3(args)
+        return
+    if args.workload == "c4":
+        bench_c4(args)
         return
     import torch
     import kmeans_dpcpp_amd as kd
