@@ -307,6 +307,27 @@ int kmd_engine_take_finished(kmd_engine_t engine, int64_t *ids,
                              int64_t capacity, int64_t *count);
 int kmd_engine_resident_bytes(kmd_engine_t engine, int64_t *bytes);
 
+/* Launch autotuner (no counterpart in the reference, whose launch shape is
+ * the caller's work_group_size / centroids_window_height).  Times the
+ * driver's candidate plans for this problem -- LDS-resident or tiled
+ * centroids (where each serves the shape) x persistent blocks per CU
+ * {occupancy limit, 1, 2, 4} -- for `iters` Lloyd iterations from
+ * init_centroids_t (not modified) and records the fastest for
+ * (n_features, n_clusters, precision, log2 n_samples); later driver calls
+ * of that shape use it.  Results never depend on the plan.  Writes the best
+ * plan and up to `capacity` candidates (path: 1 LDS-resident, 2 tiled,
+ * 0 generic; blocks_per_cu 0 = occupancy limit; fused-step ms/launch). */
+int kmd_autotune(int dtype, int itype, int precision, int64_t n_samples,
+                 int64_t n_features, int64_t n_clusters, const void *X_t,
+                 const void *sample_weight, const void *init_centroids_t,
+                 int iters, int *best_path, int *best_blocks_per_cu,
+                 double *best_ms, int *cand_path, int *cand_blocks_per_cu,
+                 double *cand_ms, int64_t capacity, int64_t *n_candidates,
+                 kmd_stream_t stream);
+/* Process-wide override of the plan driver calls use (path -1 clears it:
+ * back to the autotuner's table, else the automatic choice). */
+int kmd_set_tuning(int path, int blocks_per_cu);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,6 +26,7 @@ from ._kmeans_dpcpp_amd import (  # noqa: E402
     Engine,
     Event,
     assignment,
+    autotune,
     broadcast_divide,
     compute_centroid_shifts_squared,
     compute_centroid_to_sample_distances,
@@ -43,6 +44,7 @@ from ._kmeans_dpcpp_amd import (  # noqa: E402
     reduce_vector_blocking,
     relocate_empty_clusters,
     select_samples_far_from_centroid,
+    set_tuning,
     version,
 )
 

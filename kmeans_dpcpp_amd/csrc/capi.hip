@@ -743,4 +743,52 @@ int kmd_engine_resident_bytes(kmd_engine_t engine, int64_t *bytes) {
   });
 }
 
+int kmd_autotune(int dtype, int itype, int precision, int64_t n_samples,
+                 int64_t n_features, int64_t n_clusters, const void *X_t,
+                 const void *sample_weight, const void *init_centroids_t,
+                 int iters, int *best_path, int *best_blocks_per_cu,
+                 double *best_ms, int *cand_path, int *cand_blocks_per_cu,
+                 double *cand_ms, int64_t capacity, int64_t *n_candidates,
+                 kmd_stream_t stream) {
+  return guarded([&] {
+    check_types(dtype, itype);
+    require(precision == KMD_PREC_FP32 || precision == KMD_PREC_BF16X3 ||
+                precision == KMD_PREC_BF16,
+            "precision must be KMD_PREC_FP32, KMD_PREC_BF16X3 or KMD_PREC_BF16");
+    require(n_samples >= 1 && n_features >= 1 && n_clusters >= 1,
+            "Array dimensions are not consistent");
+    require(iters >= 1, "iters must be positive");
+    require(capacity >= 0, "capacity must be non-negative");
+    std::vector<TuneResult> res;
+    dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      using I = std::remove_pointer_t<decltype(ip)>;
+      res = autotune_lloyd<T, I>(n_samples, n_features, n_clusters, (const T *)X_t,
+                                 (const T *)sample_weight,
+                                 (const T *)init_centroids_t, precision, iters,
+                                 as_stream(stream));
+    });
+    const TuneResult *best = res.empty() ? nullptr : &res[0];
+    for (const TuneResult &r : res)
+      if (r.ms > 0.0 && (best->ms <= 0.0 || r.ms < best->ms)) best = &r;
+    if (best_path) *best_path = best ? best->path : 0;
+    if (best_blocks_per_cu) *best_blocks_per_cu = best ? best->blocks_per_cu : 0;
+    if (best_ms) *best_ms = best ? best->ms : 0.0;
+    const int64_t m = std::min<int64_t>(capacity, (int64_t)res.size());
+    for (int64_t i = 0; i < m; ++i) {
+      if (cand_path) cand_path[i] = res[i].path;
+      if (cand_blocks_per_cu) cand_blocks_per_cu[i] = res[i].blocks_per_cu;
+      if (cand_ms) cand_ms[i] = res[i].ms;
+    }
+    if (n_candidates) *n_candidates = (int64_t)res.size();
+  });
+}
+
+int kmd_set_tuning(int path, int blocks_per_cu) {
+  return guarded([&] {
+    require(path >= -1 && path <= 2, "path must be -1, 0, 1 or 2");
+    set_tuning_override(path, blocks_per_cu);
+  });
+}
+
 }  // extern "C"

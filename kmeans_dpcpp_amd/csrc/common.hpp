@@ -56,6 +56,17 @@ int device_cu_count();
 // global_mem_cache_size, src/lloyd_single_step.hpp:23).
 int64_t device_l2_bytes(int device);
 
+// Launch tuning shared by the persistent kernels' planners: a cap on blocks
+// per CU (0 = the occupancy limit) and the path choice of the driver (0 =
+// automatic, 1 = LDS-resident kernels, 2 = tiled kernels).  Set by the
+// autotuner (driver.hip) or kmd_set_tuning; read when a launch is planned.
+int tuned_blocks_per_cu();
+int tuned_path();
+inline int apply_blocks_cap(int per_cu) {
+  const int cap = tuned_blocks_per_cu();
+  return (cap > 0 && per_cu > cap) ? cap : per_cu;
+}
+
 // Stream-ordered temporary device buffer (hipMallocAsync / hipFreeAsync).
 // Freed in stream order when the object is destroyed, so work enqueued on the
 // same stream before destruction may still use it.

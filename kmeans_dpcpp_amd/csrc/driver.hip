@@ -25,8 +25,12 @@
 //    labels (the reference evaluates it against the undivided sums).
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <sstream>
+#include <tuple>
+#include <vector>
 
 #include "common.hpp"
 #include "lloyd_fast.hpp"
@@ -148,6 +152,56 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_tiled_workspace(
   }
 }
 
+// ---- launch tuning ----------------------------------------------------------
+// The choice in force on this thread (read by the planners while a driver
+// call runs), the autotuner's table per (d, k, precision, log2 n) and a
+// process-wide override (kmd_set_tuning).
+namespace {
+thread_local int t_blocks_cap = 0;
+thread_local int t_path = 0;
+struct TuneChoice {
+  int path = 0, blocks_per_cu = 0;
+};
+struct TuneGuard {
+  int pb, pp;
+  explicit TuneGuard(const TuneChoice &c) : pb(t_blocks_cap), pp(t_path) {
+    t_blocks_cap = c.blocks_per_cu;
+    t_path = c.path;
+  }
+  ~TuneGuard() {
+    t_blocks_cap = pb;
+    t_path = pp;
+  }
+};
+std::mutex g_tune_mu;
+std::map<std::tuple<int64_t, int64_t, int, int>, TuneChoice> g_tune;
+bool g_override = false;
+TuneChoice g_override_choice;
+
+int log2_bucket(int64_t n) {
+  int b = 0;
+  while ((int64_t(1) << (b + 1)) <= n) ++b;
+  return b;
+}
+
+TuneChoice choose_tuning(int64_t n, int64_t d, int64_t k, int precision) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (g_override) return g_override_choice;
+  auto it = g_tune.find(std::make_tuple(d, k, precision, log2_bucket(n)));
+  return it == g_tune.end() ? TuneChoice{} : it->second;
+}
+}  // namespace
+
+int tuned_blocks_per_cu() { return t_blocks_cap; }
+int tuned_path() { return t_path; }
+
+void set_tuning_override(int path, int blocks_per_cu) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_override = path >= 0;
+  g_override_choice.path = path < 0 ? 0 : path;
+  g_override_choice.blocks_per_cu = blocks_per_cu < 0 ? 0 : blocks_per_cu;
+}
+
 // Test hook: KMD_FORCE_TILED=1 selects the tiled path for every shape it
 // serves (the LDS-resident kernels are otherwise preferred).
 static bool force_tiled() {
@@ -163,7 +217,7 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_workspace(int64_t n,
                                                             const T *w,
                                                             int precision,
                                                             hipStream_t s) {
-  const bool tiled_first = force_tiled();
+  const bool tiled_first = force_tiled() || tuned_path() == 2;
   if (precision != KMD_PREC_FP32 && !tiled_first) {
     auto ws = make_mixed_workspace<T, I>(
         n, d, k, precision == KMD_PREC_BF16X3 ? 3 : 1, X, w, s);
@@ -332,11 +386,81 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
 }
 
 template <typename T, typename I>
+static void driver_impl(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
+                        T *init, I *labels, T *res, double tol, bool verbose,
+                        int64_t max_iter, int64_t *n_iters_out,
+                        double *inertia_out, kmd_print_fn print_fn,
+                        void *print_user, hipStream_t s, int precision,
+                        DriverStats *stats);
+
+template <typename T, typename I>
 void driver_lloyd(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
                   T *init, I *labels, T *res, double tol, bool verbose,
                   int64_t max_iter, int64_t *n_iters_out, double *inertia_out,
                   kmd_print_fn print_fn, void *print_user, hipStream_t s,
                   int precision, DriverStats *stats) {
+  TuneGuard g(choose_tuning(n, d, k, precision));
+  driver_impl<T, I>(n, d, k, X, w, init, labels, res, tol, verbose, max_iter,
+                    n_iters_out, inertia_out, print_fn, print_user, s, precision,
+                    stats);
+}
+
+// Times the driver's candidate launch plans for this problem and records the
+// fastest for its (d, k, precision, log2 n).  Candidates: LDS-resident or
+// tiled centroids (where each serves the shape) x persistent blocks per CU
+// {occupancy limit, 1, 2, 4}; each runs `iters` Lloyd iterations from the
+// given init on scratch copies, timed by the fused step's HIP events.  The
+// choice never changes results (every plan computes the same labels and
+// exact sums).
+template <typename T, typename I>
+std::vector<TuneResult> autotune_lloyd(int64_t n, int64_t d, int64_t k,
+                                       const T *X, const T *w, const T *init,
+                                       int precision, int iters, hipStream_t s) {
+  std::vector<TuneResult> out;
+  if (n <= 0) return out;
+  std::vector<int> paths;
+  if (fast_fused_supported<T>(d, k)) paths.push_back(1);
+  if (std::is_same<T, float>::value && tiled_supported(d, k)) paths.push_back(2);
+  if (paths.empty()) paths.push_back(0);
+  TempBuffer c0(sizeof(T) * d * k, s), c1(sizeof(T) * d * k, s),
+      lab(sizeof(I) * n, s);
+  for (int path : paths) {
+    for (int cap : {0, 1, 2, 4}) {
+      KMD_HIP_CHECK(hipMemcpyAsync(c0.as<T>(), init, sizeof(T) * d * k,
+                                   hipMemcpyDeviceToDevice, s));
+      TuneGuard g(TuneChoice{path, cap});
+      DriverStats st;
+      int64_t it = 0;
+      double inertia = 0.0;
+      driver_impl<T, I>(n, d, k, X, w, c0.as<T>(), lab.as<I>(), c1.as<T>(), 0.0,
+                        false, iters, &it, &inertia, nullptr, nullptr, s,
+                        precision, &st);
+      TuneResult r;
+      r.path = path;
+      r.blocks_per_cu = cap;
+      r.ms = st.fused_launches > 0 ? st.fused_kernel_ms / st.fused_launches : 0.0;
+      out.push_back(r);
+    }
+  }
+  const TuneResult *best = &out[0];
+  for (const TuneResult &r : out)
+    if (r.ms > 0.0 && (best->ms <= 0.0 || r.ms < best->ms)) best = &r;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[std::make_tuple(d, k, precision, log2_bucket(n))] =
+        TuneChoice{best->path, best->blocks_per_cu};
+  }
+  KMD_HIP_CHECK(hipStreamSynchronize(s));
+  return out;
+}
+
+template <typename T, typename I>
+static void driver_impl(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
+                        T *init, I *labels, T *res, double tol, bool verbose,
+                        int64_t max_iter, int64_t *n_iters_out,
+                        double *inertia_out, kmd_print_fn print_fn,
+                        void *print_user, hipStream_t s, int precision,
+                        DriverStats *stats) {
   auto ws_ptr = make_workspace<T, I>(n, d, k, X, w, precision, s);
   FusedTimer timer(stats != nullptr);
   auto &ws = *ws_ptr;
@@ -441,7 +565,10 @@ void driver_lloyd(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
                                    const T *, T *, I *, T *, double, bool,    \
                                    int64_t, int64_t *, double *,              \
                                    kmd_print_fn, void *, hipStream_t, int,    \
-                                   DriverStats *);
+                                   DriverStats *);                            \
+  template std::vector<TuneResult> autotune_lloyd<T, I>(                     \
+      int64_t, int64_t, int64_t, const T *, const T *, const T *, int, int,   \
+      hipStream_t);
 KMD_INST_DRIVER(float, int32_t)
 KMD_INST_DRIVER(float, int64_t)
 KMD_INST_DRIVER(double, int32_t)

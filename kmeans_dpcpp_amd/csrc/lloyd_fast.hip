@@ -486,6 +486,7 @@ MfmaLaunch plan_launch(int64_t n, int64_t d, int64_t k) {
   int per_cu = 0;
   KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
       &per_cu, reinterpret_cast<const void *>(L.fn), L.threads, L.lds));
+  per_cu = apply_blocks_cap(per_cu);
   if (per_cu < 1) {
     L.fn = nullptr;
     return L;
@@ -666,6 +667,7 @@ MfmaLaunch plan_rescore(int64_t d, int64_t k) {
   int per_cu = 0;
   KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
       &per_cu, reinterpret_cast<const void *>(L.fn), L.threads, L.lds));
+  per_cu = apply_blocks_cap(per_cu);
   if (per_cu < 1) {
     L.fn = nullptr;
     return L;

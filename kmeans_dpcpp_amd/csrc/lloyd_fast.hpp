@@ -14,6 +14,19 @@ struct DriverStats {
   int64_t n_rescored = 0;       // samples re-scored in FP32 (mixed path)
 };
 
+// One autotuner candidate: path (1 LDS-resident, 2 tiled, 0 generic),
+// persistent blocks per CU (0 = occupancy limit), fused-step ms per launch.
+struct TuneResult {
+  int path = 0, blocks_per_cu = 0;
+  double ms = 0.0;
+};
+template <typename T, typename I>
+std::vector<TuneResult> autotune_lloyd(int64_t n, int64_t d, int64_t k,
+                                       const T *X, const T *w, const T *init,
+                                       int precision, int iters, hipStream_t s);
+// Process-wide override of the tuning choice (path < 0 clears it).
+void set_tuning_override(int path, int blocks_per_cu);
+
 // Launch plan of a persistent fused-step kernel: grid size, dynamic LDS and
 // whether centroid sums are privatised in LDS.
 struct FusedPlan {

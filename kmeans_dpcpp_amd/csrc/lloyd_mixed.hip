@@ -473,6 +473,7 @@ int mixed_blocks(int64_t n, int64_t d, int64_t k, int T) {
   int per_cu = 0;
   KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
       &per_cu, reinterpret_cast<const void *>(fn), nt_of(d_bucket(d)), lds));
+  per_cu = apply_blocks_cap(per_cu);
   if (per_cu < 1) return 0;
   const int64_t need =
       ceil_div<int64_t>(ceil_div<int64_t>(n > 0 ? n : 1, 32 * kS),

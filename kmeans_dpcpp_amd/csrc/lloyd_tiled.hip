@@ -769,6 +769,7 @@ int occupancy_grid(F fn, size_t lds) {
   }
   int per_cu = 0;
   KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p, kTNT, lds));
+  per_cu = apply_blocks_cap(per_cu);
   if (per_cu < 1) return 0;
   return device_cu_count() * per_cu;
 }

@@ -922,6 +922,49 @@ class Engine {
   std::map<int64_t, std::unique_ptr<Entry>> entries_;
 };
 
+py::dict py_autotune(py::object X_, py::object w_, py::object init_,
+                     const std::string &precision, int iters, py::object q) {
+  kmd_stream_t s = resolve_stream(q);
+  Arr X = to_arr(X_, s, "X_t"), w = to_arr(w_, s, "sample_weight");
+  Arr init = to_arr(init_, s, "init_centroid_t");
+  if (X.ndim != 2 || w.ndim != 1 || init.ndim != 2)
+    throw py::value_error("Unsupported array dimensionalities");
+  if (!all_contig({&X, &w, &init}))
+    throw py::value_error("All input arrays must be C-contiguous");
+  if (!X.is_float() || !X.same_type(w) || !X.same_type(init))
+    throw py::value_error("Unsupport elemental data type");
+  const int64_t d = X.dim(0), n = X.dim(1), k = init.dim(1);
+  if (d != init.dim(0) || n != w.dim(0))
+    throw py::value_error("Array dimensions are not consistent");
+  int prec = precision == "fp32" ? KMD_PREC_FP32
+             : precision == "bf16x3" ? KMD_PREC_BF16X3
+             : precision == "bf16" ? KMD_PREC_BF16 : -1;
+  if (prec < 0) throw py::value_error("precision must be 'fp32', 'bf16x3' or 'bf16'");
+  use_device(X);
+  int bp = 0, bb = 0;
+  double bms = 0.0;
+  int cp[16], cb[16];
+  double cm[16];
+  int64_t nc = 0;
+  int rc;
+  {
+    py::gil_scoped_release nogil;
+    rc = kmd_autotune(X.dtype(), KMD_I32, prec, n, d, k, X.data, w.data, init.data,
+                      iters, &bp, &bb, &bms, cp, cb, cm, 16, &nc, s);
+  }
+  check_rc(rc);
+  static const char *kPath[] = {"generic", "lds-resident", "tiled"};
+  py::list cands;
+  for (int64_t i = 0; i < std::min<int64_t>(nc, 16); ++i)
+    cands.append(py::make_tuple(kPath[cp[i]], cb[i], cm[i]));
+  py::dict out;
+  out["path"] = kPath[bp];
+  out["blocks_per_cu"] = bb;
+  out["ms"] = bms;
+  out["candidates"] = cands;
+  return out;
+}
+
 PYBIND11_MODULE(_kmeans_dpcpp_amd, m) {
   m.doc() =
       "MI355X (gfx950) HIP implementation of the Lloyd k-means entry points "
@@ -1074,6 +1117,20 @@ PYBIND11_MODULE(_kmeans_dpcpp_amd, m) {
       .def("__len__", &Engine::size)
       .def_static("footprint", &Engine::footprint, py::arg("n_samples"),
                   py::arg("n_features"), py::arg("n_clusters"));
+  m.def("autotune", &py_autotune,
+        "Time the driver's launch plans (LDS-resident or tiled centroids x "
+        "persistent blocks per CU) for this problem and keep the fastest for "
+        "later fits of its shape; results never depend on the plan.  Returns "
+        "{'path', 'blocks_per_cu', 'ms', 'candidates'}.",
+        py::arg("X_t"), py::arg("sample_weight"), py::arg("init_centroid_t"),
+        py::arg("precision") = "fp32", py::arg("iters") = 3,
+        py::arg("sycl_queue") = py::none());
+  m.def("set_tuning",
+        [](int path, int blocks_per_cu) { check_rc(kmd_set_tuning(path, blocks_per_cu)); },
+        "Override the launch plan of driver calls: path 0 automatic, 1 "
+        "LDS-resident, 2 tiled (-1 clears the override); blocks_per_cu 0 = "
+        "occupancy limit.",
+        py::arg("path"), py::arg("blocks_per_cu") = 0);
   m.def("kmeans_lloyd_batched", &py_kmeans_lloyd_batched,
         "Many independent Lloyd fits at once: fits is a list of (X_t, "
         "sample_weight, init_centroid_t, assignments_id, res_centroids_t, tol, "

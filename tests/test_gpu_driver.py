@@ -299,3 +299,35 @@ def test_assignment_wide_shape_matches_c_restatement(kd, torch_dev):
     bad = got != ref
     assert (margin[bad] < 1e-3).all(), margin[bad]
     assert bad.mean() < 1e-3
+
+
+# ---- launch plans and the autotuner
+
+def test_launch_plans_do_not_change_results(kd, torch_dev):
+    """Every plan the autotuner can pick -- LDS-resident or tiled centroids,
+    any cap on persistent blocks per CU -- gives the same fit bit for bit."""
+    Xt, init = lo.gaussian_blobs(50_000, 32, 64, seed=61, std=1.5)
+    w = np.random.default_rng(2).uniform(0.5, 2.0, 50_000).astype(np.float32)
+    try:
+        for prec in ("fp32", "bf16x3"):
+            runs = []
+            for path, cap in ((1, 0), (1, 1), (2, 0), (2, 2)):
+                kd.set_tuning(path, cap)
+                runs.append(run_driver_ex(kd, Xt, w, init, 1e-4, 12, prec))
+            assert runs[0][4]["path"].startswith(("fp32", "bf16"))
+            assert runs[2][4]["path"].startswith("tiled")
+            for r in runs[1:]:
+                same_fit(r, runs[0])
+    finally:
+        kd.set_tuning(-1)
+
+
+def test_autotune_records_a_plan_the_driver_uses(kd, torch_dev):
+    Xt, init = lo.gaussian_blobs(40_000, 20, 50, seed=62, std=1.0)
+    w = np.ones(40_000, np.float32)
+    r = kd.autotune(to_dev(Xt), to_dev(w), to_dev(init), "fp32", 2, q())
+    assert r["path"] in ("lds-resident", "tiled")
+    assert len(r["candidates"]) == 8 and all(c[2] > 0 for c in r["candidates"])
+    assert r["ms"] == min(c[2] for c in r["candidates"])
+    got = run_driver_ex(kd, Xt, w, init, 1e-4, 5, "fp32")
+    assert got[4]["path"] == ("fp32-mfma" if r["path"] == "lds-resident" else "tiled-fp32")
