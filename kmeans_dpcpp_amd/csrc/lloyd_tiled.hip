@@ -41,11 +41,32 @@ typedef unsigned long long u64;
 
 constexpr int kTNT = 512;                 // threads per block (8 waves)
 constexpr int kTW = kTNT / kWave;         // waves per block
-constexpr int kChunkBytes = 64 * 1024;    // LDS bytes of one centroid chunk
+constexpr int kChunkBytes = 32 * 1024;    // LDS bytes of one centroid chunk
+                                          // (two of them: double buffered)
 constexpr int kMaxSegments = 1024;        // queue segments the re-score reads
 constexpr int kMaxTiledD = 128;
 constexpr int kMembers = 256;             // members per update work item
 constexpr size_t kLdsPerCu = 160 * 1024;
+
+// Copies `bytes` from global `src` (16-byte aligned, readable for 1 KB past
+// `bytes`) to LDS `dst` with global_load_lds_dwordx4: wave w moves the 1 KB
+// pieces w, w + kTW, ... (64 lanes x 16 B, lane l's bytes landing at
+// piece * 1 KB + 16 l).  Does not wait: the caller waits (vmcnt) and
+// synchronises before reading `dst`.
+__device__ inline void lds_dma(const unsigned char *src, unsigned char *dst,
+                               int bytes, int wave, int lane) {
+  const int pieces = (bytes + 1023) >> 10;
+  for (int p = wave; p < pieces; p += kTW)
+    __builtin_amdgcn_global_load_lds(
+        (const void *)(src + (size_t)p * 1024 + lane * 16),
+        (__attribute__((address_space(3))) void *)(dst + (size_t)p * 1024), 16, 0,
+        0);
+}
+
+__device__ inline void wait_dma_and_sync() {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+}
 
 // ---- FP32 assignment --------------------------------------------------------
 
@@ -76,8 +97,8 @@ constexpr int fp32_s() {
 struct TiledArgs {
   int64_t n;
   int d, k, n_ct;
-  const float *X;  // X_t (d, n); gather mode: X_r (n, d)
-  const float *C, *hn;
+  const float *X;      // X_t (d, n); gather mode: X_r (n, d)
+  const float *tiles;  // [n_ct][tile_floats] in the LDS layout (+1 KB pad)
   void *labels;
   const int *queue, *queue_counts;  // gather mode
   int n_segments;
@@ -93,9 +114,9 @@ __global__ void __launch_bounds__(kTNT) tiled_fp32_krn(TiledArgs a) {
   constexpr int S = fp32_s<KS>();
   constexpr int TF = tile_floats<KS>();
   constexpr int CPC = fp32_tiles_per_chunk<KS>();
+  constexpr int kBuf = ((CPC * TF * 4 + 1023) / 1024) * 1024;  // bytes per buffer
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  float *Cl = reinterpret_cast<float *>(lds_raw);         // [CPC][TF]
-  int *seg_off = reinterpret_cast<int *>(Cl + CPC * TF);  // [n_segments + 1]
+  int *seg_off = reinterpret_cast<int *>(lds_raw + 2 * kBuf);  // [n_segments + 1]
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1), wave = tid / kWave;
   const int col = lane & 31, half = lane >> 5;
@@ -133,6 +154,14 @@ __global__ void __launch_bounds__(kTNT) tiled_fp32_krn(TiledArgs a) {
   const bool vec = !GATHER && S == 2 &&
                    ((reinterpret_cast<uintptr_t>(a.X) & 7u) == 0) && (n % 2 == 0);
 
+  // centroid chunks stream through two LDS buffers: chunk g + 1 is copied
+  // while chunk g is used (the sequence repeats every round)
+  const int n_chunks = (n_ct + CPC - 1) / CPC;
+  const int64_t total_chunks = rounds * n_chunks;
+  const unsigned char *tiles = reinterpret_cast<const unsigned char *>(a.tiles);
+  auto chunk_bytes = [&](int c) { return min(CPC, n_ct - c * CPC) * TF * 4; };
+  if (total_chunks > 0) lds_dma(tiles, lds_raw, chunk_bytes(0), wave, lane);
+  int64_t g = 0;
   for (int64_t r = 0; r < rounds; ++r) {
     const int64_t sb = r * per_round + (int64_t)blockIdx.x * kTW + wave;
     const bool active = sb < n_super;
@@ -191,26 +220,16 @@ __global__ void __launch_bounds__(kTNT) tiled_fp32_krn(TiledArgs a) {
       bestv[q] = __builtin_huge_valf();
       besti[q] = 0;
     }
-    for (int c0 = 0; c0 < n_ct; c0 += CPC) {
+    for (int ci = 0; ci < n_chunks; ++ci, ++g) {
+      const int c0 = ci * CPC;
       const int cnt = min(CPC, n_ct - c0);
-      __syncthreads();
-      for (int i = tid; i < cnt * TF; i += kTNT) {
-        const int t = i / TF, e = i - t * TF;
-        const int ct = c0 + t;
-        if (e < 2 * KS * 32) {
-          // source order: feature-major (consecutive threads read
-          // consecutive centroids of one feature row)
-          const int f = e >> 5, cc = e & 31;
-          const int c = ct * 32 + cc;
-          const int ks = f >> 1, hf = f & 1;
-          Cl[t * TF + ((ks >> 2) * 2 + hf) * 128 + cc * 4 + (ks & 3)] =
-              (f < d && c < k) ? a.C[(int64_t)f * k + c] : 0.f;
-        } else {
-          const int c = ct * 32 + (e - 2 * KS * 32);
-          Cl[t * TF + e] = (c < k) ? a.hn[c] : __builtin_huge_valf();
-        }
+      wait_dma_and_sync();
+      if (g + 1 < total_chunks) {
+        const int cn = (int)((g + 1) % n_chunks);
+        lds_dma(tiles + (size_t)cn * CPC * TF * 4, lds_raw + ((g + 1) & 1) * kBuf,
+                chunk_bytes(cn), wave, lane);
       }
-      __syncthreads();
+      const float *Cl = reinterpret_cast<const float *>(lds_raw + (g & 1) * kBuf);
       if (active) {
         for (int t = 0; t < cnt; ++t) {
           const float *At = Cl + t * TF;
@@ -266,6 +285,31 @@ __global__ void __launch_bounds__(kTNT) tiled_fp32_krn(TiledArgs a) {
   }
 }
 
+// Centroid tiles in the FP32 kernel's LDS layout, so that a chunk is one
+// contiguous copy: tile t holds C[2 ks + half][t*32 + col] at
+// ((ks / 4) * 2 + half) * 128 + col * 4 + ks % 4, then hn (+inf padded).
+template <int KS>
+__global__ void __launch_bounds__(256)
+    tiled_fp32_prep_krn(int d, int k, int n_ct, const float *__restrict__ C,
+                        const float *__restrict__ hn, float *__restrict__ tiles) {
+  constexpr int TF = tile_floats<KS>();
+  const int64_t gid = blockIdx.x * 256ll + threadIdx.x;
+  if (gid >= (int64_t)n_ct * TF) return;
+  const int t = (int)(gid / TF), e = (int)(gid - (int64_t)t * TF);
+  if (e < 2 * KS * 32) {
+    // source order: feature-major (consecutive threads read consecutive
+    // centroids of one feature row)
+    const int f = e >> 5, cc = e & 31;
+    const int c = t * 32 + cc;
+    const int ks = f >> 1, hf = f & 1;
+    tiles[(int64_t)t * TF + ((ks >> 2) * 2 + hf) * 128 + cc * 4 + (ks & 3)] =
+        (f < d && c < k) ? C[(int64_t)f * k + c] : 0.f;
+  } else {
+    const int c = t * 32 + (e - 2 * KS * 32);
+    tiles[(int64_t)t * TF + e] = (c < k) ? hn[c] : __builtin_huge_valf();
+  }
+}
+
 // ---- BF16 assignment with certified margin ---------------------------------
 
 inline int bf16_d_of(int64_t d) {
@@ -280,7 +324,7 @@ struct TiledMixedArgs {
   int64_t n;
   int d, k, n_ct, idx_bits, fp32_len;
   const float *X;        // X_t (d, n)
-  const bf16x8 *aop;     // [n_ct][KK][64]
+  const bf16x8 *aop;     // [n_ct][KK][64] (+1 KB pad)
   const float *bound;    // [2]: max ||c||, max hn
   void *labels;
   int *queue;            // [block][seg_cap]
@@ -302,9 +346,9 @@ __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
   constexpr int CPC = (kChunkBytes / (KK * 64 * 16)) < 1
                           ? 1
                           : kChunkBytes / (KK * 64 * 16);
+  constexpr int kBuf = CPC * KK * 64 * 16;  // bytes per buffer (1 KB multiple)
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  bf16x8 *Al = reinterpret_cast<bf16x8 *>(lds_raw);           // [CPC][KK][64]
-  int &q_used = *reinterpret_cast<int *>(Al + CPC * KK * 64);
+  int &q_used = *reinterpret_cast<int *>(lds_raw + 2 * kBuf);
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1), wave = tid / kWave;
   const int col = lane & 31, half = lane >> 5;
@@ -330,6 +374,14 @@ __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
   const int64_t n_super = ceil_div<int64_t>(n, kSuper);
   const int64_t per_round = (int64_t)gridDim.x * kTW;
   const int64_t rounds = ceil_div<int64_t>(n_super, per_round);
+  // A-operand chunks stream through two LDS buffers (chunk g + 1 copied
+  // while chunk g is used; the sequence repeats every round)
+  const int n_chunks = (n_ct + CPC - 1) / CPC;
+  const int64_t total_chunks = rounds * n_chunks;
+  const unsigned char *aop = reinterpret_cast<const unsigned char *>(a.aop);
+  auto chunk_bytes = [&](int c) { return min(CPC, n_ct - c * CPC) * KK * 1024; };
+  if (total_chunks > 0) lds_dma(aop, lds_raw, chunk_bytes(0), wave, lane);
+  int64_t g = 0;
   for (int64_t r = 0; r < rounds; ++r) {
     const int64_t sb = r * per_round + (int64_t)blockIdx.x * kTW + wave;
     const bool active = sb < n_super;
@@ -383,12 +435,16 @@ __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
       b1[q] = -__builtin_huge_valf();
       b2[q] = -__builtin_huge_valf();
     }
-    for (int c0 = 0; c0 < n_ct; c0 += CPC) {
+    for (int ci = 0; ci < n_chunks; ++ci, ++g) {
+      const int c0 = ci * CPC;
       const int cnt = min(CPC, n_ct - c0);
-      __syncthreads();
-      const bf16x8 *src = a.aop + (int64_t)c0 * KK * 64;
-      for (int i = tid; i < cnt * KK * 64; i += kTNT) Al[i] = src[i];
-      __syncthreads();
+      wait_dma_and_sync();
+      if (g + 1 < total_chunks) {
+        const int cn = (int)((g + 1) % n_chunks);
+        lds_dma(aop + (size_t)cn * CPC * KK * 1024, lds_raw + ((g + 1) & 1) * kBuf,
+                chunk_bytes(cn), wave, lane);
+      }
+      const bf16x8 *Al = reinterpret_cast<const bf16x8 *>(lds_raw + (g & 1) * kBuf);
       if (active) {
         for (int t = 0; t < cnt; ++t) {
           const bf16x8 *At = Al + t * KK * 64 + lane;
@@ -729,7 +785,34 @@ size_t fp32_lds_bytes(int64_t d, bool gather) {
   const int TF = 2 * KS * 32 + 32;
   int cpc = kChunkBytes / (4 * TF);
   if (cpc < 1) cpc = 1;
-  return (size_t)cpc * TF * 4 + (gather ? sizeof(int) * (kMaxSegments + 1) : 0);
+  const size_t buf = ((size_t)cpc * TF * 4 + 1023) / 1024 * 1024;
+  return 2 * buf + (gather ? sizeof(int) * (kMaxSegments + 1) : 0);
+}
+
+// FP32 centroid tiles in the kernel's LDS layout (+1 KB read-ahead pad).
+TempBuffer prep_fp32_tiles(int64_t d, int64_t k, const float *C, const float *hn,
+                           hipStream_t s) {
+  const int KS = ks_of(d);
+  const int TF = 2 * KS * 32 + 32;
+  const int n_ct = (int)ceil_div<int64_t>(k, 32);
+  const int64_t total = (int64_t)n_ct * TF;
+  TempBuffer t(sizeof(float) * (size_t)total + 1024, s);
+  const dim3 grid((unsigned)ceil_div<int64_t>(total, 256));
+  switch (KS) {
+    case 16:
+      hipLaunchKernelGGL(tiled_fp32_prep_krn<16>, grid, dim3(256), 0, s, (int)d,
+                         (int)k, n_ct, C, hn, t.as<float>());
+      break;
+    case 32:
+      hipLaunchKernelGGL(tiled_fp32_prep_krn<32>, grid, dim3(256), 0, s, (int)d,
+                         (int)k, n_ct, C, hn, t.as<float>());
+      break;
+    default:
+      hipLaunchKernelGGL(tiled_fp32_prep_krn<64>, grid, dim3(256), 0, s, (int)d,
+                         (int)k, n_ct, C, hn, t.as<float>());
+  }
+  KMD_LAUNCH_CHECK();
+  return t;
 }
 
 template <typename I>
@@ -749,7 +832,7 @@ size_t mixed_tiled_lds_bytes(int64_t d, int T) {
   const int KK = kk_of(bf16_d_of(d), T);
   int cpc = kChunkBytes / (KK * 64 * 16);
   if (cpc < 1) cpc = 1;
-  return (size_t)cpc * KK * 64 * 16 + 16;
+  return 2 * (size_t)cpc * KK * 64 * 16 + 16;
 }
 
 template <typename F>
@@ -810,14 +893,14 @@ void launch_tiled_assign(int64_t n, int64_t d, int64_t k, const float *X_t,
   const int S = ks_of(d) <= 32 ? 2 : 1;
   const int64_t need = ceil_div<int64_t>(ceil_div<int64_t>(n, 32 * S), kTW);
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(grid_max, need));
+  TempBuffer tiles = prep_fp32_tiles(d, k, C, hn, s);
   TiledArgs a{};
   a.n = n;
   a.d = (int)d;
   a.k = (int)k;
   a.n_ct = (int)ceil_div<int64_t>(k, 32);
   a.X = X_t;
-  a.C = C;
-  a.hn = hn;
+  a.tiles = tiles.as<const float>();
   a.labels = labels;
   hipLaunchKernelGGL(fn, dim3(blocks), dim3(kTNT), lds, s, a);
   KMD_LAUNCH_CHECK();
@@ -835,14 +918,14 @@ void launch_tiled_rescore(int64_t n, int64_t d, int64_t k, const float *X_r,
   const size_t lds = fp32_lds_bytes(d, true);
   const int grid = occupancy_grid(fn, lds);
   if (grid < 1) throw RuntimeError("tiled re-score: no launch plan");
+  TempBuffer tiles = prep_fp32_tiles(d, k, C, hn, s);
   TiledArgs a{};
   a.n = n;
   a.d = (int)d;
   a.k = (int)k;
   a.n_ct = (int)ceil_div<int64_t>(k, 32);
   a.X = X_r;
-  a.C = C;
-  a.hn = hn;
+  a.tiles = tiles.as<const float>();
   a.labels = labels;
   a.queue = queue;
   a.queue_counts = queue_counts;
@@ -876,7 +959,8 @@ int64_t tiled_mixed_seg_cap(int64_t n, int64_t d, int T, int blocks) {
 }
 
 size_t tiled_aop_bytes(int64_t d, int64_t k, int T) {
-  return (size_t)ceil_div<int64_t>(k, 32) * kk_of(bf16_d_of(d), T) * 64 * 16;
+  // + 1 KB: the chunk copies read whole 1 KB pieces
+  return (size_t)ceil_div<int64_t>(k, 32) * kk_of(bf16_d_of(d), T) * 64 * 16 + 1024;
 }
 
 void launch_tiled_mixed_prep(int64_t d, int64_t k, int T, const float *C,
