@@ -52,6 +52,7 @@ struct LloydWorkspace {
   TempBuffer aop{0, nullptr}, bound{0, nullptr}, queue{0, nullptr},
       queue_counts{0, nullptr};
   TempBuffer xr{0, nullptr}, upd{0, nullptr};  // tiled: X sample-major, update scratch
+  TempBuffer qmask{0, nullptr}, cand{0, nullptr};  // tiled BF16: candidate tiles
   int64_t seg_cap = 0;     // capacity of each block's re-score queue segment
   // copies_bytes / sizes_bytes: private copies of the generic kernel, or the
   // fast path's per-block partials (in `copies`, sizes_bytes = 0)
@@ -143,6 +144,10 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_tiled_workspace(
         ws->bound = TempBuffer(2 * sizeof(float), s);
         ws->queue = TempBuffer(sizeof(int) * (size_t)ba * ws->seg_cap, s);
         ws->queue_counts = TempBuffer(sizeof(int) * (size_t)ba, s);
+        if (tiled_cand_supported(k)) {
+          ws->qmask = TempBuffer(sizeof(unsigned) * (size_t)ba * ws->seg_cap, s);
+          ws->cand = TempBuffer(tiled_cand_scratch_bytes(n, k), s);
+        }
       }
     }
     return ws;
@@ -303,15 +308,26 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
                                 ws.hn.template as<const T>(),
                                 ws.aop.template as<void>(),
                                 ws.bound.template as<float>(), s);
+        const bool cand = ws.qmask.bytes() > 0;
         launch_tiled_mixed_assign<I>(n, d, k, ws.mixed_terms, ws.blocks_b, X,
                                      ws.aop.template as<const void>(),
                                      ws.bound.template as<const float>(), labels,
                                      ws.queue.template as<int>(),
-                                     ws.queue_counts.template as<int>(), s);
-        launch_tiled_rescore<I>(n, d, k, xr, cur, ws.hn.template as<const T>(),
-                                labels, ws.queue.template as<const int>(),
-                                ws.queue_counts.template as<const int>(),
-                                ws.blocks_b, ws.seg_cap, ws.queue_count(), s);
+                                     ws.queue_counts.template as<int>(),
+                                     cand ? ws.qmask.template as<unsigned>() : nullptr,
+                                     s);
+        if (cand)
+          launch_tiled_cand_rescore<I>(
+              n, d, k, xr, cur, ws.hn.template as<const T>(), labels,
+              ws.queue.template as<const int>(),
+              ws.queue_counts.template as<const int>(),
+              ws.qmask.template as<const unsigned>(), ws.blocks_b, ws.seg_cap,
+              ws.cand.template as<void>(), ws.queue_count(), s);
+        else
+          launch_tiled_rescore<I>(n, d, k, xr, cur, ws.hn.template as<const T>(),
+                                  labels, ws.queue.template as<const int>(),
+                                  ws.queue_counts.template as<const int>(),
+                                  ws.blocks_b, ws.seg_cap, ws.queue_count(), s);
       } else {
         launch_tiled_assign<I>(n, d, k, X, cur, ws.hn.template as<const T>(),
                                labels, s);

@@ -162,7 +162,20 @@ template <typename I>
 void launch_tiled_mixed_assign(int64_t n, int64_t d, int64_t k, int T,
                                int blocks, const float *X_t, const void *aop,
                                const float *bound, I *labels, int *queue,
-                               int *queue_counts, hipStream_t s);
+                               int *queue_counts, unsigned *qmask, hipStream_t s);
+// Re-score of the queued samples restricted to their candidate tiles (the
+// BF16 kernel's qmask; needs k <= 1024), with the FP32 kernels' arithmetic;
+// falls back to the full gather re-score when the (sample, tile) pairs
+// overflow the scratch.
+bool tiled_cand_supported(int64_t k);
+size_t tiled_cand_scratch_bytes(int64_t n, int64_t k);
+template <typename I>
+void launch_tiled_cand_rescore(int64_t n, int64_t d, int64_t k, const float *X_r,
+                               const float *C, const float *hn, I *labels,
+                               const int *queue, const int *queue_counts,
+                               const unsigned *qmask, int n_segments,
+                               int64_t seg_cap, void *scratch, int *n_rescored,
+                               hipStream_t s);
 // Exact int64 fixed-point sums [d*k + k] of w x per label from X_r
 size_t tiled_update_scratch_bytes(int64_t n, int64_t k);
 template <typename I>

@@ -104,6 +104,7 @@ struct TiledArgs {
   int n_segments;
   int64_t seg_cap;
   int *n_rescored;
+  const int *only_if;  // gather mode: run only when *only_if != 0 (or null)
 };
 
 // Tile layout in LDS: element (ks, half, col) = C[2 ks + half][tile*32 + col]
@@ -125,6 +126,7 @@ __global__ void __launch_bounds__(kTNT) tiled_fp32_krn(TiledArgs a) {
   I *labels = reinterpret_cast<I *>(a.labels);
 
   if constexpr (GATHER) {
+    if (a.only_if != nullptr && a.only_if[0] == 0) return;  // block-uniform
     if (tid < kWave) {
       int carry = 0;
       for (int g0 = 0; g0 < a.n_segments; g0 += kWave) {
@@ -310,6 +312,199 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ---- candidate-tile re-score (BF16 path, at most 32 centroid tiles) ---------
+// Each queued sample lists the centroid tiles that can hold its FP32 argmin;
+// (sample, tile) pairs are bucketed by tile and each wave re-scores 32 pairs
+// of one tile with the FP32 kernels' arithmetic, keeping the best (value,
+// index) of a sample with a 64-bit atomicMin on an order-preserving key
+// (ties: the lower index, as the FP32 path's strict < over ascending
+// centroids).  Tiles outside a sample's list hold only centroids certified
+// worse than its best, so the minimum over the listed tiles is the FP32
+// path's label.
+
+struct CandArgs {
+  int n_segments, n_ct, d;
+  int64_t seg_cap;
+  const int *queue, *queue_counts;
+  const unsigned *qmask;
+  int *tile_cnt, *tile_cursor, *tile_start, *tile_witem;  // [n_ct + 1]
+  int *pairs;
+  int64_t pair_cap;
+  int *overflow;                 // pairs did not fit: full re-score instead
+  unsigned long long *keys;      // [n] per sample
+  const float *Xr, *tiles;
+  void *labels;
+  int *n_rescored;
+};
+
+__device__ inline unsigned order_key(float v) {
+  const unsigned b = __float_as_uint(v + 0.0f);  // -0 -> +0
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__global__ void __launch_bounds__(256) cand_count_krn(CandArgs a) {
+  __shared__ int cnt[32];
+  const int seg = blockIdx.x;
+  if (threadIdx.x < 32) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int m = a.queue_counts[seg];
+  const int64_t base = (int64_t)seg * a.seg_cap;
+  for (int j = threadIdx.x; j < m; j += 256) {
+    unsigned bits = a.qmask[base + j];
+    a.keys[a.queue[base + j]] = ~0ull;
+    while (bits) {
+      const int t = __ffs(bits) - 1;
+      bits &= bits - 1u;
+      atomicAdd(&cnt[t], 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < a.n_ct && cnt[threadIdx.x] != 0)
+    atomicAdd(&a.tile_cnt[threadIdx.x], cnt[threadIdx.x]);
+}
+
+__global__ void cand_scan_krn(CandArgs a) {
+  if (threadIdx.x != 0) return;
+  int run = 0, wrun = 0;
+  for (int t = 0; t < a.n_ct; ++t) {
+    const int c = a.tile_cnt[t];
+    a.tile_start[t] = run;
+    a.tile_witem[t] = wrun;
+    run += c;
+    wrun += (c + 31) / 32;
+  }
+  a.tile_start[a.n_ct] = run;
+  a.tile_witem[a.n_ct] = wrun;
+  a.overflow[0] = (int64_t)run > a.pair_cap ? 1 : 0;
+  int q = 0;
+  for (int g = 0; g < a.n_segments; ++g) q += a.queue_counts[g];
+  a.n_rescored[0] = q;
+}
+
+__global__ void __launch_bounds__(256) cand_scatter_krn(CandArgs a) {
+  __shared__ int cnt[32];
+  if (a.overflow[0]) return;
+  const int seg = blockIdx.x;
+  if (threadIdx.x < 32) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int m = a.queue_counts[seg];
+  const int64_t base = (int64_t)seg * a.seg_cap;
+  for (int j = threadIdx.x; j < m; j += 256) {
+    unsigned bits = a.qmask[base + j];
+    while (bits) {
+      const int t = __ffs(bits) - 1;
+      bits &= bits - 1u;
+      atomicAdd(&cnt[t], 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < a.n_ct)
+    cnt[threadIdx.x] = cnt[threadIdx.x] != 0
+                           ? a.tile_start[threadIdx.x] +
+                                 atomicAdd(&a.tile_cursor[threadIdx.x], cnt[threadIdx.x])
+                           : 0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < m; j += 256) {
+    unsigned bits = a.qmask[base + j];
+    const int smp = a.queue[base + j];
+    while (bits) {
+      const int t = __ffs(bits) - 1;
+      bits &= bits - 1u;
+      a.pairs[atomicAdd(&cnt[t], 1)] = smp;
+    }
+  }
+}
+
+template <int KS>
+__global__ void __launch_bounds__(kTNT) cand_rescore_krn(CandArgs a) {
+  constexpr int TF = tile_floats<KS>();
+  if (a.overflow[0]) return;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int col = lane & 31, half = lane >> 5;
+  const int total = a.tile_witem[a.n_ct];
+  const int d = a.d;
+  for (int wi = blockIdx.x * kTW + wave; wi < total; wi += gridDim.x * kTW) {
+    int lo = 0, hi = a.n_ct - 1;  // the last tile whose first item is <= wi
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.tile_witem[mid] <= wi)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int t = lo;
+    const int j0 = a.tile_start[t] + (wi - a.tile_witem[t]) * 32;
+    const int cnt = min(32, a.tile_start[t + 1] - j0);
+    const bool valid = col < cnt;
+    const int smp = valid ? a.pairs[j0 + col] : 0;
+    float xr[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int f = 2 * ks + half;
+      xr[ks] = (valid && f < d) ? a.Xr[(int64_t)smp * d + f] : 0.f;
+    }
+    const float *At = a.tiles + (int64_t)t * TF;
+    f32x16 acc;
+#pragma unroll
+    for (int k4 = 0; k4 < KS / 4; ++k4) {
+      const f32x4 av =
+          *reinterpret_cast<const f32x4 *>(At + (k4 * 2 + half) * 128 + col * 4);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (k4 == 0 && j == 0) {
+          const f32x16 z = {};
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[0], z, 0, 0, 0);
+        } else {
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[4 * k4 + j], acc, 0,
+                                                     0, 0);
+        }
+      }
+    }
+    const float *Ht = At + 2 * KS * 32;
+    float bestv = __builtin_huge_valf();
+    int besti = 0;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int m0 = 8 * g + 4 * half;
+      const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + m0);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float v = h4[e] - acc[4 * g + e];
+        const bool upd = v < bestv;
+        bestv = upd ? v : bestv;
+        besti = upd ? (t * 32 + m0 + e) : besti;
+      }
+    }
+    const float ov = __shfl_xor(bestv, 32, kWave);
+    const int oi = __shfl_xor(besti, 32, kWave);
+    const bool take = (ov < bestv) || (ov == bestv && oi < besti);
+    bestv = take ? ov : bestv;
+    besti = take ? oi : besti;
+    if (half == 0 && valid && bestv < __builtin_huge_valf())
+      atomicMin(&a.keys[smp],
+                ((unsigned long long)order_key(bestv) << 32) | (unsigned)besti);
+  }
+}
+
+template <typename I>
+__global__ void __launch_bounds__(256) cand_finalize_krn(CandArgs a) {
+  if (a.overflow[0]) return;
+  const int seg = blockIdx.x;
+  const int m = a.queue_counts[seg];
+  const int64_t base = (int64_t)seg * a.seg_cap;
+  I *labels = reinterpret_cast<I *>(a.labels);
+  for (int j = threadIdx.x; j < m; j += 256) {
+    const int smp = a.queue[base + j];
+    const unsigned long long key = a.keys[smp];
+    // no finite score anywhere: label 0, as the FP32 path
+    labels[smp] = (key == ~0ull) ? (I)0 : (I)(unsigned)(key & 0xffffffffull);
+  }
+}
+
+__global__ void __launch_bounds__(256) fill_zero_i32_krn(int m, int *p) {
+  for (int i = threadIdx.x; i < m; i += 256) p[i] = 0;
+}
+
 // ---- BF16 assignment with certified margin ---------------------------------
 
 inline int bf16_d_of(int64_t d) {
@@ -329,6 +524,7 @@ struct TiledMixedArgs {
   void *labels;
   int *queue;            // [block][seg_cap]
   int *queue_counts;     // [block]
+  unsigned *qmask;       // [block][seg_cap]: candidate tiles (n_ct <= 32)
   int64_t seg_cap;
   float pinf;
 };
@@ -347,8 +543,12 @@ __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
                           ? 1
                           : kChunkBytes / (KK * 64 * 16);
   constexpr int kBuf = CPC * KK * 64 * 16;  // bytes per buffer (1 KB multiple)
+  // S == 1: each lane's per-tile best is kept in LDS (bf16, rounded up) so
+  // the candidate tiles are chosen against the final best
+  constexpr bool kTM = (S == 1);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   int &q_used = *reinterpret_cast<int *>(lds_raw + 2 * kBuf);
+  unsigned short *tml = reinterpret_cast<unsigned short *>(lds_raw + 2 * kBuf + 16);
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1), wave = tid / kWave;
   const int col = lane & 31, half = lane >> 5;
@@ -430,10 +630,20 @@ __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
     for (int j = 0; j < 8; ++j) bh[j] = (__bf16)((half == 0 && j < 3) ? -1.f : 0.f);
 
     float b1[S], b2[S];
+    // Candidate tiles of a sample: bit t is set when tile t's best (packed)
+    // score reaches this lane's running best minus 3 e_big, where e_big
+    // bounds the margin test's 2 (E1 + E2) term from above (|D| <=
+    // 1.01 (||x|| max||c|| + max hn)).  The running best never exceeds the
+    // final one, so every tile that can hold the FP32 argmin is set.
+    float ecut[S];
+    unsigned cmask[S];
 #pragma unroll
     for (int q = 0; q < S; ++q) {
       b1[q] = -__builtin_huge_valf();
       b2[q] = -__builtin_huge_valf();
+      const float big = 1.01f * (xnorm[q] * cmax + hnmax);
+      ecut[q] = 3.f * (xnorm[q] * cmax * alpha + hnmax * beta + 2.f * pack_err * big);
+      cmask[q] = 0u;
     }
     for (int ci = 0; ci < n_chunks; ++ci, ++g) {
       const int c0 = ci * CPC;
@@ -473,6 +683,9 @@ __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
             }
           }
           const int ct = c0 + t;
+          float tmax[S];
+#pragma unroll
+          for (int q = 0; q < S; ++q) tmax[q] = -__builtin_huge_valf();
 #pragma unroll
           for (int rr = 0; rr < 16; rr += 2) {
             const unsigned i0 =
@@ -486,8 +699,21 @@ __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
               const float m = __builtin_amdgcn_fmed3f(b1[q], uu, vv);
               b1[q] = max3f(b1[q], uu, vv);
               b2[q] = __builtin_amdgcn_fmed3f(b2[q], m, pinf);
+              tmax[q] = max3f(tmax[q], uu, vv);
             }
           }
+#pragma unroll
+          for (int q = 0; q < S; ++q)
+            if constexpr (kTM) {
+              if (a.qmask != nullptr && ct < 32) {
+                // round toward +inf to bf16: never below the tile's best
+                const unsigned b = __float_as_uint(tmax[q]);
+                const unsigned up = (b & 0x80000000u) ? (b >> 16) : ((b + 0xffffu) >> 16);
+                tml[(wave * 32 + ct) * kWave + lane] = (unsigned short)up;
+              }
+            } else {
+              if (tmax[q] >= b1[q] - ecut[q]) cmask[q] |= 1u << (ct & 31);
+            }
         }
       }
     }
@@ -507,6 +733,24 @@ __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
       const bool valid = active && s < n;
       if (half == 0 && valid && !amb) labels[s] = (I)lab;
       const bool enq = (half == 0) && valid && amb;
+      if constexpr (kTM) {
+        // candidate tiles against the final best: a centroid can be the FP32
+        // argmin only if its score is within 2 (E1 + E2) of the best; with
+        // the packing error of both values this is 2 e_part + 6 pack M
+        if (a.qmask != nullptr) {
+          const float big = 1.01f * (xnorm[q] * cmax + hnmax);
+          const float thr = b1[q] - (2.f * (xnorm[q] * cmax * alpha + hnmax * beta) +
+                                     6.f * pack_err * big);
+          unsigned bits = 0u;
+          const int nt = min(n_ct, 32);
+          for (int t = 0; t < nt; ++t) {
+            const float v = __uint_as_float((unsigned)tml[(wave * 32 + t) * kWave + lane] << 16);
+            if (v >= thr) bits |= 1u << t;
+          }
+          cmask[q] = bits;
+        }
+      }
+      const unsigned tiles_any = cmask[q] | (unsigned)__shfl_xor((int)cmask[q], 32, kWave);
       const uint64_t mask = __ballot(enq);
       if (mask != 0ull) {
         const int c = __popcll(mask);
@@ -517,6 +761,7 @@ __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
         if (enq) {
           const int rank = __popcll(mask & ((1ull << lane) - 1ull));
           qseg[base + rank] = (int)s;
+          if (a.qmask) a.qmask[(int64_t)blockIdx.x * a.seg_cap + base + rank] = tiles_any;
         }
       }
     }
@@ -832,7 +1077,9 @@ size_t mixed_tiled_lds_bytes(int64_t d, int T) {
   const int KK = kk_of(bf16_d_of(d), T);
   int cpc = kChunkBytes / (KK * 64 * 16);
   if (cpc < 1) cpc = 1;
-  return 2 * (size_t)cpc * KK * 64 * 16 + 16;
+  const int S = bf16_s(bf16_d_of(d), T);
+  return 2 * (size_t)cpc * KK * 64 * 16 + 16 +
+         (S == 1 ? (size_t)kTW * 32 * kWave * sizeof(unsigned short) : 0);
 }
 
 template <typename F>
@@ -906,34 +1153,115 @@ void launch_tiled_assign(int64_t n, int64_t d, int64_t k, const float *X_t,
   KMD_LAUNCH_CHECK();
 }
 
+namespace {
 template <typename I>
-void launch_tiled_rescore(int64_t n, int64_t d, int64_t k, const float *X_r,
-                          const float *C, const float *hn, I *labels,
-                          const int *queue, const int *queue_counts,
-                          int n_segments, int64_t seg_cap, int *n_rescored,
-                          hipStream_t s) {
+void launch_rescore_full(int64_t n, int64_t d, int64_t k, const float *X_r,
+                         const float *tiles, I *labels, const int *queue,
+                         const int *queue_counts, int n_segments, int64_t seg_cap,
+                         int *n_rescored, const int *only_if, hipStream_t s) {
   require(n_segments >= 1 && n_segments <= kMaxSegments,
           "tiled re-score: segment count out of range");
   TiledFn fn = pick_fp32<I, true>(d);
   const size_t lds = fp32_lds_bytes(d, true);
   const int grid = occupancy_grid(fn, lds);
   if (grid < 1) throw RuntimeError("tiled re-score: no launch plan");
-  TempBuffer tiles = prep_fp32_tiles(d, k, C, hn, s);
   TiledArgs a{};
   a.n = n;
   a.d = (int)d;
   a.k = (int)k;
   a.n_ct = (int)ceil_div<int64_t>(k, 32);
   a.X = X_r;
-  a.tiles = tiles.as<const float>();
+  a.tiles = tiles;
   a.labels = labels;
   a.queue = queue;
   a.queue_counts = queue_counts;
   a.n_segments = n_segments;
   a.seg_cap = seg_cap;
   a.n_rescored = n_rescored;
+  a.only_if = only_if;
   hipLaunchKernelGGL(fn, dim3(grid), dim3(kTNT), lds, s, a);
   KMD_LAUNCH_CHECK();
+}
+}  // namespace
+
+template <typename I>
+void launch_tiled_rescore(int64_t n, int64_t d, int64_t k, const float *X_r,
+                          const float *C, const float *hn, I *labels,
+                          const int *queue, const int *queue_counts,
+                          int n_segments, int64_t seg_cap, int *n_rescored,
+                          hipStream_t s) {
+  TempBuffer tiles = prep_fp32_tiles(d, k, C, hn, s);
+  launch_rescore_full<I>(n, d, k, X_r, tiles.as<const float>(), labels, queue,
+                         queue_counts, n_segments, seg_cap, n_rescored, nullptr, s);
+}
+
+bool tiled_cand_supported(int64_t k) { return ceil_div<int64_t>(k, 32) <= 32; }
+
+size_t tiled_cand_scratch_bytes(int64_t n, int64_t k) {
+  const int64_t n_ct = ceil_div<int64_t>(k, 32);
+  return sizeof(int) * (size_t)(4 * (n_ct + 1) + 4) +
+         sizeof(int) * (size_t)(8 * (n > 0 ? n : 1)) +
+         sizeof(unsigned long long) * (size_t)(n > 0 ? n : 1) + 512;
+}
+
+template <typename I>
+void launch_tiled_cand_rescore(int64_t n, int64_t d, int64_t k, const float *X_r,
+                               const float *C, const float *hn, I *labels,
+                               const int *queue, const int *queue_counts,
+                               const unsigned *qmask, int n_segments,
+                               int64_t seg_cap, void *scratch, int *n_rescored,
+                               hipStream_t s) {
+  require(tiled_cand_supported(k), "candidate re-score: more than 32 tiles");
+  const int n_ct = (int)ceil_div<int64_t>(k, 32);
+  TempBuffer tiles = prep_fp32_tiles(d, k, C, hn, s);
+  CandArgs a{};
+  a.n_segments = n_segments;
+  a.n_ct = n_ct;
+  a.d = (int)d;
+  a.seg_cap = seg_cap;
+  a.queue = queue;
+  a.queue_counts = queue_counts;
+  a.qmask = qmask;
+  int *ip = reinterpret_cast<int *>(scratch);
+  a.tile_cnt = ip;
+  a.tile_cursor = ip + (n_ct + 1);
+  a.tile_start = ip + 2 * (n_ct + 1);
+  a.tile_witem = ip + 3 * (n_ct + 1);
+  a.overflow = ip + 4 * (n_ct + 1);
+  a.pairs = ip + 4 * (n_ct + 1) + 4;
+  a.pair_cap = 8 * (n > 0 ? n : 1);
+  a.keys = reinterpret_cast<unsigned long long *>(
+      (reinterpret_cast<uintptr_t>(a.pairs + a.pair_cap) + 255) & ~uintptr_t(255));
+  a.Xr = X_r;
+  a.tiles = tiles.as<const float>();
+  a.labels = labels;
+  a.n_rescored = n_rescored;
+  hipLaunchKernelGGL(fill_zero_i32_krn, dim3(1), dim3(256), 0, s, 2 * (n_ct + 1),
+                     a.tile_cnt);
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(cand_count_krn, dim3(n_segments), dim3(256), 0, s, a);
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(cand_scan_krn, dim3(1), dim3(64), 0, s, a);
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(cand_scatter_krn, dim3(n_segments), dim3(256), 0, s, a);
+  KMD_LAUNCH_CHECK();
+  const int grid = device_cu_count() * 2;
+  switch (ks_of(d)) {
+    case 16:
+      hipLaunchKernelGGL(cand_rescore_krn<16>, dim3(grid), dim3(kTNT), 0, s, a);
+      break;
+    case 32:
+      hipLaunchKernelGGL(cand_rescore_krn<32>, dim3(grid), dim3(kTNT), 0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL(cand_rescore_krn<64>, dim3(grid), dim3(kTNT), 0, s, a);
+  }
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL((cand_finalize_krn<I>), dim3(n_segments), dim3(256), 0, s, a);
+  KMD_LAUNCH_CHECK();
+  // the pairs did not fit: the full gather re-score of every queued sample
+  launch_rescore_full<I>(n, d, k, X_r, tiles.as<const float>(), labels, queue,
+                         queue_counts, n_segments, seg_cap, n_rescored, a.overflow, s);
 }
 
 template <typename I>
@@ -982,7 +1310,7 @@ template <typename I>
 void launch_tiled_mixed_assign(int64_t n, int64_t d, int64_t k, int T,
                                int blocks, const float *X_t, const void *aop,
                                const float *bound, I *labels, int *queue,
-                               int *queue_counts, hipStream_t s) {
+                               int *queue_counts, unsigned *qmask, hipStream_t s) {
   TiledMixedFn fn = pick_mixed_tiled<I>(d, T);
   if (fn == nullptr || blocks < 1)
     throw RuntimeError("tiled BF16 assignment: unsupported shape");
@@ -999,6 +1327,7 @@ void launch_tiled_mixed_assign(int64_t n, int64_t d, int64_t k, int T,
   a.labels = labels;
   a.queue = queue;
   a.queue_counts = queue_counts;
+  a.qmask = tiled_cand_supported(k) ? qmask : nullptr;
   a.seg_cap = tiled_mixed_seg_cap(n, d, T, blocks);
   a.pinf = std::numeric_limits<float>::infinity();
   hipLaunchKernelGGL(fn, dim3(blocks), dim3(kTNT), mixed_tiled_lds_bytes(d, T), s,
@@ -1063,7 +1392,11 @@ void launch_tiled_update(int64_t n, int64_t d, int64_t k, const float *X_r,
   template void launch_tiled_mixed_assign<I>(int64_t, int64_t, int64_t, int,   \
                                              int, const float *, const void *, \
                                              const float *, I *, int *, int *, \
-                                             hipStream_t);                     \
+                                             unsigned *, hipStream_t);         \
+  template void launch_tiled_cand_rescore<I>(                                  \
+      int64_t, int64_t, int64_t, const float *, const float *, const float *,  \
+      I *, const int *, const int *, const unsigned *, int, int64_t, void *,   \
+      int *, hipStream_t);                                                     \
   template void launch_tiled_update<I>(int64_t, int64_t, int64_t,              \
                                        const float *, const float *,           \
                                        const I *, const float *, void *,       \
