@@ -43,9 +43,12 @@ def parse():
     p.add_argument("--k", type=int, default=K)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
-    p.add_argument("--workload", default="c1", choices=["c1", "c3", "c4"],
+    p.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4"],
                    help="c1: one plain fit n=10M d=32 k=256 (the metric's "
-                        "configuration, default); c3: 512 independent fits in "
+                        "configuration, default); c2: one fit n=2M d=128 "
+                        "k=1024 on anisotropic blobs, 30 iterations, with the "
+                        "BF16 cross term + exact re-score (--precision, default "
+                        "bf16) beside plain FP32; c3: 512 independent fits in "
                         "one batched call (fits/s), with the same fits run one "
                         "by one through the single-fit driver beside it; c4: "
                         "a Poisson stream of fit requests served with "
@@ -79,6 +82,152 @@ def make_data(n, d, k, seed, device):
     init_idx = torch.randperm(n, generator=g, device=device)[:k]
     init_t = X_t[:, init_idx].contiguous()
     return X_t, init_t
+
+
+def make_c2_data(n, d, k, seed, device):
+    """Anisotropic Gaussian blobs: blob b draws x = c_b + Q_b (s_b * z) with
+    z ~ N(0, I), Q_b a random orthogonal matrix (QR of a Gaussian matrix,
+    signs fixed by diag(R) > 0), axis scales s_b log-uniform in [0.1, 3] and
+    centers c_b uniform in [-20, 20]^d; equal-probability blob choice; seeded
+    random-sample init.  Returns X_t (d, n) and the init (d, k)."""
+    import math
+    import torch
+    gc = torch.Generator()
+    gc.manual_seed(seed)
+    a = torch.randn((k, d, d), generator=gc, dtype=torch.float64)
+    q, r = torch.linalg.qr(a)
+    q = q * torch.sign(torch.diagonal(r, dim1=-2, dim2=-1))[:, None, :]
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    qt = q.transpose(1, 2).contiguous().to(device=device, dtype=torch.float32)
+    scales = torch.exp(torch.rand((k, d), generator=g, device=device)
+                       * (math.log(3.0) - math.log(0.1)) + math.log(0.1))
+    centers = torch.rand((k, d), generator=g, device=device) * 40.0 - 20.0
+    blob = torch.randint(0, k, (n,), generator=g, device=device)
+    order = torch.argsort(blob, stable=True)
+    sb = blob[order]
+    counts = torch.bincount(sb, minlength=k)
+    start = torch.cumsum(counts, 0) - counts
+    rank = torch.arange(n, device=device) - start[sb]
+    m = int(counts.max())
+    z = torch.randn((k, m, d), generator=g, device=device) * scales[:, None, :]
+    y = torch.bmm(z, qt)                      # row j of blob b: (Q_b (s_b z))^T
+    del z
+    rows = torch.empty((n, d), dtype=torch.float32, device=device)
+    rows[order] = y[sb, rank] + centers[sb]
+    del y
+    X_t = rows.T.contiguous()
+    del rows
+    init_idx = torch.randperm(n, generator=g, device=device)[:k]
+    return X_t, X_t[:, init_idx].contiguous()
+
+
+def fp64_update(X_t, w, labels, k):
+    """One FP64 centroid update from given labels (empty clusters: NaN)."""
+    import torch
+    d = X_t.shape[0]
+    sums = torch.zeros((k, d), dtype=torch.float64, device=X_t.device)
+    cnt = torch.zeros(k, dtype=torch.float64, device=X_t.device)
+    lab = labels.long()
+    chunk = 1 << 20
+    for s in range(0, X_t.shape[1], chunk):
+        e = min(X_t.shape[1], s + chunk)
+        ws = w[s:e].double()
+        sums.index_add_(0, lab[s:e], X_t[:, s:e].T.double() * ws[:, None])
+        cnt.index_add_(0, lab[s:e], ws)
+    return (sums / cnt[:, None]).T, cnt
+
+
+def bench_c2(args):
+    """Compute-bound single fit: the mixed-precision driver against plain FP32
+    on the same data, with an FP64 check of the last centroid update."""
+    import torch
+    import kmeans_dpcpp_amd as kd
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream()
+    n, d, k = 2_000_000, 128, 1024
+    steps = args.steps if args.steps != 50 else 30
+    prec = args.precision if args.precision != "fp32" else "bf16"
+    X_t, init_t = make_c2_data(n, d, k, 2001, dev)
+    w = torch.ones(n, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    def run(p, iters, warm):
+        lab = torch.empty(n, dtype=torch.int32, device=dev)
+        c_a, c_b = init_t.clone(), torch.empty_like(init_t)
+        if warm:
+            kd.kmeans_lloyd_driver_ex(X_t, w, c_a, lab, c_b, 0.0, False,
+                                      args.warmup, p, stream)
+            c_a = init_t.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        it, inertia, stats = kd.kmeans_lloyd_driver_ex(X_t, w, c_a, lab, c_b, 0.0,
+                                                      False, iters, p, stream)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        assert it == iters, (it, iters)
+        return el, lab, c_b, inertia, stats
+
+    el_m, lab_m, cen_m, inr_m, st_m = run(prec, steps, args.warmup > 0)
+    el_p, lab_p, cen_p, inr_p, st_p = run("fp32", steps, args.warmup > 0)
+    # FP64 check: centroids after `steps` iterations against one FP64 update
+    # from the labels the mixed form gives for its centroids after steps - 1
+    _, lab_prev, _, _, _ = run(prec, steps - 1, False)
+    c64, cnt = fp64_update(X_t, w, lab_prev, k)
+    ok = cnt > 0
+    diff = (cen_m.double() - c64)[:, ok].abs()
+    rel = float(diff.max() / c64[:, ok].abs().max())
+
+    kern_ms = st_m["fused_kernel_ms"] / max(1, st_m["fused_launches"])
+    flops = 2.0 * n * k * d + 2.0 * n * d
+    tflops = flops / (kern_ms * 1e-3) / 1e12
+    bytes_per_launch = n * d * 4 + n * 4 + n * 4
+    roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": BF16_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tflops / BF16_PEAK_TFLOPS, 4),
+            "traffic": None,
+            "kernel": "assignment (BF16 MFMA) + exact re-score + exact update",
+            "kernel_ms": round(kern_ms, 4),
+            "hbm_frac": round(bytes_per_launch / (kern_ms * 1e-3) / 1e9
+                              / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes": bytes_per_launch, "algorithmic_flops": flops}
+    kp = st_p["fused_kernel_ms"] / max(1, st_p["fused_launches"])
+    line = {
+        "metric": METRIC,
+        "value": round(steps / el_m, 3),
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * el_m / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": prec,
+        "data": "synthetic",
+        "config": {"workload": f"mixed-precision Lloyd, n={n:,} d={d} k={k} "
+                               f"anisotropic Gaussian blobs, tol=0, {steps} "
+                               f"iterations",
+                   "n_samples": n, "n_features": d, "n_clusters": k,
+                   "precision": prec},
+        "roofline": roof,
+        "cpu_baseline": None,
+        "path": st_m.get("path"),
+        "rescored_per_iter": st_m.get("n_rescored", 0) / steps,
+        "plain_fp32": {"iters_per_s": round(steps / el_p, 3),
+                       "ms_per_step": round(1e3 * el_p / steps, 4),
+                       "kernel_ms": round(kp, 4),
+                       "frac_fp32_peak": round(flops / (kp * 1e-3) / 1e12
+                                               / FP32_PEAK_TFLOPS, 4),
+                       "path": st_p.get("path")},
+        "speedup_vs_fp32": round(el_p / el_m, 2),
+        "labels_identical_to_fp32": bool(torch.equal(lab_m, lab_p)),
+        "centroids_identical_to_fp32": bool(torch.equal(cen_m.view(torch.int32),
+                                                        cen_p.view(torch.int32))),
+        "inertia": [inr_m, inr_p],
+        "centroid_rel_err_vs_fp64_update": rel,
+        "empty_clusters_at_check": int((~ok).sum()),
+    }
+    print(json.dumps(line), flush=True)
 
 
 def make_c3_batch(n_fits, seed, device):
@@ -348,6 +497,9 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the Lloyd "
               f"path is single-GPU in this build", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c2":
+        bench_c2(args)
+        return
     if args.workload == "c3":
         bench_c3(args)
         return

@@ -551,10 +551,43 @@ static void driver_impl(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
     shift = static_cast<T>(host_shift());
   }
 
-  // epilogue: labels and weighted inertia of the final centroids
+  // epilogue: labels and weighted inertia of the final centroids (on the
+  // tiled mixed path with its BF16 kernel and exact re-score: the FP32
+  // labels)
   launch_half_l2_norm<T>(d, k, cur, ws.hn.template as<T>(), s);
-  launch_assignment<T, I>(n, d, k, 0, X, cur, ws.hn.template as<const T>(),
-                          labels, s);
+  bool labelled = false;
+  if constexpr (std::is_same<T, float>::value) {
+    if (ws.tiled && ws.mixed_terms > 0 && n > 0) {
+      const float *xr = ws.xr.template as<const float>();
+      launch_tiled_mixed_prep(d, k, ws.mixed_terms, cur, ws.hn.template as<const T>(),
+                              ws.aop.template as<void>(),
+                              ws.bound.template as<float>(), s);
+      const bool cand = ws.qmask.bytes() > 0;
+      launch_tiled_mixed_assign<I>(n, d, k, ws.mixed_terms, ws.blocks_b, X,
+                                   ws.aop.template as<const void>(),
+                                   ws.bound.template as<const float>(), labels,
+                                   ws.queue.template as<int>(),
+                                   ws.queue_counts.template as<int>(),
+                                   cand ? ws.qmask.template as<unsigned>() : nullptr,
+                                   s);
+      if (cand)
+        launch_tiled_cand_rescore<I>(
+            n, d, k, xr, cur, ws.hn.template as<const T>(), labels,
+            ws.queue.template as<const int>(),
+            ws.queue_counts.template as<const int>(),
+            ws.qmask.template as<const unsigned>(), ws.blocks_b, ws.seg_cap,
+            ws.cand.template as<void>(), ws.queue_count(), s);
+      else
+        launch_tiled_rescore<I>(n, d, k, xr, cur, ws.hn.template as<const T>(),
+                                labels, ws.queue.template as<const int>(),
+                                ws.queue_counts.template as<const int>(),
+                                ws.blocks_b, ws.seg_cap, ws.queue_count(), s);
+      labelled = true;
+    }
+  }
+  if (!labelled)
+    launch_assignment<T, I>(n, d, k, 0, X, cur, ws.hn.template as<const T>(),
+                            labels, s);
   launch_inertia<T, I>(n, d, k, X, w, cur, labels,
                        ws.per_sample.template as<T>(), s);
   if (cur != res)
