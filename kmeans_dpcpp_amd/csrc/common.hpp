@@ -157,13 +157,25 @@ template <typename T, typename I>
 void launch_reduce_centroids(int64_t n_copies, int64_t d, int64_t k,
                              const T *sizes_copies, const T *cent_copies,
                              T *out_sizes, T *out_cent, I *empty_list,
-                             I *n_empty, int64_t list_len, hipStream_t s);
+                             I *n_empty, int64_t list_len, hipStream_t s,
+                             bool from_zero = false);
 template <typename T>
 void launch_centroid_shifts(int64_t d, int64_t k, const T *old_c,
                             const T *new_c, T *out, hipStream_t s);
 template <typename T, typename I>
 void launch_inertia(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
                     const T *C, const I *labels, T *out, hipStream_t s);
+// End of a Lloyd iteration in one launch: next /= sizes (per column), shifts
+// against cur, FP64 sum of the shifts into out_sum[0], half squared norms of
+// the new centroids into hn_next (when not null); skipped entirely while
+// gate[0] != 0 (gate may be null).  partials: finalize_step_blocks(k) doubles;
+// ticket: one zero-initialised counter (left at zero).
+int finalize_step_blocks(int64_t k);
+template <typename T, typename I>
+void launch_finalize_step(int64_t d, int64_t k, const I *gate, const T *sizes,
+                          const T *cur, T *next, T *shifts, T *hn_next,
+                          double *partials, unsigned *ticket, double *out_sum,
+                          hipStream_t s);
 template <typename T>
 void launch_reduce_sum(int64_t n, const T *data, double *dev_result,
                        hipStream_t s);

@@ -7,20 +7,24 @@
 // list, the relocation of empty clusters (on unnormalised sums, before the
 // division), the division, the shifts and their sum; the epilogue recomputes
 // labels and the weighted inertia of the final centroids and copies them to
-// res_centroids_t.  init_centroids_t is used as the other ping-pong buffer
+// res_centroids_t.  init_centroids_t is one of the rotating centroid buffers
 // and is overwritten.
 //
 // Differences (all on the host side, none in results):
-//  * one host synchronisation per iteration instead of two: the next
-//    iteration's front half (half-norms, fused step, reduction) is enqueued
-//    before reading back {shift sum of this iteration, empty count of the
-//    next}; when the fit stops, that speculative work is discarded (labels are
-//    recomputed by the epilogue, the written buffer is the scratch one);
+//  * the host is off the GPU's critical path: the division, shifts and
+//    shift sum run in one kernel that skips itself when the iteration found
+//    empty clusters, so a whole iteration is enqueued without a host
+//    decision, and iteration i + 1 is enqueued before iteration i's scalars
+//    {shift sum, empty count} are read back (one wait per iteration, on an
+//    event, while the GPU runs the next iteration).  The work enqueued ahead
+//    is discarded when the fit stops, or redone after a relocation; the
+//    centroid buffers rotate over three (init, res, a scratch one) so it
+//    never overwrites centroids still needed;
 //  * private copies are one per block of the persistent fused kernel and are
 //    written with plain stores, so the sums are deterministic;
-//  * the relocation path reuses the labels of the fused step instead of
-//    recomputing them (the reference recomputes them from the same centroids
-//    and half-norms, src/kmeans_lloyd_driver.hpp:226-250);
+//  * the relocation path reuses the labels of the fused step (kept per
+//    iteration parity) instead of recomputing them from the same centroids
+//    and half-norms (src/kmeans_lloyd_driver.hpp:226-250);
 //  * the verbose line reports the inertia of the centroids that produced the
 //    labels (the reference evaluates it against the undivided sums).
 #include <cstdlib>
@@ -54,22 +58,34 @@ struct LloydWorkspace {
   TempBuffer xr{0, nullptr}, upd{0, nullptr};  // tiled: X sample-major, update scratch
   TempBuffer qmask{0, nullptr}, cand{0, nullptr};  // tiled BF16: candidate tiles
   int64_t seg_cap = 0;     // capacity of each block's re-score queue segment
+  // Two iterations can be in flight (the next one is enqueued before this
+  // one's scalars are read back): cluster sizes, the empty list and the
+  // scalars {shift sum @0, n_empty @8, re-scored count @16} have one slot per
+  // iteration parity `par`.  cbuf: the third centroid buffer of the rotation.
+  int par = 0;
+  TempBuffer cbuf, fin_part, fin_ticket;
   // copies_bytes / sizes_bytes: private copies of the generic kernel, or the
   // fast path's per-block partials (in `copies`, sizes_bytes = 0)
   LloydWorkspace(int64_t n_, int64_t d_, int64_t k_, int blocks_, bool fast_,
                  bool store_, const FusedPlan &plan_, size_t copies_bytes,
                  size_t sizes_bytes, hipStream_t s)
       : n(n_), d(d_), k(k_), blocks(blocks_), fast(fast_), store_flush(store_),
-        plan(plan_), hn(sizeof(T) * k_, s), sizes(sizeof(T) * k_, s),
+        plan(plan_), hn(sizeof(T) * k_, s), sizes(2 * sizeof(T) * k_, s),
         shifts(sizeof(T) * k_, s), per_sample(sizeof(T) * (n_ > 0 ? n_ : 1), s),
         copies(copies_bytes, s), sizes_copies(sizes_bytes, s),
-        list(sizeof(I) * (k_ + 1), s), scalars(32, s),
-        scales(fast_ ? sizeof(float) * (d_ + 1) + 16 : 0, s) {}
-  I *n_empty() const { return reinterpret_cast<I *>(scalars.as<char>() + 8); }
-  int *queue_count() const {
-    return reinterpret_cast<int *>(scalars.as<char>() + 16);
+        list(2 * sizeof(I) * (k_ + 1), s), scalars(64, s),
+        scales(fast_ ? sizeof(float) * (d_ + 1) + 16 : 0, s),
+        cbuf(sizeof(T) * d_ * k_, s),
+        fin_part(sizeof(double) * finalize_step_blocks(k_), s),
+        fin_ticket(sizeof(unsigned), s) {
+    KMD_HIP_CHECK(hipMemsetAsync(fin_ticket.as<unsigned>(), 0, sizeof(unsigned), s));
   }
-  double *shift_sum() const { return scalars.as<double>(); }
+  T *sizes_p() const { return sizes.as<T>() + par * k; }
+  I *list_p() const { return list.as<I>() + par * (k + 1); }
+  char *scalars_p() const { return scalars.as<char>() + 32 * par; }
+  I *n_empty() const { return reinterpret_cast<I *>(scalars_p() + 8); }
+  int *queue_count() const { return reinterpret_cast<int *>(scalars_p() + 16); }
+  double *shift_sum() const { return reinterpret_cast<double *>(scalars_p()); }
   float *scale() const { return scales.as<float>(); }
   int *nonfinite() const {
     return reinterpret_cast<int *>(scales.as<char>() + sizeof(float) * (d + 1));
@@ -255,50 +271,107 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_workspace(int64_t n,
 }
 
 // Optional timing of the fused-step kernel with HIP events on the driver's
-// stream (one event pair per iteration, read back after each host sync).
+// stream: one event pair per iteration parity (two iterations can be in
+// flight), read back once that iteration is known to have finished.
 struct FusedTimer {
   bool on = false;
-  hipEvent_t start = nullptr, stop = nullptr;
-  bool pending = false;
+  hipEvent_t start[2] = {nullptr, nullptr}, stop[2] = {nullptr, nullptr};
+  bool pending[2] = {false, false};
+  int slot = 0;
   double total_ms = 0.0;
   int64_t launches = 0;
   explicit FusedTimer(bool enable) : on(enable) {
     if (on) {
-      KMD_HIP_CHECK(hipEventCreate(&start));
-      KMD_HIP_CHECK(hipEventCreate(&stop));
+      for (int i = 0; i < 2; ++i) {
+        KMD_HIP_CHECK(hipEventCreate(&start[i]));
+        KMD_HIP_CHECK(hipEventCreate(&stop[i]));
+      }
     }
   }
   ~FusedTimer() {
-    if (start) (void)hipEventDestroy(start);
-    if (stop) (void)hipEventDestroy(stop);
+    for (int i = 0; i < 2; ++i) {
+      if (start[i]) (void)hipEventDestroy(start[i]);
+      if (stop[i]) (void)hipEventDestroy(stop[i]);
+    }
   }
   void begin(hipStream_t s) {
-    if (on) KMD_HIP_CHECK(hipEventRecord(start, s));
+    if (on) KMD_HIP_CHECK(hipEventRecord(start[slot], s));
   }
   void end(hipStream_t s) {
     if (!on) return;
-    KMD_HIP_CHECK(hipEventRecord(stop, s));
-    pending = true;
+    KMD_HIP_CHECK(hipEventRecord(stop[slot], s));
+    pending[slot] = true;
   }
-  // call after a host synchronisation of the stream
-  void collect() {
-    if (!on || !pending) return;
+  // call once the work timed in slot i has finished
+  void collect(int i) {
+    if (!on || !pending[i]) return;
     float ms = 0.f;
-    KMD_HIP_CHECK(hipEventElapsedTime(&ms, start, stop));
+    KMD_HIP_CHECK(hipEventElapsedTime(&ms, start[i], stop[i]));
     total_ms += ms;
     ++launches;
-    pending = false;
+    pending[i] = false;
   }
+  void collect_all() {
+    collect(0);
+    collect(1);
+  }
+  void discard(int i) { pending[i] = false; }
 };
 
-// hn, fused step and reduction for centroids `cur`, sums into `next`.
+// Pinned host slots for the per-iteration scalars (32 bytes per parity), so
+// their device-to-host copies are asynchronous.  Pinning and unpinning cost
+// hundreds of microseconds, so the 64-byte blocks are kept in a process-wide
+// free list and reused (one per driver call running at a time).
+std::mutex g_pinned_mu;
+std::vector<char *> g_pinned_free;
+
+struct PinnedScalars {
+  char *p = nullptr;
+  PinnedScalars() {
+    {
+      std::lock_guard<std::mutex> lk(g_pinned_mu);
+      if (!g_pinned_free.empty()) {
+        p = g_pinned_free.back();
+        g_pinned_free.pop_back();
+      }
+    }
+    if (p == nullptr)
+      KMD_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&p), 64, hipHostMallocDefault));
+    std::memset(p, 0, 64);
+  }
+  ~PinnedScalars() {
+    if (p == nullptr) return;
+    std::lock_guard<std::mutex> lk(g_pinned_mu);
+    g_pinned_free.push_back(p);
+  }
+  PinnedScalars(const PinnedScalars &) = delete;
+  PinnedScalars &operator=(const PinnedScalars &) = delete;
+  char *slot(int i) const { return p + 32 * i; }
+};
+
+struct IterEvents {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  IterEvents() {
+    for (int i = 0; i < 2; ++i)
+      KMD_HIP_CHECK(hipEventCreateWithFlags(&e[i], hipEventDisableTiming));
+  }
+  ~IterEvents() {
+    for (int i = 0; i < 2; ++i)
+      if (e[i]) (void)hipEventDestroy(e[i]);
+  }
+  IterEvents(const IterEvents &) = delete;
+  IterEvents &operator=(const IterEvents &) = delete;
+};
+
+// hn (unless the previous iteration's finalize computed it), fused step and
+// reduction for centroids `cur`, sums into `next`; the empty count is
+// written, not accumulated.
 template <typename T, typename I>
 static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
                         const T *cur, T *next, I *labels, hipStream_t s,
-                        FusedTimer &timer) {
+                        FusedTimer &timer, bool hn_ready, const int *skip) {
   const int64_t n = ws.n, d = ws.d, k = ws.k;
-  launch_half_l2_norm<T>(d, k, cur, ws.hn.template as<T>(), s);
-  KMD_HIP_CHECK(hipMemsetAsync(ws.n_empty(), 0, sizeof(I), s));
+  if (!hn_ready) launch_half_l2_norm<T>(d, k, cur, ws.hn.template as<T>(), s);
   if constexpr (std::is_same<T, float>::value) {
     if (ws.tiled) {
       const float *xr = ws.xr.template as<const float>();
@@ -315,7 +388,7 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
                                      ws.queue.template as<int>(),
                                      ws.queue_counts.template as<int>(),
                                      cand ? ws.qmask.template as<unsigned>() : nullptr,
-                                     s);
+                                     s, skip);
         if (cand)
           launch_tiled_cand_rescore<I>(
               n, d, k, xr, cur, ws.hn.template as<const T>(), labels,
@@ -330,7 +403,7 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
                                   ws.blocks_b, ws.seg_cap, ws.queue_count(), s);
       } else {
         launch_tiled_assign<I>(n, d, k, X, cur, ws.hn.template as<const T>(),
-                               labels, s);
+                               labels, s, skip);
       }
       launch_tiled_update<I>(n, d, k, xr, w, labels, ws.scale(),
                              ws.upd.template as<void>(),
@@ -338,8 +411,7 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
       timer.end(s);
       launch_reduce_fixed_partials<T, I>(
           1, d, k, ws.copies.template as<const void>(), ws.scale(),
-          ws.nonfinite(), ws.sizes.template as<T>(), next,
-          ws.list.template as<I>(), ws.n_empty(), k, s);
+          ws.nonfinite(), ws.sizes_p(), next, ws.list_p(), ws.n_empty(), k, s, true);
       return;
     }
     if (ws.mixed_terms > 0) {
@@ -352,7 +424,7 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
                            ws.bound.template as<const float>(), ws.scale(),
                            labels, ws.copies.template as<void>(),
                            ws.queue.template as<int>(),
-                           ws.queue_counts.template as<int>(), s);
+                           ws.queue_counts.template as<int>(), s, skip);
       launch_fast_rescore<I>(n, d, k, X, w, cur, ws.hn.template as<const T>(),
                              labels, ws.scale(), ws.nonfinite(),
                              ws.copies.template as<void>(), ws.blocks,
@@ -362,8 +434,7 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
       timer.end(s);
       launch_reduce_fixed_partials<T, I>(
           ws.blocks + ws.blocks_b, d, k, ws.copies.template as<const void>(),
-          ws.scale(), ws.nonfinite(), ws.sizes.template as<T>(), next,
-          ws.list.template as<I>(), ws.n_empty(), k, s);
+          ws.scale(), ws.nonfinite(), ws.sizes_p(), next, ws.list_p(), ws.n_empty(), k, s, true);
       return;
     }
   }
@@ -372,12 +443,11 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
     launch_fast_fused_fixed<T, I>(n, d, k, X, w, cur,
                                   ws.hn.template as<const T>(), labels,
                                   ws.scale(), ws.nonfinite(),
-                                  ws.copies.template as<void>(), ws.blocks, s);
+                                  ws.copies.template as<void>(), ws.blocks, s, skip);
     timer.end(s);
     launch_reduce_fixed_partials<T, I>(
         ws.blocks, d, k, ws.copies.template as<const void>(), ws.scale(),
-        ws.nonfinite(), ws.sizes.template as<T>(), next,
-        ws.list.template as<I>(), ws.n_empty(), k, s);
+        ws.nonfinite(), ws.sizes_p(), next, ws.list_p(), ws.n_empty(), k, s, true);
     return;
   }
   if (!ws.store_flush || n == 0) {
@@ -397,8 +467,8 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
   launch_reduce_centroids<T, I>(ws.blocks, d, k,
                                 ws.sizes_copies.template as<const T>(),
                                 ws.copies.template as<const T>(),
-                                ws.sizes.template as<T>(), next,
-                                ws.list.template as<I>(), ws.n_empty(), k, s);
+                                ws.sizes_p(), next, ws.list_p(), ws.n_empty(),
+                                k, s, true);
 }
 
 template <typename T, typename I>
@@ -480,43 +550,76 @@ static void driver_impl(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
   auto ws_ptr = make_workspace<T, I>(n, d, k, X, w, precision, s);
   FusedTimer timer(stats != nullptr);
   auto &ws = *ws_ptr;
-  T *cur = init;
-  T *next = res;
+  // centroid buffers rotate init -> res -> cbuf: iteration i reads
+  // bufs[i % 3] and writes bufs[(i + 1) % 3], so an iteration enqueued ahead
+  // never writes the centroids the one before it read
+  T *bufs[3] = {init, res, ws.cbuf.template as<T>()};
   const T tol_t = static_cast<T>(tol);
   int64_t it = 0;
   T shift = std::numeric_limits<T>::infinity();
-  unsigned char host_scalars[24];
   int64_t rescored = 0;
-  auto read_scalars = [&]() {
-    KMD_HIP_CHECK(hipMemcpyAsync(host_scalars, ws.scalars.template as<char>(),
-                                 24, hipMemcpyDeviceToHost, s));
-    KMD_HIP_CHECK(hipStreamSynchronize(s));
-    timer.collect();
-  };
-  auto host_n_empty = [&]() -> int64_t {
-    I v;
-    std::memcpy(&v, host_scalars + 8, sizeof(I));
-    return (int64_t)v;
-  };
-  auto host_shift = [&]() -> double {
-    double v;
-    std::memcpy(&v, host_scalars, sizeof(double));
-    return v;
+  PinnedScalars host;
+  IterEvents ev;
+  // labels of odd iterations go to a second buffer, so the relocation of
+  // iteration i still has its labels when iteration i + 1 was enqueued ahead
+  // (the epilogue writes the caller's labels)
+  TempBuffer lab2(sizeof(I) * (size_t)(n > 0 ? n : 1), s);
+  I *labs[2] = {labels, lab2.as<I>()};
+  // one iteration: front half, then (unless it found empty clusters) the
+  // division, shifts, their sum and the next half-norms; its scalars go to
+  // host slot `par`
+  auto enqueue_iter = [&](int64_t i) {
+    ws.par = (int)(i & 1);
+    timer.slot = ws.par;
+    T *cur = bufs[i % 3];
+    T *next = bufs[(i + 1) % 3];
+    // the previous iteration's empty count: when it found empty clusters,
+    // this iteration's main kernel returns at once (it will be redone)
+    const int *skip =
+        i > 0 ? reinterpret_cast<const int *>(ws.scalars.template as<char>() +
+                                              32 * (ws.par ^ 1) + 8)
+              : nullptr;
+    lloyd_front<T, I>(ws, X, w, cur, next, labs[ws.par], s, timer, i > 0, skip);
+    launch_finalize_step<T, I>(d, k, ws.n_empty(), ws.sizes_p(), cur, next,
+                               ws.shifts.template as<T>(),
+                               ws.hn.template as<T>(),
+                               ws.fin_part.template as<double>(),
+                               ws.fin_ticket.template as<unsigned>(),
+                               ws.shift_sum(), s);
+    KMD_HIP_CHECK(hipMemcpyAsync(host.slot(ws.par), ws.scalars_p(), 32,
+                                 hipMemcpyDeviceToHost, s));
+    KMD_HIP_CHECK(hipEventRecord(ev.e[ws.par], s));
   };
 
-  if (max_iter > 0) {
-    lloyd_front<T, I>(ws, X, w, cur, next, labels, s, timer);
-    read_scalars();
-  }
+  // Iteration it + 1 is enqueued before iteration it's scalars are read
+  // (not in verbose mode, which prints between iterations).  It is discarded
+  // when the fit stops after iteration it (it wrote only the rotation's third
+  // buffer, labels and scratch, all rewritten before use), or when iteration
+  // it found empty clusters (then it is enqueued again after the relocation).
+  const bool ahead = !verbose;
+  if (max_iter > 0) enqueue_iter(0);
   while (it < max_iter && shift > tol_t) {
-    const int64_t n_empty = host_n_empty();
+    bool next_queued = false;
+    if (ahead && it + 1 < max_iter) {
+      enqueue_iter(it + 1);
+      next_queued = true;
+    }
+    const int p = (int)(it & 1);
+    KMD_HIP_CHECK(hipEventSynchronize(ev.e[p]));
+    timer.collect(p);
+    I ne;
+    std::memcpy(&ne, host.slot(p) + 8, sizeof(I));
+    const int64_t n_empty = (int64_t)ne;
     if (ws.mixed_terms > 0) {
       int qc;
-      std::memcpy(&qc, host_scalars + 16, sizeof(int));
+      std::memcpy(&qc, host.slot(p) + 16, sizeof(int));
       rescored += qc;
     }
+    T *cur = bufs[it % 3];
+    T *next = bufs[(it + 1) % 3];
+    I *lab = labs[p];
     if (verbose) {
-      launch_inertia<T, I>(n, d, k, X, w, cur, labels,
+      launch_inertia<T, I>(n, d, k, X, w, cur, lab,
                            ws.per_sample.template as<T>(), s);
       double tot = 0.0;
       TempBuffer dres(sizeof(double), s);
@@ -530,26 +633,48 @@ static void driver_impl(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
       if (print_fn) print_fn(ss.str().c_str(), print_user);
     }
     if (n_empty > 0) {
+      // the division was skipped; the iteration enqueued ahead (if any) ran
+      // on undivided sums and is redone below
+      KMD_HIP_CHECK(hipStreamSynchronize(s));
+      timer.discard(p ^ 1);  // a skipped launch, when one was enqueued
+      ws.par = p;
       // squared distance to the nearest centroid (unit weights), then move
       // the farthest samples into the empty clusters
-      launch_inertia<T, I>(n, d, k, X, nullptr, cur, labels,
+      launch_inertia<T, I>(n, d, k, X, nullptr, cur, lab,
                            ws.per_sample.template as<T>(), s);
-      launch_relocate<T, I>(n, d, k, n_empty, X, w, labels,
-                            ws.list.template as<const I>(),
+      launch_relocate<T, I>(n, d, k, n_empty, X, w, lab, ws.list_p(),
                             ws.per_sample.template as<const T>(), next,
-                            ws.sizes.template as<T>(),
-                            ws.per_sample.template as<T>(), s);
+                            ws.sizes_p(), ws.per_sample.template as<T>(), s);
+      // the redone next iteration must not take itself for one to skip
+      KMD_HIP_CHECK(hipMemsetAsync(ws.n_empty(), 0, sizeof(I), s));
+      launch_finalize_step<T, I>(d, k, static_cast<const I *>(nullptr),
+                                 ws.sizes_p(), cur, next,
+                                 ws.shifts.template as<T>(),
+                                 ws.hn.template as<T>(),
+                                 ws.fin_part.template as<double>(),
+                                 ws.fin_ticket.template as<unsigned>(),
+                                 ws.shift_sum(), s);
+      KMD_HIP_CHECK(hipMemcpyAsync(host.slot(p), ws.scalars_p(), 32,
+                                   hipMemcpyDeviceToHost, s));
+      KMD_HIP_CHECK(hipEventRecord(ev.e[p], s));
+      KMD_HIP_CHECK(hipEventSynchronize(ev.e[p]));
+      next_queued = false;
     }
-    launch_broadcast_divide<T>(d, k, next, ws.sizes.template as<const T>(), s);
-    launch_centroid_shifts<T>(d, k, cur, next, ws.shifts.template as<T>(), s);
-    launch_reduce_sum<T>(k, ws.shifts.template as<const T>(), ws.shift_sum(),
-                         s);
-    std::swap(cur, next);
+    double sh;
+    std::memcpy(&sh, host.slot(p), sizeof(double));
+    shift = static_cast<T>(sh);
     ++it;
-    if (it < max_iter) lloyd_front<T, I>(ws, X, w, cur, next, labels, s, timer);
-    read_scalars();
-    shift = static_cast<T>(host_shift());
+    if (it < max_iter && shift > tol_t && !next_queued) enqueue_iter(it);
   }
+  T *cur = bufs[it % 3];
+  ws.par = 0;
+  auto read_sum = [&]() -> double {
+    double v = 0.0;
+    KMD_HIP_CHECK(hipMemcpyAsync(&v, ws.shift_sum(), sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+    KMD_HIP_CHECK(hipStreamSynchronize(s));
+    return v;
+  };
 
   // epilogue: labels and weighted inertia of the final centroids (on the
   // tiled mixed path with its BF16 kernel and exact re-score: the FP32
@@ -595,8 +720,8 @@ static void driver_impl(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
                                  hipMemcpyDeviceToDevice, s));
   launch_reduce_sum<T>(n, ws.per_sample.template as<const T>(),
                        ws.shift_sum(), s);
-  read_scalars();
-  *inertia_out = static_cast<double>(static_cast<T>(host_shift()));
+  *inertia_out = static_cast<double>(static_cast<T>(read_sum()));
+  timer.collect_all();
   *n_iters_out = it;
   if (stats) {
     stats->fused_kernel_ms = timer.total_ms;

@@ -151,6 +151,7 @@ struct FastArgs {
   int n_segments;           // gather mode: <= kMaxSegments
   int64_t seg_cap;
   int *n_rescored;          // gather mode: total queued samples (block 0)
+  const int *skip;          // when non-null and *skip != 0 the launch does nothing
 };
 
 constexpr int kMaxSegments = 1024;
@@ -162,6 +163,8 @@ constexpr int kMaxSegments = 1024;
 template <int NT, int KS, int S, bool FULL, typename I, bool UPDATE,
           bool STORE_FLUSH, bool GATHER = false>
 __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
+  // an iteration enqueued ahead of a relocation: nothing to compute
+  if (a.skip != nullptr && *a.skip != 0) return;
   constexpr int kWavesPerBlock = NT / kWave;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int64_t n = a.n;
@@ -367,9 +370,31 @@ __global__ void __launch_bounds__(256)
   const int f = blockIdx.y;  // 0..d (d = weights)
   const float *row = (f < d) ? X + (int64_t)f * n : w;
   unsigned m = 0u;
-  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * 256)
-    m = max(m, __float_as_uint(row[i]) & 0x7fffffffu);
+  if ((reinterpret_cast<uintptr_t>(row) & 15u) == 0) {
+    // 16-byte loads, two in flight per thread; the tail element by element
+    const int64_t n4 = n / 4;
+    const uint4 *r4 = reinterpret_cast<const uint4 *>(row);
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    for (; i + stride < n4; i += 2 * stride) {
+      const uint4 a = r4[i], b = r4[i + stride];
+      m = max(m, max(max(a.x & 0x7fffffffu, a.y & 0x7fffffffu),
+                     max(a.z & 0x7fffffffu, a.w & 0x7fffffffu)));
+      m = max(m, max(max(b.x & 0x7fffffffu, b.y & 0x7fffffffu),
+                     max(b.z & 0x7fffffffu, b.w & 0x7fffffffu)));
+    }
+    for (; i < n4; i += stride) {
+      const uint4 a = r4[i];
+      m = max(m, max(max(a.x & 0x7fffffffu, a.y & 0x7fffffffu),
+                     max(a.z & 0x7fffffffu, a.w & 0x7fffffffu)));
+    }
+    for (int64_t j = 4 * n4 + blockIdx.x * 256ll + threadIdx.x; j < n; j += stride)
+      m = max(m, __float_as_uint(row[j]) & 0x7fffffffu);
+  } else {
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256)
+      m = max(m, __float_as_uint(row[i]) & 0x7fffffffu);
+  }
   for (int o = kWave / 2; o > 0; o >>= 1)
     m = max(m, (unsigned)__shfl_xor((int)m, o, kWave));
   if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(&maxbits[f], m);
@@ -535,8 +560,8 @@ void launch_fixed_point_scales(int64_t n, int64_t d, const float *X,
                                hipStream_t s) {
   TempBuffer bits(sizeof(unsigned) * (d + 1), s);
   KMD_HIP_CHECK(hipMemsetAsync(bits.as<unsigned>(), 0, bits.bytes(), s));
-  int gx = (int)ceil_div<int64_t>(n > 0 ? n : 1, 256 * 8);
-  const int cap = 4096 / (int)(d + 1) + 1;
+  int gx = (int)ceil_div<int64_t>(n > 0 ? n : 1, 256 * 32);
+  const int cap = 8192 / (int)(d + 1) + 1;
   if (gx > cap) gx = cap;
   if (gx < 1) gx = 1;
   if (n > 0) {
@@ -568,7 +593,7 @@ void launch_fast_fused_fixed(int64_t n, int64_t d, int64_t k, const T *X,
                              const T *w, const T *C, const T *hn, I *labels,
                              const float *scale, const int *nonfinite,
                              void *partials, int64_t n_partials,
-                             hipStream_t s) {
+                             hipStream_t s, const int *skip) {
   if constexpr (std::is_same<T, float>::value) {
     if (n <= 0) return;
     const MfmaLaunch L = plan_launch<I, true, true>(n, d, k);
@@ -580,8 +605,10 @@ void launch_fast_fused_fixed(int64_t n, int64_t d, int64_t k, const T *X,
     a.nonfinite = nonfinite;
     a.copies = partials;
     a.n_copies = n_partials;
+    a.skip = skip;
     run(L, a, s);
   } else {
+    (void)skip;
     throw RuntimeError("MFMA fused step is FP32 only");
   }
 }
@@ -737,7 +764,8 @@ template bool fast_fused_supported<double>(int64_t, int64_t);
                                         hipStream_t);                         \
   template void launch_fast_fused_fixed<T, I>(                                \
       int64_t, int64_t, int64_t, const T *, const T *, const T *, const T *,  \
-      I *, const float *, const int *, void *, int64_t, hipStream_t);        \
+      I *, const float *, const int *, void *, int64_t, hipStream_t,         \
+      const int *);                                                           \
   template void launch_fast_assign<T, I>(int64_t, int64_t, int64_t,           \
                                          const T *, const T *, const T *,     \
                                          I *, hipStream_t);

@@ -78,7 +78,7 @@ void launch_fast_fused_fixed(int64_t n, int64_t d, int64_t k, const T *X,
                              const T *w, const T *C, const T *hn, I *labels,
                              const float *scale, const int *nonfinite,
                              void *partials, int64_t n_partials,
-                             hipStream_t s);
+                             hipStream_t s, const int *skip = nullptr);
 
 // Sums the per-block partials of launch_fast_fused_fixed (exactly, in int64,
 // when the fixed-point path ran) into out_cent (d x k) and out_sizes (k),
@@ -88,7 +88,8 @@ void launch_reduce_fixed_partials(int64_t n_blocks, int64_t d, int64_t k,
                                   const void *partials, const float *scale,
                                   const int *nonfinite, T *out_sizes,
                                   T *out_cent, I *empty_list, I *n_empty,
-                                  int64_t list_len, hipStream_t s);
+                                  int64_t list_len, hipStream_t s,
+                                  bool from_zero = false);
 
 }  // namespace kmd
 
@@ -116,7 +117,7 @@ void launch_mixed_step(int64_t n, int64_t d, int64_t k, int T, int blocks,
                        const float *X, const float *w, const void *aop,
                        const float *bound, const float *scale, I *labels,
                        void *partials, int *queue, int *queue_counts,
-                       hipStream_t s);
+                       hipStream_t s, const int *skip = nullptr);
 // FP32 MFMA re-score of the queued samples (the FP32 path's arithmetic):
 // reads the n_segments segments of the queue, writes
 // fast_rescore_blocks<I>(d, k, n_segments) partial slots from slot0 and the
@@ -140,7 +141,7 @@ void launch_transpose(int64_t n, int64_t d, const float *X_t, float *X_r,
 template <typename I>
 void launch_tiled_assign(int64_t n, int64_t d, int64_t k, const float *X_t,
                          const float *C, const float *hn, I *labels,
-                         hipStream_t s);
+                         hipStream_t s, const int *skip = nullptr);
 // FP32 re-score of the samples in the queue segments (same arithmetic as
 // launch_tiled_assign); samples read from X_r; *n_rescored = queue length
 template <typename I>
@@ -162,7 +163,8 @@ template <typename I>
 void launch_tiled_mixed_assign(int64_t n, int64_t d, int64_t k, int T,
                                int blocks, const float *X_t, const void *aop,
                                const float *bound, I *labels, int *queue,
-                               int *queue_counts, unsigned *qmask, hipStream_t s);
+                               int *queue_counts, unsigned *qmask, hipStream_t s,
+                               const int *skip = nullptr);
 // Re-score of the queued samples restricted to their candidate tiles (the
 // BF16 kernel's qmask; needs k <= 1024), with the FP32 kernels' arithmetic;
 // falls back to the full gather re-score when the (sample, tile) pairs

@@ -61,6 +61,7 @@ struct MixedArgs {
   int64_t seg_cap;
   float pinf;            // +inf, passed at run time so that med3(b, v, pinf)
                          // stays one v_med3 (no max pattern, no canonicalise)
+  const int *skip;       // when non-null and *skip != 0 the launch does nothing
 };
 
 __device__ inline __bf16 to_bf16(float x) { return (__bf16)x; }
@@ -138,6 +139,8 @@ __global__ void __launch_bounds__(256)
 // waves per SIMD that hide each other's load and matrix-core latency.
 template <int D, int T, bool FULL, typename I, int NT>
 __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
+  // an iteration enqueued ahead of a relocation: nothing to compute
+  if (a.skip != nullptr && *a.skip != 0) return;
   using Sh = MixedShape<D, T>;
   constexpr int KD = Sh::KD, KK = Sh::KK, XH = Sh::XH;
   constexpr int kWavesPerBlock = NT / kWave;
@@ -515,7 +518,7 @@ void launch_mixed_step(int64_t n, int64_t d, int64_t k, int T, int blocks,
                        const float *X, const float *w, const void *aop,
                        const float *bound, const float *scale, I *labels,
                        void *partials, int *queue, int *queue_counts,
-                       hipStream_t s) {
+                       hipStream_t s, const int *skip) {
   MixedFn fn = pick_mixed<I>(d, T);
   if (fn == nullptr || blocks < 1)
     throw RuntimeError("mixed-precision step: unsupported shape");
@@ -536,6 +539,7 @@ void launch_mixed_step(int64_t n, int64_t d, int64_t k, int T, int blocks,
   a.queue_counts = queue_counts;
   a.seg_cap = mixed_seg_cap(n, d, blocks);
   a.pinf = std::numeric_limits<float>::infinity();
+  a.skip = skip;
   hipLaunchKernelGGL(fn, dim3(blocks), dim3(nt_of(d_bucket(d))),
                      mixed_lds_bytes(d, k, T), s,
                      a);
@@ -548,11 +552,13 @@ template void launch_mixed_step<int32_t>(int64_t, int64_t, int64_t, int, int,
                                          const float *, const float *,
                                          const void *, const float *,
                                          const float *, int32_t *, void *,
-                                         int *, int *, hipStream_t);
+                                         int *, int *, hipStream_t,
+                                         const int *);
 template void launch_mixed_step<int64_t>(int64_t, int64_t, int64_t, int, int,
                                          const float *, const float *,
                                          const void *, const float *,
                                          const float *, int64_t *, void *,
-                                         int *, int *, hipStream_t);
+                                         int *, int *, hipStream_t,
+                                         const int *);
 
 }  // namespace kmd

@@ -105,6 +105,7 @@ struct TiledArgs {
   int64_t seg_cap;
   int *n_rescored;
   const int *only_if;  // gather mode: run only when *only_if != 0 (or null)
+  const int *skip;     // when non-null and *skip != 0 the launch does nothing
 };
 
 // Tile layout in LDS: element (ks, half, col) = C[2 ks + half][tile*32 + col]
@@ -112,6 +113,8 @@ struct TiledArgs {
 // k-steps with one 16-byte load; then the 32 half norms (+inf padded).
 template <int KS, typename I, bool GATHER>
 __global__ void __launch_bounds__(kTNT) tiled_fp32_krn(TiledArgs a) {
+  // an iteration enqueued ahead of a relocation: nothing to compute
+  if (a.skip != nullptr && *a.skip != 0) return;
   constexpr int S = fp32_s<KS>();
   constexpr int TF = tile_floats<KS>();
   constexpr int CPC = fp32_tiles_per_chunk<KS>();
@@ -527,6 +530,7 @@ struct TiledMixedArgs {
   unsigned *qmask;       // [block][seg_cap]: candidate tiles (n_ct <= 32)
   int64_t seg_cap;
   float pinf;
+  const int *skip;       // when non-null and *skip != 0 the launch does nothing
 };
 
 __device__ inline float max3f(float a, float b, float c) {
@@ -537,6 +541,8 @@ __device__ inline float max3f(float a, float b, float c) {
 
 template <int D, int T, typename I>
 __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
+  // an iteration enqueued ahead of a relocation: nothing to compute
+  if (a.skip != nullptr && *a.skip != 0) return;
   constexpr int KD = D / 16, KK = (T * D + 3 + 15) / 16, XH = D / 2;
   constexpr int S = bf16_s(D, T);
   constexpr int CPC = (kChunkBytes / (KK * 64 * 16)) < 1
@@ -1130,7 +1136,7 @@ void launch_transpose(int64_t n, int64_t d, const float *X_t, float *X_r,
 template <typename I>
 void launch_tiled_assign(int64_t n, int64_t d, int64_t k, const float *X_t,
                          const float *C, const float *hn, I *labels,
-                         hipStream_t s) {
+                         hipStream_t s, const int *skip) {
   if (n <= 0) return;
   require(tiled_supported(d, k), "tiled assignment: unsupported shape");
   TiledFn fn = pick_fp32<I, false>(d);
@@ -1149,6 +1155,7 @@ void launch_tiled_assign(int64_t n, int64_t d, int64_t k, const float *X_t,
   a.X = X_t;
   a.tiles = tiles.as<const float>();
   a.labels = labels;
+  a.skip = skip;
   hipLaunchKernelGGL(fn, dim3(blocks), dim3(kTNT), lds, s, a);
   KMD_LAUNCH_CHECK();
 }
@@ -1310,7 +1317,8 @@ template <typename I>
 void launch_tiled_mixed_assign(int64_t n, int64_t d, int64_t k, int T,
                                int blocks, const float *X_t, const void *aop,
                                const float *bound, I *labels, int *queue,
-                               int *queue_counts, unsigned *qmask, hipStream_t s) {
+                               int *queue_counts, unsigned *qmask, hipStream_t s,
+                               const int *skip) {
   TiledMixedFn fn = pick_mixed_tiled<I>(d, T);
   if (fn == nullptr || blocks < 1)
     throw RuntimeError("tiled BF16 assignment: unsupported shape");
@@ -1330,6 +1338,7 @@ void launch_tiled_mixed_assign(int64_t n, int64_t d, int64_t k, int T,
   a.qmask = tiled_cand_supported(k) ? qmask : nullptr;
   a.seg_cap = tiled_mixed_seg_cap(n, d, T, blocks);
   a.pinf = std::numeric_limits<float>::infinity();
+  a.skip = skip;
   hipLaunchKernelGGL(fn, dim3(blocks), dim3(kTNT), mixed_tiled_lds_bytes(d, T), s,
                      a);
   KMD_LAUNCH_CHECK();
@@ -1382,7 +1391,8 @@ void launch_tiled_update(int64_t n, int64_t d, int64_t k, const float *X_r,
 #define KMD_INST_TILED(I)                                                      \
   template void launch_tiled_assign<I>(int64_t, int64_t, int64_t,              \
                                        const float *, const float *,           \
-                                       const float *, I *, hipStream_t);       \
+                                       const float *, I *, hipStream_t,        \
+                                       const int *);                           \
   template void launch_tiled_rescore<I>(int64_t, int64_t, int64_t,             \
                                         const float *, const float *,          \
                                         const float *, I *, const int *,       \
@@ -1392,7 +1402,8 @@ void launch_tiled_update(int64_t n, int64_t d, int64_t k, const float *X_r,
   template void launch_tiled_mixed_assign<I>(int64_t, int64_t, int64_t, int,   \
                                              int, const float *, const void *, \
                                              const float *, I *, int *, int *, \
-                                             unsigned *, hipStream_t);         \
+                                             unsigned *, hipStream_t,          \
+                                             const int *);                     \
   template void launch_tiled_cand_rescore<I>(                                  \
       int64_t, int64_t, int64_t, const float *, const float *, const float *,  \
       I *, const int *, const int *, const unsigned *, int, int64_t, void *,   \

@@ -144,10 +144,10 @@ __global__ void __launch_bounds__(kRedElems * kRedGroups)
 template <typename T, typename I>
 __global__ void __launch_bounds__(1024)
     empty_list_krn(int64_t k, const T *__restrict__ sizes, I *empty_list,
-                   I *n_empty, int64_t list_len) {
+                   I *n_empty, int64_t list_len, bool from_zero) {
   __shared__ int wave_counts[1024 / kWave];
   __shared__ int64_t running;
-  if (threadIdx.x == 0) running = (int64_t)n_empty[0];
+  if (threadIdx.x == 0) running = from_zero ? 0 : (int64_t)n_empty[0];
   __syncthreads();
   const int nwaves = blockDim.x / kWave;
   const int w = threadIdx.x / kWave;
@@ -186,6 +186,77 @@ __global__ void __launch_bounds__(kBlock)
     acc = fma(diff, diff, acc);
   }
   out[c] = acc;
+}
+
+// ---- end of a Lloyd iteration: divide, shifts, their sum -------------------
+// Thread c: column c of `next` divided by sizes[c] (the broadcast divide,
+// src/util_kernels.hpp:9-41), then shift[c] = sum_f (cur - next)^2 in feature
+// order (src/util_kernels.hpp:478-516); the k shifts are summed in FP64 (block
+// sums, then the last block adds the block sums in block order).  With a
+// non-null `gate`, the kernel does nothing while gate[0] != 0: the iteration
+// has empty clusters, which the host relocates (on the undivided sums) before
+// running it again ungated.
+template <typename T, typename I>
+__global__ void __launch_bounds__(kBlock)
+    finalize_step_krn(int64_t d, int64_t k, const I *gate,
+                      const T *__restrict__ sizes, const T *__restrict__ cur,
+                      T *__restrict__ next, T *__restrict__ shifts,
+                      T *__restrict__ hn_next, double *partials,
+                      unsigned *ticket, double *out_sum) {
+  __shared__ double lds[kBlock / kWave];
+  __shared__ bool last;
+  if (gate != nullptr && gate[0] != I(0)) return;
+  const int64_t c = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  double v = 0.0;
+  if (c < k) {
+    const T sz = sizes[c];
+    T acc = T(0), h = T(0);
+    constexpr int kF = 8;  // features loaded ahead of their use
+    for (int64_t f0 = 0; f0 < d; f0 += kF) {
+      T nv[kF], cv[kF];
+#pragma unroll
+      for (int j = 0; j < kF; ++j) {
+        if (f0 + j < d) {
+          nv[j] = next[(f0 + j) * k + c];
+          cv[j] = cur[(f0 + j) * k + c];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kF; ++j) {
+        if (f0 + j < d) {
+          const T q = nv[j] / sz;
+          next[(f0 + j) * k + c] = q;
+          const T diff = cv[j] - q;
+          acc = fma(diff, diff, acc);
+          h = fma(q, q, h);
+        }
+      }
+    }
+    shifts[c] = acc;
+    // half squared norm of the new centroid, as half_l2_norm_krn computes it
+    if (hn_next != nullptr) hn_next[c] = h / T(2);
+    v = (double)acc;
+  }
+  const double t = block_sum(v, lds);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = t;
+    __threadfence();
+    last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double a = 0.0;
+  const unsigned long long *pb =
+      reinterpret_cast<const unsigned long long *>(partials);
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
+    a += __longlong_as_double((long long)__hip_atomic_load(
+        &pb[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  const double tot = block_sum(a, lds);
+  if (threadIdx.x == 0) {
+    out_sum[0] = tot;
+    ticket[0] = 0u;
+  }
 }
 
 // ---- per-sample inertia (src/compute_inertia.hpp:11-99) --------------------
@@ -467,7 +538,8 @@ template <typename T, typename I>
 void launch_reduce_centroids(int64_t n_copies, int64_t d, int64_t k,
                              const T *sizes_copies, const T *cent_copies,
                              T *out_sizes, T *out_cent, I *empty_list,
-                             I *n_empty, int64_t list_len, hipStream_t s) {
+                             I *n_empty, int64_t list_len, hipStream_t s,
+                             bool from_zero) {
   const int64_t dk = d * k;
   if (dk > 0) {
     hipLaunchKernelGGL(sum_copies_krn<T>, dim3(ceil_div<int64_t>(dk, kRedElems)),
@@ -482,7 +554,8 @@ void launch_reduce_centroids(int64_t n_copies, int64_t d, int64_t k,
     KMD_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL((empty_list_krn<T, I>), dim3(1), dim3(1024), 0, s, k,
-                     (const T *)out_sizes, empty_list, n_empty, list_len);
+                     (const T *)out_sizes, empty_list, n_empty, list_len,
+                     from_zero);
   KMD_LAUNCH_CHECK();
 }
 
@@ -491,14 +564,16 @@ void launch_reduce_fixed_partials(int64_t n_blocks, int64_t d, int64_t k,
                                   const void *partials, const float *scale,
                                   const int *nonfinite, T *out_sizes,
                                   T *out_cent, I *empty_list, I *n_empty,
-                                  int64_t list_len, hipStream_t s) {
+                                  int64_t list_len, hipStream_t s,
+                                  bool from_zero) {
   const int64_t m = d * k + k;
   hipLaunchKernelGGL(sum_partials_fixed_krn<T>, dim3(ceil_div<int64_t>(m, kRedElems)),
                      dim3(kRedElems * kRedGroups), 0, s, n_blocks, d, k,
                      partials, scale, nonfinite, out_sizes, out_cent);
   KMD_LAUNCH_CHECK();
   hipLaunchKernelGGL((empty_list_krn<T, I>), dim3(1), dim3(1024), 0, s, k,
-                     (const T *)out_sizes, empty_list, n_empty, list_len);
+                     (const T *)out_sizes, empty_list, n_empty, list_len,
+                     from_zero);
   KMD_LAUNCH_CHECK();
 }
 
@@ -508,6 +583,21 @@ void launch_centroid_shifts(int64_t d, int64_t k, const T *a, const T *b,
   if (k == 0) return;
   hipLaunchKernelGGL(centroid_shifts_krn<T>, dim3(ceil_div<int64_t>(k, kBlock)),
                      dim3(kBlock), 0, s, d, k, a, b, out);
+  KMD_LAUNCH_CHECK();
+}
+
+int finalize_step_blocks(int64_t k) {
+  return (int)ceil_div<int64_t>(k > 0 ? k : 1, kBlock);
+}
+
+template <typename T, typename I>
+void launch_finalize_step(int64_t d, int64_t k, const I *gate, const T *sizes,
+                          const T *cur, T *next, T *shifts, T *hn_next,
+                          double *partials, unsigned *ticket, double *out_sum,
+                          hipStream_t s) {
+  hipLaunchKernelGGL((finalize_step_krn<T, I>), dim3(finalize_step_blocks(k)),
+                     dim3(kBlock), 0, s, d, k, gate, sizes, cur, next, shifts,
+                     hn_next, partials, ticket, out_sum);
   KMD_LAUNCH_CHECK();
 }
 
@@ -627,7 +717,7 @@ KMD_INST_T(double)
 #define KMD_INST_TI(T, I)                                                     \
   template void launch_reduce_centroids<T, I>(                                \
       int64_t, int64_t, int64_t, const T *, const T *, T *, T *, I *, I *,    \
-      int64_t, hipStream_t);                                                  \
+      int64_t, hipStream_t, bool);                                                  \
   template void launch_inertia<T, I>(int64_t, int64_t, int64_t, const T *,   \
                                      const T *, const T *, const I *, T *,    \
                                      hipStream_t);                            \
@@ -640,7 +730,11 @@ KMD_INST_T(double)
                                       hipStream_t);                           \
   template void launch_reduce_fixed_partials<T, I>(                           \
       int64_t, int64_t, int64_t, const void *, const float *, const int *,   \
-      T *, T *, I *, I *, int64_t, hipStream_t);
+      T *, T *, I *, I *, int64_t, hipStream_t, bool);                              \
+  template void launch_finalize_step<T, I>(int64_t, int64_t, const I *,      \
+                                           const T *, const T *, T *, T *,    \
+                                           T *, double *, unsigned *,         \
+                                           double *, hipStream_t);
 KMD_INST_TI(float, int32_t)
 KMD_INST_TI(float, int64_t)
 KMD_INST_TI(double, int32_t)

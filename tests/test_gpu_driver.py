@@ -177,15 +177,52 @@ def test_driver_nonfinite_input_propagates(kd, torch_dev):
     assert np.isnan(C).any()
 
 
-def run_driver_ex(kd, Xt, w, init, tol, max_iter, precision):
+def run_driver_ex(kd, Xt, w, init, tol, max_iter, precision, verbose=False):
     dev = torch.device("cuda", 0)
     X_d, w_d = to_dev(Xt), to_dev(w)
     init_d = to_dev(init)
     res_d = torch.empty_like(init_d)
     lab_d = torch.empty(Xt.shape[1], dtype=torch.int32, device=dev)
     n_iter, inertia, stats = kd.kmeans_lloyd_driver_ex(X_d, w_d, init_d, lab_d, res_d, tol,
-                                                       False, max_iter, precision, q())
+                                                       verbose, max_iter, precision, q())
     return to_np(lab_d).astype(np.int64), to_np(res_d), int(n_iter), inertia, stats
+
+
+@pytest.mark.parametrize("tiled", [False, True])
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "bf16x3"])
+def test_relocation_with_next_iteration_enqueued_ahead(kd, torch_dev, monkeypatch, capfd,
+                                                       prec, tiled):
+    """FP32 data whose init leaves five clusters empty in the first iteration:
+    the driver has already enqueued the second iteration when it learns of
+    them, so that iteration's main kernel must skip itself and be redone after
+    the relocation.  The fit equals the verbose one (which enqueues nothing
+    ahead), is the same on every path and precision, and follows the FP64
+    oracle."""
+    n, d, k = 40_000, 16, 24
+    Xt, init = lo.gaussian_blobs(n, d, k, seed=51, std=1.0)
+    init = init.copy()
+    init[:, 3] = 1e4
+    init[:, 7] = -1e4
+    init[:, 11:14] = 5e3
+    w = np.ones(n, np.float32)
+    if tiled:
+        monkeypatch.setenv("KMD_FORCE_TILED", "1")
+    got = run_driver_ex(kd, Xt, w, init, 1e-7, 30, prec)
+    assert got[4]["path"] == ("tiled-" if tiled else "") + (
+        "fp32" + ("" if tiled else "-mfma") if prec == "fp32" else f"{prec}+rescore")
+    seq = run_driver_ex(kd, Xt, w, init, 1e-7, 30, prec, verbose=True)
+    capfd.readouterr()
+    same_fit(got, seq)
+    monkeypatch.delenv("KMD_FORCE_TILED", raising=False)
+    ref = run_driver_ex(kd, Xt, w, init, 1e-7, 30, "fp32")
+    same_fit(got, ref)
+    # FP32 against FP64: a sample within rounding of a tie may go the other
+    # way (then the trajectories can differ by an iteration)
+    l64, C64, it64, in64 = lo.kmeans_lloyd(Xt, w, init, 1e-7, 30, dtype=np.float64)
+    assert abs(got[2] - it64) <= 1
+    assert (got[0] != l64).sum() <= 5
+    if got[2] == it64:
+        assert np.abs(got[1] - C64).max() <= 1e-5 * np.abs(C64).max()
 
 
 @pytest.mark.parametrize("shape,seed,std", [((50_000, 14, 127), 1000, 1.0),
