@@ -407,7 +407,7 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
       }
       launch_tiled_update<I>(n, d, k, xr, w, labels, ws.scale(),
                              ws.upd.template as<void>(),
-                             ws.copies.template as<void>(), s);
+                             ws.copies.template as<void>(), s, skip);
       timer.end(s);
       launch_reduce_fixed_partials<T, I>(
           1, d, k, ws.copies.template as<const void>(), ws.scale(),
@@ -564,6 +564,9 @@ static void driver_impl(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
   // iteration i still has its labels when iteration i + 1 was enqueued ahead
   // (the epilogue writes the caller's labels)
   TempBuffer lab2(sizeof(I) * (size_t)(n > 0 ? n : 1), s);
+  // valid (zero) labels from the start: kernels of an iteration whose main
+  // kernel skipped itself may still read them as cluster indices
+  KMD_HIP_CHECK(hipMemsetAsync(lab2.as<I>(), 0, lab2.bytes(), s));
   I *labs[2] = {labels, lab2.as<I>()};
   // one iteration: front half, then (unless it found empty clusters) the
   // division, shifts, their sum and the next half-norms; its scalars go to

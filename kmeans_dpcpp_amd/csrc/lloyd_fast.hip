@@ -90,11 +90,12 @@ __device__ inline void load_x(const float *__restrict__ X, int64_t n, int d,
 }
 
 // acc[q] = C[:, tile] . X[:, columns of block q] for one 32-centroid tile.
+// Ct = this lane's A operands of the tile: Ct[64 ks] = C[2 ks + half][col]
+// (the LDS layout below), so every k-step reads at an immediate offset.
 template <int KS, int S>
-__device__ inline void tile_mfma(const float *__restrict__ Ccol, int kpad,
-                                 int half, const float (&xr)[KS][S],
-                                 f32x16 (&acc)[S]) {
-  const float a0 = Ccol[half * kpad];
+__device__ inline void tile_mfma(const float *__restrict__ Ct,
+                                 const float (&xr)[KS][S], f32x16 (&acc)[S]) {
+  const float a0 = Ct[0];
 #pragma unroll
   for (int q = 0; q < S; ++q) {
     f32x16 z = {};
@@ -102,7 +103,7 @@ __device__ inline void tile_mfma(const float *__restrict__ Ccol, int kpad,
   }
 #pragma unroll
   for (int ks = 1; ks < KS; ++ks) {
-    const float a = Ccol[(2 * ks + half) * kpad];
+    const float a = Ct[64 * ks];
 #pragma unroll
     for (int q = 0; q < S; ++q)
       acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, xr[ks][q], acc[q], 0, 0,
@@ -110,12 +111,24 @@ __device__ inline void tile_mfma(const float *__restrict__ Ccol, int kpad,
   }
 }
 
-// Running argmin over the 16 rows this lane holds in tile `ct`:
-// m = 8 (r / 4) + 4 half + (r % 4), ascending in r.
+// Argmin over the 16 rows this lane holds in tile `ct` (row r is centroid
+// 32 ct + 8 (r / 4) + 4 half + r % 4, ascending in r), merged into the
+// running best: the first minimum in (tile, row) order, so ties keep the
+// lowest centroid, as one running scan in that order would.  Rows are
+// tracked as r (an inline constant) and tiles as ct; the centroid index is
+// formed once at the end (best_index).
 template <int S>
 __device__ inline void tile_argmin(const float *__restrict__ Hl, int ct,
                                    int half, const f32x16 (&acc)[S],
-                                   float (&bestv)[S], int (&besti)[S]) {
+                                   float (&bestv)[S], int (&bestt)[S],
+                                   int (&bestr)[S]) {
+  float tv[S];
+  int tr[S];
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    tv[q] = __builtin_huge_valf();
+    tr[q] = 0;
+  }
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
     const int m0 = ct * 32 + 8 * g + 4 * half;
@@ -126,12 +139,23 @@ __device__ inline void tile_argmin(const float *__restrict__ Hl, int ct,
 #pragma unroll
       for (int q = 0; q < S; ++q) {
         const float v = hv[e] - acc[q][4 * g + e];
-        const bool upd = v < bestv[q];
-        bestv[q] = upd ? v : bestv[q];
-        besti[q] = upd ? (m0 + e) : besti[q];
+        const bool upd = v < tv[q];
+        tv[q] = upd ? v : tv[q];
+        tr[q] = upd ? (4 * g + e) : tr[q];
       }
     }
   }
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    const bool upd = tv[q] < bestv[q];
+    bestv[q] = upd ? tv[q] : bestv[q];
+    bestt[q] = upd ? ct : bestt[q];
+    bestr[q] = upd ? tr[q] : bestr[q];
+  }
+}
+
+__device__ inline int best_index(int t, int r, int half) {
+  return t * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
 }
 
 // Launch parameters shared by the kernel variants.
@@ -169,7 +193,8 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int64_t n = a.n;
   const int d = a.d, k = a.k, kpad = a.kpad;
-  float *Cl = reinterpret_cast<float *>(lds_raw);  // [2*KS][kpad]
+  // Cl[((ct KS + ks) 2 + half) 32 + col] = C[2 ks + half][32 ct + col]
+  float *Cl = reinterpret_cast<float *>(lds_raw);  // 2 KS kpad floats
   float *Hl = Cl + 2 * KS * kpad;                   // [kpad]
   float *Sc = Hl + kpad;                                  // [d + 1] scales
   u64 *S64 = reinterpret_cast<u64 *>(Sc + (kMaxD + 2));  // [d*k + k] sums
@@ -202,7 +227,9 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
   }
 
   for (int i = tid; i < 2 * KS * kpad; i += NT) {
-    const int f = i / kpad, c = i - f * kpad;
+    const int cl = i & 31, hf = (i >> 5) & 1, blk = i >> 6;
+    const int ks = blk % KS, t = blk / KS;
+    const int f = 2 * ks + hf, c = 32 * t + cl;
     Cl[i] = (f < d && c < k) ? a.C[(int64_t)f * k + c] : 0.f;
   }
   for (int c = tid; c < kpad; c += NT)
@@ -264,17 +291,25 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
     }
 
     float bestv[S];
-    int besti[S];
+    int bestt[S], bestr[S], besti[S];
 #pragma unroll
     for (int q = 0; q < S; ++q) {
       bestv[q] = __builtin_huge_valf();
-      besti[q] = 0;
+      bestt[q] = 0;
+      bestr[q] = 0;
     }
+    const float *Cw = Cl + lane;
     for (int ct = 0; ct < n_ct; ++ct) {
       f32x16 acc[S];
-      tile_mfma<KS, S>(Cl + ct * 32 + col, kpad, half, xr, acc);
-      tile_argmin<S>(Hl, ct, half, acc, bestv, besti);
+      tile_mfma<KS, S>(Cw + ct * (KS * 64), xr, acc);
+      tile_argmin<S>(Hl, ct, half, acc, bestv, bestt, bestr);
     }
+    // no row of any tile below +inf: centroid 0, as the scan would give
+#pragma unroll
+    for (int q = 0; q < S; ++q)
+      besti[q] = (bestv[q] < __builtin_huge_valf())
+                     ? best_index(bestt[q], bestr[q], half)
+                     : 0;
     // merge the two halves of the column (lanes l and l ^ 32)
 #pragma unroll
     for (int q = 0; q < S; ++q) {

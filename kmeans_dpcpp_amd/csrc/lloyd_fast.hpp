@@ -183,7 +183,8 @@ size_t tiled_update_scratch_bytes(int64_t n, int64_t k);
 template <typename I>
 void launch_tiled_update(int64_t n, int64_t d, int64_t k, const float *X_r,
                          const float *w, const I *labels, const float *scale,
-                         void *scratch, void *sums, hipStream_t s);
+                         void *scratch, void *sums, hipStream_t s,
+                         const int *skip = nullptr);
 
 // ---- batched fits (batched.hip) -------------------------------------------
 struct BatchedFit {

@@ -863,7 +863,8 @@ __global__ void __launch_bounds__(256)
 template <typename I>
 __global__ void __launch_bounds__(256)
     label_hist_krn(int64_t n, int k, int64_t chunk, const I *__restrict__ labels,
-                   int *__restrict__ totals) {
+                   int *__restrict__ totals, const int *skip) {
+  if (skip != nullptr && *skip != 0) return;
   extern __shared__ int h[];
   for (int c = threadIdx.x; c < k; c += 256) h[c] = 0;
   __syncthreads();
@@ -881,7 +882,9 @@ __global__ void __launch_bounds__(256)
 // block.
 __global__ void __launch_bounds__(1024)
     cluster_scan_krn(int k, int members, const int *__restrict__ totals,
-                     int *__restrict__ start, int *__restrict__ items) {
+                     int *__restrict__ start, int *__restrict__ items,
+                     const int *skip) {
+  if (skip != nullptr && *skip != 0) return;
   __shared__ int wsum_a[1024 / kWave], wsum_b[1024 / kWave];
   __shared__ int carry_a, carry_b;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
@@ -937,7 +940,8 @@ template <typename I>
 __global__ void __launch_bounds__(256)
     scatter_krn(int64_t n, int k, int64_t chunk, const I *__restrict__ labels,
                 const int *__restrict__ start, int *__restrict__ cursor,
-                int *__restrict__ perm) {
+                int *__restrict__ perm, const int *skip) {
+  if (skip != nullptr && *skip != 0) return;
   extern __shared__ int h[];
   for (int c = threadIdx.x; c < k; c += 256) h[c] = 0;
   __syncthreads();
@@ -974,9 +978,11 @@ __global__ void __launch_bounds__(320)
     segment_sum_krn(int d, int k, const float *__restrict__ X_r,
                     const float *__restrict__ w, const float *__restrict__ scale,
                     const int *__restrict__ perm, const int *__restrict__ start,
-                    const int *__restrict__ items, u64 *__restrict__ sums) {
+                    const int *__restrict__ items, u64 *__restrict__ sums,
+                    const int *skip) {
   __shared__ int sidx[kMembers];
   __shared__ float sw[kMembers];
+  if (skip != nullptr && *skip != 0) return;
   const int item = blockIdx.x;
   if (item >= items[k]) return;
   int lo = 0, hi = k - 1;  // the last cluster whose first item is <= item
@@ -1351,7 +1357,8 @@ size_t tiled_update_scratch_bytes(int64_t n, int64_t k) {
 template <typename I>
 void launch_tiled_update(int64_t n, int64_t d, int64_t k, const float *X_r,
                          const float *w, const I *labels, const float *scale,
-                         void *scratch, void *sums, hipStream_t s) {
+                         void *scratch, void *sums, hipStream_t s,
+                         const int *skip) {
   const int64_t nb = update_blocks(n);
   const int64_t chunk = ceil_div<int64_t>(n > 0 ? n : 1, nb);
   int *totals = reinterpret_cast<int *>(scratch);  // [k + 1]
@@ -1371,20 +1378,20 @@ void launch_tiled_update(int64_t n, int64_t d, int64_t k, const float *X_r,
   const size_t hist_lds = sizeof(int) * (size_t)k;
   require(hist_lds <= kLdsPerCu, "tiled update: too many clusters for LDS counts");
   hipLaunchKernelGGL((label_hist_krn<I>), dim3((unsigned)nb), dim3(256), hist_lds, s,
-                     n, (int)k, chunk, labels, totals);
+                     n, (int)k, chunk, labels, totals, skip);
   KMD_LAUNCH_CHECK();
   hipLaunchKernelGGL(cluster_scan_krn, dim3(1), dim3(1024), 0, s, (int)k, kMembers,
-                     totals, start, items);
+                     totals, start, items, skip);
   KMD_LAUNCH_CHECK();
   hipLaunchKernelGGL((scatter_krn<I>), dim3((unsigned)nb), dim3(256), hist_lds, s, n,
-                     (int)k, chunk, labels, start, cursor, perm);
+                     (int)k, chunk, labels, start, cursor, perm, skip);
   KMD_LAUNCH_CHECK();
   // at most n / kMembers + k work items; blocks beyond items[k] return
   const int64_t max_items = n / kMembers + k;
   const int threads =
       (int)std::min<int64_t>(320, ceil_div<int64_t>(d + 1, kWave) * kWave);
   hipLaunchKernelGGL(segment_sum_krn, dim3((unsigned)max_items), dim3(threads), 0, s,
-                     (int)d, (int)k, X_r, w, scale, perm, start, items, S);
+                     (int)d, (int)k, X_r, w, scale, perm, start, items, S, skip);
   KMD_LAUNCH_CHECK();
 }
 
@@ -1411,7 +1418,7 @@ void launch_tiled_update(int64_t n, int64_t d, int64_t k, const float *X_r,
   template void launch_tiled_update<I>(int64_t, int64_t, int64_t,              \
                                        const float *, const float *,           \
                                        const I *, const float *, void *,       \
-                                       void *, hipStream_t);
+                                       void *, hipStream_t, const int *);
 KMD_INST_TILED(int32_t)
 KMD_INST_TILED(int64_t)
 
