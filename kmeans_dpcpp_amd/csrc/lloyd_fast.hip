@@ -253,8 +253,20 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
   const bool vec = a.vec != 0;
   I *labels = reinterpret_cast<I *>(a.labels);
 
+  // Streaming mode: the next super-block's samples and weights are loaded
+  // while this one is scored, so no wave waits on HBM at its start.
+  const int64_t sb_stride = (int64_t)gridDim.x * kWavesPerBlock;
+  float xn[KS][S], wn[S];
+  auto prefetch = [&](int64_t b) {
+    const int64_t pf = b * kSuper + (int64_t)S * col;
+    load_x<KS, S, FULL>(X, n, d, pf, half, vec, xn);
+#pragma unroll
+    for (int q = 0; q < S; ++q) wn[q] = (UPDATE && pf + q < n) ? w[pf + q] : 0.f;
+  };
+  if (!GATHER && (int64_t)blockIdx.x * kWavesPerBlock + wave < n_super)
+    prefetch((int64_t)blockIdx.x * kWavesPerBlock + wave);
   for (int64_t sb = (int64_t)blockIdx.x * kWavesPerBlock + wave; sb < n_super;
-       sb += (int64_t)gridDim.x * kWavesPerBlock) {
+       sb += sb_stride) {
     const int64_t pos_first = sb * kSuper + (int64_t)S * col;
     int64_t sidx[S];
     bool valid[S];
@@ -276,7 +288,7 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
         sidx[q] = a.queue[(int64_t)lo * a.seg_cap + (p - seg_off[lo])];
       }
     }
-    float xr[KS][S];
+    float xr[KS][S], wv[S];
     if (GATHER) {
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
@@ -286,8 +298,16 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
           xr[ks][q] = ((FULL || f < d) && valid[q]) ? X[(int64_t)f * n + sidx[q]]
                                                      : 0.f;
       }
+#pragma unroll
+      for (int q = 0; q < S; ++q) wv[q] = (UPDATE && valid[q]) ? w[sidx[q]] : 0.f;
     } else {
-      load_x<KS, S, FULL>(X, n, d, pos_first, half, vec, xr);
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int q = 0; q < S; ++q) xr[ks][q] = xn[ks][q];
+#pragma unroll
+      for (int q = 0; q < S; ++q) wv[q] = wn[q];
+      if (sb + sb_stride < n_super) prefetch(sb + sb_stride);
     }
 
     float bestv[S];
@@ -328,7 +348,7 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
 #pragma unroll
       for (int q = 0; q < S; ++q) {
         if (valid[q]) {
-          const float ws = w[sidx[q]];
+          const float ws = wv[q];
           const int lab = besti[q];
           if (fixed) {
             // |w x| 2^E_f < 2^30: the rounded element fits an int32; the
@@ -510,7 +530,7 @@ KernFn pick_nt(int64_t d) {
 
 // Threads per block of the MFMA kernels: 8 waves share one copy of the
 // centroids and of the per-block sums in LDS.
-constexpr int kThreads = 512;
+constexpr int kThreads = 768;
 
 // A resolved launch: kernel variant, block size, dynamic LDS and the
 // persistent grid (occupancy-limited, never more blocks than work).
