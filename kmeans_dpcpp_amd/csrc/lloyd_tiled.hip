@@ -41,7 +41,7 @@ typedef unsigned long long u64;
 
 constexpr int kTNT = 512;                 // threads per block (8 waves)
 constexpr int kTW = kTNT / kWave;         // waves per block
-constexpr int kChunkBytes = 32 * 1024;    // LDS bytes of one centroid chunk
+constexpr int kChunkBytes = 52 * 1024;    // LDS bytes of one centroid chunk
                                           // (two of them: double buffered)
 constexpr int kMaxSegments = 1024;        // queue segments the re-score reads
 constexpr int kMaxTiledD = 128;
@@ -516,7 +516,11 @@ inline int bf16_d_of(int64_t d) {
   return 128;
 }
 inline int kk_of(int D, int T) { return (T * D + 3 + 15) / 16; }
-constexpr int bf16_s(int D, int T) { return D * T < 128 ? 2 : 1; }
+constexpr int bf16_s(int D, int T) { return D * T <= 128 ? 2 : 1; }
+// per-tile maxima kept in LDS (candidate tiles against the final best): for
+// one sample block per wave, and for two at D * T == 128 (where the two
+// blocks' maxima and the double-buffered chunks still fit in LDS)
+constexpr bool bf16_tm(int D, int T) { return bf16_s(D, T) == 1 || D * T == 128; }
 
 struct TiledMixedArgs {
   int64_t n;
@@ -540,7 +544,8 @@ __device__ inline float max3f(float a, float b, float c) {
 }
 
 template <int D, int T, typename I>
-__global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
+__global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    tiled_mixed_krn(TiledMixedArgs a) {
   // an iteration enqueued ahead of a relocation: nothing to compute
   if (a.skip != nullptr && *a.skip != 0) return;
   constexpr int KD = D / 16, KK = (T * D + 3 + 15) / 16, XH = D / 2;
@@ -551,7 +556,7 @@ __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
   constexpr int kBuf = CPC * KK * 64 * 16;  // bytes per buffer (1 KB multiple)
   // S == 1: each lane's per-tile best is kept in LDS (bf16, rounded up) so
   // the candidate tiles are chosen against the final best
-  constexpr bool kTM = (S == 1);
+  constexpr bool kTM = bf16_tm(D, T);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   int &q_used = *reinterpret_cast<int *>(lds_raw + 2 * kBuf);
   unsigned short *tml = reinterpret_cast<unsigned short *>(lds_raw + 2 * kBuf + 16);
@@ -601,11 +606,17 @@ __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
 #pragma unroll
       for (int q = 0; q < S; ++q) ss[q] = 0.f;
       const bool whole = vec && active && (s0 + S <= n);
+      // the row stride, opaque to loop-invariant code motion: otherwise the
+      // XH row pointers are hoisted out of the round loop and stay live
+      // (2 XH registers) through the MFMA loop
+      int64_t n_row = n;
+      asm volatile("" : "+s"(n_row));
+      const float *xs = a.X + s0;
 #pragma unroll
       for (int i = 0; i < XH; ++i) {
         const int f = 16 * (i / 8) + 8 * half + (i % 8);
         float xv[S];
-        const float *row = a.X + (int64_t)f * n + s0;
+        const float *row = xs + (int64_t)f * n_row;
         if (f < d && whole) {
           if constexpr (S == 2) {
             const float2 v = *reinterpret_cast<const float2 *>(row);
@@ -665,26 +676,44 @@ __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
         for (int t = 0; t < cnt; ++t) {
           const bf16x8 *At = Al + t * KK * 64 + lane;
           f32x16 acc[S];
+          // The tile's A operands are read from LDS in groups of up to kG,
+          // each group issued back to back and waited for once: an empty
+          // statement that takes the group's values keeps the scheduler from
+          // moving each read next to its MFMA (which exposes the LDS latency
+          // once per k-step).
+          constexpr int kG = KK < 10 ? KK : 9;
 #pragma unroll
-          for (int kk = 0; kk < KK; ++kk) {
-            const bf16x8 av = At[kk * 64];
+          for (int k0 = 0; k0 < KK; k0 += kG) {
+            bf16x8 ava[kG];
 #pragma unroll
-            for (int q = 0; q < S; ++q) {
-              bf16x8 bv;
-              if (kk < KD)
-                bv = bhi[kk][q];
-              else if (T == 3 && kk < 2 * KD)
-                bv = bhi[(kk - KD) % KD][q];
-              else if (T == 3 && kk < 3 * KD)
-                bv = blo[(kk - 2 * KD) % KD][q];
-              else
-                bv = bh;
-              if (kk == 0) {
-                const f32x16 z = {};
-                acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, z, 0, 0, 0);
-              } else {
-                acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[q], 0,
-                                                                  0, 0);
+            for (int j = 0; j < kG; ++j)
+              if (k0 + j < KK) ava[j] = At[(k0 + j) * 64];
+#pragma unroll
+            for (int j = 0; j < kG; ++j)
+              if (k0 + j < KK) asm volatile("" : : "v"(ava[j]));
+#pragma unroll
+            for (int j = 0; j < kG; ++j) {
+              const int kk = k0 + j;
+              if (kk >= KK) break;
+              const bf16x8 av = ava[j];
+#pragma unroll
+              for (int q = 0; q < S; ++q) {
+                bf16x8 bv;
+                if (kk < KD)
+                  bv = bhi[kk][q];
+                else if (T == 3 && kk < 2 * KD)
+                  bv = bhi[(kk - KD) % KD][q];
+                else if (T == 3 && kk < 3 * KD)
+                  bv = blo[(kk - 2 * KD) % KD][q];
+                else
+                  bv = bh;
+                if (kk == 0) {
+                  const f32x16 z = {};
+                  acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, z, 0, 0, 0);
+                } else {
+                  acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[q], 0,
+                                                                    0, 0);
+                }
               }
             }
           }
@@ -715,7 +744,7 @@ __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
                 // round toward +inf to bf16: never below the tile's best
                 const unsigned b = __float_as_uint(tmax[q]);
                 const unsigned up = (b & 0x80000000u) ? (b >> 16) : ((b + 0xffffu) >> 16);
-                tml[(wave * 32 + ct) * kWave + lane] = (unsigned short)up;
+                tml[((wave * S + q) * 32 + ct) * kWave + lane] = (unsigned short)up;
               }
             } else {
               if (tmax[q] >= b1[q] - ecut[q]) cmask[q] |= 1u << (ct & 31);
@@ -750,7 +779,8 @@ __global__ void __launch_bounds__(kTNT) tiled_mixed_krn(TiledMixedArgs a) {
           unsigned bits = 0u;
           const int nt = min(n_ct, 32);
           for (int t = 0; t < nt; ++t) {
-            const float v = __uint_as_float((unsigned)tml[(wave * 32 + t) * kWave + lane] << 16);
+            const float v = __uint_as_float(
+                (unsigned)tml[((wave * S + q) * 32 + t) * kWave + lane] << 16);
             if (v >= thr) bits |= 1u << t;
           }
           cmask[q] = bits;
@@ -1091,7 +1121,9 @@ size_t mixed_tiled_lds_bytes(int64_t d, int T) {
   if (cpc < 1) cpc = 1;
   const int S = bf16_s(bf16_d_of(d), T);
   return 2 * (size_t)cpc * KK * 64 * 16 + 16 +
-         (S == 1 ? (size_t)kTW * 32 * kWave * sizeof(unsigned short) : 0);
+         (bf16_tm(bf16_d_of(d), T)
+              ? (size_t)kTW * S * 32 * kWave * sizeof(unsigned short)
+              : 0);
 }
 
 template <typename F>
