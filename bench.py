@@ -55,6 +55,8 @@ def parse():
                         "continuous batching (fits/s, p50/p99 latency) beside "
                         "the plain per-request FP32 driver")
     p.add_argument("--rate", type=float, default=200.0, help="c4 arrivals per s")
+    p.add_argument("--large-precision", default="bf16", choices=["fp32", "bf16x3", "bf16"],
+                   help="c4: precision of the large requests' fits")
     p.add_argument("--requests", type=int, default=400, help="c4 requests")
     p.add_argument("--fits", type=int, default=512)
     p.add_argument("--copy-probe-mib", type=int, default=0,
@@ -375,11 +377,13 @@ def bench_c4(args):
     torch.cuda.synchronize()
     tol, max_iter = 1e-4, 100
 
-    # served: continuous batching; the large fits use BF16x3 + exact re-score
-    # (labels and centroids identical to FP32) on their own stream
+    # served: continuous batching; the large fits use the BF16 cross term +
+    # exact re-score (labels and centroids identical to FP32) on their own
+    # stream
     budget = int(0.6 * torch.cuda.get_device_properties(0).total_memory)
     out = []
-    with serving.Server(budget, iterations_per_tick=1, large_precision="bf16x3") as srv:
+    with serving.Server(budget, iterations_per_tick=1,
+                        large_precision=args.large_precision) as srv:
         t0 = time.perf_counter()
         for (ta, X_t, w, init) in reqs:
             now = time.perf_counter() - t0
@@ -427,7 +431,8 @@ def bench_c4(args):
         "config": {"workload": f"served stream: {len(out)} Poisson arrivals at "
                                f"{args.rate:g} req/s, 95% batched-fit sizes, 5% "
                                f"n=2M d=96 k=512; tol 1e-4, max_iter 100; "
-                               f"continuous batching, footprint admission"},
+                               f"continuous batching, footprint admission; "
+                               f"large fits in {args.large_precision}"},
         "latency_s": {"p50": round(lat["p50"], 4), "p99": round(lat["p99"], 4),
                       "mean": round(lat["mean"], 4)},
         "plain_per_request_fp32": {

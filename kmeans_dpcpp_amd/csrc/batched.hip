@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "fixed_point.hpp"
 #include "lloyd_fast.hpp"
 
 namespace kmd {
@@ -74,6 +75,7 @@ struct BFit {
   float *hn, *sizes;                        // (k + 1)
   int *list, *totals, *cursor, *start, *citems;  // (k + 1)
   int *nonfinite;   // (1)
+  unsigned *maxbits;  // (d + 1) bit patterns of max |x_f| and max |w|
 };
 
 struct AItem {  // assignment tile
@@ -83,6 +85,10 @@ struct AItem {  // assignment tile
 struct CItem {  // update chunk
   int fit;
   int first, count;
+};
+struct TItem {  // transpose tile: 32 samples x 32 features of one fit
+  int fit;
+  int s0, f0;
 };
 
 __device__ inline double block_sum256(double v, double *lds) {
@@ -642,6 +648,84 @@ __global__ void __launch_bounds__(256)
   for (int64_t i = threadIdx.x; i < m; i += 256) F.C[1][i] = F.C[0][i];
 }
 
+// ---- set-up of many new fits at once ---------------------------------------
+// (the batched call adds all its fits first; these replace the per-fit
+// transpose, maxima and scales launches with one launch each)
+
+// state (active while max_iter > 0) and cleared maxima of each new fit
+__global__ void __launch_bounds__(256)
+    bt_prep_state_krn(int m, const BFit *__restrict__ fits) {
+  const int f = blockIdx.x * 256 + threadIdx.x;
+  if (f >= m) return;
+  const BFit F = fits[f];
+  BState S{};
+  S.active = F.max_iter > 0 ? 1 : 0;
+  F.st[0] = S;
+  for (int j = 0; j <= F.d; ++j) F.maxbits[j] = 0u;
+}
+
+// X_r = transpose of X_t for one 32 x 32 tile of one fit (transpose_krn's
+// tile, per item)
+__global__ void __launch_bounds__(256)
+    bt_transpose_krn(const BFit *__restrict__ fits, const TItem *__restrict__ items) {
+  __shared__ float tile[32][33];
+  const TItem it = items[blockIdx.x];
+  const BFit F = fits[it.fit];
+  const int64_t n = F.n;
+  const int d = F.d;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int f = it.f0 + r;
+    const int64_t s = (int64_t)it.s0 + tx;
+    tile[r][tx] = (f < d && s < n) ? F.X[(int64_t)f * n + s] : 0.f;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t s = (int64_t)it.s0 + r;
+    const int f = it.f0 + tx;
+    if (s < n && f < d) F.xr[s * d + f] = tile[tx][r];
+  }
+}
+
+// maxbits[f] = max of (bits of X[f, s]) & 0x7fffffff over the item's
+// samples (f == d: the weights), as absmax_bits_krn computes it per fit
+__global__ void __launch_bounds__(256)
+    bt_absmax_krn(const BFit *__restrict__ fits, const CItem *__restrict__ items) {
+  const CItem it = items[blockIdx.x];
+  const BFit F = fits[it.fit];
+  for (int f = 0; f <= F.d; ++f) {
+    const float *row = (f < F.d) ? F.X + (int64_t)f * F.n : F.w;
+    unsigned m = 0u;
+    for (int i = threadIdx.x; i < it.count; i += 256)
+      m = max(m, __float_as_uint(row[it.first + i]) & 0x7fffffffu);
+    for (int o = kWave / 2; o > 0; o >>= 1)
+      m = max(m, (unsigned)__shfl_xor((int)m, o, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(&F.maxbits[f], m);
+  }
+}
+
+// each new fit's fixed-point scales and non-finite flag (fixed_point.hpp)
+__global__ void __launch_bounds__(256)
+    bt_scales_krn(int m, const BFit *__restrict__ fits) {
+  const int f = blockIdx.x * 256 + threadIdx.x;
+  if (f >= m) return;
+  const BFit F = fits[f];
+  fixed_point_scales_from_bits(F.n, F.d, F.maxbits, F.scale, F.nonfinite);
+}
+
+// out[i] = *flags[i]: the new fits' non-finite flags read back with one copy
+__global__ void __launch_bounds__(256)
+    bt_gather_flags_krn(int m, const int *const *__restrict__ flags, int *out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < m) out[i] = *flags[i];
+}
+
+// out[f] = fit f's state: the host reads every fit's state with one copy
+__global__ void __launch_bounds__(256)
+    bt_gather_state_krn(int nf, const BFit *__restrict__ fits, BState *out) {
+  const int f = blockIdx.x * 256 + threadIdx.x;
+  if (f < nf) out[f] = fits[f].st[0];
+}
 
 // ---- host ---------------------------------------------------------------------
 inline int ks_bucket(int d) { return d <= 16 ? 8 : (d <= 32 ? 16 : 32); }
@@ -684,7 +768,7 @@ inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 // Byte layout of one fit's workspace arena.
 struct Arena {
   size_t st, xr, perm, dist, sums, scale, hn, sizes, list, totals, cursor, start,
-      citems, nonfinite, total;
+      citems, nonfinite, maxbits, total;
   Arena(int64_t n, int64_t d, int64_t k) {
     size_t o = 0;
     auto take = [&](size_t b) {
@@ -706,6 +790,7 @@ struct Arena {
     start = take(sizeof(int) * (size_t)(k + 1));
     citems = take(sizeof(int) * (size_t)(k + 1));
     nonfinite = take(sizeof(int));
+    maxbits = take(sizeof(unsigned) * (size_t)(d + 1));
     total = o;
   }
 };
@@ -732,7 +817,19 @@ struct BatchEngine {
   ~BatchEngine();
   BatchEngine(const BatchEngine &) = delete;
   BatchEngine &operator=(const BatchEngine &) = delete;
-  int64_t add(const BatchedFit &f, bool check_now);
+  // defer_prep: the fit's set-up (transpose, maxima, scales) waits for
+  // prep_pending(), which does it for all deferred fits in a few launches
+  // arena: when given, the fit's workspace (arena_bytes(f) bytes, 256-byte
+  // aligned) instead of an allocation of its own
+  int64_t add(const BatchedFit &f, bool check_now, bool defer_prep = false,
+              char *arena = nullptr);
+  static size_t arena_bytes(const BatchedFit &f);
+  void prep_pending();
+  // defer_epilogue: fits that stop are set aside and get their epilogue
+  // together at flush_stopped() (the one-call batched path)
+  bool defer_epilogue_ = false;
+  std::vector<Slot *> stopped_;
+  void flush_stopped();
   int64_t step(int iterations, bool use_graph);
   std::vector<int64_t> take_finished();
   int64_t resident_bytes() const;
@@ -745,11 +842,13 @@ struct BatchEngine {
   hipStream_t user_ = nullptr, s_ = nullptr;
   hipEvent_t ev_ = nullptr;
   std::vector<std::unique_ptr<Slot>> slots_;
-  std::vector<Slot *> unchecked_;
+  std::vector<Slot *> unchecked_, pending_prep_;
   std::vector<BFit> host_fits_;
   std::vector<AItem> ai_[3];
   std::vector<CItem> ci_;
-  std::unique_ptr<TempBuffer> fits_d_, ai_d_[3], ci_d_, seg_total_;
+  std::unique_ptr<TempBuffer> fits_d_, ai_d_[3], ci_d_, seg_total_, st_d_;
+  // every fit's state in one copy: gathered on the device, read back once
+  void read_states(const BFit *dfits, int nf, std::vector<BState> &out);
   int64_t seg_bound_ = 1, next_id_ = 0, launches_ = 0;
   bool dirty_ = true;
   hipGraph_t graph_ = nullptr;
@@ -763,6 +862,7 @@ struct BatchEngine::Slot {
   TempBuffer arena{0, nullptr};
   BFit dev{};
   bool done = false;
+  bool stopped = false;  // stopped, epilogue deferred
 };
 
 BatchEngine::BatchEngine(hipStream_t user_stream) : user_(user_stream) {
@@ -783,6 +883,7 @@ BatchEngine::~BatchEngine() {
   for (auto &b : ai_d_) b.reset();
   ci_d_.reset();
   seg_total_.reset();
+  st_d_.reset();
   (void)hipStreamSynchronize(s_);
   (void)hipEventDestroy(ev_);
   (void)hipStreamDestroy(s_);
@@ -797,17 +898,27 @@ void BatchEngine::drop_graph() {
 
 // check_now: verify the fit's inputs are finite before returning (one host
 // synchronisation); otherwise that happens at the next step for all new fits.
-int64_t BatchEngine::add(const BatchedFit &f, bool check_now) {
+size_t BatchEngine::arena_bytes(const BatchedFit &f) {
+  return Arena(f.n, f.d, f.k).total;
+}
+
+int64_t BatchEngine::add(const BatchedFit &f, bool check_now, bool defer_prep,
+                         char *arena) {
   require(batched_fit_supported(f.n, f.d, f.k), "batched fit: unsupported shape");
   auto slot = std::make_unique<Slot>();
   slot->desc = f;
   slot->id = next_id_++;
   const Arena A(f.n, f.d, f.k);
-  // the caller's arrays are ready once its stream reaches this point
-  KMD_HIP_CHECK(hipEventRecord(ev_, user_));
-  KMD_HIP_CHECK(hipStreamWaitEvent(s_, ev_, 0));
-  slot->arena = TempBuffer(A.total, s_);
-  char *b = slot->arena.as<char>();
+  if (!defer_prep) {
+    // the caller's arrays are ready once its stream reaches this point
+    KMD_HIP_CHECK(hipEventRecord(ev_, user_));
+    KMD_HIP_CHECK(hipStreamWaitEvent(s_, ev_, 0));
+  }
+  char *b = arena;
+  if (b == nullptr) {
+    slot->arena = TempBuffer(A.total, s_);
+    b = slot->arena.as<char>();
+  }
   BFit &F = slot->dev;
   F.n = f.n;
   F.d = (int)f.d;
@@ -833,12 +944,17 @@ int64_t BatchEngine::add(const BatchedFit &f, bool check_now) {
   F.start = reinterpret_cast<int *>(b + A.start);
   F.citems = reinterpret_cast<int *>(b + A.citems);
   F.nonfinite = reinterpret_cast<int *>(b + A.nonfinite);
-  hipLaunchKernelGGL(bt_set_state_krn, dim3(1), dim3(1), 0, s_, F.st,
-                     f.max_iter > 0 ? 1 : 0);
-  KMD_LAUNCH_CHECK();
-  // sample-major copy and the single-fit driver's fixed-point scales
-  launch_transpose(f.n, f.d, f.X, F.xr, s_);
-  launch_fixed_point_scales(f.n, f.d, f.X, f.w, F.scale, F.nonfinite, s_);
+  F.maxbits = reinterpret_cast<unsigned *>(b + A.maxbits);
+  if (defer_prep) {
+    pending_prep_.push_back(slot.get());
+  } else {
+    hipLaunchKernelGGL(bt_set_state_krn, dim3(1), dim3(1), 0, s_, F.st,
+                       f.max_iter > 0 ? 1 : 0);
+    KMD_LAUNCH_CHECK();
+    // sample-major copy and the single-fit driver's fixed-point scales
+    launch_transpose(f.n, f.d, f.X, F.xr, s_);
+    launch_fixed_point_scales(f.n, f.d, f.X, f.w, F.scale, F.nonfinite, s_);
+  }
   unchecked_.push_back(slot.get());
   slots_.push_back(std::move(slot));
   dirty_ = true;
@@ -847,13 +963,71 @@ int64_t BatchEngine::add(const BatchedFit &f, bool check_now) {
   return id;
 }
 
-void BatchEngine::check_inputs() {
-  if (unchecked_.empty()) return;
-  std::vector<int> bad(unchecked_.size());
-  for (size_t i = 0; i < unchecked_.size(); ++i)
-    KMD_HIP_CHECK(hipMemcpyAsync(&bad[i], unchecked_[i]->dev.nonfinite, sizeof(int),
-                                 hipMemcpyDeviceToHost, s_));
+void BatchEngine::prep_pending() {
+  if (pending_prep_.empty()) return;
+  // the caller's arrays are ready once its stream reaches this point
+  KMD_HIP_CHECK(hipEventRecord(ev_, user_));
+  KMD_HIP_CHECK(hipStreamWaitEvent(s_, ev_, 0));
+  const int m = (int)pending_prep_.size();
+  std::vector<BFit> fits(m);
+  std::vector<TItem> ti;
+  std::vector<CItem> ci;
+  for (int f = 0; f < m; ++f) {
+    const BFit &F = pending_prep_[f]->dev;
+    fits[f] = F;
+    for (int64_t s0 = 0; s0 < F.n; s0 += 32)
+      for (int f0 = 0; f0 < F.d; f0 += 32) ti.push_back(TItem{f, (int)s0, f0});
+    for (int64_t p = 0; p < F.n; p += kChunk)
+      ci.push_back(CItem{f, (int)p, (int)std::min<int64_t>(kChunk, F.n - p)});
+  }
+  TempBuffer dF(sizeof(BFit) * m, s_), dT(sizeof(TItem) * std::max<size_t>(1, ti.size()), s_),
+      dC(sizeof(CItem) * std::max<size_t>(1, ci.size()), s_);
+  KMD_HIP_CHECK(hipMemcpyAsync(dF.as<void>(), fits.data(), sizeof(BFit) * m,
+                               hipMemcpyHostToDevice, s_));
+  if (!ti.empty())
+    KMD_HIP_CHECK(hipMemcpyAsync(dT.as<void>(), ti.data(), sizeof(TItem) * ti.size(),
+                                 hipMemcpyHostToDevice, s_));
+  if (!ci.empty())
+    KMD_HIP_CHECK(hipMemcpyAsync(dC.as<void>(), ci.data(), sizeof(CItem) * ci.size(),
+                                 hipMemcpyHostToDevice, s_));
+  const unsigned gm = (unsigned)ceil_div<int64_t>(m, 256);
+  hipLaunchKernelGGL(bt_prep_state_krn, dim3(gm), dim3(256), 0, s_, m, dF.as<const BFit>());
+  KMD_LAUNCH_CHECK();
+  if (!ti.empty()) {
+    hipLaunchKernelGGL(bt_transpose_krn, dim3((unsigned)ti.size()), dim3(256), 0, s_,
+                       dF.as<const BFit>(), dT.as<const TItem>());
+    KMD_LAUNCH_CHECK();
+  }
+  if (!ci.empty()) {
+    hipLaunchKernelGGL(bt_absmax_krn, dim3((unsigned)ci.size()), dim3(256), 0, s_,
+                       dF.as<const BFit>(), dC.as<const CItem>());
+    KMD_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(bt_scales_krn, dim3(gm), dim3(256), 0, s_, m, dF.as<const BFit>());
+  KMD_LAUNCH_CHECK();
+  // the host vectors are copied before they go out of scope
   KMD_HIP_CHECK(hipStreamSynchronize(s_));
+  pending_prep_.clear();
+}
+
+void BatchEngine::check_inputs() {
+  prep_pending();
+  if (unchecked_.empty()) return;
+  const int m = (int)unchecked_.size();
+  std::vector<int> bad(m);
+  std::vector<const int *> ptrs(m);
+  for (int i = 0; i < m; ++i) ptrs[i] = unchecked_[i]->dev.nonfinite;
+  {
+    TempBuffer dp(sizeof(const int *) * m, s_), dv(sizeof(int) * m, s_);
+    KMD_HIP_CHECK(hipMemcpyAsync(dp.as<void>(), ptrs.data(), sizeof(const int *) * m,
+                                 hipMemcpyHostToDevice, s_));
+    hipLaunchKernelGGL(bt_gather_flags_krn, dim3((unsigned)ceil_div<int64_t>(m, 256)),
+                       dim3(256), 0, s_, m, dp.as<const int *const>(), dv.as<int>());
+    KMD_LAUNCH_CHECK();
+    KMD_HIP_CHECK(hipMemcpyAsync(bad.data(), dv.as<int>(), sizeof(int) * m,
+                                 hipMemcpyDeviceToHost, s_));
+    KMD_HIP_CHECK(hipStreamSynchronize(s_));
+  }
   std::vector<Slot *> failed;
   for (size_t i = 0; i < unchecked_.size(); ++i)
     if (bad[i]) failed.push_back(unchecked_[i]);
@@ -990,11 +1164,8 @@ void BatchEngine::epilogue(const std::vector<Slot *> &done) {
   KMD_LAUNCH_CHECK();
   hipLaunchKernelGGL(bt_copy_result_krn, dim3(nf), dim3(256), 0, s_, F);
   KMD_LAUNCH_CHECK();
-  std::vector<BState> st(nf);
-  for (int f = 0; f < nf; ++f)
-    KMD_HIP_CHECK(hipMemcpyAsync(&st[f], fits[f].st, sizeof(BState),
-                                 hipMemcpyDeviceToHost, s_));
-  KMD_HIP_CHECK(hipStreamSynchronize(s_));
+  std::vector<BState> st;
+  read_states(F, nf, st);
   for (int f = 0; f < nf; ++f) {
     Slot *sl = done[f];
     if (sl->desc.out_iters) *sl->desc.out_iters = st[f].it;
@@ -1002,6 +1173,20 @@ void BatchEngine::epilogue(const std::vector<Slot *> &done) {
     sl->done = true;
     finished_.push_back(sl->id);
   }
+}
+
+void BatchEngine::read_states(const BFit *dfits, int nf, std::vector<BState> &out) {
+  out.resize(nf);
+  if (nf == 0) return;
+  const size_t bytes = sizeof(BState) * (size_t)nf;
+  if (!st_d_ || st_d_->bytes() < bytes)
+    st_d_ = std::make_unique<TempBuffer>(bytes, s_);
+  hipLaunchKernelGGL(bt_gather_state_krn, dim3((unsigned)ceil_div<int64_t>(nf, 256)),
+                     dim3(256), 0, s_, nf, dfits, st_d_->as<BState>());
+  KMD_LAUNCH_CHECK();
+  KMD_HIP_CHECK(hipMemcpyAsync(out.data(), st_d_->as<BState>(), bytes,
+                               hipMemcpyDeviceToHost, s_));
+  KMD_HIP_CHECK(hipStreamSynchronize(s_));
 }
 
 int64_t BatchEngine::step(int iterations, bool use_graph) {
@@ -1022,22 +1207,31 @@ int64_t BatchEngine::step(int iterations, bool use_graph) {
   }
   launches_ += iterations;
   // read the masks; fits that stopped get their epilogue
-  std::vector<BState> st(host_fits_.size());
-  for (size_t f = 0; f < host_fits_.size(); ++f)
-    KMD_HIP_CHECK(hipMemcpyAsync(&st[f], host_fits_[f].st, sizeof(BState),
-                                 hipMemcpyDeviceToHost, s_));
-  KMD_HIP_CHECK(hipStreamSynchronize(s_));
+  std::vector<BState> st;
+  read_states(fits_d_->as<const BFit>(), (int)host_fits_.size(), st);
   std::vector<Slot *> stopping;
   int64_t active = 0;
   for (size_t f = 0; f < slots_.size(); ++f) {
-    if (slots_[f]->done) continue;
+    if (slots_[f]->done || slots_[f]->stopped) continue;
     if (st[f].active)
       ++active;
     else
       stopping.push_back(slots_[f].get());
   }
-  epilogue(stopping);
+  if (defer_epilogue_) {
+    for (Slot *sl : stopping) {
+      sl->stopped = true;
+      stopped_.push_back(sl);
+    }
+  } else {
+    epilogue(stopping);
+  }
   return active;
+}
+
+void BatchEngine::flush_stopped() {
+  epilogue(stopped_);
+  stopped_.clear();
 }
 
 std::vector<int64_t> BatchEngine::take_finished() {
@@ -1073,21 +1267,43 @@ int64_t batch_engine_launches(const BatchEngine *e) { return e->launches_; }
 void batched_lloyd(const std::vector<BatchedFit> &in, int graph_replays,
                    hipStream_t user_stream, BatchedStats *stats) {
   if (in.empty()) return;
+  // one allocation for all fits' workspaces (declared before the engine, so
+  // it is released after the engine has drained its stream)
+  std::vector<size_t> off(in.size());
+  size_t total = 0;
+  for (size_t i = 0; i < in.size(); ++i) {
+    require(batched_fit_supported(in[i].n, in[i].d, in[i].k),
+            "batched fit: unsupported shape");
+    off[i] = total;
+    total += (BatchEngine::arena_bytes(in[i]) + 255) & ~size_t(255);
+  }
+  TempBuffer pool(total, user_stream);
   BatchEngine eng(user_stream);
   int64_t max_iter = 0;
-  for (const BatchedFit &f : in) {
-    eng.add(f, false);
-    max_iter = std::max(max_iter, f.max_iter);
+  for (size_t i = 0; i < in.size(); ++i) {
+    eng.add(in[i], false, true, pool.as<char>() + off[i]);
+    max_iter = std::max(max_iter, in[i].max_iter);
   }
   const int R = std::max(1, graph_replays);
   int64_t replays = 0;
+  int64_t in_set = (int64_t)in.size();
+  eng.defer_epilogue_ = true;
   // a fit with max_iter 0 (or converged) finishes at the first check
   for (;;) {
     const int64_t r = std::min<int64_t>(R, std::max<int64_t>(0, max_iter - replays));
     const int64_t active = eng.step((int)r, true);
     replays += r;
     if (active == 0) break;
+    // once half the set has stopped, drop the stopped fits (their results are
+    // written) so the work lists, grids and the captured iteration cover only
+    // the fits still running
+    if (2 * active <= in_set) {
+      eng.flush_stopped();
+      (void)eng.take_finished();
+      in_set = active;
+    }
   }
+  eng.flush_stopped();
   if (stats) stats->graph_replays = replays;
   (void)hipStreamSynchronize(user_stream);
 }

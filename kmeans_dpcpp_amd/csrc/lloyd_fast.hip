@@ -38,6 +38,7 @@
 // index on ties) are those of src/device_functions.hpp:139-162.  Relative to
 // the sequential FMA chain of the reference the dot products differ by a few
 // ulps, so labels can differ only on sub-ulp margins.
+#include "fixed_point.hpp"
 #include "lloyd_fast.hpp"
 
 namespace kmd {
@@ -455,31 +456,10 @@ __global__ void __launch_bounds__(256)
   if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(&maxbits[f], m);
 }
 
-// scale[f] = 2^E_f with max|w| max|x_f| 2^E_f < 2^30 (each rounded element
-// fits an int32) and n max|w| max|x_f| 2^E_f < 2^61 (the sum fits an
-// int64); scale[d] likewise for the counts.  nonfinite[0] = 1 if any value
-// is infinite or NaN.
+// Scales from the maxima (fixed_point.hpp), one thread.
 __global__ void scales_krn(int64_t n, int d, const unsigned *maxbits,
                            float *scale, int *nonfinite) {
-  const unsigned wb = maxbits[d];
-  int bad = wb >= 0x7f800000u;
-  const double wmax = (double)__uint_as_float(wb);
-  for (int f = 0; f <= d; ++f) {
-    const unsigned b = maxbits[f];
-    bad |= b >= 0x7f800000u;
-    const double m = wmax * (f < d ? (double)__uint_as_float(b) : 1.0);
-    int e = 0;
-    if (m > 0.0 && m < 1e300) {
-      int ex_elem, ex_sum;
-      frexp(m, &ex_elem);                       // m < 2^ex_elem
-      frexp(m * (double)(n > 0 ? n : 1), &ex_sum);
-      const int e1 = 30 - ex_elem, e2 = 61 - ex_sum;
-      e = e1 < e2 ? e1 : e2;
-      e = e > 120 ? 120 : (e < -120 ? -120 : e);
-    }
-    scale[f] = ldexpf(1.f, e);
-  }
-  nonfinite[0] = bad;
+  fixed_point_scales_from_bits(n, d, maxbits, scale, nonfinite);
 }
 
 // ---- launch planning -----------------------------------------------------
