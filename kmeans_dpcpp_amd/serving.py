@@ -13,8 +13,9 @@ requests on top of the same arithmetic:
   is in flight; every tick advances all of them by ``iterations_per_tick``
   Lloyd iterations and retires the ones that stopped.  Their samples and
   centroid state stay on the device from admission to completion.
-* Larger requests run through the single-fit driver on a worker thread with
-  a stream of its own, so their work overlaps the batch ticks on the GPU.
+* Larger requests run through the single-fit driver on worker threads, each
+  with a stream of its own, so their work overlaps the batch ticks (and each
+  other) on the GPU.
 
 Each request's result (labels, centroids, iteration count, inertia) is the
 one ``kmeans_lloyd_driver`` gives it alone (the engine's results are
@@ -92,13 +93,14 @@ class Server:
 
     ``submit`` may be called from any thread; a loop thread admits requests
     (first come first served, by footprint against ``hbm_budget_bytes``),
-    ticks the batch engine and hands large fits to a worker thread.
+    ticks the batch engine and hands large fits to ``large_workers`` worker
+    threads.
     ``engine_factory`` / ``large_runner`` replace the GPU engine and the
     single-fit runner (for host-only tests of the loop's logic)."""
 
     def __init__(self, hbm_budget_bytes, iterations_per_tick=1, use_graph=False,
                  large_precision="fp32", engine_factory=None, large_runner=None,
-                 footprint=request_footprint, serves=engine_serves):
+                 footprint=request_footprint, serves=engine_serves, large_workers=2):
         self.budget = int(hbm_budget_bytes)
         self.iterations_per_tick = int(iterations_per_tick)
         self.use_graph = bool(use_graph)
@@ -115,12 +117,15 @@ class Server:
         self.ticks = 0
         self._running = True
         self._inflight = {}
+        self._local = threading.local()
         self._loop = threading.Thread(target=self._run_loop, name="kmd-serve",
                                       daemon=True)
-        self._worker = threading.Thread(target=self._run_worker,
-                                        name="kmd-serve-large", daemon=True)
+        self._workers = [threading.Thread(target=self._run_worker,
+                                          name=f"kmd-serve-large-{i}", daemon=True)
+                         for i in range(max(1, int(large_workers)))]
         self._loop.start()
-        self._worker.start()
+        for t in self._workers:
+            t.start()
 
     # ---- client side ------------------------------------------------------
     def submit(self, X_t, sample_weight, init_centroids_t, tol=1e-4,
@@ -143,7 +148,8 @@ class Server:
             self._running = False
             self._lock.notify_all()
         self._loop.join()
-        self._worker.join()
+        for t in self._workers:
+            t.join()
 
     def __enter__(self):
         return self
@@ -242,8 +248,9 @@ class Server:
     def _run_large(self, req):
         import torch
         from . import Stream
-        if not hasattr(self, "_large_stream"):
-            self._large_stream = Stream()
+        if not hasattr(self._local, "stream"):  # one stream per worker thread
+            self._local.stream = Stream()
+        stream = self._local.stream
         X = req.X_t
         n, d, k = req.shape
         req.labels = torch.empty(n, dtype=torch.int32, device=X.device)
@@ -252,8 +259,8 @@ class Server:
         torch.cuda.current_stream().synchronize()  # clone done before the driver
         n_iter, inertia, _ = _native.kmeans_lloyd_driver_ex(
             X, req.sample_weight, work, req.labels, req.centroids, req.tol, False,
-            req.max_iter, req.precision, self._large_stream.handle)
-        self._large_stream.synchronize()
+            req.max_iter, req.precision, stream.handle)
+        stream.synchronize()
         return int(n_iter), float(inertia)
 
 
