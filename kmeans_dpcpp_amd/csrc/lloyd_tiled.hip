@@ -513,14 +513,22 @@ __global__ void __launch_bounds__(256) fill_zero_i32_krn(int m, int *p) {
 inline int bf16_d_of(int64_t d) {
   if (d <= 32) return 32;
   if (d <= 64) return 64;
+  if (d <= 96) return 96;
   return 128;
 }
-inline int kk_of(int D, int T) { return (T * D + 3 + 15) / 16; }
+constexpr int kk_of(int D, int T) { return (T * D + 3 + 15) / 16; }
 constexpr int bf16_s(int D, int T) { return D * T <= 128 ? 2 : 1; }
-// per-tile maxima kept in LDS (candidate tiles against the final best): for
-// one sample block per wave, and for two at D * T == 128 (where the two
-// blocks' maxima and the double-buffered chunks still fit in LDS)
-constexpr bool bf16_tm(int D, int T) { return bf16_s(D, T) == 1 || D * T == 128; }
+// LDS of the BF16 kernel: the per-tile maxima of every lane's sample blocks
+// (bf16, for the candidate tiles), two centroid chunks of mixed_cpc tiles
+// each (as large as kChunkBytes and the rest of LDS allow) and a counter
+constexpr int mixed_tml_bytes(int D, int T) {
+  return kTW * bf16_s(D, T) * 32 * kWave * (int)sizeof(unsigned short);
+}
+constexpr int mixed_cpc(int D, int T) {
+  const int cap = (int)kLdsPerCu - mixed_tml_bytes(D, T) - 1024;
+  const int chunk = kChunkBytes < cap / 2 ? kChunkBytes : cap / 2;
+  return chunk / (kk_of(D, T) * 1024) < 1 ? 1 : chunk / (kk_of(D, T) * 1024);
+}
 
 struct TiledMixedArgs {
   int64_t n;
@@ -550,13 +558,8 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
   if (a.skip != nullptr && *a.skip != 0) return;
   constexpr int KD = D / 16, KK = (T * D + 3 + 15) / 16, XH = D / 2;
   constexpr int S = bf16_s(D, T);
-  constexpr int CPC = (kChunkBytes / (KK * 64 * 16)) < 1
-                          ? 1
-                          : kChunkBytes / (KK * 64 * 16);
+  constexpr int CPC = mixed_cpc(D, T);
   constexpr int kBuf = CPC * KK * 64 * 16;  // bytes per buffer (1 KB multiple)
-  // S == 1: each lane's per-tile best is kept in LDS (bf16, rounded up) so
-  // the candidate tiles are chosen against the final best
-  constexpr bool kTM = bf16_tm(D, T);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   int &q_used = *reinterpret_cast<int *>(lds_raw + 2 * kBuf);
   unsigned short *tml = reinterpret_cast<unsigned short *>(lds_raw + 2 * kBuf + 16);
@@ -647,19 +650,13 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
     for (int j = 0; j < 8; ++j) bh[j] = (__bf16)((half == 0 && j < 3) ? -1.f : 0.f);
 
     float b1[S], b2[S];
-    // Candidate tiles of a sample: bit t is set when tile t's best (packed)
-    // score reaches this lane's running best minus 3 e_big, where e_big
-    // bounds the margin test's 2 (E1 + E2) term from above (|D| <=
-    // 1.01 (||x|| max||c|| + max hn)).  The running best never exceeds the
-    // final one, so every tile that can hold the FP32 argmin is set.
-    float ecut[S];
+    // candidate tiles of each sample (bit t: tile t), from the per-tile
+    // maxima once the sample's best is known
     unsigned cmask[S];
 #pragma unroll
     for (int q = 0; q < S; ++q) {
       b1[q] = -__builtin_huge_valf();
       b2[q] = -__builtin_huge_valf();
-      const float big = 1.01f * (xnorm[q] * cmax + hnmax);
-      ecut[q] = 3.f * (xnorm[q] * cmax * alpha + hnmax * beta + 2.f * pack_err * big);
       cmask[q] = 0u;
     }
     for (int ci = 0; ci < n_chunks; ++ci, ++g) {
@@ -737,17 +734,13 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
               tmax[q] = max3f(tmax[q], uu, vv);
             }
           }
+          // the tile's best, rounded toward +inf to bf16 (never below it)
 #pragma unroll
           for (int q = 0; q < S; ++q)
-            if constexpr (kTM) {
-              if (a.qmask != nullptr && ct < 32) {
-                // round toward +inf to bf16: never below the tile's best
-                const unsigned b = __float_as_uint(tmax[q]);
-                const unsigned up = (b & 0x80000000u) ? (b >> 16) : ((b + 0xffffu) >> 16);
-                tml[((wave * S + q) * 32 + ct) * kWave + lane] = (unsigned short)up;
-              }
-            } else {
-              if (tmax[q] >= b1[q] - ecut[q]) cmask[q] |= 1u << (ct & 31);
+            if (a.qmask != nullptr && ct < 32) {
+              const unsigned b = __float_as_uint(tmax[q]);
+              const unsigned up = (b & 0x80000000u) ? (b >> 16) : ((b + 0xffffu) >> 16);
+              tml[((wave * S + q) * 32 + ct) * kWave + lane] = (unsigned short)up;
             }
         }
       }
@@ -768,7 +761,7 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
       const bool valid = active && s < n;
       if (half == 0 && valid && !amb) labels[s] = (I)lab;
       const bool enq = (half == 0) && valid && amb;
-      if constexpr (kTM) {
+      {
         // candidate tiles against the final best: a centroid can be the FP32
         // argmin only if its score is within 2 (E1 + E2) of the best; with
         // the packing error of both values this is 2 e_part + 6 pack M
@@ -1110,20 +1103,16 @@ TiledMixedFn pick_mixed_tiled(int64_t d, int T) {
     return T == 3 ? &tiled_mixed_krn<DD, 3, I> : &tiled_mixed_krn<DD, 1, I>;
   KMD_TM_CASE(32)
   KMD_TM_CASE(64)
+  KMD_TM_CASE(96)
   KMD_TM_CASE(128)
 #undef KMD_TM_CASE
   return nullptr;
 }
 
 size_t mixed_tiled_lds_bytes(int64_t d, int T) {
-  const int KK = kk_of(bf16_d_of(d), T);
-  int cpc = kChunkBytes / (KK * 64 * 16);
-  if (cpc < 1) cpc = 1;
-  const int S = bf16_s(bf16_d_of(d), T);
-  return 2 * (size_t)cpc * KK * 64 * 16 + 16 +
-         (bf16_tm(bf16_d_of(d), T)
-              ? (size_t)kTW * S * 32 * kWave * sizeof(unsigned short)
-              : 0);
+  const int D = bf16_d_of(d);
+  return 2 * (size_t)mixed_cpc(D, T) * kk_of(D, T) * 64 * 16 + 16 +
+         (size_t)mixed_tml_bytes(D, T);
 }
 
 template <typename F>

@@ -304,7 +304,9 @@ def test_tiled_fp32_wide_shape_vs_oracle(kd, torch_dev):
 
 @pytest.mark.parametrize("shape,seed,std", [((60_000, 100, 300), 41, 2.0),
                                             ((50_001, 128, 1024), 43, 1.0),
-                                            ((30_000, 48, 2000), 45, 1.5)])
+                                            ((30_000, 48, 2000), 45, 1.5),
+                                            ((40_000, 96, 512), 47, 1.0),
+                                            ((25_001, 70, 200), 49, 2.0)])
 def test_tiled_mixed_matches_tiled_fp32_bit_exactly(kd, torch_dev, shape, seed, std):
     """BF16 and BF16x3 on the tiled path, with the certified margin and the
     FP32 gather re-score, give the tiled FP32 fit bit for bit."""
