@@ -57,6 +57,7 @@ struct LloydWorkspace {
       queue_counts{0, nullptr};
   TempBuffer xr{0, nullptr}, upd{0, nullptr};  // tiled: X sample-major, update scratch
   TempBuffer qmask{0, nullptr}, cand{0, nullptr};  // tiled BF16: candidate tiles
+  TempBuffer xb{0, nullptr}, xnb{0, nullptr};  // tiled BF16 (T = 1): BF16 samples
   int64_t seg_cap = 0;     // capacity of each block's re-score queue segment
   // Two iterations can be in flight (the next one is enqueued before this
   // one's scalars are read back): cluster sizes, the empty list and the
@@ -163,6 +164,12 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_tiled_workspace(
         if (tiled_cand_supported(k)) {
           ws->qmask = TempBuffer(sizeof(unsigned) * (size_t)ba * ws->seg_cap, s);
           ws->cand = TempBuffer(tiled_cand_scratch_bytes(n, k), s);
+        }
+        if (terms == 1) {
+          ws->xb = TempBuffer(tiled_xb_bytes(n, d), s);
+          ws->xnb = TempBuffer(sizeof(float) * (size_t)n, s);
+          launch_tiled_xb_prep(n, d, X, ws->xb.template as<void>(),
+                               ws->xnb.template as<float>(), s);
         }
       }
     }
@@ -388,7 +395,8 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
                                      ws.queue.template as<int>(),
                                      ws.queue_counts.template as<int>(),
                                      cand ? ws.qmask.template as<unsigned>() : nullptr,
-                                     s, skip);
+                                     s, skip, ws.xb.template as<const void>(),
+                                     ws.xnb.template as<const float>());
         if (cand)
           launch_tiled_cand_rescore<I>(
               n, d, k, xr, cur, ws.hn.template as<const T>(), labels,
@@ -697,7 +705,8 @@ static void driver_impl(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
                                    ws.queue.template as<int>(),
                                    ws.queue_counts.template as<int>(),
                                    cand ? ws.qmask.template as<unsigned>() : nullptr,
-                                   s);
+                                   s, nullptr, ws.xb.template as<const void>(),
+                                   ws.xnb.template as<const float>());
       if (cand)
         launch_tiled_cand_rescore<I>(
             n, d, k, xr, cur, ws.hn.template as<const T>(), labels,

@@ -164,7 +164,13 @@ void launch_tiled_mixed_assign(int64_t n, int64_t d, int64_t k, int T,
                                int blocks, const float *X_t, const void *aop,
                                const float *bound, I *labels, int *queue,
                                int *queue_counts, unsigned *qmask, hipStream_t s,
-                               const int *skip = nullptr);
+                               const int *skip = nullptr, const void *xb = nullptr,
+                               const float *xnb = nullptr);
+// BF16 samples (sample-major, d rounded up to the kernel's bucket) and norm
+// bounds for launch_tiled_mixed_assign with T = 1
+size_t tiled_xb_bytes(int64_t n, int64_t d);
+void launch_tiled_xb_prep(int64_t n, int64_t d, const float *X_t, void *xb,
+                          float *xnb, hipStream_t s);
 // Re-score of the queued samples restricted to their candidate tiles (the
 // BF16 kernel's qmask; needs k <= 1024), with the FP32 kernels' arithmetic;
 // falls back to the full gather re-score when the (sample, tile) pairs

@@ -543,6 +543,11 @@ struct TiledMixedArgs {
   int64_t seg_cap;
   float pinf;
   const int *skip;       // when non-null and *skip != 0 the launch does nothing
+  // T == 1: the samples as BF16, sample-major (n, D), and their norm bounds
+  // (prepared once per fit; required); the next round's operands are loaded
+  // while the current round is scored
+  const bf16x8 *xb;
+  const float *xnb;
 };
 
 __device__ inline float max3f(float a, float b, float c) {
@@ -596,6 +601,32 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
   auto chunk_bytes = [&](int c) { return min(CPC, n_ct - c * CPC) * KK * 1024; };
   if (total_chunks > 0) lds_dma(aop, lds_raw, chunk_bytes(0), wave, lane);
   int64_t g = 0;
+  // streamed BF16 operands (T == 1): lane (col, half) of sample s reads
+  // features 16 kk + 8 half + 0..7 as one 16-byte load per k-step
+  constexpr bool kXB = (T == 1);
+  constexpr bool use_xb = kXB;
+  bf16x8 nbh[kXB ? KD : 1][S];
+  float nxn[S];
+  auto load_xb = [&](int64_t rr) {
+    const int64_t sbr = rr * per_round + (int64_t)blockIdx.x * kTW + wave;
+    const int64_t sr = sbr * kSuper + (int64_t)S * col;
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      const bool ok = sbr < n_super && sr + q < n;
+      const bf16x8 *row = a.xb + (sr + q) * (D / 8) + half;
+#pragma unroll
+      for (int kk = 0; kk < (kXB ? KD : 1); ++kk) {
+        if (ok) {
+          nbh[kk][q] = row[2 * kk];
+        } else {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) nbh[kk][q][j] = (__bf16)0.f;
+        }
+      }
+      nxn[q] = ok ? a.xnb[sr + q] : 0.f;
+    }
+  };
+  if (use_xb && rounds > 0) load_xb(0);
   for (int64_t r = 0; r < rounds; ++r) {
     const int64_t sb = r * per_round + (int64_t)blockIdx.x * kTW + wave;
     const bool active = sb < n_super;
@@ -604,7 +635,16 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
     float xnorm[S];
     bf16x8 bhi[KD][S];
     bf16x8 blo[(T == 3) ? KD : 1][S];
-    {
+    if constexpr (use_xb) {
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        xnorm[q] = nxn[q];
+#pragma unroll
+        for (int kk = 0; kk < KD; ++kk) bhi[kk][q] = nbh[kk < (kXB ? KD : 1) ? kk : 0][q];
+      }
+      // the next round's operands, in flight while this round is scored
+      if (r + 1 < rounds) load_xb(r + 1);
+    } else {
       float ss[S];
 #pragma unroll
       for (int q = 0; q < S; ++q) ss[q] = 0.f;
@@ -841,6 +881,30 @@ __global__ void __launch_bounds__(256)
     v[j] = (__bf16)val;
   }
   aop[gid] = v;
+}
+
+// xb[s] = the sample's features as BF16 (row of D, zero past d) and xnb[s]
+// = 1.0001 ||x_s|| (FP32; an upper bound of the norm), for the BF16 kernel's
+// streamed operands.  One thread per sample; the features are read
+// feature-major (coalesced over samples) and written 16 bytes at a time.
+__global__ void __launch_bounds__(256)
+    tiled_xb_prep_krn(int64_t n, int d, int D, const float *__restrict__ X_t,
+                      bf16x8 *__restrict__ xb, float *__restrict__ xnb) {
+  const int64_t s = blockIdx.x * 256ll + threadIdx.x;
+  if (s >= n) return;
+  float ss = 0.f;
+  for (int f0 = 0; f0 < D; f0 += 8) {
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int f = f0 + j;
+      const float x = (f < d) ? X_t[(int64_t)f * n + s] : 0.f;
+      ss = fmaf(x, x, ss);
+      v[j] = (__bf16)x;
+    }
+    xb[s * (D / 8) + f0 / 8] = v;
+  }
+  xnb[s] = sqrtf(ss) * 1.0001f;
 }
 
 __global__ void __launch_bounds__(256)
@@ -1320,6 +1384,19 @@ int64_t tiled_mixed_seg_cap(int64_t n, int64_t d, int T, int blocks) {
   return rounds * kTW * 32 * S;
 }
 
+size_t tiled_xb_bytes(int64_t n, int64_t d) {
+  return sizeof(unsigned short) * (size_t)(n > 0 ? n : 1) * bf16_d_of(d) + 256;
+}
+
+void launch_tiled_xb_prep(int64_t n, int64_t d, const float *X_t, void *xb,
+                          float *xnb, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(tiled_xb_prep_krn, dim3((unsigned)ceil_div<int64_t>(n, 256)),
+                     dim3(256), 0, s, n, (int)d, bf16_d_of(d), X_t,
+                     reinterpret_cast<bf16x8 *>(xb), xnb);
+  KMD_LAUNCH_CHECK();
+}
+
 size_t tiled_aop_bytes(int64_t d, int64_t k, int T) {
   // + 1 KB: the chunk copies read whole 1 KB pieces
   return (size_t)ceil_div<int64_t>(k, 32) * kk_of(bf16_d_of(d), T) * 64 * 16 + 1024;
@@ -1345,7 +1422,7 @@ void launch_tiled_mixed_assign(int64_t n, int64_t d, int64_t k, int T,
                                int blocks, const float *X_t, const void *aop,
                                const float *bound, I *labels, int *queue,
                                int *queue_counts, unsigned *qmask, hipStream_t s,
-                               const int *skip) {
+                               const int *skip, const void *xb, const float *xnb) {
   TiledMixedFn fn = pick_mixed_tiled<I>(d, T);
   if (fn == nullptr || blocks < 1)
     throw RuntimeError("tiled BF16 assignment: unsupported shape");
@@ -1366,6 +1443,10 @@ void launch_tiled_mixed_assign(int64_t n, int64_t d, int64_t k, int T,
   a.seg_cap = tiled_mixed_seg_cap(n, d, T, blocks);
   a.pinf = std::numeric_limits<float>::infinity();
   a.skip = skip;
+  if (T == 1 && (xb == nullptr || xnb == nullptr))
+    throw RuntimeError("tiled BF16 assignment: the BF16 sample copy is required");
+  a.xb = (T == 1) ? reinterpret_cast<const bf16x8 *>(xb) : nullptr;
+  a.xnb = (T == 1) ? xnb : nullptr;
   hipLaunchKernelGGL(fn, dim3(blocks), dim3(kTNT), mixed_tiled_lds_bytes(d, T), s,
                      a);
   KMD_LAUNCH_CHECK();
@@ -1431,7 +1512,8 @@ void launch_tiled_update(int64_t n, int64_t d, int64_t k, const float *X_r,
                                              int, const float *, const void *, \
                                              const float *, I *, int *, int *, \
                                              unsigned *, hipStream_t,          \
-                                             const int *);                     \
+                                             const int *, const void *,        \
+                                             const float *);                   \
   template void launch_tiled_cand_rescore<I>(                                  \
       int64_t, int64_t, int64_t, const float *, const float *, const float *,  \
       I *, const int *, const int *, const unsigned *, int, int64_t, void *,   \
