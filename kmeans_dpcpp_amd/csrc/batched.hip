@@ -42,7 +42,10 @@ typedef unsigned long long u64;
 
 constexpr int kNT = 512;                 // assignment block (8 waves)
 constexpr int kW = kNT / kWave;
-constexpr int kTileSamples = kW * 64;     // samples per assignment item
+constexpr int kSubTiles = 8;              // 512-sample blocks per assignment item
+constexpr int kTileSamples = kSubTiles * kW * 64;  // samples per assignment item
+                                          // (the fit's centroids are loaded into
+                                          // LDS once per item)
 constexpr int kChunk = 4096;              // samples per update item
 constexpr int kMembers = 256;             // members per segment-sum item
 constexpr int kMaxK = 256;
@@ -158,7 +161,9 @@ __global__ void __launch_bounds__(kNT)
     }
     __syncthreads();
     const int64_t n = F.n;
-    const int64_t p0 = (int64_t)item.first + wave * 64 + 2 * col;
+    for (int sub = 0; sub < kSubTiles; ++sub) {
+    const int64_t p0 = (int64_t)item.first + (int64_t)sub * kW * 64 + wave * 64 + 2 * col;
+    if ((int64_t)item.first + (int64_t)sub * kW * 64 >= n) break;  // block-uniform
     float xr[KS][2];
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
@@ -213,6 +218,7 @@ __global__ void __launch_bounds__(kNT)
       const bool take = (ov < bestv[q]) || (ov == bestv[q] && oi < besti[q]);
       besti[q] = take ? oi : besti[q];
       if (half == 0 && p0 + q < n) F.labels[p0 + q] = besti[q];
+    }
     }
   }
 }
