@@ -176,6 +176,7 @@ struct FastArgs {
   int n_segments;           // gather mode: <= kMaxSegments
   int64_t seg_cap;
   int *n_rescored;          // gather mode: total queued samples (block 0)
+  int n_priv;               // copies of the LDS int64 sums (power of 2; 0 = 1)
   const int *skip;          // when non-null and *skip != 0 the launch does nothing
 };
 
@@ -198,12 +199,16 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
   float *Cl = reinterpret_cast<float *>(lds_raw);  // 2 KS kpad floats
   float *Hl = Cl + 2 * KS * kpad;                   // [kpad]
   float *Sc = Hl + kpad;                                  // [d + 1] scales
-  u64 *S64 = reinterpret_cast<u64 *>(Sc + (kMaxD + 2));  // [d*k + k] sums
+  // [R][d*k + k] int64 sums: lane l adds into copy l % R, so lanes of one
+  // instruction whose samples share a label mostly hit different addresses
+  // (few centroids: many lanes per label); the copies are summed at the end
+  u64 *S64 = reinterpret_cast<u64 *>(Sc + (kMaxD + 2));
   float *S32 = reinterpret_cast<float *>(S64);            // non-finite path
   const int tid = threadIdx.x;
   const int64_t dk = (int64_t)d * k;
+  const int R = (UPDATE && a.n_priv > 1) ? a.n_priv : 1;
   const bool fixed = UPDATE ? (a.nonfinite[0] == 0) : false;
-  int *seg_off = reinterpret_cast<int *>(S64 + dk + k);  // gather: [nseg+1]
+  int *seg_off = reinterpret_cast<int *>(S64 + R * (dk + k));  // gather: [nseg+1]
   if constexpr (GATHER) {
     // exclusive prefix sum of the segment counts (one wave, 64 per step)
     if (tid < kWave) {
@@ -237,7 +242,7 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
     Hl[c] = (c < k) ? a.hn[c] : __builtin_huge_valf();
   if (UPDATE) {
     for (int f = tid; f <= d; f += NT) Sc[f] = a.scale[f];
-    for (int64_t i = tid; i < dk + k; i += NT) S64[i] = 0ull;
+    for (int64_t i = tid; i < R * (dk + k); i += NT) S64[i] = 0ull;
   }
   __syncthreads();
 
@@ -354,16 +359,17 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
           if (fixed) {
             // |w x| 2^E_f < 2^30: the rounded element fits an int32; the
             // sign-extended value is added into the 64-bit sum
+            u64 *Sp = S64 + (int64_t)(lane & (R - 1)) * (dk + k);
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
               const int f = 2 * ks + half;
               if (FULL || f < d) {
                 const int qv = __float2int_rn((xr[ks][q] * ws) * Sc[f]);
-                atomicAdd(&S64[(int64_t)f * k + lab], (u64)(long long)qv);
+                atomicAdd(&Sp[(int64_t)f * k + lab], (u64)(long long)qv);
               }
             }
             if (half == 0)
-              atomicAdd(&S64[dk + lab],
+              atomicAdd(&Sp[dk + lab],
                         (u64)(long long)__float2int_rn(ws * Sc[d]));
           } else {
 #pragma unroll
@@ -386,7 +392,11 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
       if (fixed) {
         u64 *cp = reinterpret_cast<u64 *>(a.copies) +
                   (a.slot0 + blockIdx.x) * (dk + k);
-        for (int64_t i = tid; i < dk + k; i += NT) cp[i] = S64[i];
+        for (int64_t i = tid; i < dk + k; i += NT) {
+          u64 t = S64[i];
+          for (int r = 1; r < R; ++r) t += S64[r * (dk + k) + i];
+          cp[i] = t;
+        }
       } else {
         float *cp = reinterpret_cast<float *>(a.copies) +
                     (a.slot0 + blockIdx.x) * (dk + k);
@@ -400,7 +410,9 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
         float v;
         if (fixed) {
           const int f = (int)(i / k);  // f == d for the counts
-          v = (float)((double)(long long)S64[i] / (double)Sc[f]);
+          u64 t = S64[i];
+          for (int r = 1; r < R; ++r) t += S64[r * (dk + k) + i];
+          v = (float)((double)(long long)t / (double)Sc[f]);
         } else {
           v = S32[i];
         }
@@ -475,13 +487,22 @@ inline int ks_bucket(int64_t d) {
   return 32;
 }
 
-inline size_t lds_bytes(int64_t d, int64_t k, bool update) {
+inline size_t lds_bytes(int64_t d, int64_t k, bool update, int copies = 1) {
   const int kpad = pad32(k);
   const int KS = ks_bucket(d);
   size_t b = sizeof(float) * ((size_t)2 * KS * kpad + kpad);
   if (update)
-    b += sizeof(float) * (kMaxD + 2) + sizeof(u64) * (size_t)(d + 1) * k;
+    b += sizeof(float) * (kMaxD + 2) +
+         sizeof(u64) * (size_t)copies * (d + 1) * k;
   return b;
+}
+
+// Copies of the LDS sums the fused step keeps (up to 8, as LDS allows: the
+// kernel runs one block per CU).
+inline int sum_copies(int64_t d, int64_t k) {
+  int r = 1;
+  while (r < 8 && lds_bytes(d, k, true, 2 * r) <= kLdsPerCu) r *= 2;
+  return r;
 }
 
 using KernFn = void (*)(FastArgs);
@@ -519,13 +540,15 @@ struct MfmaLaunch {
   int threads = kThreads;
   int blocks = 0;
   size_t lds = 0;
+  int copies = 1;  // copies of the LDS sums (UPDATE)
 };
 
 template <typename I, bool UPDATE, bool STORE>
 MfmaLaunch plan_launch(int64_t n, int64_t d, int64_t k) {
   MfmaLaunch L;
   if (d < 1 || d > kMaxD || k < 1 || k > (1 << 20)) return L;
-  L.lds = lds_bytes(d, k, UPDATE);
+  L.copies = UPDATE ? sum_copies(d, k) : 1;
+  L.lds = lds_bytes(d, k, UPDATE, L.copies);
   if (L.lds > kLdsPerCu) return L;
   L.threads = kThreads;
   L.fn = pick_nt<kThreads, I, UPDATE, STORE>(d);
@@ -641,6 +664,7 @@ void launch_fast_fused_fixed(int64_t n, int64_t d, int64_t k, const T *X,
     a.copies = partials;
     a.n_copies = n_partials;
     a.skip = skip;
+    a.n_priv = L.copies;
     run(L, a, s);
   } else {
     (void)skip;
@@ -669,6 +693,7 @@ void launch_fast_fused(int64_t n, int64_t d, int64_t k, int64_t n_copies,
     a.copies = copies;
     a.sizes_copies = sizes_copies;
     a.n_copies = n_copies;
+    a.n_priv = L.copies;
     run(L, a, s);
   } else {
     throw RuntimeError("MFMA fused step is FP32 only");
