@@ -561,8 +561,11 @@ def main():
     t_hbm = bytes_per_launch / (HBM_PEAK_GBS * 1e9)
     t_cmp = flops_per_launch / (compute_peak * 1e12)
     here = os.path.dirname(os.path.abspath(__file__))
-    traffic = traffic_from_profile(os.path.join(here, "profiles",
-                                                "fused_step_n10M_d32_k256_pmc.md"))
+    # HBM bytes per launch measured with rocprofv3 for this workload (only
+    # when the shape is the one profiled)
+    traffic = (traffic_from_profile(os.path.join(here, "profiles",
+                                                 "fused_step_n10M_d32_k256_pmc.md"))
+               if (n, d, k, args.precision) == (N, D, K, "fp32") else None)
     if t_cmp >= t_hbm:
         roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": compute_peak,
                 "unit": "TFLOP/s", "frac": round(tflops / compute_peak, 4)}
