@@ -328,6 +328,50 @@ int kmd_autotune(int dtype, int itype, int precision, int64_t n_samples,
  * back to the autotuner's table, else the automatic choice). */
 int kmd_set_tuning(int path, int blocks_per_cu);
 
+/* ---- estimator front-end: data preparation and seeding ----------------
+ * No counterpart in the reference, whose driver takes init_centroids_t from
+ * its caller (python_api/_kmeans_lloyd.cpp:1437); these serve the caller the
+ * reference's README (README.md:3-4) names: an sklearn-style estimator that
+ * holds sample-major data. */
+
+/* Per-feature mean and variance (FP64, two passes; variance = mean squared
+ * deviation, numpy's np.var) of a row-major X (n_samples, n_features).
+ * mean and var: device, n_features doubles; var may be NULL. */
+int kmd_column_mean_var(int dtype, int64_t n_samples, int64_t n_features,
+                        const void *X, double *mean, double *var,
+                        kmd_stream_t stream);
+
+/* dst (cols, rows) = src (rows, cols)^T - shift[col]; shift: device doubles
+ * (cols) rounded to dtype before the subtraction, or NULL for a plain
+ * transpose.  Turns sample-major X into X_t (and distances (k, n) into
+ * (n, k)). */
+int kmd_transpose(int dtype, int64_t rows, int64_t cols, const void *src,
+                  const double *shift, void *dst, kmd_stream_t stream);
+
+/* out_t[:, j] = X_t[:, indices[j]] for j < n_indices (indices: device
+ * int64, each in [0, n_samples)). */
+int kmd_gather_columns(int dtype, int64_t n_samples, int64_t n_features,
+                       int64_t n_indices, const void *X_t,
+                       const int64_t *indices, void *out_t,
+                       kmd_stream_t stream);
+
+/* Greedy k-means++ seeding, the algorithm of scikit-learn 1.7.2
+ * (sklearn/cluster/_kmeans.py:174-277) with its random draws supplied:
+ * uniforms (HOST memory, 1 + (n_clusters - 1) * n_local_trials doubles in
+ * [0, 1), copied before the call returns): uniforms[0] picks the first
+ * center (probability proportional to sample_weight, numpy choice's rule),
+ * uniforms[1 + (c - 1) n_local_trials + j] the j-th candidate of center c
+ * (probability proportional to w D^2); the candidate with the smallest
+ * potential wins.  n_local_trials in [1, 16] (sklearn's default is
+ * 2 + floor(ln n_clusters)).  Writes centers_t (n_features, n_clusters) and
+ * the chosen sample indices (device int64, n_clusters).  Distances,
+ * potentials and prefix sums are FP64. */
+int kmd_kmeans_plusplus(int dtype, int64_t n_samples, int64_t n_features,
+                        int64_t n_clusters, int64_t n_local_trials,
+                        const void *X_t, const void *sample_weight,
+                        const double *uniforms, void *centers_t,
+                        int64_t *indices, kmd_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,25 +28,30 @@ from ._kmeans_dpcpp_amd import (  # noqa: E402
     assignment,
     autotune,
     broadcast_divide,
+    column_mean_var,
     compute_centroid_shifts_squared,
     compute_centroid_to_sample_distances,
     compute_inertia,
     compute_number_of_private_copies,
     compute_threshold,
     fused_lloyd_single_step,
+    gather_columns,
     half_l2_norm_squared,
     iterative_merge_sort,
     kmeans_lloyd_batched,
     kmeans_lloyd_driver,
     kmeans_lloyd_driver_ex,
+    kmeans_plusplus_t,
     record_event,
     reduce_centroids_data,
     reduce_vector_blocking,
     relocate_empty_clusters,
     select_samples_far_from_centroid,
     set_tuning,
+    transpose,
     version,
 )
+from .estimator import KMeans, kmeans_plusplus  # noqa: E402
 
 
 class Stream:

@@ -784,6 +784,97 @@ int kmd_autotune(int dtype, int itype, int precision, int64_t n_samples,
   });
 }
 
+int kmd_column_mean_var(int dtype, int64_t n_samples, int64_t n_features,
+                        const void *X, double *mean, double *var,
+                        kmd_stream_t stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(n_samples >= 1 && n_features >= 1,
+            "n_samples and n_features must be positive");
+    check_ptr(X, n_samples * n_features, "X");
+    check_ptr(mean, n_features, "mean");
+    dispatch_t(dtype, [&](auto *tp) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      launch_column_mean_var<T>(n_samples, n_features, (const T *)X, mean, var,
+                                as_stream(stream));
+    });
+  });
+}
+
+int kmd_transpose(int dtype, int64_t rows, int64_t cols, const void *src,
+                  const double *shift, void *dst, kmd_stream_t stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(rows >= 0 && cols >= 0, "negative dimensions");
+    check_ptr(src, rows * cols, "src");
+    check_ptr(dst, rows * cols, "dst");
+    dispatch_t(dtype, [&](auto *tp) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      launch_transpose_shift<T>(rows, cols, (const T *)src, shift, (T *)dst,
+                                as_stream(stream));
+    });
+  });
+}
+
+int kmd_gather_columns(int dtype, int64_t n_samples, int64_t n_features,
+                       int64_t n_indices, const void *X_t,
+                       const int64_t *indices, void *out_t,
+                       kmd_stream_t stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(n_samples >= 1 && n_features >= 1 && n_indices >= 0,
+            "Inputs have inconsistent dimensions.");
+    check_ptr(X_t, n_samples * n_features, "X_t");
+    check_ptr(indices, n_indices, "indices");
+    check_ptr(out_t, n_indices * n_features, "out_t");
+    dispatch_t(dtype, [&](auto *tp) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      launch_gather_columns<T>(n_samples, n_features, n_indices, (const T *)X_t,
+                               indices, (T *)out_t, as_stream(stream));
+    });
+  });
+}
+
+int kmd_kmeans_plusplus(int dtype, int64_t n_samples, int64_t n_features,
+                        int64_t n_clusters, int64_t n_local_trials,
+                        const void *X_t, const void *sample_weight,
+                        const double *uniforms, void *centers_t,
+                        int64_t *indices, kmd_stream_t stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(n_features >= 1 && n_clusters >= 1,
+            "n_features and n_clusters must be positive");
+    require(n_samples >= n_clusters,
+            "n_samples must be at least n_clusters");
+    require(n_local_trials >= 1 && n_local_trials <= kpp_max_trials(),
+            "n_local_trials must be between 1 and 16");
+    check_ptr(X_t, n_samples * n_features, "X_t");
+    check_ptr(sample_weight, n_samples, "sample_weight");
+    check_ptr(uniforms, 1, "uniforms");
+    check_ptr(centers_t, n_clusters * n_features, "centers_t");
+    check_ptr(indices, n_clusters, "indices");
+    const int64_t nu = 1 + (n_clusters - 1) * n_local_trials;
+    for (int64_t i = 0; i < nu; ++i)
+      require(uniforms[i] >= 0.0 && uniforms[i] < 1.0,
+              "uniforms must lie in [0, 1)");
+    hipStream_t s = as_stream(stream);
+    TempBuffer u_dev((size_t)nu * sizeof(double), s);
+    // pageable source: the copy is complete when the stream reaches it, and
+    // the caller's buffer may be released when this call returns
+    KMD_HIP_CHECK(hipMemcpyAsync(u_dev.as<double>(), uniforms,
+                                 (size_t)nu * sizeof(double),
+                                 hipMemcpyHostToDevice, s));
+    KMD_HIP_CHECK(hipStreamSynchronize(s));
+    dispatch_t(dtype, [&](auto *tp) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      launch_kmeans_plusplus<T>(n_samples, n_features, n_clusters,
+                                (int)n_local_trials, (const T *)X_t,
+                                (const T *)sample_weight, u_dev.as<double>(),
+                                (T *)centers_t, indices, s);
+    });
+  });
+}
+
 int kmd_set_tuning(int path, int blocks_per_cu) {
   return guarded([&] {
     require(path >= -1 && path <= 2, "path must be -1, 0, 1 or 2");

@@ -199,6 +199,22 @@ void launch_relocate(int64_t n, int64_t d, int64_t k, int64_t n_empty,
                      const I *empty_list, const T *sq_dist, T *centroids,
                      T *sizes, T *per_sample_inertia, hipStream_t s);
 
+// Data preparation and k-means++ seeding (seeding.hip)
+template <typename T>
+void launch_column_mean_var(int64_t n, int64_t d, const T *X, double *mean,
+                            double *var, hipStream_t s);
+template <typename T>
+void launch_transpose_shift(int64_t rows, int64_t cols, const T *src,
+                            const double *shift, T *dst, hipStream_t s);
+template <typename T>
+void launch_gather_columns(int64_t n, int64_t d, int64_t m, const T *X_t,
+                           const int64_t *idx, T *out_t, hipStream_t s);
+int kpp_max_trials();
+template <typename T>
+void launch_kmeans_plusplus(int64_t n, int64_t d, int64_t k, int trials,
+                            const T *X_t, const T *w, const double *u_dev,
+                            T *centers_t, int64_t *indices, hipStream_t s);
+
 // Lloyd kernels (lloyd.hip)
 template <typename T, typename I>
 void launch_assignment(int64_t n, int64_t d, int64_t k, int64_t h,

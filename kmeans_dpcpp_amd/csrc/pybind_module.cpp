@@ -163,12 +163,12 @@ Arr to_arr(const py::handle &obj, kmd_stream_t s, const char *name) {
   return a;
 }
 
-bool all_contig(std::initializer_list<const Arr *> xs) {
+bool all_contig(const std::vector<const Arr *> &xs) {
   for (auto *x : xs)
     if (!x->c_contig) return false;
   return true;
 }
-bool same_device(std::initializer_list<const Arr *> xs) {
+bool same_device(const std::vector<const Arr *> &xs) {
   int d = -2;
   for (auto *x : xs) {
     if (d == -2) d = x->device;
@@ -741,6 +741,127 @@ py::tuple py_kmeans_lloyd_driver_ex(py::object X_, py::object w_,
   return py::make_tuple((size_t)n_iters, inertia, stats);
 }
 
+// ---- estimator front-end: preparation and seeding -------------------------
+
+std::pair<Event, Event> py_column_mean_var(py::object X_, py::object mean_,
+                                           py::object var_, py::object q,
+                                           py::list depends) {
+  kmd_stream_t s = resolve_stream(q);
+  Arr X = to_arr(X_, s, "X"), mean = to_arr(mean_, s, "mean");
+  const bool has_var = !var_.is_none();
+  Arr var = has_var ? to_arr(var_, s, "var") : Arr();
+  if (X.ndim != 2 || mean.ndim != 1 || (has_var && var.ndim != 1))
+    throw py::value_error("X must be a matrix, mean and var vectors");
+  std::vector<const Arr *> all = {&X, &mean};
+  if (has_var) all.push_back(&var);
+  if (!all_contig(all)) throw py::value_error("All input arrays must be C-contiguous");
+  if (!same_device(all)) throw py::value_error(kQueueMsg);
+  if (!X.is_float() || mean.code != kDLFloat || mean.bits != 64 ||
+      (has_var && (var.code != kDLFloat || var.bits != 64)))
+    throw py::value_error("X must be float32/float64, mean and var float64");
+  if (mean.dim(0) != X.dim(1) || (has_var && var.dim(0) != X.dim(1)))
+    throw py::value_error("Array dimensions are not consistent");
+  use_device(X);
+  wait_depends(s, depends);
+  check_rc(kmd_column_mean_var(X.dtype(), X.dim(0), X.dim(1), X.data,
+                               static_cast<double *>(mean.data),
+                               has_var ? static_cast<double *>(var.data) : nullptr,
+                               s));
+  return record(s, has_var ? keep_of({&X, &mean, &var}) : keep_of({&X, &mean}));
+}
+
+std::pair<Event, Event> py_transpose(py::object src_, py::object dst_,
+                                     py::object shift_, py::object q,
+                                     py::list depends) {
+  kmd_stream_t s = resolve_stream(q);
+  Arr src = to_arr(src_, s, "src"), dst = to_arr(dst_, s, "dst");
+  const bool has_shift = !shift_.is_none();
+  Arr shift = has_shift ? to_arr(shift_, s, "shift") : Arr();
+  if (src.ndim != 2 || dst.ndim != 2 || (has_shift && shift.ndim != 1))
+    throw py::value_error("src and dst must be matrices, shift a vector");
+  std::vector<const Arr *> all = {&src, &dst};
+  if (has_shift) all.push_back(&shift);
+  if (!all_contig(all)) throw py::value_error("All input arrays must be C-contiguous");
+  if (!same_device(all)) throw py::value_error(kQueueMsg);
+  if (!src.is_float() || !src.same_type(dst) ||
+      (has_shift && (shift.code != kDLFloat || shift.bits != 64)))
+    throw py::value_error("src and dst must share a float dtype, shift float64");
+  if (dst.dim(0) != src.dim(1) || dst.dim(1) != src.dim(0) ||
+      (has_shift && shift.dim(0) != src.dim(1)))
+    throw py::value_error("Array dimensions are not consistent");
+  use_device(src);
+  wait_depends(s, depends);
+  check_rc(kmd_transpose(src.dtype(), src.dim(0), src.dim(1), src.data,
+                         has_shift ? static_cast<const double *>(shift.data) : nullptr,
+                         dst.data, s));
+  return record(s, has_shift ? keep_of({&src, &dst, &shift}) : keep_of({&src, &dst}));
+}
+
+std::pair<Event, Event> py_gather_columns(py::object X_, py::object idx_,
+                                          py::object out_, py::object q,
+                                          py::list depends) {
+  kmd_stream_t s = resolve_stream(q);
+  Arr X = to_arr(X_, s, "X_t"), idx = to_arr(idx_, s, "indices"),
+      out = to_arr(out_, s, "out_t");
+  if (X.ndim != 2 || idx.ndim != 1 || out.ndim != 2)
+    throw py::value_error("X_t and out_t must be matrices, indices a vector");
+  if (!all_contig({&X, &idx, &out}))
+    throw py::value_error("All input arrays must be C-contiguous");
+  if (!same_device({&X, &idx, &out})) throw py::value_error(kQueueMsg);
+  if (!X.is_float() || !X.same_type(out) || idx.code != kDLInt || idx.bits != 64)
+    throw py::value_error("X_t and out_t must share a float dtype, indices int64");
+  if (out.dim(0) != X.dim(0) || out.dim(1) != idx.dim(0))
+    throw py::value_error("Array dimensions are not consistent");
+  use_device(X);
+  wait_depends(s, depends);
+  check_rc(kmd_gather_columns(X.dtype(), X.dim(1), X.dim(0), idx.dim(0), X.data,
+                              static_cast<const int64_t *>(idx.data), out.data, s));
+  return record(s, keep_of({&X, &idx, &out}));
+}
+
+std::pair<Event, Event> py_kmeans_plusplus(py::object X_, py::object w_,
+                                           py::array_t<double, py::array::c_style |
+                                                                   py::array::forcecast>
+                                               uniforms,
+                                           py::object C_, py::object idx_,
+                                           py::object q, py::list depends) {
+  kmd_stream_t s = resolve_stream(q);
+  Arr X = to_arr(X_, s, "X_t"), w = to_arr(w_, s, "sample_weight"),
+      C = to_arr(C_, s, "centers_t"), idx = to_arr(idx_, s, "indices");
+  if (X.ndim != 2 || w.ndim != 1 || C.ndim != 2 || idx.ndim != 1)
+    throw py::value_error("Array dimensionality is not consistent");
+  if (!all_contig({&X, &w, &C, &idx}))
+    throw py::value_error("All input arrays must be C-contiguous");
+  if (!same_device({&X, &w, &C, &idx})) throw py::value_error(kQueueMsg);
+  if (!X.is_float() || !X.same_type(w) || !X.same_type(C) ||
+      idx.code != kDLInt || idx.bits != 64)
+    throw py::value_error("X_t, sample_weight, centers_t must share a float "
+                          "dtype; indices must be int64");
+  const int64_t d = X.dim(0), n = X.dim(1), k = C.dim(1);
+  if (C.dim(0) != d || w.dim(0) != n || idx.dim(0) != k)
+    throw py::value_error("Array dimensions are not consistent");
+  if (uniforms.ndim() != 2 || uniforms.shape(0) < 1)
+    throw py::value_error("uniforms must be a (1 + (k - 1), T) array: row 0 "
+                          "holds the first center's draw");
+  const int64_t T = uniforms.shape(1);
+  if (uniforms.shape(0) != k)
+    throw py::value_error("uniforms must have n_clusters rows");
+  // flatten to [u0, row 1, row 2, ...]
+  std::vector<double> u((size_t)(1 + (k - 1) * T));
+  const double *up = uniforms.data();
+  u[0] = up[0];
+  for (int64_t c = 1; c < k; ++c)
+    for (int64_t j = 0; j < T; ++j) u[(size_t)(1 + (c - 1) * T + j)] = up[c * T + j];
+  use_device(X);
+  wait_depends(s, depends);
+  {
+    py::gil_scoped_release nogil;
+    check_rc(kmd_kmeans_plusplus(X.dtype(), n, d, k, T, X.data, w.data, u.data(),
+                                 C.data, static_cast<int64_t *>(idx.data), s));
+  }
+  return record(s, keep_of({&X, &w, &C, &idx}));
+}
+
 }  // namespace
 
 py::tuple py_kmeans_lloyd_batched(py::list fits, const std::string &precision,
@@ -1148,4 +1269,29 @@ PYBIND11_MODULE(_kmeans_dpcpp_amd, m) {
         py::arg("assignments_id"), py::arg("res_centroids_t"), py::arg("tol"),
         py::arg("verbose"), py::arg("max_iter"), py::arg("precision"),
         py::arg("sycl_queue"), py::arg("depends") = py::list());
+  m.def("column_mean_var", &py_column_mean_var,
+        "column_mean_var(X, mean, var=None, sycl_queue=None, depends=[]): "
+        "per-feature mean and variance (float64 outputs) of a row-major "
+        "(n_samples, n_features) X.",
+        py::arg("X"), py::arg("mean"), py::arg("var") = py::none(),
+        py::arg("sycl_queue") = py::none(), py::arg("depends") = py::list());
+  m.def("transpose", &py_transpose,
+        "transpose(src, dst, shift=None, sycl_queue=None, depends=[]): "
+        "dst = src.T - shift (shift: float64 per column of src, or None).",
+        py::arg("src"), py::arg("dst"), py::arg("shift") = py::none(),
+        py::arg("sycl_queue") = py::none(), py::arg("depends") = py::list());
+  m.def("gather_columns", &py_gather_columns,
+        "gather_columns(X_t, indices, out_t, sycl_queue=None, depends=[]): "
+        "out_t[:, j] = X_t[:, indices[j]] (indices int64 on the device).",
+        py::arg("X_t"), py::arg("indices"), py::arg("out_t"),
+        py::arg("sycl_queue") = py::none(), py::arg("depends") = py::list());
+  m.def("kmeans_plusplus_t", &py_kmeans_plusplus,
+        "kmeans_plusplus_t(X_t, sample_weight, uniforms, centers_t, indices, "
+        "sycl_queue=None, depends=[]): greedy k-means++ (scikit-learn's "
+        "algorithm) with the random draws given: uniforms is a host float64 "
+        "array (n_clusters, n_local_trials) whose [0, 0] picks the first "
+        "center and row c the candidates of center c.",
+        py::arg("X_t"), py::arg("sample_weight"), py::arg("uniforms"),
+        py::arg("centers_t"), py::arg("indices"),
+        py::arg("sycl_queue") = py::none(), py::arg("depends") = py::list());
 }
