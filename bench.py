@@ -43,7 +43,7 @@ def parse():
     p.add_argument("--k", type=int, default=K)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
-    p.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4"],
+    p.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4", "fit"],
                    help="c1: one plain fit n=10M d=32 k=256 (the metric's "
                         "configuration, default); c2: one fit n=2M d=128 "
                         "k=1024 on anisotropic blobs, 30 iterations, with the "
@@ -53,7 +53,11 @@ def parse():
                         "by one through the single-fit driver beside it; c4: "
                         "a Poisson stream of fit requests served with "
                         "continuous batching (fits/s, p50/p99 latency) beside "
-                        "the plain per-request FP32 driver")
+                        "the plain per-request FP32 driver; fit: the "
+                        "sklearn-style estimator on sample-major data of the "
+                        "c1 shape (column moments + centred transpose, GPU "
+                        "k-means++ seeding, --steps Lloyd iterations), with "
+                        "scikit-learn's k-means++ on a host sample beside it")
     p.add_argument("--rate", type=float, default=200.0, help="c4 arrivals per s")
     p.add_argument("--large-precision", default="bf16", choices=["fp32", "bf16x3", "bf16"],
                    help="c4: precision of the large requests' fits")
@@ -447,6 +451,136 @@ def bench_c4(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_fit(args):
+    """The estimator front-end end to end: KMeans(k, init='k-means++',
+    n_init=1, max_iter=steps, tol=0).fit(X) on sample-major float32 X of the
+    c1 shape, with its parts timed separately (HIP events on the stream the
+    kernels run on) and the k-means++ seeding priced against the HBM roof."""
+    import numpy as np
+    import torch
+    import kmeans_dpcpp_amd as kd
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream()
+    n, d, k = args.n, args.d, args.k
+    steps = args.steps
+    X_t0, _ = make_data(n, d, k, SEED, dev)
+    X = X_t0.t().contiguous()                      # (n, d) sample-major
+    del X_t0
+    w = torch.ones(n, dtype=torch.float32, device=dev)
+    T = 2 + int(np.log(k))
+    torch.cuda.synchronize()
+    # warm-up: every kernel of the fit once on a slice
+    if args.warmup > 0:
+        kd.KMeans(k, random_state=0, max_iter=2, tol=0).fit(X[:max(4 * k, 200_000)])
+        torch.cuda.synchronize()
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        out = fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1), out
+
+    # parts, each on the full data
+    mean = torch.empty(d, dtype=torch.float64, device=dev)
+    var = torch.empty(d, dtype=torch.float64, device=dev)
+    X_t = torch.empty((d, n), dtype=torch.float32, device=dev)
+
+    def prep():
+        kd.column_mean_var(X, mean, var, stream)
+        kd.transpose(X, X_t, mean, stream)
+    prep_ms, _ = timed(prep)
+    u = np.random.RandomState(0).random_sample((k, T))
+    C = torch.empty((d, k), dtype=torch.float32, device=dev)
+    idx = torch.empty(k, dtype=torch.int64, device=dev)
+    kpp_ms, _ = timed(lambda: kd.kmeans_plusplus_t(X_t, w, u, C, idx, stream))
+    del X_t
+    torch.cuda.synchronize()
+    # the whole fit
+    t0 = time.perf_counter()
+    est = kd.KMeans(k, init="k-means++", n_init=1, max_iter=steps, tol=0,
+                    random_state=0).fit(X)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+
+    # k-means++ traffic: the first center reads w; the pass of center 0
+    # reads X and w and writes D^2; every later pass reads X, w and D^2 and
+    # writes D^2 (FP64)
+    kpp_bytes = (n * 4 + n * (4 * d + 4 + 8)
+                 + (k - 1) * n * (4 * d + 4 + 8 + 8))
+    kpp_gbs = kpp_bytes / (kpp_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(kpp_gbs, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(kpp_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "k-means++ seeding (kpp_score_krn + one-block kpp_pick_krn per center)",
+            "kernel_ms": round(kpp_ms, 3),
+            "algorithmic_bytes": kpp_bytes}
+    prep_bytes = 2 * n * d * 4 + 2 * n * d * 4      # two moment passes + transpose
+    line = {
+        "metric": METRIC,
+        "value": round(1.0 / fit_s, 4),
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * fit_s / max(1, est.n_iter_), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (seeding: f64)",
+        "data": "synthetic",
+        "config": {"workload": f"estimator fit: KMeans(n_clusters={k}, "
+                               f"init='k-means++', n_init=1, max_iter={steps}, "
+                               f"tol=0) on sample-major float32 blobs, "
+                               f"n={n:,} d={d}",
+                   "n_samples": n, "n_features": d, "n_clusters": k,
+                   "n_local_trials": T},
+        "roofline": roof,
+        "fit_s": round(fit_s, 4),
+        "n_iter": est.n_iter_,
+        "parts_ms": {"moments_and_centred_transpose": round(prep_ms, 3),
+                     "kmeans_plusplus": round(kpp_ms, 3),
+                     "lloyd_and_rest": round(1e3 * fit_s - prep_ms - kpp_ms, 3)},
+        "prep_hbm_frac": round(prep_bytes / (prep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "cpu_baseline": None if args.no_cpu_baseline else
+        cpu_kmeanspp_baseline(X, k, T, n, args.cpu_seconds),
+    }
+    if line["cpu_baseline"]:
+        line["kmeanspp_gpu_over_cpu"] = round(
+            (1e3 / kpp_ms) / line["cpu_baseline"]["value"], 1)
+    print(json.dumps(line), flush=True)
+
+
+def cpu_kmeanspp_baseline(X, k, T, n, budget_s):
+    """scikit-learn's k-means++ (its own BLAS threads, all host cores) on a
+    host sample of the same data, scaled linearly in n to the full set."""
+    import numpy as np
+    try:
+        from sklearn.cluster import kmeans_plusplus
+    except ImportError:
+        return None
+    cores = len(os.sched_getaffinity(0))
+    n_s = min(n, 20_000)
+    Xs = X[:n_s].cpu().numpy().copy()
+    t0 = time.perf_counter()
+    kmeans_plusplus(Xs, k, random_state=0, n_local_trials=T)
+    el = time.perf_counter() - t0
+    while el < budget_s / 4 and n_s < n:
+        n_s = min(n, n_s * 4)
+        Xs = X[:n_s].cpu().numpy().copy()
+        t0 = time.perf_counter()
+        kmeans_plusplus(Xs, k, random_state=0, n_local_trials=T)
+        el = time.perf_counter() - t0
+    full_s = el * (n / n_s)
+    return {"value": round(1.0 / full_s, 5), "unit": "k-means++ seedings/s",
+            "cores": cores, "kind": "port",
+            "sample": (f"scikit-learn {__import__('sklearn').__version__} "
+                       f"kmeans_plusplus (float32, k={k}, {T} local trials) on "
+                       f"the first {n_s:,} samples in {el:.2f} s, scaled "
+                       f"linearly to n={n:,}")}
+
+
 def traffic_from_profile(path):
     """HBM bytes per fused-step launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
@@ -510,6 +644,9 @@ def main():
         return
     if args.workload == "c4":
         bench_c4(args)
+        return
+    if args.workload == "fit":
+        bench_fit(args)
         return
     import torch
     import kmeans_dpcpp_amd as kd

@@ -20,44 +20,28 @@
 //    caller's random stream fixes the centers.  Distances, potentials and
 //    prefix sums are FP64.
 //
-// k-means++ layout.  Samples are cut into segments of kSeg; one launch per
-// center: every block scores one segment against the T candidates, writes
-// m_t(i) = min(D_i^2, |x_i - cand_t|^2) (FP64, row t of a T x n buffer) and
-// the segment sums of w m_t; the last block to finish (atomic ticket) sums
-// the potentials, picks the winner t*, copies the center and searches the
-// prefix sums of w m_t* (segment level, then inside one segment) for the
-// next center's candidates.  Row t* of the buffer is the next launch's D^2,
-// so a center costs one pass: read X_t, w and D^2, write T rows.
+// k-means++ layout.  Samples are cut into segments of kSeg; two launches per
+// center: a scoring pass where every block scores one segment against the
+// T candidates and writes m_t(i) = min(D_i^2, |x_i - cand_t|^2) (FP64, row
+// t of a T x n buffer) and the segment sums of w m_t; then a one-block pick
+// that sums the potentials, takes the winner t*, copies the center and
+// searches the prefix sums of w m_t* (segment level, then inside one
+// segment) for the next center's candidates.  Row t* of the buffer is the
+// next center's D^2, so a center costs one pass over the data: read X_t, w
+// and D^2, write T rows.  (The kernel boundary publishes the rows; a
+// last-block ticket inside one launch would need a device-scope release in
+// every block, which stalls each CU on an L2 write-back.)
 #include "common.hpp"
 
 namespace kmd {
 
 namespace {
 
-constexpr int kNT = 256;             // threads per block
-constexpr int kSeg = 4 * kNT;        // samples per segment (4 per thread)
+constexpr int kNT = 256;             // threads per block (data preparation)
 constexpr int kFC = 256;             // candidate features staged per LDS chunk
 constexpr int kMaxTrials = 16;
 
 __device__ inline int lane_of() { return threadIdx.x & (kWave - 1); }
-
-__device__ inline double block_sum_d(double v, double *lds /*[kNT / kWave]*/) {
-  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  __syncthreads();
-  if (lane_of() == 0) lds[threadIdx.x / kWave] = v;
-  __syncthreads();
-  double t = 0.0;
-#pragma unroll
-  for (int i = 0; i < kNT / kWave; ++i) t += lds[i];
-  return t;
-}
-
-// Loads written by other blocks of the same launch, read after the ticket.
-__device__ inline double load_agent(const double *p) {
-  return __longlong_as_double((long long)__hip_atomic_load(
-      reinterpret_cast<const unsigned long long *>(p), __ATOMIC_RELAXED,
-      __HIP_MEMORY_SCOPE_AGENT));
-}
 
 // ---- per-feature mean and variance -----------------------------------------
 // Block (x, y) covers features [y FC, y FC + FC) (FC = min(d, 256)) of rows
@@ -140,302 +124,434 @@ __global__ void __launch_bounds__(kNT)
 }
 
 // ---- k-means++ ----------------------------------------------------------------
+constexpr int kSNT = 512;                 // threads per scoring block (2 per CU)
+constexpr int kPNT = 1024;                // threads of the one-block picks
+constexpr int kSPT = 4;                   // samples per scoring thread
+constexpr int kSeg = kSPT * kSNT;         // samples per segment (2048)
+constexpr int kSub = 64;                  // samples per sub-segment
+constexpr int kSubPerSeg = kSeg / kSub;   // 32
+constexpr int kUF = 4;                    // feature rows loaded together
+
+static_assert(kSubPerSeg <= kWave, "one lane per sub-segment of a segment");
+static_assert(kSub == kWave, "one lane per sample of a sub-segment");
+
+template <typename T>
+struct Quad {
+  T v[kSPT];
+};
+
+// Samples i0 .. i0 + 3 of one feature row (zero past n).
+template <typename T>
+__device__ inline Quad<T> load_quad(const T *row, int64_t i0, int64_t n, bool vec) {
+  Quad<T> q;
+  if (vec) {
+    if constexpr (sizeof(T) == 4) {
+      const float4 v = *reinterpret_cast<const float4 *>(row);
+      q.v[0] = v.x; q.v[1] = v.y; q.v[2] = v.z; q.v[3] = v.w;
+    } else {
+      const double2 v0 = *reinterpret_cast<const double2 *>(row);
+      const double2 v1 = *reinterpret_cast<const double2 *>(row + 2);
+      q.v[0] = v0.x; q.v[1] = v0.y; q.v[2] = v1.x; q.v[3] = v1.y;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSPT; ++j) q.v[j] = (i0 + j < n) ? row[j] : (T)0;
+  }
+  return q;
+}
+
 struct KppArgs {
-  int64_t n, d, k, nb;
+  int64_t n, d, k, nb, nsub;
   int trials;                 // T (candidates per center after the first)
   const void *X_t, *w;
   const double *u;            // uniforms: [0] first center, 1 + (c-1) T + j
   void *centers_t;            // (d, k)
   int64_t *indices;           // (k)
-  double *m[2];               // [T][n] each: m_t(i) of even / odd centers
+  double *d2[2];              // D^2 to centers 0..c-1, written by pass c into d2[c & 1]
   double *part;               // [T][nb] segment sums of w m_t
+  double *part2;              // [T][nsub] sub-segment sums of w m_t
   int64_t *cand;              // [T] candidates of the current center
-  int *tstar;                 // winner of the previous center
-  unsigned *ticket;
+  double *csq;                // [T + 1] |staged row|^2 of the last scoring pass
 };
 
-// Per-block search of the prefix sums of v (segment sums seg[nb]; values
-// v_i = w_i, or w_i m_i when m != null) for nt targets r[j]: the first i with
-// P(i) > r (right) or P(i) >= r (left), else n - 1.  P(i) = (prefix of the
-// segment sums up to i's segment, as this block computes it) + (prefix of v
-// inside the segment).
-template <typename T>
-__device__ void block_search(int64_t n, int64_t nb, const double *seg,
-                             const T *w, const double *m, const double *r,
-                             int nt, bool right, int64_t *out,
-                             double *s_chunk /*[kNT]*/, int *s_seg /*[kMaxTrials]*/,
-                             double *s_before /*[kMaxTrials]*/) {
-  const int tid = threadIdx.x;
-  const int64_t per = ceil_div<int64_t>(nb, kNT);
-  const int64_t lo = tid * per, hi = (lo + per < nb) ? lo + per : nb;
+struct SearchLds {
+  double wtot[kPNT / kWave];  // wave totals, then their exclusive prefix
+  double r[kMaxTrials];
+  double before[kMaxTrials];
+  int seg[kMaxTrials];
+};
+
+__device__ inline double wave_incl_scan(double y, int lane) {
+  for (int o = 1; o < kWave; o <<= 1) {
+    const double t = __shfl_up(y, o, kWave);
+    if (lane >= o) y += t;
+  }
+  return y;
+}
+
+// Search of the prefix sums of v (value(i) = w_i or w_i m_i) for nt targets
+// L.r[j]: the first i with P(i) > r (right) or P(i) >= r (left), else n - 1.
+// Three levels, each defined by this search's own arithmetic: the segment
+// sums seg[nb], each thread owning a contiguous range (block-scanned range
+// totals, then a sequential walk), the first segment whose inclusive prefix
+// reaches r; inside it the sub-segment sums sub[nsub] (32 per segment), the
+// first reaching r from the segment's exclusive prefix (else its last);
+// inside that the 64 values, the first reaching r (else the last).  One
+// block; writes out[0 : nt].
+template <typename VF>
+__device__ void pick_search(int64_t n, int64_t nb, int64_t nsub, const double *seg,
+                            const double *sub, VF value, int nt, bool right,
+                            int64_t *out, SearchLds &L) {
+  const int tid = threadIdx.x, lane = lane_of(), wv = tid / kWave;
+  const int64_t per = ceil_div<int64_t>(nb, kPNT);
+  const int64_t lo = ((int64_t)tid * per < nb) ? (int64_t)tid * per : nb;
+  const int64_t hi = (lo + per < nb) ? lo + per : nb;
   double tsum = 0.0;
-  for (int64_t s = lo; s < hi; ++s) tsum += load_agent(seg + s);
-  if (tid < nt) s_seg[tid] = INT32_MAX;
-  s_chunk[tid] = tsum;
+  for (int64_t s = lo; s < hi; ++s) tsum += seg[s];
+  const double y = wave_incl_scan(tsum, lane);
+  double yp = __shfl_up(y, 1, kWave);
+  if (lane == 0) yp = 0.0;
+  if (lane == kWave - 1) L.wtot[wv] = y;
+  if (tid < nt) L.seg[tid] = INT32_MAX;
   __syncthreads();
-  if (tid == 0) {  // exclusive scan of the chunk sums, in chunk order
+  if (tid == 0) {                                   // exclusive wave prefix
     double run = 0.0;
-    for (int t = 0; t < kNT; ++t) {
-      const double v = s_chunk[t];
-      s_chunk[t] = run;
-      run += v;
+    for (int i = 0; i < kPNT / kWave; ++i) {
+      const double t = L.wtot[i];
+      L.wtot[i] = run;
+      run += t;
     }
   }
   __syncthreads();
-  // first segment whose walked prefix reaches r[j]: every thread walks its
-  // chunk (from the scanned chunk offset); the lowest hit wins, and its
-  // thread records the prefix before that segment
-  for (int j = 0; j < nt; ++j) {
-    const double rj = r[j];
-    int fnd = INT32_MAX;
-    double bef = 0.0, run = s_chunk[tid];
-    for (int64_t s = lo; s < hi; ++s) {
-      const double nxt = run + load_agent(seg + s);
-      if (right ? (nxt > rj) : (nxt >= rj)) {
-        fnd = (int)s;
-        bef = run;
-        break;
+  int found[kMaxTrials];
+  double bef[kMaxTrials], rj[kMaxTrials];
+#pragma unroll
+  for (int j = 0; j < kMaxTrials; ++j) {
+    found[j] = INT32_MAX;
+    bef[j] = 0.0;
+    rj[j] = (j < nt) ? L.r[j] : 0.0;
+  }
+  double run = L.wtot[wv] + yp;                     // prefix before this range
+  for (int64_t s = lo; s < hi; ++s) {
+    const double nxt = run + seg[s];
+#pragma unroll
+    for (int j = 0; j < kMaxTrials; ++j) {
+      const bool hit = right ? (nxt > rj[j]) : (nxt >= rj[j]);
+      if (j < nt && hit && found[j] == INT32_MAX) {
+        found[j] = (int)s;
+        bef[j] = run;
       }
-      run = nxt;
     }
-    if (fnd != INT32_MAX) atomicMin(&s_seg[j], fnd);
-    __syncthreads();
-    if (fnd != INT32_MAX && fnd == s_seg[j]) s_before[j] = bef;
+    run = nxt;
   }
+#pragma unroll
+  for (int j = 0; j < kMaxTrials; ++j)
+    if (j < nt && found[j] != INT32_MAX) atomicMin(&L.seg[j], found[j]);
   __syncthreads();
-  // inside the segment: wave j % 4 takes target j, 16 samples per lane
-  const int wv = tid / kWave, lane = lane_of();
-  constexpr int kPer = kSeg / kWave;
-  for (int j = wv; j < nt; j += kNT / kWave) {
-    const int s = s_seg[j];
-    if (s == INT32_MAX) {
+#pragma unroll
+  for (int j = 0; j < kMaxTrials; ++j)
+    if (j < nt && found[j] != INT32_MAX && found[j] == L.seg[j]) L.before[j] = bef[j];
+  __syncthreads();
+  for (int j = wv; j < nt; j += kPNT / kWave) {    // one wave per target
+    const int sj = L.seg[j];
+    if (sj == INT32_MAX) {
       if (lane == 0) out[j] = n - 1;
       continue;
     }
-    const int64_t base = (int64_t)s * kSeg;
-    const int64_t end = (base + kSeg < n) ? base + kSeg : n;
-    const int64_t i0 = base + (int64_t)lane * kPer;
-    double v[kPer];
-    double lsum = 0.0;
-#pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-      const int64_t i = i0 + e;
-      double x = 0.0;
-      if (i < end) {
-        x = (double)w[i];
-        if (m) x *= load_agent(m + i);
-      }
-      v[e] = x;
-      lsum += x;
-    }
-    // inclusive scan of the lane sums
-    double inc = lsum;
-    for (int o = 1; o < kWave; o <<= 1) {
-      const double y = __shfl_up(inc, o, kWave);
-      if (lane >= o) inc += y;
-    }
-    double q = __shfl_up(inc, 1, kWave);
-    if (lane == 0) q = 0.0;
-    q += s_before[j];
-    int hit = -1;
-#pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-      q += v[e];
-      const bool h = (i0 + e < end) && (right ? (q > r[j]) : (q >= r[j]));
-      if (h && hit < 0) hit = e;
-    }
-    const unsigned long long mask = __ballot(hit >= 0);
-    if (mask == 0ull) {
-      if (lane == 0) out[j] = end - 1;
-    } else {
-      const int first = __ffsll((long long)mask) - 1;
-      if (lane == first) out[j] = i0 + hit;
-    }
+    const double r = L.r[j];
+    // sub-segments of segment sj
+    const int64_t b0 = (int64_t)sj * kSubPerSeg;
+    const int64_t b_end = (b0 + kSubPerSeg < nsub) ? b0 + kSubPerSeg : nsub;
+    const bool bin = (lane < kSubPerSeg) && (b0 + lane < b_end);
+    const double yb = wave_incl_scan(bin ? sub[b0 + lane] : 0.0, lane);
+    double ybp = __shfl_up(yb, 1, kWave);
+    if (lane == 0) ybp = 0.0;
+    const double ib = L.before[j] + yb;
+    unsigned long long mask = __ballot(bin && (right ? (ib > r) : (ib >= r)));
+    const int lb = mask ? __ffsll((long long)mask) - 1 : (int)(b_end - b0 - 1);
+    const int64_t bsel = b0 + lb;
+    const double before2 = L.before[j] + __shfl(ybp, lb, kWave);
+    // samples of sub-segment bsel
+    const int64_t i = bsel * kSub + lane;
+    const int64_t i_end = (bsel * kSub + kSub < n) ? bsel * kSub + kSub : n;
+    const double vi = (i < i_end) ? value(i) : 0.0;
+    const double qi = before2 + wave_incl_scan(vi, lane);
+    mask = __ballot(i < i_end && (right ? (qi > r) : (qi >= r)));
+    if (lane == 0) out[j] = mask ? bsel * kSub + (__ffsll((long long)mask) - 1) : i_end - 1;
   }
 }
 
-// Segment sums of w, then (last block) the first center.
+// Segment and sub-segment sums of w.
 template <typename T>
-__global__ void __launch_bounds__(kNT) kpp_first_krn(KppArgs a) {
-  __shared__ double lds[kNT / kWave];
-  __shared__ double s_chunk[kNT];
-  __shared__ int s_seg[kMaxTrials];
-  __shared__ double s_before[kMaxTrials];
-  __shared__ double s_r[1];
-  __shared__ bool last;
+__global__ void __launch_bounds__(kSNT) kpp_wsum_krn(KppArgs a) {
+  __shared__ double lds[kSNT / kWave];
   const T *w = static_cast<const T *>(a.w);
-  const int64_t i0 = (int64_t)blockIdx.x * kSeg + 4 * threadIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.x * kSeg + kSPT * threadIdx.x;
   double v = 0.0;
-  for (int j = 0; j < 4; ++j)
+  for (int j = 0; j < kSPT; ++j)
     if (i0 + j < a.n) v += (double)w[i0 + j];
-  const double s = block_sum_d(v, lds);
+  for (int o = 1; o < kSub / kSPT; o <<= 1) v += __shfl_xor(v, o, kWave);
+  const int64_t sb = i0 / kSub;
+  if ((threadIdx.x & (kSub / kSPT - 1)) == 0 && sb < a.nsub) a.part2[sb] = v;
+  for (int o = kSub / kSPT; o < kWave; o <<= 1) v += __shfl_xor(v, o, kWave);
+  if (lane_of() == 0) lds[threadIdx.x / kWave] = v;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    a.part[blockIdx.x] = s;
-    __threadfence();
-    last = atomicAdd(a.ticket, 1u) == gridDim.x - 1;
+    double t = 0.0;
+    for (int i = 0; i < kSNT / kWave; ++i) t += lds[i];
+    a.part[blockIdx.x] = t;
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double t = 0.0;
-  for (int64_t b = threadIdx.x; b < a.nb; b += kNT) t += load_agent(a.part + b);
-  const double tot = block_sum_d(t, lds);
-  if (threadIdx.x == 0) s_r[0] = a.u[0] * tot;
-  __syncthreads();
-  block_search<T>(a.n, a.nb, a.part, w, nullptr, s_r, 1, true, a.cand, s_chunk,
-                  s_seg, s_before);
-  if (threadIdx.x == 0) a.ticket[0] = 0u;
 }
 
-// One center: scores the segment against the candidates, then (last block)
-// picks the winner and the next candidates.
+// One block: the first center, drawn with probability proportional to w.
+template <typename T>
+__global__ void __launch_bounds__(kPNT) kpp_first_pick_krn(KppArgs a) {
+  __shared__ double lds[kPNT / kWave];
+  __shared__ SearchLds L;
+  const T *w = static_cast<const T *>(a.w);
+  double t = 0.0;
+  for (int64_t b = threadIdx.x; b < a.nb; b += kPNT) t += a.part[b];
+  for (int o = kWave / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);
+  if (lane_of() == 0) lds[threadIdx.x / kWave] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int i = 0; i < kPNT / kWave; ++i) tot += lds[i];
+    L.r[0] = a.u[0] * tot;
+  }
+  pick_search(a.n, a.nb, a.nsub, a.part, a.part2,
+              [&](int64_t i) { return (double)w[i]; }, 1, true, a.cand, L);
+}
+
+// Squared distance in the scoring pass's arithmetic: |x|^2 + |c|^2 - 2 x.c
+// in FP64 with the three sums accumulated by fused multiply-adds over
+// features in ascending order, clamped at 0.
+__device__ inline double kpp_dist(double xsq, double csq, double xc) {
+  const double dd = fma(-2.0, xc, xsq + csq);
+  return dd > 0.0 ? dd : 0.0;
+}
+
+// One center, scoring pass (pure streaming; the pick is the next launch).
+// Row 0 of the staged points is center c - 1 (the first center when
+// c == 0): D^2 <- min(D^2, dist0) is written to d2[c & 1].  Rows 1..T are
+// the candidates of center c: m_t = min(D^2, dist_t) gives the segment and
+// sub-segment sums of w m_t (for c == 0, of w D^2 itself).
 template <typename T, int TT>
-__global__ void __launch_bounds__(kNT) kpp_step_krn(KppArgs a, int c) {
-  __shared__ double cl[TT * kFC];
-  __shared__ double lds[kNT / kWave];
-  __shared__ double s_chunk[kNT];
-  __shared__ int s_seg[kMaxTrials];
-  __shared__ double s_before[kMaxTrials];
-  __shared__ double s_pot[kMaxTrials];
-  __shared__ double s_r[kMaxTrials];
-  __shared__ int64_t s_idx;
-  __shared__ int s_ts;
-  __shared__ bool last;
+__global__ void __launch_bounds__(kSNT, 4) kpp_score_krn(KppArgs a, int c) {
+  __shared__ double cl[(TT + 1) * kFC];
+  __shared__ double csq[TT + 1];
   const T *X = static_cast<const T *>(a.X_t);
   const T *w = static_cast<const T *>(a.w);
   const int64_t n = a.n, d = a.d;
-  const int nc = (c == 0) ? 1 : a.trials;  // candidates of this center
-  const double *dold =
-      (c == 0) ? nullptr : a.m[(c - 1) & 1] + (int64_t)a.tstar[0] * n;
-  double *mout = a.m[c & 1];
-  const int64_t i0 = (int64_t)blockIdx.x * kSeg + 4 * threadIdx.x;
-  const bool vec = (i0 + 3 < n) && (n % 4 == 0) &&
+  const int nc = (c == 0) ? 0 : a.trials;   // candidates of this center
+  const int rows = nc + 1;
+  const int64_t idx0 = (c == 0) ? a.cand[0] : a.indices[c - 1];
+  const double *dold = (c == 0) ? nullptr : a.d2[(c - 1) & 1];
+  double *dnew = a.d2[c & 1];
+  const int64_t i0 = (int64_t)blockIdx.x * kSeg + kSPT * threadIdx.x;
+  const bool vec = (i0 + kSPT - 1 < n) && (n % kSPT == 0) &&
                    ((reinterpret_cast<uintptr_t>(X) & 15u) == 0);
-  double acc[4][TT];
+  double acc0[kSPT], acc[kSPT][TT], xsq[kSPT];
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
+  for (int j = 0; j < kSPT; ++j) {
+    xsq[j] = 0.0;
+    acc0[j] = 0.0;
 #pragma unroll
     for (int t = 0; t < TT; ++t) acc[j][t] = 0.0;
+  }
+  if (threadIdx.x <= TT) csq[threadIdx.x] = 0.0;
+  auto step = [&](const Quad<T> &xq, int f) {
+    double x[kSPT];
+#pragma unroll
+    for (int j = 0; j < kSPT; ++j) x[j] = (double)xq.v[j];
+    const double c0 = cl[f];
+#pragma unroll
+    for (int j = 0; j < kSPT; ++j) {
+      xsq[j] = fma(x[j], x[j], xsq[j]);
+      acc0[j] = fma(x[j], c0, acc0[j]);
+    }
+#pragma unroll
+    for (int t = 0; t < TT; ++t) {
+      if (t < nc) {
+        const double cv = cl[(t + 1) * kFC + f];
+#pragma unroll
+        for (int j = 0; j < kSPT; ++j) acc[j][t] = fma(x[j], cv, acc[j][t]);
+      }
+    }
+  };
 
   for (int64_t f0 = 0; f0 < d; f0 += kFC) {
     const int fcn = (int)((d - f0 < kFC) ? d - f0 : kFC);
     __syncthreads();
-    for (int e = threadIdx.x; e < TT * fcn; e += kNT) {
-      const int t = e / fcn, f = e % fcn;
-      cl[t * kFC + f] = (t < nc) ? (double)X[(f0 + f) * n + a.cand[t]] : 0.0;
+    for (int e = threadIdx.x; e < (TT + 1) * fcn; e += kSNT) {
+      const int r = e / fcn, f = e % fcn;
+      const int64_t src = (r == 0) ? idx0 : ((r < rows) ? a.cand[r - 1] : -1);
+      cl[r * kFC + f] = (src >= 0) ? (double)X[(f0 + f) * n + src] : 0.0;
     }
     __syncthreads();
-    for (int f = 0; f < fcn; ++f) {
-      const T *row = X + (f0 + f) * n + i0;
-      double x[4];
-      if constexpr (sizeof(T) == 4) {
-        if (vec) {
-          const float4 v = *reinterpret_cast<const float4 *>(row);
-          x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
-        } else {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) x[j] = (i0 + j < n) ? (double)row[j] : 0.0;
-        }
-      } else {
-        if (vec) {
-          const double2 v0 = *reinterpret_cast<const double2 *>(row);
-          const double2 v1 = *reinterpret_cast<const double2 *>(row + 2);
-          x[0] = v0.x; x[1] = v0.y; x[2] = v1.x; x[3] = v1.y;
-        } else {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) x[j] = (i0 + j < n) ? (double)row[j] : 0.0;
-        }
+    if (threadIdx.x <= TT) {    // |c|^2 of each staged row, features ascending
+      double q = csq[threadIdx.x];
+      for (int f = 0; f < fcn; ++f) {
+        const double cv = cl[threadIdx.x * kFC + f];
+        q = fma(cv, cv, q);
       }
-#pragma unroll
-      for (int t = 0; t < TT; ++t) {
-        if (t < nc) {
-          const double cv = cl[t * kFC + f];
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const double dx = x[j] - cv;
-            acc[j][t] = fma(dx, dx, acc[j][t]);
-          }
-        }
-      }
+      csq[threadIdx.x] = q;
     }
-  }
-
-  // m_t = min(D^2, dist_t); segment sums of w m_t
-  double wv[4], dv[4];
+    // groups of UF feature rows, the next group's loads issued before the
+    // current group is scored (2 UF rows in flight per lane)
+    const T *base = X + f0 * n + i0;
+    constexpr int UF0 = (TT <= 4) ? kUF : ((TT <= 7) ? kUF / 2 : 1);
+    constexpr int UF = (sizeof(T) == 4 || UF0 == 1) ? UF0 : UF0 / 2;  // registers
+    const int fg = fcn - fcn % UF;
+    Quad<T> cur[UF], nxt[UF];
+    if (fg > 0) {
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+      for (int u = 0; u < UF; ++u) cur[u] = load_quad(base + (int64_t)u * n, i0, n, vec);
+    }
+    for (int f = 0; f < fg; f += UF) {
+      if (f + UF < fg) {
+#pragma unroll
+        for (int u = 0; u < UF; ++u)
+          nxt[u] = load_quad(base + (int64_t)(f + UF + u) * n, i0, n, vec);
+      }
+#pragma unroll
+      for (int u = 0; u < UF; ++u) step(cur[u], f + u);
+#pragma unroll
+      for (int u = 0; u < UF; ++u) cur[u] = nxt[u];
+    }
+    for (int f = fg; f < fcn; ++f) step(load_quad(base + (int64_t)f * n, i0, n, vec), f);
+  }
+  __syncthreads();            // csq complete, cl free
+  if (blockIdx.x == 0 && threadIdx.x <= TT) a.csq[threadIdx.x] = csq[threadIdx.x];
+
+  double wv[kSPT], dn[kSPT];
+  const double c0sq = csq[0];
+#pragma unroll
+  for (int j = 0; j < kSPT; ++j) {
     const bool in = i0 + j < n;
     wv[j] = in ? (double)w[i0 + j] : 0.0;
-    dv[j] = (in && dold) ? dold[i0 + j] : __builtin_inf();
+    const double dv = (in && dold) ? dold[i0 + j] : __builtin_inf();
+    const double d0 = kpp_dist(xsq[j], c0sq, acc0[j]);
+    dn[j] = (dv <= d0) ? dv : d0;
   }
+  if (vec) {
+    *reinterpret_cast<double2 *>(dnew + i0) = make_double2(dn[0], dn[1]);
+    *reinterpret_cast<double2 *>(dnew + i0 + 2) = make_double2(dn[2], dn[3]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSPT; ++j)
+      if (i0 + j < n) dnew[i0 + j] = dn[j];
+  }
+
+  double *wsum = cl;          // [kSNT / kWave][TT]
+  const int64_t sb = i0 / kSub;
+  const bool sub_lead = (threadIdx.x & (kSub / kSPT - 1)) == 0 && sb < a.nsub;
+  const int ns = (c == 0) ? 1 : nc;        // sums to produce
 #pragma unroll
   for (int t = 0; t < TT; ++t) {
-    if (t < nc) {
-      double mv[4], p = 0.0;
+    if (t < ns) {
+      double p = 0.0;
+      const double ct = csq[t + 1];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const double dd = acc[j][t];
-        mv[j] = (dv[j] <= dd) ? dv[j] : dd;
-        p += (i0 + j < n) ? wv[j] * mv[j] : 0.0;
+      for (int j = 0; j < kSPT; ++j) {
+        double mv = dn[j];
+        if (c != 0) {
+          const double dd = kpp_dist(xsq[j], ct, acc[j][t]);
+          mv = (dn[j] <= dd) ? dn[j] : dd;
+        }
+        p += (i0 + j < n) ? wv[j] * mv : 0.0;
       }
-      double *mo = mout + (int64_t)t * n + i0;
-      if (vec) {
-        *reinterpret_cast<double2 *>(mo) = make_double2(mv[0], mv[1]);
-        *reinterpret_cast<double2 *>(mo + 2) = make_double2(mv[2], mv[3]);
-      } else {
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (i0 + j < n) mo[j] = mv[j];
-      }
-      const double s = block_sum_d(p, lds);
-      if (threadIdx.x == 0) a.part[(int64_t)t * a.nb + blockIdx.x] = s;
+      // sub-segment sum (16 lanes), then the wave sum
+      for (int o = 1; o < kSub / kSPT; o <<= 1) p += __shfl_xor(p, o, kWave);
+      if (sub_lead) a.part2[(int64_t)t * a.nsub + sb] = p;
+      for (int o = kSub / kSPT; o < kWave; o <<= 1) p += __shfl_xor(p, o, kWave);
+      if (lane_of() == 0) wsum[(threadIdx.x / kWave) * TT + t] = p;
     }
   }
-  // every thread wrote m rows the last block reads: release from all waves
-  __threadfence();
   __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(a.ticket, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+  if (threadIdx.x < ns) {
+    double tot = 0.0;
+    for (int i = 0; i < kSNT / kWave; ++i) tot += wsum[i * TT + threadIdx.x];
+    a.part[(int64_t)threadIdx.x * a.nb + blockIdx.x] = tot;
+  }
+}
 
-  // potentials, winner, center
-  for (int t = 0; t < nc; ++t) {
+// One center, pick: potentials, winner (first minimum), center copy, and the
+// next center's candidates.  One block.  The values w_i m_t*(i) the last
+// search level needs are recomputed for 64 samples per target with the
+// scoring pass's arithmetic (the pass does not store m).
+template <typename T>
+__global__ void __launch_bounds__(kPNT) kpp_pick_krn(KppArgs a, int c) {
+  __shared__ double red[kPNT / kWave][kMaxTrials];
+  __shared__ SearchLds L;
+  __shared__ double s_pot[kMaxTrials];
+  __shared__ double s_csq;
+  __shared__ int64_t s_idx;
+  __shared__ int s_ts;
+  const T *X = static_cast<const T *>(a.X_t);
+  const T *w = static_cast<const T *>(a.w);
+  const int64_t n = a.n, d = a.d, k = a.k;
+  const int ns = (c == 0) ? 1 : a.trials;
+#pragma unroll
+  for (int t = 0; t < kMaxTrials; ++t) {       // all potentials in one pass
     double v = 0.0;
-    for (int64_t b = threadIdx.x; b < a.nb; b += kNT)
-      v += load_agent(a.part + (int64_t)t * a.nb + b);
-    const double tot = block_sum_d(v, lds);
-    if (threadIdx.x == 0) s_pot[t] = tot;
+    if (t < ns)
+      for (int64_t b = threadIdx.x; b < a.nb; b += kPNT) v += a.part[(int64_t)t * a.nb + b];
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if (lane_of() == 0) red[threadIdx.x / kWave][t] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < ns) {
+    double tot = 0.0;
+    for (int i = 0; i < kPNT / kWave; ++i) tot += red[i][threadIdx.x];
+    s_pot[threadIdx.x] = tot;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     int ts = 0;
-    for (int t = 1; t < nc; ++t)
+    for (int t = 1; t < ns; ++t)
       if (s_pot[t] < s_pot[ts]) ts = t;
     s_ts = ts;
     s_idx = a.cand[ts];
-    a.tstar[0] = ts;
     a.indices[c] = s_idx;
+    s_csq = a.csq[ts + 1];                     // |center c|^2 as the pass has it
   }
   __syncthreads();
+  const int64_t idx = s_idx;
   T *C = static_cast<T *>(a.centers_t);
-  for (int64_t f = threadIdx.x; f < d; f += kNT) C[f * a.k + c] = X[f * n + s_idx];
-  if (c + 1 < a.k) {
+  for (int64_t f = threadIdx.x; f < d; f += kPNT) C[f * k + c] = X[f * n + idx];
+  if (c + 1 < k) {
     const int ts = s_ts;
     if (threadIdx.x < a.trials)
-      s_r[threadIdx.x] = a.u[1 + (int64_t)c * a.trials + threadIdx.x] * s_pot[ts];
-    __syncthreads();
-    block_search<T>(n, a.nb, a.part + (int64_t)ts * a.nb, w,
-                    mout + (int64_t)ts * n, s_r, a.trials, false, a.cand,
-                    s_chunk, s_seg, s_before);
+      L.r[threadIdx.x] = a.u[1 + (int64_t)c * a.trials + threadIdx.x] * s_pot[ts];
+    const double *dn = a.d2[c & 1];
+    const double ccsq = s_csq;
+    auto value = [&](int64_t i) {
+      double m = dn[i];
+      if (c != 0) {                            // min(D^2, |x_i - center c|^2)
+        double xsq = 0.0, xc = 0.0;
+#pragma unroll 8
+        for (int64_t f = 0; f < d; ++f) {
+          const double x = (double)X[f * n + i];
+          xsq = fma(x, x, xsq);
+          xc = fma(x, (double)X[f * n + idx], xc);
+        }
+        const double dd = kpp_dist(xsq, ccsq, xc);
+        m = (m <= dd) ? m : dd;
+      }
+      return (double)w[i] * m;
+    };
+    pick_search(n, a.nb, a.nsub, a.part + (int64_t)ts * a.nb,
+                a.part2 + (int64_t)ts * a.nsub, value, a.trials, false, a.cand, L);
   }
-  if (threadIdx.x == 0) a.ticket[0] = 0u;
 }
 
 template <typename T, int TT>
 void launch_kpp_step(const KppArgs &a, int c, hipStream_t s) {
-  hipLaunchKernelGGL((kpp_step_krn<T, TT>), dim3((unsigned)a.nb), dim3(kNT), 0,
+  hipLaunchKernelGGL((kpp_score_krn<T, TT>), dim3((unsigned)a.nb), dim3(kSNT), 0,
                      s, a, c);
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(kpp_pick_krn<T>, dim3(1), dim3(kPNT), 0, s, a, c);
   KMD_LAUNCH_CHECK();
 }
 
@@ -502,36 +618,44 @@ void launch_kmeans_plusplus(int64_t n, int64_t d, int64_t k, int trials,
           "n_local_trials must be between 1 and 16");
   require(k >= 1 && n >= k, "n_samples must be >= n_clusters >= 1");
   const int64_t nb = ceil_div<int64_t>(n, kSeg);
+  const int64_t nsub = ceil_div<int64_t>(n, kSub);
   require(nb <= INT32_MAX, "too many samples");
-  const size_t mrow = (size_t)trials * (size_t)n;
-  TempBuffer mbuf(2 * mrow * sizeof(double), s);
-  TempBuffer part((size_t)trials * (size_t)nb * sizeof(double), s);
-  TempBuffer small(256, s);
+  TempBuffer dbuf(2 * (size_t)n * sizeof(double), s);
+  TempBuffer part((size_t)trials * (size_t)(nb + nsub) * sizeof(double), s);
+  TempBuffer small(512, s);
   KppArgs a{};
-  a.n = n; a.d = d; a.k = k; a.nb = nb; a.trials = trials;
+  a.n = n; a.d = d; a.k = k; a.nb = nb; a.nsub = nsub; a.trials = trials;
   a.X_t = X_t; a.w = w; a.u = u_dev; a.centers_t = centers_t;
   a.indices = indices;
-  a.m[0] = mbuf.as<double>();
-  a.m[1] = mbuf.as<double>() + mrow;
+  a.d2[0] = dbuf.as<double>();
+  a.d2[1] = dbuf.as<double>() + n;
   a.part = part.as<double>();
+  a.part2 = part.as<double>() + (size_t)trials * (size_t)nb;
   char *sm = small.as<char>();
   a.cand = reinterpret_cast<int64_t *>(sm);              // 16 x 8 bytes
-  a.tstar = reinterpret_cast<int *>(sm + 128);
-  a.ticket = reinterpret_cast<unsigned *>(sm + 136);
-  KMD_HIP_CHECK(hipMemsetAsync(sm, 0, 256, s));
-  hipLaunchKernelGGL(kpp_first_krn<T>, dim3((unsigned)nb), dim3(kNT), 0, s, a);
+  a.csq = reinterpret_cast<double *>(sm + 128);          // 17 x 8 bytes
+  KMD_HIP_CHECK(hipMemsetAsync(sm, 0, 512, s));
+  hipLaunchKernelGGL(kpp_wsum_krn<T>, dim3((unsigned)nb), dim3(kSNT), 0, s, a);
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(kpp_first_pick_krn<T>, dim3(1), dim3(kPNT), 0, s, a);
   KMD_LAUNCH_CHECK();
   launch_kpp_step<T, 1>(a, 0, s);
-  for (int64_t c = 1; c < k; ++c) {
-    if (trials <= 2)
-      launch_kpp_step<T, 2>(a, (int)c, s);
-    else if (trials <= 4)
-      launch_kpp_step<T, 4>(a, (int)c, s);
-    else if (trials <= 8)
-      launch_kpp_step<T, 8>(a, (int)c, s);
-    else
-      launch_kpp_step<T, 16>(a, (int)c, s);
-  }
+  // candidate registers sized to the trial count (2 + floor(ln k) is 2..8
+  // for k < 1097)
+  auto step = [&](int c) {
+    switch (trials) {
+      case 1: launch_kpp_step<T, 1>(a, c, s); break;
+      case 2: launch_kpp_step<T, 2>(a, c, s); break;
+      case 3: launch_kpp_step<T, 3>(a, c, s); break;
+      case 4: launch_kpp_step<T, 4>(a, c, s); break;
+      case 5: launch_kpp_step<T, 5>(a, c, s); break;
+      case 6: launch_kpp_step<T, 6>(a, c, s); break;
+      case 7: launch_kpp_step<T, 7>(a, c, s); break;
+      case 8: launch_kpp_step<T, 8>(a, c, s); break;
+      default: launch_kpp_step<T, 16>(a, c, s); break;
+    }
+  };
+  for (int64_t c = 1; c < k; ++c) step((int)c);
 }
 
 #define KMD_SEED_INST(T)                                                       \
