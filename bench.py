@@ -451,6 +451,19 @@ def bench_c4(args):
     print(json.dumps(line), flush=True)
 
 
+def copy_probe(mib, dev):
+    """One device-to-device copy of `mib` MiB: a kernel of known HBM traffic
+    that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    if mib <= 0:
+        return
+    import torch
+    src = torch.full((mib << 18,), 1.0, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def bench_fit(args):
     """The estimator front-end end to end: KMeans(k, init='k-means++',
     n_init=1, max_iter=steps, tol=0).fit(X) on sample-major float32 X of the
@@ -466,6 +479,7 @@ def bench_fit(args):
     X_t0, _ = make_data(n, d, k, SEED, dev)
     X = X_t0.t().contiguous()                      # (n, d) sample-major
     del X_t0
+    copy_probe(args.copy_probe_mib, dev)
     w = torch.ones(n, dtype=torch.float32, device=dev)
     T = 2 + int(np.log(k))
     torch.cuda.synchronize()
@@ -510,9 +524,14 @@ def bench_fit(args):
     kpp_bytes = (n * 4 + n * (4 * d + 4 + 8)
                  + (k - 1) * n * (4 * d + 4 + 8 + 8))
     kpp_gbs = kpp_bytes / (kpp_ms * 1e-3) / 1e9
+    here = os.path.dirname(os.path.abspath(__file__))
+    # HBM bytes per scoring launch measured with rocprofv3 (profiled shape only)
+    traffic = (traffic_from_profile(os.path.join(here, "profiles",
+                                                 "kmeanspp_fit_n10M_d32_k256.md"))
+               if (n, d, k) == (N, D, K) else None)
     roof = {"bound": "hbm", "achieved": round(kpp_gbs, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(kpp_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "k-means++ seeding (kpp_score_krn + one-block kpp_pick_krn per center)",
             "kernel_ms": round(kpp_ms, 3),
             "algorithmic_bytes": kpp_bytes}
@@ -657,13 +676,7 @@ def main():
     w = torch.ones(n, dtype=torch.float32, device=dev)
     labels = torch.empty(n, dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream()
-    if args.copy_probe_mib > 0:
-        src = torch.full((args.copy_probe_mib << 18,), 1.0, dtype=torch.float32,
-                         device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+    copy_probe(args.copy_probe_mib, dev)
 
     # warmup: W iterations (untimed)
     c_a = init_t.clone()
