@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(kNT)
             (f < d && c < k) ? C[(int64_t)f * k + c] : 0.f;
       } else {
         const int c = t * 32 + (e - 2 * KS * 32);
-        Cl[t * TF + e] = (c < k) ? hn[c] : __builtin_huge_valf();
+        Cl[t * TF + e] = (c < k) ? -hn[c] : -__builtin_huge_valf();  // -hn
       }
     }
     __syncthreads();
@@ -176,6 +176,15 @@ __global__ void __launch_bounds__(kNT)
     int besti[2] = {0, 0};
     for (int t = 0; t < n_ct; ++t) {
       const float *At = Cl + t * TF;
+      // the chain starts from the tile's -hn, so -acc = hn - c.x
+      const float *Ht = At + 2 * KS * 32;
+      f32x16 h;
+#pragma unroll
+      for (int gg = 0; gg < 4; ++gg) {
+        const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + 8 * gg + 4 * half);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) h[4 * gg + e] = h4[e];
+      }
       f32x16 acc[2];
 #pragma unroll
       for (int k4 = 0; k4 < KS / 4; ++k4) {
@@ -186,8 +195,7 @@ __global__ void __launch_bounds__(kNT)
 #pragma unroll
           for (int q = 0; q < 2; ++q) {
             if (k4 == 0 && j == 0) {
-              const f32x16 z = {};
-              acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[0][q], z, 0,
+              acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[0][q], h, 0,
                                                             0, 0);
             } else {
               acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(
@@ -195,16 +203,14 @@ __global__ void __launch_bounds__(kNT)
             }
           }
       }
-      const float *Ht = At + 2 * KS * 32;
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
         const int m0 = 8 * g + 4 * half;
-        const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + m0);
 #pragma unroll
         for (int e = 0; e < 4; ++e)
 #pragma unroll
           for (int q = 0; q < 2; ++q) {
-            const float v = h4[e] - acc[q][4 * g + e];
+            const float v = -acc[q][4 * g + e];
             const bool upd = v < bestv[q];
             bestv[q] = upd ? v : bestv[q];
             besti[q] = upd ? (t * 32 + m0 + e) : besti[q];

@@ -13,9 +13,9 @@
 // q < S, read once from HBM with one S-wide load per feature (the column a
 // sample occupies is a free choice of the product, so neighbouring samples
 // sit in one lane).  The centroid matrix (zero-padded to 32
-// columns and an even number of rows) and the half-norms (+inf padded) stay
-// in LDS for the life of the persistent block; each A operand read from LDS
-// feeds S MFMAs.
+// columns and an even number of rows) and the negated half-norms (-inf
+// padded) stay in LDS for the life of the persistent block; each A operand
+// read from LDS feeds S MFMAs.
 //
 // Update.  w_s x_s is accumulated into per-block sums in LDS as 64-bit
 // fixed-point integers (ds_add_u64): q = rint(w x 2^E_f), with one exponent
@@ -33,11 +33,14 @@
 // FP32 and added with global atomics into private copy (block % n_copies)
 // (the fused_lloyd_single_step entry point).
 //
-// Arithmetic of the labels: FP32 products accumulated in FP32 by the matrix
-// core; the pseudo-inertia hn[c] - dot and the argmin rule (strict <, lowest
-// index on ties) are those of src/device_functions.hpp:139-162.  Relative to
-// the sequential FMA chain of the reference the dot products differ by a few
-// ulps, so labels can differ only on sub-ulp margins.
+// Arithmetic of the labels: the matrix core accumulates -hn[c] + sum_f
+// C[f, c] X[f, s] in FP32 (the chain starts from -hn as its C operand), and
+// its exact negation is the pseudo-inertia hn[c] - dot; the argmin rule
+// (strict <, lowest index on ties) is that of
+// src/device_functions.hpp:139-162.  Relative to the sequential FMA chain of
+// the reference the scores differ by a few ulps, so labels can differ only
+// on sub-ulp margins.  Every FP32 path (tiled, batched, re-score) uses this
+// same sequence, so their labels are identical.
 #include "fixed_point.hpp"
 #include "lloyd_fast.hpp"
 
@@ -90,18 +93,29 @@ __device__ inline void load_x(const float *__restrict__ X, int64_t n, int d,
   }
 }
 
-// acc[q] = C[:, tile] . X[:, columns of block q] for one 32-centroid tile.
-// Ct = this lane's A operands of the tile: Ct[64 ks] = C[2 ks + half][col]
-// (the LDS layout below), so every k-step reads at an immediate offset.
+// acc[q] = -hn[tile] + C[:, tile] . X[:, columns of block q] for one
+// 32-centroid tile: the chain starts from the negated half norms (the C
+// operand of the first MFMA), so -acc is the score hn - c.x with no vector
+// instruction per score.  Ct = this lane's A operands of the tile:
+// Ct[64 ks] = C[2 ks + half][col] (the LDS layout below), so every k-step
+// reads at an immediate offset; Ht = the tile's -hn at this lane's first row.
 template <int KS, int S>
 __device__ inline void tile_mfma(const float *__restrict__ Ct,
+                                 const float *__restrict__ Ht,
                                  const float (&xr)[KS][S], f32x16 (&acc)[S]) {
+  f32x16 h;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const float4 h4 = *reinterpret_cast<const float4 *>(Ht + 8 * g);
+    h[4 * g + 0] = h4.x;
+    h[4 * g + 1] = h4.y;
+    h[4 * g + 2] = h4.z;
+    h[4 * g + 3] = h4.w;
+  }
   const float a0 = Ct[0];
 #pragma unroll
-  for (int q = 0; q < S; ++q) {
-    f32x16 z = {};
-    acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, xr[0][q], z, 0, 0, 0);
-  }
+  for (int q = 0; q < S; ++q)
+    acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, xr[0][q], h, 0, 0, 0);
 #pragma unroll
   for (int ks = 1; ks < KS; ++ks) {
     const float a = Ct[64 * ks];
@@ -119,8 +133,7 @@ __device__ inline void tile_mfma(const float *__restrict__ Ct,
 // tracked as r (an inline constant) and tiles as ct; the centroid index is
 // formed once at the end (best_index).
 template <int S>
-__device__ inline void tile_argmin(const float *__restrict__ Hl, int ct,
-                                   int half, const f32x16 (&acc)[S],
+__device__ inline void tile_argmin(int ct, const f32x16 (&acc)[S],
                                    float (&bestv)[S], int (&bestt)[S],
                                    int (&bestr)[S]) {
   float tv[S];
@@ -132,14 +145,11 @@ __device__ inline void tile_argmin(const float *__restrict__ Hl, int ct,
   }
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
-    const int m0 = ct * 32 + 8 * g + 4 * half;
-    const float4 h4 = *reinterpret_cast<const float4 *>(Hl + m0);
-    const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
 #pragma unroll
       for (int q = 0; q < S; ++q) {
-        const float v = hv[e] - acc[q][4 * g + e];
+        const float v = -acc[q][4 * g + e];   // hn - c.x (exact negation)
         const bool upd = v < tv[q];
         tv[q] = upd ? v : tv[q];
         tr[q] = upd ? (4 * g + e) : tr[q];
@@ -239,7 +249,7 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
     Cl[i] = (f < d && c < k) ? a.C[(int64_t)f * k + c] : 0.f;
   }
   for (int c = tid; c < kpad; c += NT)
-    Hl[c] = (c < k) ? a.hn[c] : __builtin_huge_valf();
+    Hl[c] = (c < k) ? -a.hn[c] : -__builtin_huge_valf();   // -hn, -inf padded
   if (UPDATE) {
     for (int f = tid; f <= d; f += NT) Sc[f] = a.scale[f];
     for (int64_t i = tid; i < R * (dk + k); i += NT) S64[i] = 0ull;
@@ -327,8 +337,8 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
     const float *Cw = Cl + lane;
     for (int ct = 0; ct < n_ct; ++ct) {
       f32x16 acc[S];
-      tile_mfma<KS, S>(Cw + ct * (KS * 64), xr, acc);
-      tile_argmin<S>(Hl, ct, half, acc, bestv, bestt, bestr);
+      tile_mfma<KS, S>(Cw + ct * (KS * 64), Hl + ct * 32 + 4 * half, xr, acc);
+      tile_argmin<S>(ct, acc, bestv, bestt, bestr);
     }
     // no row of any tile below +inf: centroid 0, as the scan would give
 #pragma unroll

@@ -238,6 +238,16 @@ __global__ void __launch_bounds__(kTNT) tiled_fp32_krn(TiledArgs a) {
       if (active) {
         for (int t = 0; t < cnt; ++t) {
           const float *At = Cl + t * TF;
+          // rows of this lane: m = 8 (r / 4) + 4 half + (r % 4), ascending;
+          // the chain starts from -hn (the tile stores it), so -acc = hn - c.x
+          const float *Ht = At + 2 * KS * 32;
+          f32x16 h;
+#pragma unroll
+          for (int gg = 0; gg < 4; ++gg) {
+            const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + 8 * gg + 4 * half);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) h[4 * gg + e] = h4[e];
+          }
           f32x16 acc[S];
 #pragma unroll
           for (int k4 = 0; k4 < KS / 4; ++k4) {
@@ -248,8 +258,7 @@ __global__ void __launch_bounds__(kTNT) tiled_fp32_krn(TiledArgs a) {
 #pragma unroll
               for (int q = 0; q < S; ++q) {
                 if (k4 == 0 && j == 0) {
-                  const f32x16 z = {};
-                  acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[0][q], z,
+                  acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[0][q], h,
                                                                 0, 0, 0);
                 } else {
                   acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(
@@ -258,17 +267,14 @@ __global__ void __launch_bounds__(kTNT) tiled_fp32_krn(TiledArgs a) {
               }
             }
           }
-          // rows of this lane: m = 8 (r / 4) + 4 half + (r % 4), ascending
-          const float *Ht = At + 2 * KS * 32;
 #pragma unroll
           for (int g = 0; g < 4; ++g) {
             const int m0 = 8 * g + 4 * half;
-            const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + m0);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
 #pragma unroll
               for (int q = 0; q < S; ++q) {
-                const float v = h4[e] - acc[q][4 * g + e];
+                const float v = -acc[q][4 * g + e];
                 const bool upd = v < bestv[q];
                 bestv[q] = upd ? v : bestv[q];
                 besti[q] = upd ? ((c0 + t) * 32 + m0 + e) : besti[q];
@@ -292,7 +298,8 @@ __global__ void __launch_bounds__(kTNT) tiled_fp32_krn(TiledArgs a) {
 
 // Centroid tiles in the FP32 kernel's LDS layout, so that a chunk is one
 // contiguous copy: tile t holds C[2 ks + half][t*32 + col] at
-// ((ks / 4) * 2 + half) * 128 + col * 4 + ks % 4, then hn (+inf padded).
+// ((ks / 4) * 2 + half) * 128 + col * 4 + ks % 4, then -hn (-inf padded),
+// the C operand the MFMA chain starts from.
 template <int KS>
 __global__ void __launch_bounds__(256)
     tiled_fp32_prep_krn(int d, int k, int n_ct, const float *__restrict__ C,
@@ -311,7 +318,7 @@ __global__ void __launch_bounds__(256)
         (f < d && c < k) ? C[(int64_t)f * k + c] : 0.f;
   } else {
     const int c = t * 32 + (e - 2 * KS * 32);
-    tiles[(int64_t)t * TF + e] = (c < k) ? hn[c] : __builtin_huge_valf();
+    tiles[(int64_t)t * TF + e] = (c < k) ? -hn[c] : -__builtin_huge_valf();
   }
 }
 
@@ -447,6 +454,14 @@ __global__ void __launch_bounds__(kTNT) cand_rescore_krn(CandArgs a) {
       xr[ks] = (valid && f < d) ? a.Xr[(int64_t)smp * d + f] : 0.f;
     }
     const float *At = a.tiles + (int64_t)t * TF;
+    const float *Ht = At + 2 * KS * 32;
+    f32x16 h;
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+      const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + 8 * gg + 4 * half);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) h[4 * gg + e] = h4[e];
+    }
     f32x16 acc;
 #pragma unroll
     for (int k4 = 0; k4 < KS / 4; ++k4) {
@@ -455,24 +470,21 @@ __global__ void __launch_bounds__(kTNT) cand_rescore_krn(CandArgs a) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         if (k4 == 0 && j == 0) {
-          const f32x16 z = {};
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[0], z, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[0], h, 0, 0, 0);
         } else {
           acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[4 * k4 + j], acc, 0,
                                                      0, 0);
         }
       }
     }
-    const float *Ht = At + 2 * KS * 32;
     float bestv = __builtin_huge_valf();
     int besti = 0;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       const int m0 = 8 * g + 4 * half;
-      const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + m0);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const float v = h4[e] - acc[4 * g + e];
+        const float v = -acc[4 * g + e];
         const bool upd = v < bestv;
         bestv = upd ? v : bestv;
         besti = upd ? (t * 32 + m0 + e) : besti;
