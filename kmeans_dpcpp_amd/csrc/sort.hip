@@ -1,5 +1,6 @@
-// Stable ascending sort of floating-point values (NaN last) and the
-// top-k threshold built on it.
+// Stable ascending sort of floating-point values (NaN last) and the top-k
+// threshold (the element the sort puts at n - topk, found by radix select
+// over the same order keys, without sorting).
 //
 // Replaces iterative_merge_sort (src/iterative_merge_sort.hpp:204-368) and
 // compute_threshold_kernel (src/util_kernels.hpp:163-206).  The reference sorts
@@ -179,10 +180,6 @@ __global__ void __launch_bounds__(kSortBlock)
     out[o0 + t] = stage[t];
 }
 
-template <typename T>
-__global__ void copy_elem_krn(const T *src, int64_t idx, T *dst) {
-  dst[0] = src[idx];
-}
 
 }  // namespace
 
@@ -213,6 +210,167 @@ void launch_merge_sort(int64_t n, const T *in, T *out, hipStream_t s) {
   }
 }
 
+// ---- order statistic by radix select ---------------------------------------
+// The element at position p of the stable ascending sort, without sorting:
+// an MSB-first radix select over the sort's order keys (8-bit digits) finds
+// its key K and its rank r among the elements whose key is K; the r-th such
+// element in input order is the one the stable sort puts at p (elements with
+// equal keys differ only as -0 / +0 or NaN payloads, and keep input order).
+constexpr int kSelBlock = 256;
+constexpr int kSelChunk = 16 * kSelBlock;   // elements per block in the scans
+
+template <typename T>
+struct SelState {
+  typename Key<T>::U prefix, mask;
+  int64_t rank;                       // rank of the target among the prefix
+  unsigned long long hist[256];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kSelBlock) sel_init_krn(SelState<T> *st, int64_t p) {
+  st->hist[threadIdx.x] = 0ull;
+  if (threadIdx.x == 0) {
+    st->prefix = 0;
+    st->mask = 0;
+    st->rank = p;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kSelBlock)
+    sel_hist_krn(int64_t n, const T *__restrict__ data, SelState<T> *st, int shift) {
+  using U = typename Key<T>::U;
+  __shared__ unsigned h[256];
+  h[threadIdx.x] = 0u;
+  __syncthreads();
+  const U prefix = st->prefix, mask = st->mask;
+  for (int64_t i = blockIdx.x * (int64_t)kSelBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kSelBlock) {
+    const U k = Key<T>::of(data[i]);
+    if ((k & mask) == prefix) atomicAdd(&h[(unsigned)(k >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&st->hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+// one block: the digit whose bucket holds the target; clears the histogram
+template <typename T>
+__global__ void __launch_bounds__(kSelBlock) sel_pick_krn(SelState<T> *st, int shift) {
+  using U = typename Key<T>::U;
+  __shared__ unsigned long long h[256];
+  h[threadIdx.x] = st->hist[threadIdx.x];
+  st->hist[threadIdx.x] = 0ull;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t r = st->rank;
+    int b = 0;
+    for (; b < 255; ++b) {
+      if (r < (int64_t)h[b]) break;
+      r -= (int64_t)h[b];
+    }
+    st->prefix |= (U)b << shift;
+    st->mask |= (U)255 << shift;
+    st->rank = r;
+  }
+}
+
+// per chunk: elements whose key is the selected key
+template <typename T>
+__global__ void __launch_bounds__(kSelBlock)
+    sel_count_krn(int64_t n, const T *__restrict__ data, const SelState<T> *st,
+                  unsigned *counts) {
+  using U = typename Key<T>::U;
+  __shared__ unsigned c;
+  if (threadIdx.x == 0) c = 0u;
+  __syncthreads();
+  const U K = st->prefix;
+  const int64_t base = (int64_t)blockIdx.x * kSelChunk;
+  unsigned mine = 0u;
+  for (int j = threadIdx.x; j < kSelChunk; j += kSelBlock) {
+    const int64_t i = base + j;
+    mine += (i < n && Key<T>::of(data[i]) == K) ? 1u : 0u;
+  }
+  if (mine) atomicAdd(&c, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+// one block: the chunk holding the rank-th match, then the match itself
+template <typename T>
+__global__ void __launch_bounds__(kSelBlock)
+    sel_finish_krn(int64_t n, int64_t n_chunks, const T *__restrict__ data,
+                   const SelState<T> *st, const unsigned *counts, T *out) {
+  using U = typename Key<T>::U;
+  __shared__ int64_t s_chunk, s_r;
+  if (threadIdx.x == 0) {
+    int64_t r = st->rank, b = 0;
+    for (; b < n_chunks - 1; ++b) {
+      if (r < (int64_t)counts[b]) break;
+      r -= (int64_t)counts[b];
+    }
+    s_chunk = b;
+    s_r = r;
+  }
+  __syncthreads();
+  const U K = st->prefix;
+  const int64_t base = s_chunk * kSelChunk;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (threadIdx.x < kWave) {          // one wave walks the chunk in order
+    int64_t r = s_r;
+    for (int j0 = 0; j0 < kSelChunk; j0 += kWave) {
+      const int64_t i = base + j0 + lane;
+      const bool m = i < n && Key<T>::of(data[i]) == K;
+      const unsigned long long bal = __ballot(m);
+      const int cnt = __popcll(bal);
+      if (r < cnt) {
+        // the (r+1)-th set bit of bal
+        unsigned long long b = bal;
+        for (int q = 0; q < r; ++q) b &= b - 1ull;
+        const int pos = __ffsll((long long)b) - 1;
+        if (lane == pos) out[0] = data[i];
+        break;
+      }
+      r -= cnt;
+    }
+  }
+}
+
+template <typename T>
+void launch_order_statistic(int64_t n, const T *data, int64_t p, T *out,
+                            hipStream_t s) {
+  using U = typename Key<T>::U;
+  constexpr int kBits = 8 * (int)sizeof(U);
+  TempBuffer st(sizeof(SelState<T>), s);
+  hipLaunchKernelGGL(sel_init_krn<T>, dim3(1), dim3(kSelBlock), 0, s,
+                     st.as<SelState<T>>(), p);
+  KMD_LAUNCH_CHECK();
+  const int64_t want = ceil_div<int64_t>(n, kSelBlock * 16);
+  const unsigned grid = (unsigned)std::min<int64_t>(
+      std::max<int64_t>(want, 1), 8 * (int64_t)device_cu_count());
+  for (int shift = kBits - 8; shift >= 0; shift -= 8) {
+    hipLaunchKernelGGL(sel_hist_krn<T>, dim3(grid), dim3(kSelBlock), 0, s, n,
+                       data, st.as<SelState<T>>(), shift);
+    KMD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(sel_pick_krn<T>, dim3(1), dim3(kSelBlock), 0, s,
+                       st.as<SelState<T>>(), shift);
+    KMD_LAUNCH_CHECK();
+  }
+  const int64_t n_chunks = ceil_div<int64_t>(n, kSelChunk);
+  require(n_chunks <= INT32_MAX, "too many elements");
+  TempBuffer counts(sizeof(unsigned) * (size_t)n_chunks, s);
+  hipLaunchKernelGGL(sel_count_krn<T>, dim3((unsigned)n_chunks), dim3(kSelBlock),
+                     0, s, n, data, st.as<const SelState<T>>(),
+                     counts.as<unsigned>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(sel_finish_krn<T>, dim3(1), dim3(kSelBlock), 0, s, n,
+                     n_chunks, data, st.as<const SelState<T>>(),
+                     counts.as<const unsigned>(), out);
+  KMD_LAUNCH_CHECK();
+}
+
+// sorted_asc[n - topk] (src/util_kernels.hpp:163-206): the maximum for
+// topk == 1, else the order statistic by radix select (the same element the
+// stable sort would put there).
 template <typename T>
 void launch_threshold(int64_t n, const T *data, int64_t topk, T *threshold,
                       hipStream_t s) {
@@ -221,11 +379,7 @@ void launch_threshold(int64_t n, const T *data, int64_t topk, T *threshold,
     launch_max<T>(n, data, threshold, s);
     return;
   }
-  TempBuffer sorted(sizeof(T) * n, s);
-  launch_merge_sort<T>(n, data, sorted.as<T>(), s);
-  hipLaunchKernelGGL(copy_elem_krn<T>, dim3(1), dim3(1), 0, s,
-                     sorted.as<const T>(), n - topk, threshold);
-  KMD_LAUNCH_CHECK();
+  launch_order_statistic<T>(n, data, n - topk, threshold, s);
 }
 
 template void launch_merge_sort<float>(int64_t, const float *, float *,

@@ -119,6 +119,36 @@ def test_compute_threshold(kd, golden, torch_dev, dt):
 
 
 @pytest.mark.parametrize("dt", FDT)
+@pytest.mark.parametrize("n", [2, 3, 2049, 100_003, 1_000_000])
+def test_compute_threshold_is_the_stable_sort_element(kd, torch_dev, dt, n):
+    """topk >= 2: the element at n - topk of the stable ascending sort (NaN
+    last, -0 and +0 equal and kept in input order), bit for bit -- found by
+    radix select, checked against np.sort(kind="stable") at positions in the
+    middle, at both ends of the run of zeros and inside the NaNs."""
+    rng = np.random.default_rng(n + 7)
+    x = (rng.integers(-50, 50, size=n) / 4).astype(dt)          # heavy ties
+    z = rng.choice(n, min(n, 40), replace=False)
+    x[z] = np.where(rng.random(len(z)) < 0.5, -0.0, 0.0).astype(dt)
+    if n > 10:
+        x[rng.choice(n, 3, replace=False)] = np.array([np.inf, -np.inf, np.nan], dtype=dt)
+        nan2 = np.array([np.nan], dtype=dt)
+        nan2.view(np.uint32 if dt == np.float32 else np.uint64)[0] ^= 1  # another payload
+        x[rng.choice(n, 1)] = nan2[0]
+    srt = np.sort(x, kind="stable")
+    ut = np.uint32 if dt == np.float32 else np.uint64
+    zpos = np.nonzero(srt == 0)[0]
+    pos = {0, n - 2, n // 2, int(zpos[0]), int(zpos[-1]), int(zpos[len(zpos) // 2])}
+    if n > 10:
+        pos |= {n - 2, n - 3}                                      # the NaNs / +inf
+    xd = to_dev(x)
+    for p in sorted(q for q in pos if 0 <= q <= n - 2):
+        thr = to_dev(np.full((), -127, dtype=dt))
+        kd.compute_threshold(xd, n - p, thr, sycl_queue=q())[0].wait()
+        got = to_np(thr).reshape(()).view(ut)
+        assert got == srt[p].view(ut), (n, p, srt[p])
+
+
+@pytest.mark.parametrize("dt", FDT)
 @pytest.mark.parametrize("n", [1, 5, 2047, 2048, 2049, 4096 + 3, 100_000, 1_000_003])
 def test_iterative_merge_sort_bit_exact(kd, torch_dev, dt, n):
     rng = np.random.default_rng(n)
