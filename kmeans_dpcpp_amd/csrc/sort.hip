@@ -244,11 +244,25 @@ __global__ void __launch_bounds__(kSelBlock)
   h[threadIdx.x] = 0u;
   __syncthreads();
   const U prefix = st->prefix, mask = st->mask;
+  // runs of one digit are counted in a register and added once (the first
+  // digit is nearly constant over distances, and per-element LDS atomics on
+  // one bin would serialise)
+  unsigned cur = 0u, run = 0u;
   for (int64_t i = blockIdx.x * (int64_t)kSelBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kSelBlock) {
     const U k = Key<T>::of(data[i]);
-    if ((k & mask) == prefix) atomicAdd(&h[(unsigned)(k >> shift) & 255u], 1u);
+    if ((k & mask) == prefix) {
+      const unsigned dg = (unsigned)(k >> shift) & 255u;
+      if (dg == cur) {
+        ++run;
+      } else {
+        if (run) atomicAdd(&h[cur], run);
+        cur = dg;
+        run = 1u;
+      }
+    }
   }
+  if (run) atomicAdd(&h[cur], run);
   __syncthreads();
   if (h[threadIdx.x]) atomicAdd(&st->hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
@@ -302,14 +316,35 @@ __global__ void __launch_bounds__(kSelBlock)
                    const SelState<T> *st, const unsigned *counts, T *out) {
   using U = typename Key<T>::U;
   __shared__ int64_t s_chunk, s_r;
+  __shared__ unsigned long long part[kSelBlock];
+  // counts of each thread's contiguous range of chunks, scanned in order
+  const int64_t per = ceil_div<int64_t>(n_chunks, kSelBlock);
+  const int64_t lo = std::min<int64_t>((int64_t)threadIdx.x * per, n_chunks);
+  const int64_t hi = std::min<int64_t>(lo + per, n_chunks);
+  unsigned long long mine = 0ull;
+  for (int64_t b = lo; b < hi; ++b) mine += counts[b];
+  part[threadIdx.x] = mine;
+  if (threadIdx.x == 0) s_chunk = -1;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    int64_t r = st->rank, b = 0;
-    for (; b < n_chunks - 1; ++b) {
+    int64_t r = st->rank;
+    int t = 0;
+    for (; t < kSelBlock - 1; ++t) {
+      if (r < (int64_t)part[t]) break;
+      r -= (int64_t)part[t];
+    }
+    s_r = r;
+    s_chunk = t;                      // the thread whose range holds it
+  }
+  __syncthreads();
+  if ((int)threadIdx.x == s_chunk) {
+    int64_t r = s_r, b = lo;
+    for (; b < hi - 1; ++b) {
       if (r < (int64_t)counts[b]) break;
       r -= (int64_t)counts[b];
     }
-    s_chunk = b;
     s_r = r;
+    s_chunk = b;
   }
   __syncthreads();
   const U K = st->prefix;
