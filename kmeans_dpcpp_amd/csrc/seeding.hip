@@ -424,15 +424,33 @@ __global__ void __launch_bounds__(kSNT, 4) kpp_score_krn(KppArgs a, int c) {
   __syncthreads();            // csq complete, cl free
   if (blockIdx.x == 0 && threadIdx.x <= TT) a.csq[threadIdx.x] = csq[threadIdx.x];
 
-  double wv[kSPT], dn[kSPT];
+  double wv[kSPT], dv[kSPT], dn[kSPT];
+  const bool wvec = vec && ((reinterpret_cast<uintptr_t>(w) & 15u) == 0);
+  if (wvec) {                 // vector loads of w and D^2 (aligned like X)
+    const Quad<T> wq = load_quad(w + i0, i0, n, true);
+#pragma unroll
+    for (int j = 0; j < kSPT; ++j) wv[j] = (double)wq.v[j];
+    if (dold) {
+      const double2 d01 = *reinterpret_cast<const double2 *>(dold + i0);
+      const double2 d23 = *reinterpret_cast<const double2 *>(dold + i0 + 2);
+      dv[0] = d01.x; dv[1] = d01.y; dv[2] = d23.x; dv[3] = d23.y;
+    } else {
+#pragma unroll
+      for (int j = 0; j < kSPT; ++j) dv[j] = __builtin_inf();
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSPT; ++j) {
+      const bool in = i0 + j < n;
+      wv[j] = in ? (double)w[i0 + j] : 0.0;
+      dv[j] = (in && dold) ? dold[i0 + j] : __builtin_inf();
+    }
+  }
   const double c0sq = csq[0];
 #pragma unroll
   for (int j = 0; j < kSPT; ++j) {
-    const bool in = i0 + j < n;
-    wv[j] = in ? (double)w[i0 + j] : 0.0;
-    const double dv = (in && dold) ? dold[i0 + j] : __builtin_inf();
     const double d0 = kpp_dist(xsq[j], c0sq, acc0[j]);
-    dn[j] = (dv <= d0) ? dv : d0;
+    dn[j] = (dv[j] <= d0) ? dv[j] : d0;
   }
   if (vec) {
     *reinterpret_cast<double2 *>(dnew + i0) = make_double2(dn[0], dn[1]);
