@@ -38,7 +38,7 @@ namespace kmd {
 namespace {
 
 constexpr int kNT = 256;             // threads per block (data preparation)
-constexpr int kFC = 256;             // candidate features staged per LDS chunk
+constexpr int kFC = 64;              // candidate features staged per LDS chunk
 constexpr int kMaxTrials = 16;
 
 __device__ inline int lane_of() { return threadIdx.x & (kWave - 1); }
@@ -131,6 +131,7 @@ constexpr int kSeg = kSPT * kSNT;         // samples per segment (2048)
 constexpr int kSub = 64;                  // samples per sub-segment
 constexpr int kSubPerSeg = kSeg / kSub;   // 32
 constexpr int kUF = 4;                    // feature rows loaded together
+constexpr int kStage = 4;                 // rows per LDS DMA stage (float)
 
 static_assert(kSubPerSeg <= kWave, "one lane per sub-segment of a segment");
 static_assert(kSub == kWave, "one lane per sample of a sub-segment");
@@ -336,10 +337,15 @@ __device__ inline double kpp_dist(double xsq, double csq, double xc) {
 // c == 0): D^2 <- min(D^2, dist0) is written to d2[c & 1].  Rows 1..T are
 // the candidates of center c: m_t = min(D^2, dist_t) gives the segment and
 // sub-segment sums of w m_t (for c == 0, of w D^2 itself).
-template <typename T, int TT>
-__global__ void __launch_bounds__(kSNT, 4) kpp_score_krn(KppArgs a, int c) {
+template <typename T, int TT, bool DMA>
+__global__ void __launch_bounds__(kSNT, 4) kpp_score_krn(KppArgs a, int c, int64_t blk0) {
   __shared__ double cl[(TT + 1) * kFC];
   __shared__ double csq[TT + 1];
+  // DMA (float, whole aligned segments only): rows staged in LDS,
+  // [2 buffers][kStage rows][kSeg samples]; each wave fills and reads only
+  // its own 256-sample slice
+  __shared__ float xs[DMA ? 2 * kStage * kSeg : 1];
+  const int64_t blk = blk0 + blockIdx.x;      // segment of this block
   const T *X = static_cast<const T *>(a.X_t);
   const T *w = static_cast<const T *>(a.w);
   const int64_t n = a.n, d = a.d;
@@ -348,7 +354,7 @@ __global__ void __launch_bounds__(kSNT, 4) kpp_score_krn(KppArgs a, int c) {
   const int64_t idx0 = (c == 0) ? a.cand[0] : a.indices[c - 1];
   const double *dold = (c == 0) ? nullptr : a.d2[(c - 1) & 1];
   double *dnew = a.d2[c & 1];
-  const int64_t i0 = (int64_t)blockIdx.x * kSeg + kSPT * threadIdx.x;
+  const int64_t i0 = blk * kSeg + kSPT * threadIdx.x;
   const bool vec = (i0 + kSPT - 1 < n) && (n % kSPT == 0) &&
                    ((reinterpret_cast<uintptr_t>(X) & 15u) == 0);
   double acc0[kSPT], acc[kSPT][TT], xsq[kSPT];
@@ -397,6 +403,57 @@ __global__ void __launch_bounds__(kSNT, 4) kpp_score_krn(KppArgs a, int c) {
       }
       csq[threadIdx.x] = q;
     }
+    if constexpr (DMA && sizeof(T) == 4) {
+      {
+        // rows by LDS DMA (global_load_lds_dwordx4): the next stage of
+        // kStage rows is in flight while the current one is scored, with no
+        // registers held for it; a wave reads back only what it loaded
+        const int wv = threadIdx.x / kWave, ln = lane_of();
+        // lanes past n (ragged last segment) copy the row's last 4 samples
+        // instead: in bounds and 16-byte aligned (n % 4 == 0); their
+        // samples are masked in the epilogue
+        const int64_t s_lane = blk * kSeg + wv * (kWave * kSPT) + ln * kSPT;
+        const float *gbase = reinterpret_cast<const float *>(X) + f0 * n +
+                             (s_lane + kSPT <= n ? s_lane : n - kSPT);
+        float *lbase = xs + wv * (kWave * kSPT);
+        const int ns = (fcn + kStage - 1) / kStage;
+        auto issue = [&](int st) {
+          float *buf = lbase + (st & 1) * (kStage * kSeg);
+#pragma unroll
+          for (int r = 0; r < kStage; ++r) {
+            // always kStage copies (a short last stage repeats its last row),
+            // so the vmcnt the reads wait for is a constant
+            const int f = (st * kStage + r < fcn) ? st * kStage + r : fcn - 1;
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(gbase + (int64_t)f * n),
+                (__attribute__((address_space(3))) void *)(buf + r * kSeg), 16, 0, 0);
+          }
+        };
+        issue(0);
+        for (int st = 0; st < ns; ++st) {
+          // earlier reads of the buffer the next stage overwrites are done
+          asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+          if (st + 1 < ns) {
+            issue(st + 1);
+            asm volatile("s_waitcnt vmcnt(%0)" ::"n"(kStage) : "memory");
+          } else {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+          }
+          const float *buf = xs + (st & 1) * (kStage * kSeg) + threadIdx.x * kSPT;
+#pragma unroll
+          for (int r = 0; r < kStage; ++r) {
+            const int f = st * kStage + r;
+            if (f < fcn) {
+              const float4 v = *reinterpret_cast<const float4 *>(buf + r * kSeg);
+              Quad<T> q;
+              q.v[0] = v.x; q.v[1] = v.y; q.v[2] = v.z; q.v[3] = v.w;
+              step(q, f);
+            }
+          }
+        }
+        continue;
+      }
+    }
     // groups of UF feature rows, the next group's loads issued before the
     // current group is scored (2 UF rows in flight per lane)
     const T *base = X + f0 * n + i0;
@@ -422,7 +479,7 @@ __global__ void __launch_bounds__(kSNT, 4) kpp_score_krn(KppArgs a, int c) {
     for (int f = fg; f < fcn; ++f) step(load_quad(base + (int64_t)f * n, i0, n, vec), f);
   }
   __syncthreads();            // csq complete, cl free
-  if (blockIdx.x == 0 && threadIdx.x <= TT) a.csq[threadIdx.x] = csq[threadIdx.x];
+  if (blk == 0 && threadIdx.x <= TT) a.csq[threadIdx.x] = csq[threadIdx.x];
 
   double wv[kSPT], dv[kSPT], dn[kSPT];
   const bool wvec = vec && ((reinterpret_cast<uintptr_t>(w) & 15u) == 0);
@@ -490,7 +547,7 @@ __global__ void __launch_bounds__(kSNT, 4) kpp_score_krn(KppArgs a, int c) {
   if (threadIdx.x < ns) {
     double tot = 0.0;
     for (int i = 0; i < kSNT / kWave; ++i) tot += wsum[i * TT + threadIdx.x];
-    a.part[(int64_t)threadIdx.x * a.nb + blockIdx.x] = tot;
+    a.part[(int64_t)threadIdx.x * a.nb + blk] = tot;
   }
 }
 
@@ -566,8 +623,15 @@ __global__ void __launch_bounds__(kPNT) kpp_pick_krn(KppArgs a, int c) {
 
 template <typename T, int TT>
 void launch_kpp_step(const KppArgs &a, int c, hipStream_t s) {
-  hipLaunchKernelGGL((kpp_score_krn<T, TT>), dim3((unsigned)a.nb), dim3(kSNT), 0,
-                     s, a, c);
+  // 16-byte aligned float rows (n % 4 == 0) stream through LDS DMA; other
+  // data through registers
+  if (std::is_same<T, float>::value && a.n % kSPT == 0 &&
+      (reinterpret_cast<uintptr_t>(a.X_t) & 15u) == 0)
+    hipLaunchKernelGGL((kpp_score_krn<T, TT, true>), dim3((unsigned)a.nb),
+                       dim3(kSNT), 0, s, a, c, (int64_t)0);
+  else
+    hipLaunchKernelGGL((kpp_score_krn<T, TT, false>), dim3((unsigned)a.nb),
+                       dim3(kSNT), 0, s, a, c, (int64_t)0);
   KMD_LAUNCH_CHECK();
   hipLaunchKernelGGL(kpp_pick_krn<T>, dim3(1), dim3(kPNT), 0, s, a, c);
   KMD_LAUNCH_CHECK();
