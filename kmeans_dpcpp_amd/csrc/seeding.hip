@@ -132,6 +132,7 @@ constexpr int kSub = 64;                  // samples per sub-segment
 constexpr int kSubPerSeg = kSeg / kSub;   // 32
 constexpr int kUF = 4;                    // feature rows loaded together
 constexpr int kStage = 4;                 // rows per LDS DMA stage (float)
+constexpr int kPickD = 1024;              // center features the pick keeps in LDS
 
 static_assert(kSubPerSeg <= kWave, "one lane per sub-segment of a segment");
 static_assert(kSub == kWave, "one lane per sample of a sub-segment");
@@ -207,8 +208,17 @@ __device__ void pick_search(int64_t n, int64_t nb, int64_t nsub, const double *s
   const int64_t per = ceil_div<int64_t>(nb, kPNT);
   const int64_t lo = ((int64_t)tid * per < nb) ? (int64_t)tid * per : nb;
   const int64_t hi = (lo + per < nb) ? lo + per : nb;
+  constexpr int kKeep = 8;              // range values kept in registers
+  double keep[kKeep];
   double tsum = 0.0;
-  for (int64_t s = lo; s < hi; ++s) tsum += seg[s];
+#pragma unroll
+  for (int q = 0; q < kKeep; ++q) {
+    keep[q] = (lo + q < hi) ? seg[lo + q] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < kKeep; ++q)
+    if (lo + q < hi) tsum += keep[q];
+  for (int64_t s = lo + kKeep; s < hi; ++s) tsum += seg[s];
   const double y = wave_incl_scan(tsum, lane);
   double yp = __shfl_up(y, 1, kWave);
   if (lane == 0) yp = 0.0;
@@ -234,7 +244,11 @@ __device__ void pick_search(int64_t n, int64_t nb, int64_t nsub, const double *s
   }
   double run = L.wtot[wv] + yp;                     // prefix before this range
   for (int64_t s = lo; s < hi; ++s) {
-    const double nxt = run + seg[s];
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < kKeep; ++q)
+      if (s == lo + q) v = keep[q];
+    const double nxt = run + ((s < lo + kKeep) ? v : seg[s]);
 #pragma unroll
     for (int j = 0; j < kMaxTrials; ++j) {
       const bool hit = right ? (nxt > rj[j]) : (nxt >= rj[j]);
@@ -563,6 +577,7 @@ __global__ void __launch_bounds__(kPNT) kpp_pick_krn(KppArgs a, int c) {
   __shared__ double s_csq;
   __shared__ int64_t s_idx;
   __shared__ int s_ts;
+  __shared__ double s_c[kPickD];               // center c (d <= kPickD)
   const T *X = static_cast<const T *>(a.X_t);
   const T *w = static_cast<const T *>(a.w);
   const int64_t n = a.n, d = a.d, k = a.k;
@@ -594,7 +609,12 @@ __global__ void __launch_bounds__(kPNT) kpp_pick_krn(KppArgs a, int c) {
   __syncthreads();
   const int64_t idx = s_idx;
   T *C = static_cast<T *>(a.centers_t);
-  for (int64_t f = threadIdx.x; f < d; f += kPNT) C[f * k + c] = X[f * n + idx];
+  for (int64_t f = threadIdx.x; f < d; f += kPNT) {
+    const T v = X[f * n + idx];
+    C[f * k + c] = v;
+    if (f < kPickD) s_c[f] = (double)v;
+  }
+  __syncthreads();
   if (c + 1 < k) {
     const int ts = s_ts;
     if (threadIdx.x < a.trials)
@@ -608,8 +628,9 @@ __global__ void __launch_bounds__(kPNT) kpp_pick_krn(KppArgs a, int c) {
 #pragma unroll 8
         for (int64_t f = 0; f < d; ++f) {
           const double x = (double)X[f * n + i];
+          const double cf = (f < kPickD) ? s_c[f] : (double)X[f * n + idx];
           xsq = fma(x, x, xsq);
-          xc = fma(x, (double)X[f * n + idx], xc);
+          xc = fma(x, cf, xc);
         }
         const double dd = kpp_dist(xsq, ccsq, xc);
         m = (m <= dd) ? m : dd;
