@@ -120,6 +120,19 @@ def test_tolerance_is_scaled_by_feature_variance(kd):
     assert est._tol == pytest.approx(_tolerance(X, 1e-3), rel=1e-12)
 
 
+@pytest.mark.parametrize("n,d,k", [(7, 1, 1), (5, 3, 5), (300, 1, 4), (64, 2, 64)])
+def test_edge_shapes_match_sklearn(kd, n, d, k):
+    """k = 1, d = 1, n = k: same labels and centres as scikit-learn from the
+    same k-means++ seeds."""
+    rng = np.random.default_rng(n * 10 + d)
+    X = rng.normal(size=(n, d)) * 3 + rng.integers(0, 4, size=(n, 1)) * 10
+    ours = kd.KMeans(k, random_state=4, max_iter=50).fit(X)
+    ref = skc.KMeans(k, random_state=4, max_iter=50, algorithm="lloyd").fit(X)
+    assert_array_equal(ours.labels_, ref.labels_)
+    assert _rel(ours.cluster_centers_, ref.cluster_centers_) < 1e-9
+    assert ours.inertia_ == pytest.approx(ref.inertia_, rel=1e-9, abs=1e-9)
+
+
 def test_bad_parameters(kd):
     X = np.zeros((5, 2))
     with pytest.raises(ValueError):

@@ -85,6 +85,8 @@ KPP_CASES = [
     (50, 2, 50, 3, np.float64, False),        # k == n
     (3001, 8, 33, 1, np.float32, False),      # one trial
     (1025, 5, 7, 2, np.float64, True),        # two segments, ragged tail
+    (6000, 200, 24, None, np.float32, False), # LDS-staged rows, 4 feature chunks
+    (4100, 64, 12, 8, np.float32, True),      # LDS-staged rows, ragged last segment
 ]
 
 
