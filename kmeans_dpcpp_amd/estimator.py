@@ -81,7 +81,8 @@ def kmeans_plusplus(X, n_clusters, *, sample_weight=None, x_squared_norms=None,
     (sklearn.cluster.kmeans_plusplus).  Returns (centers (k, d), indices (k,)).
     The same random_state gives scikit-learn's draws; the indices equal its
     indices except where a draw lands within rounding of a cumulative-sum
-    step (FP64 here)."""
+    step (FP64 here).  x_squared_norms is accepted for compatibility and not
+    used (the kernels compute the norms they need)."""
     torch = _torch()
     device = device or torch.device("cuda", torch.cuda.current_device())
     Xd, host = _as_device(X, device)
@@ -240,7 +241,10 @@ class KMeans:
         self._tol = tol
         rs = _check_random_state(self.random_state)
         k = self.n_clusters
-        inits = [self._init_centers(X_t, Xd, w, mean, rs, q) for _ in range(n_init)]
+        try:
+            inits = [self._init_centers(X_t, Xd, w, mean, rs, q) for _ in range(n_init)]
+        finally:
+            self._X_user = None    # only a callable init needs the caller's X
         if self.verbose:
             print("Initialization complete")
         runs = self._run_lloyd(X_t, w, inits, tol, q)
