@@ -52,8 +52,11 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned long long u64;
 
 // 32-sample MFMA column blocks per wave; each A operand read from LDS feeds
-// S MFMAs.
-constexpr int kS = 2;
+// S MFMAs.  One block per wave keeps a wave at ~110 VGPRs, so 16 waves (4
+// per SIMD) share a block's LDS copies and hide each other's latencies;
+// measured faster than S = 2 at 12 waves in both the MFMA-bound (k = 256)
+// and the HBM-bound (k = 8) regime.
+constexpr int kS = 1;
 constexpr int kMaxD = 64;
 constexpr size_t kLdsPerCu = 160 * 1024;
 
@@ -539,9 +542,9 @@ KernFn pick_nt(int64_t d) {
 #undef KMD_KS_CASE
 }
 
-// Threads per block of the MFMA kernels: 8 waves share one copy of the
+// Threads per block of the MFMA kernels: 16 waves share one copy of the
 // centroids and of the per-block sums in LDS.
-constexpr int kThreads = 768;
+constexpr int kThreads = 1024;
 
 // A resolved launch: kernel variant, block size, dynamic LDS and the
 // persistent grid (occupancy-limited, never more blocks than work).
