@@ -34,9 +34,11 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned long long u64;
 
-// threads per block: 12 waves (3 per SIMD) where the registers allow it
-constexpr int nt_of(int D) { return D <= 32 ? 768 : 512; }
-constexpr int kS = 2;                  // 32-sample column blocks per wave
+// threads per block: 16 waves (4 per SIMD) where the registers allow it
+// (d <= 32 at one 32-sample column block per wave: <= 128 VGPRs; measured
+// faster than 12 waves x 2 column blocks at n = 10 M, d = 32, k = 256)
+constexpr int nt_of(int D) { return D <= 32 ? 1024 : 512; }
+constexpr int kS = 1;                  // 32-sample column blocks per wave
 constexpr size_t kLdsPerCu = 160 * 1024;
 constexpr int kScPad = 66;             // scale slots in LDS (d + 1 <= 65)
 
@@ -191,9 +193,13 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
       if (FULL || f < d) {
         const float *row = X + (int64_t)f * n + s0;
         if (whole) {
-          const float2 v = *reinterpret_cast<const float2 *>(row);
-          dst[i][0] = v.x;
-          dst[i][1] = v.y;
+          if constexpr (kS == 2) {
+            const float2 v = *reinterpret_cast<const float2 *>(row);
+            dst[i][0] = v.x;
+            dst[i][1] = v.y;
+          } else {
+            dst[i][0] = row[0];
+          }
         } else {
 #pragma unroll
           for (int q = 0; q < kS; ++q) dst[i][q] = (s0 + q < n) ? row[q] : 0.f;
