@@ -1,19 +1,22 @@
 // Many independent Lloyd fits, one launch per step of an iteration for all
 // of them, the iteration captured once in a hipGraph and replayed.
 //
-// Work lists.  The host flattens the fits into (fit, 512-sample tile) items
+// Work lists.  The host flattens the fits into (fit, 4096-sample tile) items
 // for the assignment (one list per feature bucket d <= 16 / 32 / 64, one
 // persistent launch each) and (fit, 4096-sample chunk) items for the update.
 // A block takes items grid-stride, loads the item's fit's centroid tiles and
 // half norms into LDS and labels the tile's samples with the FP32 MFMA
 // arithmetic of the single-fit kernels (v_mfma_f32_32x32x2f32 over ascending
-// k-steps, argmin of hn - dot, strict <, lowest index on ties).  Items of a
-// fit whose mask is off are skipped, so converged fits cost nothing.
+// k-steps, argmin of hn - dot, strict <, lowest index on ties), one 32-sample
+// column block per wave.  Items of a fit whose mask is off are skipped, so
+// converged fits cost nothing.
 //
 // Update.  Per fit, exactly as the tiled single-fit path: labels counted,
 // cluster totals scanned, samples scattered into cluster order and summed
 // as int64 fixed point (the fit's own scales) from a sample-major copy of
-// its X.  The sums therefore equal the single-fit driver's bit for bit.
+// its X, one block per segment item (up to 256 members of one cluster; the
+// items of all fits are flattened by a map kernel).  Integer sums do not
+// depend on the order, so they equal the single-fit driver's bit for bit.
 //
 // Finalize (one block per fit): sums -> centroids and sizes, ascending
 // empty-cluster list, relocation of empty clusters to the farthest samples
@@ -40,10 +43,10 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned long long u64;
 
-constexpr int kNT = 512;                 // assignment block (8 waves)
-constexpr int kW = kNT / kWave;
-constexpr int kSubTiles = 8;              // 512-sample blocks per assignment item
-constexpr int kTileSamples = kSubTiles * kW * 64;  // samples per assignment item
+constexpr int kTileSamples = 4096;        // samples per assignment item
+// assignment block: one 32-sample column block per wave; 16 waves, or 8 for
+// d > 32, whose 32 x-registers per lane do not fit the 128 VGPRs of 16 waves
+constexpr int assign_threads(int KS) { return KS <= 16 ? 1024 : 512; }
                                           // (the fit's centroids are loaded into
                                           // LDS once per item)
 constexpr int kChunk = 4096;              // samples per update item
@@ -81,6 +84,18 @@ struct BFit {
   unsigned *maxbits;  // (d + 1) bit patterns of max |x_f| and max |w|
 };
 
+// A pointer read from a BFit, marked as global memory: loads through it are
+// global_load (counted by vmcnt only), not flat loads, whose lgkmcnt share
+// makes every LDS wait also wait for them.
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T *gptr(const T *p) {
+  return (const __attribute__((address_space(1))) T *)p;
+}
+
+// The current (c = 0) or other centroid buffer.  A select, not F.C[c]: a
+// dynamic index into a struct held in registers puts the struct in scratch.
+__device__ inline float *cbuf(const BFit &F, int c) { return c ? F.C[1] : F.C[0]; }
+
 struct AItem {  // assignment tile
   int fit;
   int first;
@@ -111,7 +126,7 @@ __global__ void __launch_bounds__(kFB)
   const int f = blockIdx.x;
   if (!all && !fits[f].st[0].active) return;
   const BFit F = fits[f];
-  const float *C = F.C[fits[f].st[0].cur];
+  const float *C = cbuf(F, fits[f].st[0].cur);
   for (int c = threadIdx.x; c < F.k; c += kFB) {
     float acc = 0.f;
     for (int j = 0; j < F.d; ++j) {
@@ -129,10 +144,12 @@ constexpr int tile_floats() {
 }
 
 template <int KS>
-__global__ void __launch_bounds__(kNT)
+__global__ void __launch_bounds__(assign_threads(KS))
     bt_assign_krn(const BFit *__restrict__ fits, const AItem *__restrict__ items, int n_items,
                   int all) {
   constexpr int TF = tile_floats<KS>();
+  constexpr int kNT = assign_threads(KS), kW = kNT / kWave;
+  constexpr int kSubTiles = kTileSamples / (kW * 32);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   float *Cl = reinterpret_cast<float *>(lds_raw);  // [kMaxK / 32][TF]
   const int tid = threadIdx.x;
@@ -142,89 +159,89 @@ __global__ void __launch_bounds__(kNT)
     const AItem item = items[it];
     if (!all && !fits[item.fit].st[0].active) continue;  // block-uniform
     const BFit F = fits[item.fit];
-    const float *C = F.C[fits[item.fit].st[0].cur];
-    const float *hn = F.hn;
+    const auto *C = gptr(cbuf(F, fits[item.fit].st[0].cur));
+    const auto *hn = gptr(F.hn);
     const int d = F.d, k = F.k, n_ct = (k + 31) / 32;
     __syncthreads();
-    for (int i = tid; i < n_ct * TF; i += kNT) {
-      const int t = i / TF, e = i - t * TF;
-      if (e < 2 * KS * 32) {
-        const int f = e >> 5, cc = e & 31;
-        const int c = t * 32 + cc;
-        const int ks = f >> 1, hf = f & 1;
-        Cl[t * TF + ((ks >> 2) * 2 + hf) * 128 + cc * 4 + (ks & 3)] =
-            (f < d && c < k) ? C[(int64_t)f * k + c] : 0.f;
-      } else {
-        const int c = t * 32 + (e - 2 * KS * 32);
-        Cl[t * TF + e] = (c < k) ? -hn[c] : -__builtin_huge_valf();  // -hn
+    // stage the centroid tiles: four loads in flight per thread, then the stores
+    constexpr int kSU = 4;
+    for (int i0 = tid; i0 < n_ct * TF; i0 += kSU * kNT) {
+      float v[kSU];
+      int at[kSU];
+#pragma unroll
+      for (int u = 0; u < kSU; ++u) {
+        const int i = i0 + u * kNT;
+        const int t = i / TF, e = i - t * TF;
+        at[u] = -1;
+        v[u] = 0.f;
+        if (i < n_ct * TF) {
+          if (e < 2 * KS * 32) {
+            const int f = e >> 5, cc = e & 31;
+            const int c = t * 32 + cc;
+            const int ks = f >> 1, hf = f & 1;
+            at[u] = t * TF + ((ks >> 2) * 2 + hf) * 128 + cc * 4 + (ks & 3);
+            v[u] = (f < d && c < k) ? C[(int64_t)f * k + c] : 0.f;
+          } else {
+            const int c = t * 32 + (e - 2 * KS * 32);
+            at[u] = t * TF + e;
+            v[u] = (c < k) ? -hn[c] : -__builtin_huge_valf();  // -hn
+          }
+        }
       }
+#pragma unroll
+      for (int u = 0; u < kSU; ++u)
+        if (at[u] >= 0) Cl[at[u]] = v[u];
     }
     __syncthreads();
     const int64_t n = F.n;
     for (int sub = 0; sub < kSubTiles; ++sub) {
-    const int64_t p0 = (int64_t)item.first + (int64_t)sub * kW * 64 + wave * 64 + 2 * col;
-    if ((int64_t)item.first + (int64_t)sub * kW * 64 >= n) break;  // block-uniform
-    float xr[KS][2];
+      const int64_t s0 = (int64_t)item.first + (int64_t)sub * kW * 32;
+      if (s0 >= n) break;  // block-uniform
+      const int64_t p = s0 + wave * 32 + col;
+      float xr[KS];
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      const int f = 2 * ks + half;
-#pragma unroll
-      for (int q = 0; q < 2; ++q)
-        xr[ks][q] = (f < d && p0 + q < n) ? F.X[(int64_t)f * n + p0 + q] : 0.f;
-    }
-    float bestv[2] = {__builtin_huge_valf(), __builtin_huge_valf()};
-    int besti[2] = {0, 0};
-    for (int t = 0; t < n_ct; ++t) {
-      const float *At = Cl + t * TF;
-      // the chain starts from the tile's -hn, so -acc = hn - c.x
-      const float *Ht = At + 2 * KS * 32;
-      f32x16 h;
-#pragma unroll
-      for (int gg = 0; gg < 4; ++gg) {
-        const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + 8 * gg + 4 * half);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) h[4 * gg + e] = h4[e];
+      for (int ks = 0; ks < KS; ++ks) {
+        const int f = 2 * ks + half;
+        xr[ks] = (f < d && p < n) ? gptr(F.X)[(int64_t)f * n + p] : 0.f;
       }
-      f32x16 acc[2];
+      float bestv = __builtin_huge_valf();
+      int besti = 0;
+      for (int t = 0; t < n_ct; ++t) {
+        const float *At = Cl + t * TF;
+        // the chain starts from the tile's -hn, so -acc = hn - c.x
+        const float *Ht = At + 2 * KS * 32;
+        f32x16 acc;
 #pragma unroll
-      for (int k4 = 0; k4 < KS / 4; ++k4) {
-        const f32x4 av =
-            *reinterpret_cast<const f32x4 *>(At + (k4 * 2 + half) * 128 + col * 4);
+        for (int gg = 0; gg < 4; ++gg) {
+          const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + 8 * gg + 4 * half);
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+          for (int e = 0; e < 4; ++e) acc[4 * gg + e] = h4[e];
+        }
 #pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            if (k4 == 0 && j == 0) {
-              acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[0][q], h, 0,
-                                                            0, 0);
-            } else {
-              acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(
-                  av[j], xr[4 * k4 + j][q], acc[q], 0, 0, 0);
-            }
+        for (int k4 = 0; k4 < KS / 4; ++k4) {
+          const f32x4 av =
+              *reinterpret_cast<const f32x4 *>(At + (k4 * 2 + half) * 128 + col * 4);
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[4 * k4 + j], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int m0 = 8 * g + 4 * half;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const float v = -acc[4 * g + e];
+            const bool upd = v < bestv;
+            bestv = upd ? v : bestv;
+            besti = upd ? (t * 32 + m0 + e) : besti;
           }
+        }
       }
-#pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const int m0 = 8 * g + 4 * half;
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-#pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            const float v = -acc[q][4 * g + e];
-            const bool upd = v < bestv[q];
-            bestv[q] = upd ? v : bestv[q];
-            besti[q] = upd ? (t * 32 + m0 + e) : besti[q];
-          }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const float ov = __shfl_xor(bestv[q], 32, kWave);
-      const int oi = __shfl_xor(besti[q], 32, kWave);
-      const bool take = (ov < bestv[q]) || (ov == bestv[q] && oi < besti[q]);
-      besti[q] = take ? oi : besti[q];
-      if (half == 0 && p0 + q < n) F.labels[p0 + q] = besti[q];
-    }
+      const float ov = __shfl_xor(bestv, 32, kWave);
+      const int oi = __shfl_xor(besti, 32, kWave);
+      const bool take = (ov < bestv) || (ov == bestv && oi < besti);
+      besti = take ? oi : besti;
+      if (half == 0 && p < n) F.labels[p] = besti;
     }
   }
 }
@@ -360,53 +377,91 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// one block = up to kMembers members of one cluster of one fit
-__global__ void __launch_bounds__(128)
-    bt_segsum_krn(int n_fits, const BFit *__restrict__ fits, const int *__restrict__ total) {
+// Segment items of every active fit, flattened: map[seg_base + i] = {fit,
+// cluster, first member (position in perm), member count} for item i of the
+// fit (items of cluster c are citems[c] .. citems[c + 1] - 1, each up to
+// kMembers consecutive members of the cluster).
+__global__ void __launch_bounds__(kFB)
+    bt_seg_map_krn(const BFit *__restrict__ fits, int4 *__restrict__ map) {
+  __shared__ int ci[kMaxK + 1];
+  const int f = blockIdx.x;
+  if (!fits[f].st[0].active) return;
+  const BFit F = fits[f];
+  const int k = F.k, base = fits[f].st[0].seg_base, items = fits[f].st[0].seg_items;
+  for (int c = threadIdx.x; c <= k; c += kFB) ci[c] = F.citems[c];
+  __syncthreads();
+  for (int i = threadIdx.x; i < items; i += kFB) {
+    int a = 0, b = k - 1;  // the last cluster whose first item is <= i
+    while (a < b) {
+      const int mid = (a + b + 1) >> 1;
+      if (ci[mid] <= i)
+        a = mid;
+      else
+        b = mid - 1;
+    }
+    const int m0 = F.start[a] + (i - ci[a]) * kMembers;
+    map[base + i] = make_int4(f, a, m0, min(kMembers, F.start[a + 1] - m0));
+  }
+}
+
+// One block = one segment item (up to kMembers members of one cluster of one
+// fit).  The d + 1 columns (features, then the weight) are split over
+// floor(256 / (d + 1)) row groups of d + 1 threads; each thread sums its
+// column over every R-th member, the groups are combined in LDS and one
+// atomic per column adds the item's sum.  Every term is an integer, so the
+// sums do not depend on the order.
+constexpr int kSegThreads = 256;
+static_assert(kSegThreads / (kMaxD + 1) >= 1, "a row group must fit a block");
+__global__ void __launch_bounds__(kSegThreads)
+    bt_segsum_krn(const BFit *__restrict__ fits, const int4 *__restrict__ map,
+                  const int *__restrict__ total) {
   __shared__ int sidx[kMembers];
   __shared__ float sw[kMembers];
+  __shared__ long long red[kSegThreads];
   const int g = blockIdx.x;
   if (g >= total[0]) return;
-  int lo = 0, hi = n_fits - 1;  // the last fit whose first item is <= g
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (fits[mid].st[0].seg_base <= g)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  const BFit F = fits[lo];
-  const int local = g - fits[lo].st[0].seg_base;
-  const int *ci = F.citems;
-  int a = 0, b = F.k - 1;  // the last cluster whose first item is <= local
-  while (a < b) {
-    const int mid = (a + b + 1) >> 1;
-    if (ci[mid] <= local)
-      a = mid;
-    else
-      b = mid - 1;
-  }
-  const int c = a;
-  const int m0 = F.start[c] + (local - ci[c]) * kMembers;
-  const int cnt = min(kMembers, F.start[c + 1] - m0);
-  for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+  const int4 e = map[g];
+  const BFit F = fits[e.x];
+  const int c = e.y, m0 = e.z, cnt = e.w;
+  const int t = threadIdx.x;
+  for (int j = t; j < cnt; j += kSegThreads) {
     const int s = F.perm[m0 + j];
     sidx[j] = s;
     sw[j] = F.w[s];
   }
   __syncthreads();
-  const float *Xr = F.xr;
-  const int d = F.d;
-  for (int f = threadIdx.x; f <= d; f += blockDim.x) {
+  const auto *Xr = gptr(F.xr);
+  const int d = F.d, D1 = d + 1, R = kSegThreads / D1;
+  long long acc = 0;
+  if (t < R * D1) {
+    const int f = t % D1, r = t / D1;
     const float sc = F.scale[f];
-    long long acc = 0;
     if (f < d) {
-      for (int j = 0; j < cnt; ++j)
+      // eight row gathers in flight per thread before the first is used
+      constexpr int kU = 8;
+      int j = r;
+      for (; j + (kU - 1) * R < cnt; j += kU * R) {
+        int ix[kU];
+        float v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) ix[u] = sidx[j + u * R];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) v[u] = Xr[(int64_t)ix[u] * d + f];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) acc += __float2int_rn((v[u] * sw[j + u * R]) * sc);
+      }
+      for (; j < cnt; j += R)
         acc += __float2int_rn((Xr[(int64_t)sidx[j] * d + f] * sw[j]) * sc);
     } else {
-      for (int j = 0; j < cnt; ++j) acc += __float2int_rn(sw[j] * sc);
+      for (int j = r; j < cnt; j += R) acc += __float2int_rn(sw[j] * sc);
     }
-    if (acc != 0) atomicAdd(&F.sums[(int64_t)f * F.k + c], (u64)acc);
+  }
+  red[t] = acc;
+  __syncthreads();
+  if (t < D1) {
+    long long sum = 0;
+    for (int r = 0; r < R; ++r) sum += red[r * D1 + t];
+    if (sum != 0) atomicAdd(&F.sums[(int64_t)t * F.k + c], (u64)sum);
   }
 }
 
@@ -443,7 +498,7 @@ __global__ void __launch_bounds__(256)
   const CItem item = items[blockIdx.x];
   if (!fits[item.fit].st[0].active || fits[item.fit].st[0].n_empty == 0) return;
   const BFit F = fits[item.fit];
-  const float *C = F.C[fits[item.fit].st[0].cur];
+  const float *C = cbuf(F, fits[item.fit].st[0].cur);
   for (int j = threadIdx.x; j < item.count; j += 256) {
     const int64_t s = item.first + j;
     const int c = F.labels[s];
@@ -473,8 +528,8 @@ __global__ void __launch_bounds__(kFB)
   const int d = F.d, k = F.k, tid = threadIdx.x;
   const int64_t n = F.n;
   const int cur = fits[f].st[0].cur;
-  const float *Cc = F.C[cur];
-  float *Cn = F.C[cur ^ 1];
+  const float *Cc = cbuf(F, cur);
+  float *Cn = cbuf(F, cur ^ 1);
   float *sizes = F.sizes;
   const int *list = F.list;
   const float *sc = F.scale;
@@ -610,7 +665,7 @@ __global__ void __launch_bounds__(256)
     bt_inertia_krn(const BFit *__restrict__ fits, const CItem *__restrict__ items) {
   const CItem item = items[blockIdx.x];
   const BFit F = fits[item.fit];
-  const float *C = F.C[fits[item.fit].st[0].cur];
+  const float *C = cbuf(F, fits[item.fit].st[0].cur);
   for (int j = threadIdx.x; j < item.count; j += 256) {
     const int64_t s = item.first + j;
     const int c = F.labels[s];
@@ -752,7 +807,7 @@ int assign_grid() {
     KMD_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)assign_lds(KS)));
     int per_cu = 0;
-    KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNT,
+    KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, assign_threads(KS),
                                                                 assign_lds(KS)));
     grid = device_cu_count() * std::max(1, per_cu);
   }
@@ -764,7 +819,7 @@ void launch_assign_bucket(const BFit *fits, const AItem *items, int n_items,
                           int all, hipStream_t s) {
   if (n_items <= 0) return;
   const int grid = std::min(assign_grid<KS>(), n_items);
-  hipLaunchKernelGGL(bt_assign_krn<KS>, dim3(grid), dim3(kNT), assign_lds(KS), s,
+  hipLaunchKernelGGL(bt_assign_krn<KS>, dim3(grid), dim3(assign_threads(KS)), assign_lds(KS), s,
                      fits, items, n_items, all);
   KMD_LAUNCH_CHECK();
 }
@@ -858,7 +913,7 @@ struct BatchEngine {
   std::vector<BFit> host_fits_;
   std::vector<AItem> ai_[3];
   std::vector<CItem> ci_;
-  std::unique_ptr<TempBuffer> fits_d_, ai_d_[3], ci_d_, seg_total_, st_d_;
+  std::unique_ptr<TempBuffer> fits_d_, ai_d_[3], ci_d_, seg_total_, seg_map_, st_d_;
   // every fit's state in one copy: gathered on the device, read back once
   void read_states(const BFit *dfits, int nf, std::vector<BState> &out);
   int64_t seg_bound_ = 1, next_id_ = 0, launches_ = 0;
@@ -895,6 +950,7 @@ BatchEngine::~BatchEngine() {
   for (auto &b : ai_d_) b.reset();
   ci_d_.reset();
   seg_total_.reset();
+  seg_map_.reset();
   st_d_.reset();
   (void)hipStreamSynchronize(s_);
   (void)hipEventDestroy(ev_);
@@ -1085,6 +1141,7 @@ void BatchEngine::rebuild() {
   for (int b = 0; b < 3; ++b) upload(ai_d_[b], ai_[b].data(), sizeof(AItem) * ai_[b].size());
   upload(ci_d_, ci_.data(), sizeof(CItem) * ci_.size());
   if (!seg_total_) seg_total_ = std::make_unique<TempBuffer>(256, s_);
+  seg_map_ = std::make_unique<TempBuffer>(sizeof(int4) * (size_t)seg_bound_, s_);
   KMD_HIP_CHECK(hipStreamSynchronize(s_));  // host vectors may change after this
   dirty_ = false;
 }
@@ -1111,8 +1168,10 @@ void BatchEngine::iteration(hipStream_t q) {
   hipLaunchKernelGGL(bt_scatter_krn, dim3(nci), dim3(256), 0, q, dF,
                      ci_d_->as<CItem>());
   KMD_LAUNCH_CHECK();
-  hipLaunchKernelGGL(bt_segsum_krn, dim3((unsigned)seg_bound_), dim3(128), 0, q, nf, dF,
-                     seg_total_->as<int>());
+  hipLaunchKernelGGL(bt_seg_map_krn, dim3(nf), dim3(kFB), 0, q, dF, seg_map_->as<int4>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bt_segsum_krn, dim3((unsigned)seg_bound_), dim3(kSegThreads), 0, q, dF,
+                     seg_map_->as<const int4>(), seg_total_->as<int>());
   KMD_LAUNCH_CHECK();
   hipLaunchKernelGGL(bt_empty_krn, dim3(nf), dim3(kFB), 0, q, dF);
   KMD_LAUNCH_CHECK();
