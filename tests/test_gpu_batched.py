@@ -110,3 +110,25 @@ def test_batched_mixes_in_unserved_shapes(kd, torch_dev):
         assert g[2] == r[2]
         assert_array_equal(g[0], r[0])
         assert_array_equal(g[1].view(np.uint32), r[1].view(np.uint32))
+
+
+@pytest.mark.parametrize("replays", [1, 3])
+def test_batched_edge_shapes_equal_single_fits(kd, torch_dev, replays):
+    """Shapes at the segment sums' and assignment's edges: d = 1 (128 row
+    groups of two columns), d = 63 (four groups of 64), d = 64 (three groups
+    of 65), k = 1 (one cluster split over many 256-member items), n below one
+    32-sample column block, n = k, and a ragged last 4096-sample tile."""
+    rng = np.random.default_rng(21)
+    shapes = [(40000, 1, 2), (3000, 63, 5), (8191, 64, 17), (20000, 3, 1),
+              (20, 3, 4), (64, 2, 64), (4097, 16, 256), (1, 1, 1)]
+    probs = []
+    for i, (n, d, k) in enumerate(shapes):
+        Xt, w, init = make_fit(rng, n, d, k, seed=900 + i)
+        probs.append((Xt, w, init, 1e-4, 30))
+    refs = [single(kd, Xt, w, init, tol, mi) for Xt, w, init, tol, mi in probs]
+    got, _ = batched(kd, probs, replays)
+    for i, (g, r) in enumerate(zip(got, refs)):
+        assert g[2] == r[2], (shapes[i], g[2], r[2])
+        assert_array_equal(g[0], r[0])
+        assert_array_equal(g[1].view(np.uint32), r[1].view(np.uint32))
+        assert g[3] == r[3], (shapes[i], g[3], r[3])
