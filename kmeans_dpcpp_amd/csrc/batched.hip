@@ -1,9 +1,10 @@
 // Many independent Lloyd fits, one launch per step of an iteration for all
 // of them, the iteration captured once in a hipGraph and replayed.
 //
-// Work lists.  The host flattens the fits into (fit, 4096-sample tile) items
+// Work lists.  The host flattens the fits into (fit, run of samples) items
 // for the assignment (one list per feature bucket d <= 16 / 32 / 64, one
-// persistent launch each) and (fit, 4096-sample chunk) items for the update.
+// persistent launch each; heaviest items first) and (fit, 4096-sample chunk)
+// items for the update.
 // A block takes items grid-stride, loads the item's fit's centroid tiles and
 // half norms into LDS and labels the tile's samples with the FP32 MFMA
 // arithmetic of the single-fit kernels (v_mfma_f32_32x32x2f32 over ascending
@@ -43,7 +44,12 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned long long u64;
 
-constexpr int kTileSamples = 4096;        // samples per assignment item
+// Assignment items are sized by their MFMA work: about kItemMacs
+// multiply-adds (samples x centroids padded to 32-tiles x features padded to
+// the kernel's feature bucket), at least kItemMin and at most kItemMax
+// samples, in multiples of kItemMin (one sub-tile of either block shape).
+constexpr int64_t kItemMacs = int64_t(2048) * 256 * 64;
+constexpr int kItemMin = 512, kItemMax = 2048;
 // assignment block: one 32-sample column block per wave; 16 waves, or 8 for
 // d > 32, whose 32 x-registers per lane do not fit the 128 VGPRs of 16 waves
 constexpr int assign_threads(int KS) { return KS <= 16 ? 1024 : 512; }
@@ -96,9 +102,9 @@ __device__ inline const __attribute__((address_space(1))) T *gptr(const T *p) {
 // dynamic index into a struct held in registers puts the struct in scratch.
 __device__ inline float *cbuf(const BFit &F, int c) { return c ? F.C[1] : F.C[0]; }
 
-struct AItem {  // assignment tile
+struct AItem {  // assignment tile: samples first .. first + count - 1 of a fit
   int fit;
-  int first;
+  int first, count;
 };
 struct CItem {  // update chunk
   int fit;
@@ -149,99 +155,114 @@ __global__ void __launch_bounds__(assign_threads(KS))
                   int all) {
   constexpr int TF = tile_floats<KS>();
   constexpr int kNT = assign_threads(KS), kW = kNT / kWave;
-  constexpr int kSubTiles = kTileSamples / (kW * 32);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   float *Cl = reinterpret_cast<float *>(lds_raw);  // [kMaxK / 32][TF]
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1), wave = tid / kWave;
   const int col = lane & 31, half = lane >> 5;
-  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    const AItem item = items[it];
-    if (!all && !fits[item.fit].st[0].active) continue;  // block-uniform
-    const BFit F = fits[item.fit];
-    const auto *C = gptr(cbuf(F, fits[item.fit].st[0].cur));
-    const auto *hn = gptr(F.hn);
-    const int d = F.d, k = F.k, n_ct = (k + 31) / 32;
-    __syncthreads();
-    // stage the centroid tiles: four loads in flight per thread, then the stores
-    constexpr int kSU = 4;
-    for (int i0 = tid; i0 < n_ct * TF; i0 += kSU * kNT) {
-      float v[kSU];
-      int at[kSU];
-#pragma unroll
-      for (int u = 0; u < kSU; ++u) {
-        const int i = i0 + u * kNT;
-        const int t = i / TF, e = i - t * TF;
-        at[u] = -1;
-        v[u] = 0.f;
-        if (i < n_ct * TF) {
-          if (e < 2 * KS * 32) {
-            const int f = e >> 5, cc = e & 31;
-            const int c = t * 32 + cc;
-            const int ks = f >> 1, hf = f & 1;
-            at[u] = t * TF + ((ks >> 2) * 2 + hf) * 128 + cc * 4 + (ks & 3);
-            v[u] = (f < d && c < k) ? C[(int64_t)f * k + c] : 0.f;
-          } else {
-            const int c = t * 32 + (e - 2 * KS * 32);
-            at[u] = t * TF + e;
-            v[u] = (c < k) ? -hn[c] : -__builtin_huge_valf();  // -hn
-          }
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kSU; ++u)
-        if (at[u] >= 0) Cl[at[u]] = v[u];
+  // The block's items are blockIdx.x, + gridDim.x, ... (grid-stride); the
+  // mask flags of the next 64 of them are read at once by wave 0 and the
+  // active ones listed in LDS, so converged fits cost one flag read per item
+  // in parallel instead of a round trip each.
+  __shared__ int act[kWave];
+  __shared__ int n_act;
+  for (int base = blockIdx.x; base < n_items; base += kWave * gridDim.x) {
+    __syncthreads();  // the previous batch's list is no longer read
+    if (wave == 0) {
+      const int it = base + lane * gridDim.x;
+      const bool a = it < n_items && (all || fits[items[it].fit].st[0].active);
+      const unsigned long long m = __ballot(a);
+      if (a) act[__popcll(m & ((1ull << lane) - 1))] = it;
+      if (lane == 0) n_act = __popcll(m);
     }
     __syncthreads();
-    const int64_t n = F.n;
-    for (int sub = 0; sub < kSubTiles; ++sub) {
-      const int64_t s0 = (int64_t)item.first + (int64_t)sub * kW * 32;
-      if (s0 >= n) break;  // block-uniform
-      const int64_t p = s0 + wave * 32 + col;
-      float xr[KS];
+    const int na = n_act;
+    for (int j = 0; j < na; ++j) {
+      const AItem item = items[act[j]];
+      const BFit F = fits[item.fit];
+      const auto *C = gptr(cbuf(F, fits[item.fit].st[0].cur));
+      const auto *hn = gptr(F.hn);
+      const int d = F.d, k = F.k, n_ct = (k + 31) / 32;
+      __syncthreads();
+      // stage the centroid tiles: four loads in flight per thread, then the stores
+      constexpr int kSU = 4;
+      for (int i0 = tid; i0 < n_ct * TF; i0 += kSU * kNT) {
+        float v[kSU];
+        int at[kSU];
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        const int f = 2 * ks + half;
-        xr[ks] = (f < d && p < n) ? gptr(F.X)[(int64_t)f * n + p] : 0.f;
-      }
-      float bestv = __builtin_huge_valf();
-      int besti = 0;
-      for (int t = 0; t < n_ct; ++t) {
-        const float *At = Cl + t * TF;
-        // the chain starts from the tile's -hn, so -acc = hn - c.x
-        const float *Ht = At + 2 * KS * 32;
-        f32x16 acc;
-#pragma unroll
-        for (int gg = 0; gg < 4; ++gg) {
-          const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + 8 * gg + 4 * half);
-#pragma unroll
-          for (int e = 0; e < 4; ++e) acc[4 * gg + e] = h4[e];
-        }
-#pragma unroll
-        for (int k4 = 0; k4 < KS / 4; ++k4) {
-          const f32x4 av =
-              *reinterpret_cast<const f32x4 *>(At + (k4 * 2 + half) * 128 + col * 4);
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[4 * k4 + j], acc, 0, 0, 0);
-        }
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const int m0 = 8 * g + 4 * half;
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const float v = -acc[4 * g + e];
-            const bool upd = v < bestv;
-            bestv = upd ? v : bestv;
-            besti = upd ? (t * 32 + m0 + e) : besti;
+        for (int u = 0; u < kSU; ++u) {
+          const int i = i0 + u * kNT;
+          const int t = i / TF, e = i - t * TF;
+          at[u] = -1;
+          v[u] = 0.f;
+          if (i < n_ct * TF) {
+            if (e < 2 * KS * 32) {
+              const int f = e >> 5, cc = e & 31;
+              const int c = t * 32 + cc;
+              const int ks = f >> 1, hf = f & 1;
+              at[u] = t * TF + ((ks >> 2) * 2 + hf) * 128 + cc * 4 + (ks & 3);
+              v[u] = (f < d && c < k) ? C[(int64_t)f * k + c] : 0.f;
+            } else {
+              const int c = t * 32 + (e - 2 * KS * 32);
+              at[u] = t * TF + e;
+              v[u] = (c < k) ? -hn[c] : -__builtin_huge_valf();  // -hn
+            }
           }
         }
+#pragma unroll
+        for (int u = 0; u < kSU; ++u)
+          if (at[u] >= 0) Cl[at[u]] = v[u];
       }
-      const float ov = __shfl_xor(bestv, 32, kWave);
-      const int oi = __shfl_xor(besti, 32, kWave);
-      const bool take = (ov < bestv) || (ov == bestv && oi < besti);
-      besti = take ? oi : besti;
-      if (half == 0 && p < n) F.labels[p] = besti;
+      __syncthreads();
+      const int64_t n = F.n;
+      const int64_t end = min((int64_t)item.first + item.count, n);
+      for (int64_t s0 = item.first; s0 < end; s0 += kW * 32) {  // block-uniform
+        const int64_t p = s0 + wave * 32 + col;
+        float xr[KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          const int f = 2 * ks + half;
+          xr[ks] = (f < d && p < end) ? gptr(F.X)[(int64_t)f * n + p] : 0.f;
+        }
+        float bestv = __builtin_huge_valf();
+        int besti = 0;
+        for (int t = 0; t < n_ct; ++t) {
+          const float *At = Cl + t * TF;
+          // the chain starts from the tile's -hn, so -acc = hn - c.x
+          const float *Ht = At + 2 * KS * 32;
+          f32x16 acc;
+#pragma unroll
+          for (int gg = 0; gg < 4; ++gg) {
+            const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + 8 * gg + 4 * half);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[4 * gg + e] = h4[e];
+          }
+#pragma unroll
+          for (int k4 = 0; k4 < KS / 4; ++k4) {
+            const f32x4 av =
+                *reinterpret_cast<const f32x4 *>(At + (k4 * 2 + half) * 128 + col * 4);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+              acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[j], xr[4 * k4 + j], acc, 0, 0, 0);
+          }
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            const int m0 = 8 * g + 4 * half;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const float v = -acc[4 * g + e];
+              const bool upd = v < bestv;
+              bestv = upd ? v : bestv;
+              besti = upd ? (t * 32 + m0 + e) : besti;
+            }
+          }
+        }
+        const float ov = __shfl_xor(bestv, 32, kWave);
+        const int oi = __shfl_xor(besti, 32, kWave);
+        const bool take = (ov < bestv) || (ov == bestv && oi < besti);
+        besti = take ? oi : besti;
+        if (half == 0 && p < end) F.labels[p] = besti;
+      }
     }
   }
 }
@@ -420,10 +441,10 @@ __global__ void __launch_bounds__(kSegThreads)
   __shared__ long long red[kSegThreads];
   const int g = blockIdx.x;
   if (g >= total[0]) return;
+  const int t = threadIdx.x;
   const int4 e = map[g];
   const BFit F = fits[e.x];
   const int c = e.y, m0 = e.z, cnt = e.w;
-  const int t = threadIdx.x;
   for (int j = t; j < cnt; j += kSegThreads) {
     const int s = F.perm[m0 + j];
     sidx[j] = s;
@@ -815,13 +836,49 @@ int assign_grid() {
 }
 
 template <int KS>
+int assign_launch_grid(int n_items) {
+  return std::max(1, std::min(assign_grid<KS>(), n_items));
+}
+
+template <int KS>
 void launch_assign_bucket(const BFit *fits, const AItem *items, int n_items,
                           int all, hipStream_t s) {
   if (n_items <= 0) return;
-  const int grid = std::min(assign_grid<KS>(), n_items);
+  const int grid = assign_launch_grid<KS>(n_items);
   hipLaunchKernelGGL(bt_assign_krn<KS>, dim3(grid), dim3(assign_threads(KS)), assign_lds(KS), s,
                      fits, items, n_items, all);
   KMD_LAUNCH_CHECK();
+}
+
+inline int assign_bucket(int d) { return ks_bucket(d) == 8 ? 0 : (ks_bucket(d) == 16 ? 1 : 2); }
+
+// Append fit f's assignment items to its bucket's list.
+void append_assign_items(const BFit &F, int f, std::vector<AItem> *ai) {
+  const int64_t macs = (int64_t)((F.k + 31) / 32 * 32) * (2 * ks_bucket(F.d));
+  int64_t len = kItemMacs / macs;
+  len = std::max<int64_t>(kItemMin, std::min<int64_t>(kItemMax, len)) / kItemMin * kItemMin;
+  for (int64_t p = 0; p < F.n; p += len)
+    ai[assign_bucket(F.d)].push_back(
+        AItem{f, (int)p, (int)std::min<int64_t>(len, F.n - p)});
+}
+
+// Order one bucket's items for the grid-stride loop of `grid` blocks:
+// heaviest first (work = samples x padded centroids; ties by fit, first),
+// then each round of `grid` items reversed every other round, so every
+// block's total work comes out about equal.
+void order_assign_items(const std::vector<BFit> &fits, int grid, std::vector<AItem> &v) {
+  auto work = [&](const AItem &a) { return (int64_t)a.count * ((fits[a.fit].k + 31) / 32); };
+  std::stable_sort(v.begin(), v.end(),
+                   [&](const AItem &a, const AItem &b) { return work(a) > work(b); });
+  if (grid <= 1) return;
+  for (size_t r0 = (size_t)grid; r0 < v.size(); r0 += 2 * (size_t)grid)
+    std::reverse(v.begin() + r0, v.begin() + std::min(v.size(), r0 + grid));
+}
+
+void order_all_assign_items(const std::vector<BFit> &fits, std::vector<AItem> *ai) {
+  order_assign_items(fits, assign_launch_grid<8>((int)ai[0].size()), ai[0]);
+  order_assign_items(fits, assign_launch_grid<16>((int)ai[1].size()), ai[1]);
+  order_assign_items(fits, assign_launch_grid<32>((int)ai[2].size()), ai[2]);
 }
 
 __global__ void bt_set_state_krn(BState *st, int active) {
@@ -1125,13 +1182,12 @@ void BatchEngine::rebuild() {
   for (size_t f = 0; f < slots_.size(); ++f) {
     const BFit &F = slots_[f]->dev;
     host_fits_.push_back(F);
-    const int bucket = ks_bucket(F.d) == 8 ? 0 : (ks_bucket(F.d) == 16 ? 1 : 2);
-    for (int64_t first = 0; first < F.n; first += kTileSamples)
-      ai_[bucket].push_back(AItem{(int)f, (int)first});
+    append_assign_items(F, (int)f, ai_);
     for (int64_t first = 0; first < F.n; first += kChunk)
       ci_.push_back(CItem{(int)f, (int)first, (int)std::min<int64_t>(kChunk, F.n - first)});
     seg_bound_ += F.n / kMembers + F.k;
   }
+  order_all_assign_items(host_fits_, ai_);
   auto upload = [&](std::unique_ptr<TempBuffer> &buf, const void *src, size_t bytes) {
     buf = std::make_unique<TempBuffer>(std::max<size_t>(bytes, 64), s_);
     if (bytes) KMD_HIP_CHECK(hipMemcpyAsync(buf->as<void>(), src, bytes,
@@ -1193,8 +1249,7 @@ void BatchEngine::epilogue(const std::vector<Slot *> &done) {
   for (size_t f = 0; f < done.size(); ++f) {
     const BFit &F = done[f]->dev;
     fits.push_back(F);
-    const int bucket = ks_bucket(F.d) == 8 ? 0 : (ks_bucket(F.d) == 16 ? 1 : 2);
-    for (int64_t p = 0; p < F.n; p += kTileSamples) ai[bucket].push_back(AItem{(int)f, (int)p});
+    append_assign_items(F, (int)f, ai);
     for (int64_t p = 0; p < F.n; p += kChunk)
       ci.push_back(CItem{(int)f, (int)p, (int)std::min<int64_t>(kChunk, F.n - p)});
     int64_t g = (F.n + 4095) / 4096;
@@ -1203,6 +1258,7 @@ void BatchEngine::epilogue(const std::vector<Slot *> &done) {
     first.push_back((int)map.size());
     for (int b = 0; b < g; ++b) map.push_back(make_int2((int)f, b));
   }
+  order_all_assign_items(fits, ai);
   const int nf = (int)fits.size();
   auto up = [&](const void *src, size_t bytes) {
     auto buf = std::make_unique<TempBuffer>(std::max<size_t>(bytes, 64), s_);
