@@ -554,15 +554,36 @@ __global__ void __launch_bounds__(kFB)
   float *sizes = F.sizes;
   const int *list = F.list;
   const float *sc = F.scale;
-  const int64_t dk = (int64_t)d * k;
+  const int dk = d * k;  // <= kMaxD * kMaxK
+  // Each loop below issues kU loads per thread before using any of them
+  // (one round trip per kU elements, not per element); the arithmetic and
+  // its order per element are unchanged.
+  constexpr int kU = 8;
   // sums -> centroid sums and sizes (the single-slot partial reduction)
-  for (int64_t i = tid; i < dk + k; i += kFB) {
-    const int64_t fe = i / k;
-    const float v = (float)((double)(long long)F.sums[i] / (double)sc[fe]);
-    if (i < dk)
-      Cn[i] = v;
-    else
-      sizes[i - dk] = v;
+  for (int i0 = tid; i0 < dk + k; i0 += kU * kFB) {
+    long long q[kU];
+    float sv[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int i = i0 + u * kFB;
+      q[u] = 0;
+      sv[u] = 1.f;
+      if (i < dk + k) {
+        q[u] = (long long)F.sums[i];
+        sv[u] = sc[i / k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int i = i0 + u * kFB;
+      if (i < dk + k) {
+        const float v = (float)((double)q[u] / (double)sv[u]);
+        if (i < dk)
+          Cn[i] = v;
+        else
+          sizes[i - dk] = v;
+      }
+    }
   }
   __syncthreads();
   const int n_empty = fits[f].st[0].n_empty;
@@ -657,14 +678,47 @@ __global__ void __launch_bounds__(kFB)
     __syncthreads();
   }
   // divide, then shifts and their sum in the single-fit order
-  for (int64_t i = tid; i < dk; i += kFB) Cn[i] = Cn[i] / sizes[i % k];
+  for (int i0 = tid; i0 < dk; i0 += kU * kFB) {
+    float a[kU], b[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int i = i0 + u * kFB;
+      a[u] = 0.f;
+      b[u] = 1.f;
+      if (i < dk) {
+        a[u] = Cn[i];
+        b[u] = sizes[i % k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int i = i0 + u * kFB;
+      if (i < dk) Cn[i] = a[u] / b[u];
+    }
+  }
   __syncthreads();
   double acc = 0.0;
   for (int c = tid; c < k; c += kFB) {
     float s2 = 0.f;
-    for (int j = 0; j < d; ++j) {
-      const float diff = Cc[(int64_t)j * k + c] - Cn[(int64_t)j * k + c];
-      s2 = fmaf(diff, diff, s2);
+    for (int j0 = 0; j0 < d; j0 += kU) {
+      float a[kU], b[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int j = j0 + u;
+        a[u] = 0.f;
+        b[u] = 0.f;
+        if (j < d) {
+          a[u] = Cc[(int64_t)j * k + c];
+          b[u] = Cn[(int64_t)j * k + c];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (j0 + u < d) {
+          const float diff = a[u] - b[u];
+          s2 = fmaf(diff, diff, s2);
+        }
+      }
     }
     acc += (double)s2;
   }
