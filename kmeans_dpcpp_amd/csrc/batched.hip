@@ -56,6 +56,8 @@ constexpr int assign_threads(int KS) { return KS <= 16 ? 1024 : 512; }
                                           // (the fit's centroids are loaded into
                                           // LDS once per item)
 constexpr int kChunk = 4096;              // samples per update item
+constexpr int kPerThread = kChunk / 256;  // of a 256-thread update block
+static_assert(kChunk % 256 == 0, "update items split evenly over 256 threads");
 constexpr int kMembers = 256;             // members per segment-sum item
 constexpr int kMaxK = 256;
 constexpr int kMaxD = 64;
@@ -133,11 +135,17 @@ __global__ void __launch_bounds__(kFB)
   if (!all && !fits[f].st[0].active) return;
   const BFit F = fits[f];
   const float *C = cbuf(F, fits[f].st[0].cur);
+  constexpr int kU = 8;  // loads in flight per thread; the fmaf order is j = 0, 1, ...
   for (int c = threadIdx.x; c < F.k; c += kFB) {
     float acc = 0.f;
-    for (int j = 0; j < F.d; ++j) {
-      const float v = C[(int64_t)j * F.k + c];
-      acc = fmaf(v, v, acc);
+    for (int j0 = 0; j0 < F.d; j0 += kU) {
+      float v[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        v[u] = (j0 + u < F.d) ? C[(int64_t)(j0 + u) * F.k + c] : 0.f;
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (j0 + u < F.d) acc = fmaf(v[u], v[u], acc);
     }
     F.hn[c] = acc / 2.f;
   }
@@ -288,9 +296,16 @@ __global__ void __launch_bounds__(256)
   if (!fits[item.fit].st[0].active) return;
   const BFit F = fits[item.fit];
   for (int c = threadIdx.x; c < F.k; c += 256) h[c] = 0;
+  int lab[kPerThread];  // all loaded before the first is used (count <= kChunk)
+#pragma unroll
+  for (int u = 0; u < kPerThread; ++u) {
+    const int j = threadIdx.x + u * 256;
+    lab[u] = (j < item.count) ? F.labels[item.first + j] : -1;
+  }
   __syncthreads();
-  for (int j = threadIdx.x; j < item.count; j += 256)
-    atomicAdd(&h[F.labels[item.first + j]], 1);
+#pragma unroll
+  for (int u = 0; u < kPerThread; ++u)
+    if (lab[u] >= 0) atomicAdd(&h[lab[u]], 1);
   __syncthreads();
   for (int c = threadIdx.x; c < F.k; c += 256)
     if (h[c] != 0) atomicAdd(&F.totals[c], h[c]);
@@ -384,18 +399,27 @@ __global__ void __launch_bounds__(256)
   if (!fits[item.fit].st[0].active) return;
   const BFit F = fits[item.fit];
   for (int c = threadIdx.x; c < F.k; c += 256) h[c] = 0;
+  // the chunk's labels, all loaded before the first is used (count <= kChunk)
+  int lab[kPerThread];
+#pragma unroll
+  for (int u = 0; u < kPerThread; ++u) {
+    const int j = threadIdx.x + u * 256;
+    lab[u] = (j < item.count) ? F.labels[item.first + j] : -1;
+  }
   __syncthreads();
-  for (int j = threadIdx.x; j < item.count; j += 256)
-    atomicAdd(&h[F.labels[item.first + j]], 1);
+#pragma unroll
+  for (int u = 0; u < kPerThread; ++u)
+    if (lab[u] >= 0) atomicAdd(&h[lab[u]], 1);
   __syncthreads();
   for (int c = threadIdx.x; c < F.k; c += 256)
     h[c] = (h[c] != 0) ? F.start[c] + atomicAdd(&F.cursor[c], h[c]) : 0;
   __syncthreads();
-  for (int j = threadIdx.x; j < item.count; j += 256) {
-    const int s = item.first + j;
-    const int pos = atomicAdd(&h[F.labels[s]], 1);
-    F.perm[pos] = s;
-  }
+#pragma unroll
+  for (int u = 0; u < kPerThread; ++u)
+    if (lab[u] >= 0) {
+      const int pos = atomicAdd(&h[lab[u]], 1);
+      F.perm[pos] = item.first + threadIdx.x + u * 256;
+    }
 }
 
 // Segment items of every active fit, flattened: map[seg_base + i] = {fit,
