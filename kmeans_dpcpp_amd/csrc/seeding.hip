@@ -582,13 +582,24 @@ __global__ void __launch_bounds__(kPNT) kpp_pick_krn(KppArgs a, int c) {
   const T *w = static_cast<const T *>(a.w);
   const int64_t n = a.n, d = a.d, k = a.k;
   const int ns = (c == 0) ? 1 : a.trials;
+  // all potentials in one pass: per thread, each trial's segment sums in
+  // ascending order; the ns loads of one segment are in flight together
+  double v[kMaxTrials];
 #pragma unroll
-  for (int t = 0; t < kMaxTrials; ++t) {       // all potentials in one pass
-    double v = 0.0;
-    if (t < ns)
-      for (int64_t b = threadIdx.x; b < a.nb; b += kPNT) v += a.part[(int64_t)t * a.nb + b];
-    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    if (lane_of() == 0) red[threadIdx.x / kWave][t] = v;
+  for (int t = 0; t < kMaxTrials; ++t) v[t] = 0.0;
+  for (int64_t b = threadIdx.x; b < a.nb; b += kPNT) {
+    double p[kMaxTrials];
+#pragma unroll
+    for (int t = 0; t < kMaxTrials; ++t) p[t] = (t < ns) ? a.part[(int64_t)t * a.nb + b] : 0.0;
+#pragma unroll
+    for (int t = 0; t < kMaxTrials; ++t)
+      if (t < ns) v[t] += p[t];
+  }
+#pragma unroll
+  for (int t = 0; t < kMaxTrials; ++t) {
+    double vt = v[t];
+    for (int o = kWave / 2; o > 0; o >>= 1) vt += __shfl_xor(vt, o, kWave);
+    if (lane_of() == 0) red[threadIdx.x / kWave][t] = vt;
   }
   __syncthreads();
   if (threadIdx.x < ns) {
@@ -625,7 +636,7 @@ __global__ void __launch_bounds__(kPNT) kpp_pick_krn(KppArgs a, int c) {
       double m = dn[i];
       if (c != 0) {                            // min(D^2, |x_i - center c|^2)
         double xsq = 0.0, xc = 0.0;
-#pragma unroll 8
+#pragma unroll 16
         for (int64_t f = 0; f < d; ++f) {
           const double x = (double)X[f * n + i];
           const double cf = (f < kPickD) ? s_c[f] : (double)X[f * n + idx];
