@@ -371,18 +371,39 @@ __global__ void __launch_bounds__(kBlock)
                        int nb, I *selected, const I *n_gt, const I *n_eq) {
   __shared__ int64_t off_gt, off_eq, tot_gt;
   __shared__ int wc[2][kBlock / kWave];
-  if (threadIdx.x == 0) {
+  __shared__ int64_t red[3][kBlock / kWave];
+  {  // this block's offsets and the total: integer sums over the blocks' counts
     int64_t a = 0, b = 0, ta = 0;
-    for (int i = 0; i < nb; ++i) {
+    for (int i = threadIdx.x; i < nb; i += kBlock) {
+      const int64_t cg = counts[2 * i], ce = counts[2 * i + 1];
       if (i < (int)blockIdx.x) {
-        a += counts[2 * i];
-        b += counts[2 * i + 1];
+        a += cg;
+        b += ce;
       }
-      ta += counts[2 * i];
+      ta += cg;
     }
-    off_gt = a;
-    off_eq = b;
-    tot_gt = ta;
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+      a += __shfl_xor(a, o, kWave);
+      b += __shfl_xor(b, o, kWave);
+      ta += __shfl_xor(ta, o, kWave);
+    }
+    if (lane_id() == 0) {
+      red[0][threadIdx.x / kWave] = a;
+      red[1][threadIdx.x / kWave] = b;
+      red[2][threadIdx.x / kWave] = ta;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t ra = 0, rb = 0, rt = 0;
+      for (int q = 0; q < kBlock / kWave; ++q) {
+        ra += red[0][q];
+        rb += red[1][q];
+        rt += red[2][q];
+      }
+      off_gt = ra;
+      off_eq = rb;
+      tot_gt = rt;
+    }
   }
   __syncthreads();
   const T t = thr[0];
@@ -432,12 +453,30 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 template <typename I>
-__global__ void select_finalize_krn(int64_t n_selected, const int64_t *counts,
-                                    int nb, I *n_gt, I *n_eq) {
+__global__ void __launch_bounds__(kBlock)
+    select_finalize_krn(int64_t n_selected, const int64_t *counts, int nb, I *n_gt,
+                        I *n_eq) {
+  __shared__ int64_t red[2][kBlock / kWave];
   int64_t a = 0, b = 0;
-  for (int i = 0; i < nb; ++i) {
+  for (int i = threadIdx.x; i < nb; i += kBlock) {
     a += counts[2 * i];
     b += counts[2 * i + 1];
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, kWave);
+    b += __shfl_xor(b, o, kWave);
+  }
+  if (lane_id() == 0) {
+    red[0][threadIdx.x / kWave] = a;
+    red[1][threadIdx.x / kWave] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  a = 0;
+  b = 0;
+  for (int q = 0; q < kBlock / kWave; ++q) {
+    a += red[0][q];
+    b += red[1][q];
   }
   int64_t eq_limit = n_selected - a;
   if (eq_limit < 0) eq_limit = 0;
@@ -664,7 +703,7 @@ void launch_select_far(int64_t n_selected, int64_t n, const T *dist,
                      n, chunk, n_selected, dist, thr,
                      counts.as<const int64_t>(), nb, selected, n_gt, n_eq);
   KMD_LAUNCH_CHECK();
-  hipLaunchKernelGGL(select_finalize_krn<I>, dim3(1), dim3(1), 0, s,
+  hipLaunchKernelGGL(select_finalize_krn<I>, dim3(1), dim3(kBlock), 0, s,
                      n_selected, counts.as<const int64_t>(), nb, n_gt, n_eq);
   KMD_LAUNCH_CHECK();
 }
