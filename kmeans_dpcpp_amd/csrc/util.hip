@@ -280,6 +280,41 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// The same with the centroids staged in LDS as k rows of d + 1 values (row
+// c = centroid c; the odd stride puts lanes with different c mod 32 on
+// different banks), for k (d + 1) values within kInertiaLds bytes: each
+// sample's d centroid reads come from LDS instead of a 64-address gather per
+// wave load.  Same arithmetic and order per sample as inertia_krn.
+constexpr size_t kInertiaLds = 64 * 1024;
+template <typename T, typename I>
+__global__ void __launch_bounds__(kBlock)
+    inertia_lds_krn(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
+                    const T *C, const I *labels, T *out) {
+  extern __shared__ __align__(16) unsigned char inertia_smem[];
+  T *Cs = reinterpret_cast<T *>(inertia_smem);
+  const int64_t ld = d + 1;
+  for (int64_t e = threadIdx.x; e < d * k; e += kBlock) {
+    const int64_t f = e / k, c = e - f * k;
+    Cs[c * ld + f] = C[e];
+  }
+  __syncthreads();
+  for (int64_t s = blockIdx.x * (int64_t)kBlock + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * kBlock) {
+    int64_t c = (int64_t)labels[s];
+    T acc = T(0);
+    if (c >= 0 && c < k) {
+      const T *row = Cs + c * ld;
+      for (int64_t f = 0; f < d; ++f) {
+        T diff = X[f * n + s] - row[f];
+        acc = fma(diff, diff, acc);
+      }
+    } else {
+      acc = std::numeric_limits<T>::quiet_NaN();
+    }
+    out[s] = (w != nullptr) ? acc * w[s] : acc;
+  }
+}
+
 // ---- deterministic two-stage sum (src/compute_inertia.hpp:104-135) ---------
 // Stage 1: fixed grid, each block sums a grid-stride slice in FP64.
 template <typename T>
@@ -644,6 +679,25 @@ template <typename T, typename I>
 void launch_inertia(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
                     const T *C, const I *labels, T *out, hipStream_t s) {
   if (n == 0) return;
+  const size_t lds = sizeof(T) * (size_t)k * (size_t)(d + 1);
+  if (lds <= kInertiaLds) {
+    static const bool attr_set = [] {
+      KMD_HIP_CHECK(hipFuncSetAttribute(
+          reinterpret_cast<const void *>(&inertia_lds_krn<T, I>),
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kInertiaLds));
+      return true;
+    }();
+    (void)attr_set;
+    // persistent blocks, each staging the centroids once: as many per CU as
+    // the 160 KB of LDS hold, at most 8 (32 waves)
+    const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(8, (160 * 1024) / (int64_t)lds));
+    const int64_t grid = std::min<int64_t>(ceil_div<int64_t>(n, kBlock),
+                                           (int64_t)device_cu_count() * per_cu);
+    hipLaunchKernelGGL((inertia_lds_krn<T, I>), dim3((unsigned)grid), dim3(kBlock), lds, s,
+                       n, d, k, X, w, C, labels, out);
+    KMD_LAUNCH_CHECK();
+    return;
+  }
   hipLaunchKernelGGL((inertia_krn<T, I>), dim3(grid_for(n, kBlock, 8192)),
                      dim3(kBlock), 0, s, n, d, k, X, w, C, labels, out);
   KMD_LAUNCH_CHECK();

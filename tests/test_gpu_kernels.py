@@ -360,6 +360,31 @@ def test_compute_inertia(kd, golden, torch_dev, dt):
     assert_allclose(to_np(out), exp, rtol=res(dt))
 
 
+@pytest.mark.parametrize("n,d,k,dt", [
+    (50_000, 32, 256, np.float32),    # centroids staged in LDS (33 KB)
+    (50_000, 32, 256, np.float64),    # 67.6 KB: read from global memory
+    (20_000, 64, 300, np.float32),    # 78 KB: global memory
+    (4_099, 3, 4_000, np.float32),    # many narrow centroids: global memory
+    (777, 7, 9, np.float64),          # LDS, ragged n
+])
+def test_compute_inertia_both_centroid_paths(kd, torch_dev, n, d, k, dt):
+    """Weighted per-sample inertia w_s * sum_f (x - c_l)^2 at sizes on both
+    sides of the 64 KB LDS threshold for the centroid table."""
+    rng = np.random.default_rng(n + d + k)
+    X = rng.normal(size=(n, d)).astype(dt) * 3
+    C = rng.normal(size=(k, d)).astype(dt) * 3
+    w = rng.uniform(0.5, 2.0, n).astype(dt)
+    ids = rng.integers(0, k, n).astype(np.int32)
+    out = torch.empty(n, dtype=to_dev(X).dtype, device=torch_dev)
+    ht, _ = kd.compute_inertia(to_dev(np.ascontiguousarray(X.T)), to_dev(w),
+                               to_dev(np.ascontiguousarray(C.T)), to_dev(ids), out,
+                               work_group_size=256, sycl_queue=q())
+    ht.wait()
+    Xd, Cd = X.astype(np.float64), C.astype(np.float64)
+    exp = np.sum(np.square(Xd - Cd[ids]), axis=1) * w.astype(np.float64)
+    assert_allclose(to_np(out), exp, rtol=1e-5 if dt == np.float32 else 1e-12)
+
+
 @pytest.mark.parametrize("dt", FDT)
 def test_reduce_vector_blocking(kd, golden, torch_dev, dt):
     vec = to_dev(golden["rv_data"], dt)
