@@ -303,10 +303,21 @@ __global__ void __launch_bounds__(kBlock)
     int64_t c = (int64_t)labels[s];
     T acc = T(0);
     if (c >= 0 && c < k) {
+      // up to kU feature loads in flight (one round for d <= kU); the fma
+      // chain still runs over f = 0, 1, ...
+      constexpr int kU = 32;
       const T *row = Cs + c * ld;
-      for (int64_t f = 0; f < d; ++f) {
-        T diff = X[f * n + s] - row[f];
-        acc = fma(diff, diff, acc);
+      for (int64_t f0 = 0; f0 < d; f0 += kU) {
+        T xv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) xv[u] = (f0 + u < d) ? X[(f0 + u) * n + s] : T(0);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          if (f0 + u < d) {
+            T diff = xv[u] - row[f0 + u];
+            acc = fma(diff, diff, acc);
+          }
+        }
       }
     } else {
       acc = std::numeric_limits<T>::quiet_NaN();
