@@ -482,8 +482,8 @@ __global__ void __launch_bounds__(kSegThreads)
     const int f = t % D1, r = t / D1;
     const float sc = F.scale[f];
     if (f < d) {
-      // eight row gathers in flight per thread before the first is used
-      constexpr int kU = 8;
+      // sixteen row gathers in flight per thread before the first is used
+      constexpr int kU = 16;
       int j = r;
       for (; j + (kU - 1) * R < cnt; j += kU * R) {
         int ix[kU];
@@ -548,9 +548,25 @@ __global__ void __launch_bounds__(256)
     const int64_t s = item.first + j;
     const int c = F.labels[s];
     float acc = 0.f;
-    for (int fe = 0; fe < F.d; ++fe) {
-      const float diff = F.X[(int64_t)fe * F.n + s] - C[(int64_t)fe * F.k + c];
-      acc = fmaf(diff, diff, acc);
+    constexpr int kU = 16;  // loads in flight; the fmaf chain runs over fe = 0, 1, ...
+    for (int f0 = 0; f0 < F.d; f0 += kU) {
+      float xv[kU], cv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        xv[u] = 0.f;
+        cv[u] = 0.f;
+        if (f0 + u < F.d) {
+          xv[u] = F.X[(int64_t)(f0 + u) * F.n + s];
+          cv[u] = C[(int64_t)(f0 + u) * F.k + c];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (f0 + u < F.d) {
+          const float diff = xv[u] - cv[u];
+          acc = fmaf(diff, diff, acc);
+        }
+      }
     }
     F.dist[s] = acc;
   }
@@ -769,9 +785,25 @@ __global__ void __launch_bounds__(256)
     const int64_t s = item.first + j;
     const int c = F.labels[s];
     float acc = 0.f;
-    for (int fe = 0; fe < F.d; ++fe) {
-      const float diff = F.X[(int64_t)fe * F.n + s] - C[(int64_t)fe * F.k + c];
-      acc = fmaf(diff, diff, acc);
+    constexpr int kU = 16;  // loads in flight; the fmaf chain runs over fe = 0, 1, ...
+    for (int f0 = 0; f0 < F.d; f0 += kU) {
+      float xv[kU], cv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        xv[u] = 0.f;
+        cv[u] = 0.f;
+        if (f0 + u < F.d) {
+          xv[u] = F.X[(int64_t)(f0 + u) * F.n + s];
+          cv[u] = C[(int64_t)(f0 + u) * F.k + c];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (f0 + u < F.d) {
+          const float diff = xv[u] - cv[u];
+          acc = fmaf(diff, diff, acc);
+        }
+      }
     }
     F.dist[s] = acc * F.w[s];
   }
