@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(kTNT) tiled_fp32_krn(TiledArgs a) {
   const int lane = tid & (kWave - 1), wave = tid / kWave;
   const int col = lane & 31, half = lane >> 5;
   const int64_t n = a.n;
-  const int d = a.d, k = a.k, n_ct = a.n_ct;
+  const int d = a.d, n_ct = a.n_ct;
   I *labels = reinterpret_cast<I *>(a.labels);
 
   if constexpr (GATHER) {
@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(256) cand_count_krn(CandArgs a) {
     }
   }
   __syncthreads();
-  if (threadIdx.x < a.n_ct && cnt[threadIdx.x] != 0)
+  if ((int)threadIdx.x < a.n_ct && cnt[threadIdx.x] != 0)
     atomicAdd(&a.tile_cnt[threadIdx.x], cnt[threadIdx.x]);
 }
 
@@ -408,7 +408,7 @@ __global__ void __launch_bounds__(256) cand_scatter_krn(CandArgs a) {
     }
   }
   __syncthreads();
-  if (threadIdx.x < a.n_ct)
+  if ((int)threadIdx.x < a.n_ct)
     cnt[threadIdx.x] = cnt[threadIdx.x] != 0
                            ? a.tile_start[threadIdx.x] +
                                  atomicAdd(&a.tile_cursor[threadIdx.x], cnt[threadIdx.x])

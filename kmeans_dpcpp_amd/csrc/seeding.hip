@@ -558,7 +558,7 @@ __global__ void __launch_bounds__(kSNT, 4) kpp_score_krn(KppArgs a, int c, int64
     }
   }
   __syncthreads();
-  if (threadIdx.x < ns) {
+  if ((int)threadIdx.x < ns) {
     double tot = 0.0;
     for (int i = 0; i < kSNT / kWave; ++i) tot += wsum[i * TT + threadIdx.x];
     a.part[(int64_t)threadIdx.x * a.nb + blk] = tot;
@@ -602,7 +602,7 @@ __global__ void __launch_bounds__(kPNT) kpp_pick_krn(KppArgs a, int c) {
     if (lane_of() == 0) red[threadIdx.x / kWave][t] = vt;
   }
   __syncthreads();
-  if (threadIdx.x < ns) {
+  if ((int)threadIdx.x < ns) {
     double tot = 0.0;
     for (int i = 0; i < kPNT / kWave; ++i) tot += red[i][threadIdx.x];
     s_pot[threadIdx.x] = tot;
@@ -628,7 +628,7 @@ __global__ void __launch_bounds__(kPNT) kpp_pick_krn(KppArgs a, int c) {
   __syncthreads();
   if (c + 1 < k) {
     const int ts = s_ts;
-    if (threadIdx.x < a.trials)
+    if ((int)threadIdx.x < a.trials)
       L.r[threadIdx.x] = a.u[1 + (int64_t)c * a.trials + threadIdx.x] * s_pot[ts];
     const double *dn = a.d2[c & 1];
     const double ccsq = s_csq;
