@@ -1105,13 +1105,14 @@ __global__ void __launch_bounds__(320)
     const float sc = scale[f];
     long long acc = 0;
     if (f < d) {
+      constexpr int kU = 16;  // row gathers in flight per thread (integer sums: any order)
       int j = 0;
-      for (; j + 4 <= cnt; j += 4) {
-        float xv[4];
+      for (; j + kU <= cnt; j += kU) {
+        float xv[kU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xv[u] = X_r[(int64_t)sidx[j + u] * d + f];
+        for (int u = 0; u < kU; ++u) xv[u] = X_r[(int64_t)sidx[j + u] * d + f];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc += __float2int_rn((xv[u] * sw[j + u]) * sc);
+        for (int u = 0; u < kU; ++u) acc += __float2int_rn((xv[u] * sw[j + u]) * sc);
       }
       for (; j < cnt; ++j)
         acc += __float2int_rn((X_r[(int64_t)sidx[j] * d + f] * sw[j]) * sc);
