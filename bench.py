@@ -61,6 +61,14 @@ def parse():
     p.add_argument("--rate", type=float, default=200.0, help="c4 arrivals per s")
     p.add_argument("--large-precision", default="bf16", choices=["fp32", "bf16x3", "bf16"],
                    help="c4: precision of the large requests' fits")
+    p.add_argument("--small-precision", default="bf16", choices=["fp32", "bf16"],
+                   help="c4: assignment precision of the batch engine")
+    p.add_argument("--no-extras", action="store_true",
+                   help="c1: leave out the batched-fit, served-stream and BF16 "
+                        "measurements that the default line carries beside the "
+                        "FP32 figure")
+    p.add_argument("--extra-requests", type=int, default=200,
+                   help="c1 extras: requests of the served stream")
     p.add_argument("--requests", type=int, default=400, help="c4 requests")
     p.add_argument("--fits", type=int, default=512)
     p.add_argument("--copy-probe-mib", type=int, default=0,
@@ -263,12 +271,15 @@ def make_c3_batch(n_fits, seed, device):
     return fits
 
 
-def bench_c3(args):
+def run_c3(n_fits, precision):
+    """n_fits independent fits in one batched call (assignment precision
+    'fp32' or 'bf16'), then the same fits one by one through the plain FP32
+    single-fit driver; every fit is compared bitwise."""
     import torch
     import kmeans_dpcpp_amd as kd
     dev = torch.device("cuda", 0)
     stream = torch.cuda.current_stream()
-    data = make_c3_batch(args.fits, 3001, dev)
+    data = make_c3_batch(n_fits, 3001, dev)
     n_total = sum(x.shape[1] for x, _, _ in data)
 
     def batch_args():
@@ -279,11 +290,11 @@ def bench_c3(args):
                         torch.empty_like(init), 1e-4, 100))
         return out
 
-    kd.kmeans_lloyd_batched(batch_args()[:8], "fp32", stream)      # warm-up
+    kd.kmeans_lloyd_batched(batch_args()[:8], precision, stream)      # warm-up
     fits = batch_args()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res, stats = kd.kmeans_lloyd_batched(fits, "fp32", stream, 4)
+    res, stats = kd.kmeans_lloyd_batched(fits, precision, stream, 4)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     iters = [r[0] for r in res]
@@ -309,23 +320,14 @@ def bench_c3(args):
                            "label_mismatch": int((lab != bf[3]).sum()),
                            "max_centroid_diff": float((out - bf[4]).abs().max())})
     same = not differ
-    line = {
-        "metric": METRIC,
-        "value": round(len(fits) / el, 3),
-        "unit": "fits/s",
-        "n_gpus": 1,
-        "steps": 1,
-        "warmup": 1,
-        "ms_per_step": round(1e3 * el, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"{len(fits)} independent Lloyd fits in one batched "
-                               f"call (n log-uniform 1e3..2e5, d in {{16,32,64}}, "
-                               f"k in {{8..256}}, tol 1e-4, max_iter 100)",
-                   "total_samples": n_total},
+    return {
+        "fits_per_s": round(len(fits) / el, 3),
+        "ms": round(1e3 * el, 3),
+        "precision": precision,
+        "workload": f"{len(fits)} independent Lloyd fits in one batched call (n "
+                    f"log-uniform 1e3..2e5, d in {{16,32,64}}, k in {{8..256}}, tol "
+                    f"1e-4, max_iter 100)",
+        "total_samples": n_total,
         "graph_launches": stats["graph_launches"],
         "iterations": {"sum": int(sum(iters)), "max": int(max(iters)),
                        "mean": round(sum(iters) / len(iters), 2)},
@@ -335,6 +337,29 @@ def bench_c3(args):
         "fits_differing": differ[:5],
         "n_fits_differing": len(differ),
     }
+
+
+def bench_c3(args):
+    prec = args.precision if args.precision in ("fp32", "bf16") else "bf16"
+    r = run_c3(args.fits, prec)
+    line = {
+        "metric": METRIC,
+        "value": r["fits_per_s"],
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": r["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == "fp32" else "bf16 cross term + f32 re-score",
+        "data": "synthetic",
+        "config": {"workload": r["workload"], "total_samples": r["total_samples"],
+                   "precision": prec},
+    }
+    line.update({k: v for k, v in r.items()
+                 if k not in ("fits_per_s", "ms", "workload", "total_samples")})
     print(json.dumps(line), flush=True)
 
 
@@ -370,14 +395,16 @@ def make_c4_requests(n_req, rate, seed, device):
     return reqs
 
 
-def bench_c4(args):
-    import threading
+def run_c4(n_requests, rate, large_precision, small_precision):
+    """A Poisson stream served with continuous batching, then the same
+    arrivals served one at a time by the plain FP32 driver; every request's
+    result is compared bitwise."""
     import torch
     import kmeans_dpcpp_amd as kd
     from kmeans_dpcpp_amd import serving
     dev = torch.device("cuda", 0)
     stream = torch.cuda.current_stream()
-    reqs = make_c4_requests(args.requests, args.rate, 4001, dev)
+    reqs = make_c4_requests(n_requests, rate, 4001, dev)
     torch.cuda.synchronize()
     tol, max_iter = 1e-4, 100
 
@@ -386,8 +413,8 @@ def bench_c4(args):
     # stream
     budget = int(0.6 * torch.cuda.get_device_properties(0).total_memory)
     out = []
-    with serving.Server(budget, iterations_per_tick=1,
-                        large_precision=args.large_precision) as srv:
+    with serving.Server(budget, iterations_per_tick=1, large_precision=large_precision,
+                        small_precision=small_precision) as srv:
         t0 = time.perf_counter()
         for (ta, X_t, w, init) in reqs:
             now = time.perf_counter() - t0
@@ -419,24 +446,16 @@ def bench_c4(args):
 
     def pct(v, p):
         return v[min(len(v) - 1, int(round(p / 100 * (len(v) - 1))))]
-    line = {
-        "metric": METRIC,
-        "value": round(len(out) / el, 3),
-        "unit": "fits/s",
-        "n_gpus": 1,
-        "steps": 1,
-        "warmup": 0,
-        "ms_per_step": round(1e3 * el, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"served stream: {len(out)} Poisson arrivals at "
-                               f"{args.rate:g} req/s, 95% batched-fit sizes, 5% "
-                               f"n=2M d=96 k=512; tol 1e-4, max_iter 100; "
-                               f"continuous batching, footprint admission; "
-                               f"large fits in {args.large_precision}"},
+    return {
+        "fits_per_s": round(len(out) / el, 3),
+        "ms": round(1e3 * el, 3),
+        "workload": f"served stream: {len(out)} Poisson arrivals at {rate:g} req/s, "
+                    f"95% batched-fit sizes, 5% n=2M d=96 k=512; tol 1e-4, max_iter "
+                    f"100; continuous batching, footprint admission; small fits "
+                    f"{small_precision}, large fits {large_precision}",
+        "rate_req_per_s": rate,
+        "requests": len(out),
+        "large_requests": sum(1 for r in out if not r.batched),
         "latency_s": {"p50": round(lat["p50"], 4), "p99": round(lat["p99"], 4),
                       "mean": round(lat["mean"], 4)},
         "plain_per_request_fp32": {
@@ -448,6 +467,26 @@ def bench_c4(args):
         "peak_resident_gb": round(srv.peak_resident / 1e9, 2),
         "identical_to_plain": bool(same),
     }
+
+
+def bench_c4(args):
+    r = run_c4(args.requests, args.rate, args.large_precision, args.small_precision)
+    line = {
+        "metric": METRIC,
+        "value": r["fits_per_s"],
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": r["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 cross term + f32 re-score",
+        "data": "synthetic",
+        "config": {"workload": r["workload"]},
+    }
+    line.update({k: v for k, v in r.items() if k not in ("fits_per_s", "ms", "workload")})
     print(json.dumps(line), flush=True)
 
 
@@ -756,7 +795,53 @@ def main():
         "path": stats.get("path"),
         "rescored_per_iter": stats.get("n_rescored", 0) / max(1, n_iter),
     }
+    if not args.no_extras and args.precision == "fp32":
+        # beside the plain FP32 figure: the same fit with the BF16 cross term
+        # (HBM roofline), the batched many-fit call and the served stream
+        line["mixed_bf16"] = run_c1_mixed(args, X_t, w, init_t, labels, c_b, inertia)
+        line["batched_c3"] = run_c3(args.fits, "bf16")
+        line["served_c4"] = run_c4(args.extra_requests, args.rate, "bf16", "bf16")
     print(json.dumps(line), flush=True)
+
+
+def run_c1_mixed(args, X_t, w, init_t, labels_fp32, cent_fp32, inertia_fp32):
+    """The timed fit again with the BF16 cross term + certified FP32 re-score:
+    iterations/s, the assignment+update kernels against the HBM roof, and
+    bitwise equality with the FP32 fit."""
+    import torch
+    import kmeans_dpcpp_amd as kd
+    stream = torch.cuda.current_stream()
+    n, d = X_t.shape[1], X_t.shape[0]
+    lab = torch.empty_like(labels_fp32)
+    c_a, c_b = init_t.clone(), torch.empty_like(init_t)
+    if args.warmup > 0:
+        kd.kmeans_lloyd_driver_ex(X_t, w, c_a, lab, c_b, 0.0, False, args.warmup, "bf16",
+                                  stream)
+        c_a = init_t.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it, inertia, st = kd.kmeans_lloyd_driver_ex(X_t, w, c_a, lab, c_b, 0.0, False,
+                                                args.steps, "bf16", stream)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kern_ms = st["fused_kernel_ms"] / max(1, st["fused_launches"])
+    bytes_per_launch = n * d * 4 + n * 4 + n * 4
+    gbs = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    return {
+        "iters_per_s": round(it / el, 3),
+        "ms_per_step": round(1e3 * el / it, 4),
+        "path": st.get("path"),
+        "rescored_per_iter": round(st.get("n_rescored", 0) / max(1, it), 1),
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "kernel": "BF16 assignment + update, then the FP32 re-score",
+                     "kernel_ms": round(kern_ms, 4),
+                     "algorithmic_bytes": bytes_per_launch},
+        "labels_identical_to_fp32": bool(torch.equal(lab, labels_fp32)),
+        "centroids_identical_to_fp32": bool(torch.equal(c_b.view(torch.int32),
+                                                        cent_fp32.view(torch.int32))),
+        "inertia_identical_to_fp32": inertia == inertia_fp32,
+    }
 
 
 if __name__ == "__main__":

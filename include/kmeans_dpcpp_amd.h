@@ -276,12 +276,19 @@ typedef struct kmd_fit {
  * launch per step of an iteration over a work list of (fit, tile) items,
  * per-fit convergence masks, the iteration captured once in a hipGraph and
  * replayed, `graph_replays` iterations between host checks of the masks
- * (<= 0: 4).  Other fits run through the single-fit driver.  precision:
- * KMD_PREC_FP32 only.  *graph_launches (may be NULL) = replays issued.
- * Blocking. */
+ * (<= 0: 4).  Other fits, and fits whose data range needs floating-point
+ * centroid sums, run through the single-fit driver.  precision:
+ * KMD_PREC_FP32, or KMD_PREC_BF16 (the batched assignment takes the cross
+ * term in BF16 on the matrix core and re-scores in FP32 every sample whose
+ * margin is below the error bound: the same labels, so the same fits, bit
+ * for bit).  The first two iterations try the assignment kernels' two
+ * launch shapes (the default threads per block, then half of them), timing
+ * them, and the rest run with the faster; results never depend on it.
+ * *graph_launches (may be NULL) = iterations issued (replays and trials),
+ * *assign_shape (may be NULL) = the shape kept.  Blocking. */
 int kmd_kmeans_lloyd_batched(int64_t n_fits, kmd_fit *fits, int precision,
                              int graph_replays, int64_t *graph_launches,
-                             kmd_stream_t stream);
+                             int *assign_shape, kmd_stream_t stream);
 
 /* Continuous batching (no counterpart in the reference; for a serving loop
  * that admits fit requests while others run).  An engine holds a running
@@ -306,27 +313,35 @@ int kmd_engine_step(kmd_engine_t engine, int iterations, int use_graph,
 int kmd_engine_take_finished(kmd_engine_t engine, int64_t *ids,
                              int64_t capacity, int64_t *count);
 int kmd_engine_resident_bytes(kmd_engine_t engine, int64_t *bytes);
+/* Assignment kernels of the engine's iterations: KMD_PREC_FP32 (default) or
+ * KMD_PREC_BF16 (BF16 cross term + certified FP32 re-score; results equal
+ * the FP32 engine's bit for bit).  Applies from the next step. */
+int kmd_engine_set_precision(kmd_engine_t engine, int precision);
 
 /* Launch autotuner (no counterpart in the reference, whose launch shape is
  * the caller's work_group_size / centroids_window_height).  Times the
  * driver's candidate plans for this problem -- LDS-resident or tiled
- * centroids (where each serves the shape) x persistent blocks per CU
- * {occupancy limit, 1, 2, 4} -- for `iters` Lloyd iterations from
- * init_centroids_t (not modified) and records the fastest for
+ * centroids (where each serves the shape) x, for the LDS-resident FP32
+ * step, its launch shape (shape 0: 16 waves per block x one 32-sample
+ * column block per wave; 1: 12 waves x two; 2: 8 waves x two) x persistent
+ * blocks per CU {occupancy limit, 1, 2, 4} -- for `iters` Lloyd iterations
+ * from init_centroids_t (not modified) and records the fastest for
  * (n_features, n_clusters, precision, log2 n_samples); later driver calls
  * of that shape use it.  Results never depend on the plan.  Writes the best
  * plan and up to `capacity` candidates (path: 1 LDS-resident, 2 tiled,
- * 0 generic; blocks_per_cu 0 = occupancy limit; fused-step ms/launch). */
+ * 0 generic; blocks_per_cu 0 = occupancy limit; shape; fused-step
+ * ms/launch). */
 int kmd_autotune(int dtype, int itype, int precision, int64_t n_samples,
                  int64_t n_features, int64_t n_clusters, const void *X_t,
                  const void *sample_weight, const void *init_centroids_t,
                  int iters, int *best_path, int *best_blocks_per_cu,
-                 double *best_ms, int *cand_path, int *cand_blocks_per_cu,
-                 double *cand_ms, int64_t capacity, int64_t *n_candidates,
-                 kmd_stream_t stream);
+                 int *best_shape, double *best_ms, int *cand_path,
+                 int *cand_blocks_per_cu, int *cand_shape, double *cand_ms,
+                 int64_t capacity, int64_t *n_candidates, kmd_stream_t stream);
 /* Process-wide override of the plan driver calls use (path -1 clears it:
- * back to the autotuner's table, else the automatic choice). */
-int kmd_set_tuning(int path, int blocks_per_cu);
+ * back to the autotuner's table, else the automatic choice; shape as in
+ * kmd_autotune). */
+int kmd_set_tuning(int path, int blocks_per_cu, int shape);
 
 /* ---- estimator front-end: data preparation and seeding ----------------
  * No counterpart in the reference, whose driver takes init_centroids_t from

@@ -42,6 +42,7 @@ namespace {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned long long u64;
 
 // Assignment items are sized by their MFMA work: about kItemMacs
@@ -88,8 +89,9 @@ struct BFit {
   float *scale;     // (d + 1) fixed-point scales
   float *hn, *sizes;                        // (k + 1)
   int *list, *totals, *cursor, *start, *citems;  // (k + 1)
-  int *nonfinite;   // (1)
-  unsigned *maxbits;  // (d + 1) bit patterns of max |x_f| and max |w|
+  int *nonfinite;   // (1) 0: fixed-point sums; 1: inf/NaN; 2: range too wide
+  unsigned *maxbits;  // (d + 2) range maxima (fixed_point.hpp)
+  u64 *estat;         // (2 (d + 1)) exponent sums and counts (fixed_point.hpp)
 };
 
 // A pointer read from a BFit, marked as global memory: loads through it are
@@ -157,12 +159,12 @@ constexpr int tile_floats() {
   return 2 * KS * 32 + 32;
 }
 
-template <int KS>
-__global__ void __launch_bounds__(assign_threads(KS))
+template <int KS, int NT = assign_threads(KS)>
+__global__ void __launch_bounds__(NT)
     bt_assign_krn(const BFit *__restrict__ fits, const AItem *__restrict__ items, int n_items,
                   int all) {
   constexpr int TF = tile_floats<KS>();
-  constexpr int kNT = assign_threads(KS), kW = kNT / kWave;
+  constexpr int kNT = NT, kW = kNT / kWave;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   float *Cl = reinterpret_cast<float *>(lds_raw);  // [kMaxK / 32][TF]
   const int tid = threadIdx.x;
@@ -270,6 +272,304 @@ __global__ void __launch_bounds__(assign_threads(KS))
         const bool take = (ov < bestv) || (ov == bestv && oi < besti);
         besti = take ? oi : besti;
         if (half == 0 && p < end) F.labels[p] = besti;
+      }
+    }
+  }
+}
+
+
+// ---- assignment with the BF16 cross term and a certified FP32 re-score ----------
+//
+// The same items, staging and labels as bt_assign_krn, with the cross term of
+// each (sample, centroid) pair first taken on the matrix core in BF16:
+//   D = sum_f bf16(x_f) bf16(c_f) - (h1 + h2 + h3)   (v_mfma_f32_32x32x16_bf16,
+// FP32 accumulation; h1 + h2 + h3 = the FP32 half norm split exactly into
+// three BF16 terms, the samples' side of those K columns is -1), so that the
+// argmin of hn - c.x is the argmax of D.  Each lane keeps the best and
+// second-best D of its sample with the centroid index packed into the low
+// mantissa bits.  With E the bound on the error of D against exact
+// arithmetic plus the bound on the FP32 path's error (from ||x||,
+// max ||c||, max hn, the index packing), a sample whose gap best - second
+// exceeds 2 E has the FP32 path's label, unique, and is labelled here.  The
+// others are listed in LDS and re-scored after the item's samples with the
+// FP32 path's instruction sequence (bt_assign_krn's chain), so every label is
+// the FP32 path's label bit for bit and the exact update that follows gives
+// bitwise the FP32 fit.
+template <int KS>
+constexpr int mixed_kk() {
+  return (2 * KS + 3 + 15) / 16;  // k-steps of 16: D = 2 KS features + 3 hn terms
+}
+template <int KS>
+constexpr size_t mixed_aop_offset() {
+  return (size_t)(kMaxK / 32) * (2 * KS * 32 + 32) * 4;   // after the FP32 tiles
+}
+template <int KS>
+constexpr size_t mixed_list_offset() {
+  return mixed_aop_offset<KS>() + (size_t)(kMaxK / 32) * mixed_kk<KS>() * 64 * 16;
+}
+template <int KS>
+constexpr size_t mixed_assign_lds() {
+  return mixed_list_offset<KS>() + sizeof(int) * kItemMax;
+}
+
+// v_max3_f32 on values known to be finite (no NaN canonicalisation in front)
+__device__ inline float max3f(float a, float b, float c) {
+  float r;
+  asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+template <int KS, int NT = assign_threads(KS)>
+__global__ void __launch_bounds__(NT)
+    bt_assign_mixed_krn(const BFit *__restrict__ fits, const AItem *__restrict__ items,
+                        int n_items, int all, float pinf) {
+  constexpr int TF = tile_floats<KS>();
+  constexpr int D = 2 * KS, KD = D / 16, KK = mixed_kk<KS>(), XH = D / 2;
+  constexpr int kNT = NT, kW = kNT / kWave;
+  constexpr int kIdxBits = 8;                      // k <= kMaxK = 256
+  constexpr unsigned kIdxMask = (1u << kIdxBits) - 1u;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  float *Cl = reinterpret_cast<float *>(lds_raw);  // FP32 tiles [kMaxK / 32][TF]
+  bf16x8 *Al = reinterpret_cast<bf16x8 *>(lds_raw + mixed_aop_offset<KS>());
+  int *list = reinterpret_cast<int *>(lds_raw + mixed_list_offset<KS>());
+  __shared__ int act[kWave];
+  __shared__ int n_act, n_list;
+  __shared__ float red[kW];
+  __shared__ int bad_hn[kW];
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1), wave = tid / kWave;
+  const int col = lane & 31, half = lane >> 5;
+  constexpr float u = 1.0f / 256.0f;   // bf16 unit roundoff
+  constexpr float acc_err = (float)(D + 3 + D + 8) * 5.9604645e-8f;
+  constexpr float alpha = 2.f * u + u * u + acc_err;
+  constexpr float beta = acc_err;
+  const float pack_err = ldexpf(1.0f, kIdxBits - 23) * 1.01f;
+  for (int base = blockIdx.x; base < n_items; base += kWave * gridDim.x) {
+    __syncthreads();
+    if (wave == 0) {
+      const int it = base + lane * gridDim.x;
+      const bool a = it < n_items && (all || fits[items[it].fit].st[0].active);
+      const unsigned long long m = __ballot(a);
+      if (a) act[__popcll(m & ((1ull << lane) - 1))] = it;
+      if (lane == 0) n_act = __popcll(m);
+    }
+    __syncthreads();
+    const int na = n_act;
+    for (int j = 0; j < na; ++j) {
+      const AItem item = items[act[j]];
+      const BFit F = fits[item.fit];
+      const auto *C = gptr(cbuf(F, fits[item.fit].st[0].cur));
+      const auto *hn = gptr(F.hn);
+      const int d = F.d, k = F.k, n_ct = (k + 31) / 32;
+      __syncthreads();
+      if (tid == 0) n_list = 0;
+      // FP32 tiles (the re-score's operands), as bt_assign_krn stages them,
+      // and max hn over the fit's centroids
+      float hmax = 0.f;
+      int hbad = 0;
+      constexpr int kSU = 4;
+      for (int i0 = tid; i0 < n_ct * TF; i0 += kSU * kNT) {
+        float v[kSU];
+        int at[kSU];
+#pragma unroll
+        for (int uu = 0; uu < kSU; ++uu) {
+          const int i = i0 + uu * kNT;
+          const int t = i / TF, e = i - t * TF;
+          at[uu] = -1;
+          v[uu] = 0.f;
+          if (i < n_ct * TF) {
+            if (e < 2 * KS * 32) {
+              const int f = e >> 5, cc = e & 31;
+              const int c = t * 32 + cc;
+              const int ks = f >> 1, hf = f & 1;
+              at[uu] = t * TF + ((ks >> 2) * 2 + hf) * 128 + cc * 4 + (ks & 3);
+              v[uu] = (f < d && c < k) ? C[(int64_t)f * k + c] : 0.f;
+            } else {
+              const int c = t * 32 + (e - 2 * KS * 32);
+              at[uu] = t * TF + e;
+              if (c < k) {
+                const float h = hn[c];
+                v[uu] = -h;
+                hbad |= !(h < 1e36f);         // also true for NaN
+                hmax = fmaxf(hmax, h);
+              } else {
+                v[uu] = -__builtin_huge_valf();
+              }
+            }
+          }
+        }
+#pragma unroll
+        for (int uu = 0; uu < kSU; ++uu)
+          if (at[uu] >= 0) Cl[at[uu]] = v[uu];
+      }
+      // BF16 A operands: element jj of lane l for (tile ct, k-step kk) is
+      // centroid ct 32 + l % 32 at K position 16 kk + 8 (l / 32) + jj
+      for (int g = tid; g < n_ct * KK * 64; g += kNT) {
+        const int l = g & 63, kk = (g >> 6) % KK, ct = g / (64 * KK);
+        const int c = ct * 32 + (l & 31);
+        bf16x8 av;
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          const int pp = 16 * kk + 8 * (l >> 5) + jj;
+          float val = 0.f;
+          if (pp < D) {
+            val = (pp < d && c < k) ? C[(int64_t)pp * k + c] : 0.f;
+          } else if (pp < D + 3) {
+            if (c < k) {
+              const float h = hn[c];
+              const float h1 = (float)(__bf16)h;
+              const float r1 = h - h1;
+              const float h2 = (float)(__bf16)r1;
+              const float h3 = (float)(__bf16)(r1 - h2);
+              val = pp == D ? h1 : (pp == D + 1 ? h2 : h3);
+            } else {
+              val = pp == D ? 3.0e38f : 0.f;   // padding centroid: never the best
+            }
+          }
+          av[jj] = (__bf16)val;
+        }
+        Al[g] = av;
+      }
+      for (int o = kWave / 2; o > 0; o >>= 1) {
+        hmax = fmaxf(hmax, __shfl_xor(hmax, o, kWave));
+        hbad |= __shfl_xor(hbad, o, kWave);
+      }
+      if (lane == 0) {
+        red[wave] = hmax;
+        bad_hn[wave] = hbad;
+      }
+      __syncthreads();
+      float hnmax = 0.f;
+      int any_bad = 0;
+#pragma unroll
+      for (int i = 0; i < kW; ++i) {
+        hnmax = fmaxf(hnmax, red[i]);
+        any_bad |= bad_hn[i];
+      }
+      const float cmax = sqrtf(2.f * hnmax);
+      const bool finite_c = !any_bad;
+      const int64_t n = F.n;
+      const int64_t end = min((int64_t)item.first + item.count, n);
+      bf16x8 bh;
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj)
+        bh[jj] = (__bf16)((half == 0 && jj < 3) ? -1.f : 0.f);
+      for (int64_t s0 = item.first; s0 < end; s0 += kW * 32) {  // block-uniform
+        const int64_t p = s0 + wave * 32 + col;
+        // lane features 16 (i / 8) + 8 half + i % 8
+        float xr[XH];
+#pragma unroll
+        for (int i = 0; i < XH; ++i) {
+          const int f = 16 * (i / 8) + 8 * half + (i % 8);
+          xr[i] = (f < d && p < end) ? gptr(F.X)[(int64_t)f * n + p] : 0.f;
+        }
+        float ss = 0.f;
+#pragma unroll
+        for (int i = 0; i < XH; ++i) ss = fmaf(xr[i], xr[i], ss);
+        ss += __shfl_xor(ss, 32, kWave);
+        const float xnorm = sqrtf(ss) * 1.0001f;
+        bf16x8 bhi[KD];
+#pragma unroll
+        for (int kd = 0; kd < KD; ++kd)
+#pragma unroll
+          for (int jj = 0; jj < 8; ++jj) bhi[kd][jj] = (__bf16)xr[8 * kd + jj];
+        float b1 = -__builtin_huge_valf(), b2 = -__builtin_huge_valf();
+        for (int ct = 0; ct < n_ct; ++ct) {
+          const bf16x8 *At = Al + ct * KK * 64 + lane;
+          f32x16 acc = {};
+#pragma unroll
+          for (int kk = 0; kk < KK; ++kk)
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(At[kk * 64],
+                                                          kk < KD ? bhi[kk < KD ? kk : 0] : bh,
+                                                          acc, 0, 0, 0);
+          // rows m = 8 (r / 4) + 4 half + r % 4 of the tile; top-2 of
+          // {b1, b2, u, v}: b1' = max3(b1, u, v), b2' = max(b2, med3(b1, u, v))
+#pragma unroll
+          for (int r = 0; r < 16; r += 2) {
+            const unsigned i0 = (unsigned)(ct * 32 + 8 * (r / 4) + 4 * half + (r % 4));
+            const float uu = __uint_as_float((__float_as_uint(acc[r]) & ~kIdxMask) | i0);
+            const float vv =
+                __uint_as_float((__float_as_uint(acc[r + 1]) & ~kIdxMask) | (i0 + 1u));
+            const float mm = __builtin_amdgcn_fmed3f(b1, uu, vv);
+            b1 = max3f(b1, uu, vv);
+            b2 = __builtin_amdgcn_fmed3f(b2, mm, pinf);
+          }
+        }
+        {
+          const float o1 = __shfl_xor(b1, 32, kWave);
+          const float o2 = __shfl_xor(b2, 32, kWave);
+          const float n2 = fmaxf(fminf(b1, o1), fmaxf(b2, o2));
+          b1 = fmaxf(b1, o1);
+          b2 = n2;
+        }
+        const float e = xnorm * cmax * alpha + hnmax * beta +
+                        pack_err * (fabsf(b1) + fabsf(b2));
+        const int lab = (int)(__float_as_uint(b1) & kIdxMask);
+        const bool amb = !(b1 - b2 > 2.f * e) || !(xnorm * cmax < 1e36f) || !finite_c ||
+                         lab >= k;
+        const bool valid = half == 0 && p < end;
+        if (valid && !amb) F.labels[p] = lab;
+        const bool enq = valid && amb;
+        const unsigned long long mask = __ballot(enq);
+        if (mask != 0ull) {
+          const int cnt = __popcll(mask);
+          const int leader = __ffsll((long long)mask) - 1;
+          int b0 = 0;
+          if (lane == leader) b0 = atomicAdd(&n_list, cnt);
+          b0 = __shfl(b0, leader, kWave);
+          if (enq) list[b0 + __popcll(mask & ((1ull << lane) - 1ull))] = (int)p;
+        }
+      }
+      __syncthreads();
+      // FP32 re-score of the listed samples: bt_assign_krn's chain
+      const int nl = n_list;
+      for (int j0 = wave * 32; j0 < nl; j0 += kW * 32) {
+        const int jl = j0 + col;
+        const bool ok = jl < nl;
+        const int64_t p = ok ? (int64_t)list[jl] : (int64_t)item.first;
+        float xr[KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          const int f = 2 * ks + half;
+          xr[ks] = (f < d && ok) ? gptr(F.X)[(int64_t)f * n + p] : 0.f;
+        }
+        float bestv = __builtin_huge_valf();
+        int besti = 0;
+        for (int t = 0; t < n_ct; ++t) {
+          const float *At = Cl + t * TF;
+          const float *Ht = At + 2 * KS * 32;
+          f32x16 acc;
+#pragma unroll
+          for (int gg = 0; gg < 4; ++gg) {
+            const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + 8 * gg + 4 * half);
+#pragma unroll
+            for (int ee = 0; ee < 4; ++ee) acc[4 * gg + ee] = h4[ee];
+          }
+#pragma unroll
+          for (int k4 = 0; k4 < KS / 4; ++k4) {
+            const f32x4 av =
+                *reinterpret_cast<const f32x4 *>(At + (k4 * 2 + half) * 128 + col * 4);
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+              acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[jj], xr[4 * k4 + jj], acc, 0, 0, 0);
+          }
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            const int m0 = 8 * g + 4 * half;
+#pragma unroll
+            for (int ee = 0; ee < 4; ++ee) {
+              const float v = -acc[4 * g + ee];
+              const bool upd = v < bestv;
+              bestv = upd ? v : bestv;
+              besti = upd ? (t * 32 + m0 + ee) : besti;
+            }
+          }
+        }
+        const float ov = __shfl_xor(bestv, 32, kWave);
+        const int oi = __shfl_xor(besti, 32, kWave);
+        const bool take = (ov < bestv) || (ov == bestv && oi < besti);
+        besti = take ? oi : besti;
+        if (half == 0 && ok) F.labels[p] = besti;
       }
     }
   }
@@ -859,7 +1159,8 @@ __global__ void __launch_bounds__(256)
   BState S{};
   S.active = F.max_iter > 0 ? 1 : 0;
   F.st[0] = S;
-  for (int j = 0; j <= F.d; ++j) F.maxbits[j] = 0u;
+  for (int j = 0; j < range_maxbits_count(F.d); ++j) F.maxbits[j] = 0u;
+  for (int j = 0; j < range_estat_count(F.d); ++j) F.estat[j] = 0ull;
 }
 
 // X_r = transpose of X_t for one 32 x 32 tile of one fit (transpose_krn's
@@ -885,20 +1186,37 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// maxbits[f] = max of (bits of X[f, s]) & 0x7fffffff over the item's
-// samples (f == d: the weights), as absmax_bits_krn computes it per fit
+// The range statistics of fixed_point.hpp over the item's samples (f == d:
+// the weights), as absmax_bits_krn gathers them per fit (integer maxima and
+// sums: the same values in any order)
 __global__ void __launch_bounds__(256)
     bt_absmax_krn(const BFit *__restrict__ fits, const CItem *__restrict__ items) {
   const CItem it = items[blockIdx.x];
   const BFit F = fits[it.fit];
   for (int f = 0; f <= F.d; ++f) {
     const float *row = (f < F.d) ? F.X + (int64_t)f * F.n : F.w;
-    unsigned m = 0u;
-    for (int i = threadIdx.x; i < it.count; i += 256)
-      m = max(m, __float_as_uint(row[it.first + i]) & 0x7fffffffu);
-    for (int o = kWave / 2; o > 0; o >>= 1)
+    unsigned m = 0u, esum = 0u, ecnt = 0u, wmin_inv = 0u;
+    for (int i = threadIdx.x; i < it.count; i += 256) {
+      const unsigned v = __float_as_uint(row[it.first + i]);
+      range_accumulate(v, m, esum, ecnt);
+      const unsigned b = v & 0x7fffffffu;
+      if (f == F.d && b != 0u) wmin_inv = max(wmin_inv, ~b);
+    }
+    u64 es = esum, ec = ecnt;
+    for (int o = kWave / 2; o > 0; o >>= 1) {
       m = max(m, (unsigned)__shfl_xor((int)m, o, kWave));
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(&F.maxbits[f], m);
+      wmin_inv = max(wmin_inv, (unsigned)__shfl_xor((int)wmin_inv, o, kWave));
+      es += __shfl_xor(es, o, kWave);
+      ec += __shfl_xor(ec, o, kWave);
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+      atomicMax(&F.maxbits[f], m);
+      if (f == F.d && wmin_inv != 0u) atomicMax(&F.maxbits[F.d + 1], wmin_inv);
+      if (ec != 0) {
+        atomicAdd(&F.estat[2 * f], es);
+        atomicAdd(&F.estat[2 * f + 1], ec);
+      }
+    }
   }
 }
 
@@ -908,7 +1226,7 @@ __global__ void __launch_bounds__(256)
   const int f = blockIdx.x * 256 + threadIdx.x;
   if (f >= m) return;
   const BFit F = fits[f];
-  fixed_point_scales_from_bits(F.n, F.d, F.maxbits, F.scale, F.nonfinite);
+  fixed_point_scales_from_bits(F.n, F.d, F.maxbits, F.estat, F.scale, F.nonfinite);
 }
 
 // out[i] = *flags[i]: the new fits' non-finite flags read back with one copy
@@ -930,19 +1248,29 @@ inline int ks_bucket(int d) { return d <= 16 ? 8 : (d <= 32 ? 16 : 32); }
 
 size_t assign_lds(int KS) { return (size_t)(kMaxK / 32) * (2 * KS * 32 + 32) * 4; }
 
+// Assignment launch shapes the batched call tunes between (results never
+// depend on them): threads per block = assign_threads(KS) (shape 0) or half
+// of it (shape 1).
+template <int KS, int NT>
+int assign_grid_nt(bool mixed) {
+  static int grid[2] = {0, 0};
+  int &g = grid[mixed ? 1 : 0];
+  if (g == 0) {
+    const void *fn = mixed ? reinterpret_cast<const void *>(&bt_assign_mixed_krn<KS, NT>)
+                           : reinterpret_cast<const void *>(&bt_assign_krn<KS, NT>);
+    const size_t lds = mixed ? mixed_assign_lds<KS>() : assign_lds(KS);
+    KMD_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    int per_cu = 0;
+    KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NT, lds));
+    g = device_cu_count() * std::max(1, per_cu);
+  }
+  return g;
+}
+
 template <int KS>
 int assign_grid() {
-  static int grid = 0;
-  if (grid == 0) {
-    const void *fn = reinterpret_cast<const void *>(&bt_assign_krn<KS>);
-    KMD_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)assign_lds(KS)));
-    int per_cu = 0;
-    KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, assign_threads(KS),
-                                                                assign_lds(KS)));
-    grid = device_cu_count() * std::max(1, per_cu);
-  }
-  return grid;
+  return assign_grid_nt<KS, assign_threads(KS)>(false);
 }
 
 template <int KS>
@@ -950,14 +1278,30 @@ int assign_launch_grid(int n_items) {
   return std::max(1, std::min(assign_grid<KS>(), n_items));
 }
 
+template <int KS, int NT>
+void launch_assign_nt(const BFit *fits, const AItem *items, int n_items, int all,
+                      hipStream_t s, bool mixed) {
+  const int grid = std::max(1, std::min(assign_grid_nt<KS, NT>(mixed), n_items));
+  if (mixed)
+    hipLaunchKernelGGL((bt_assign_mixed_krn<KS, NT>), dim3(grid), dim3(NT),
+                       mixed_assign_lds<KS>(), s, fits, items, n_items, all,
+                       __builtin_huge_valf());
+  else
+    hipLaunchKernelGGL((bt_assign_krn<KS, NT>), dim3(grid), dim3(NT), assign_lds(KS), s,
+                       fits, items, n_items, all);
+  KMD_LAUNCH_CHECK();
+}
+
+// mixed: the BF16 cross term with the certified FP32 re-score (the same
+// labels as the FP32 kernel); shape: 0 or 1 (assign_grid_nt)
 template <int KS>
 void launch_assign_bucket(const BFit *fits, const AItem *items, int n_items,
-                          int all, hipStream_t s) {
+                          int all, hipStream_t s, bool mixed, int shape) {
   if (n_items <= 0) return;
-  const int grid = assign_launch_grid<KS>(n_items);
-  hipLaunchKernelGGL(bt_assign_krn<KS>, dim3(grid), dim3(assign_threads(KS)), assign_lds(KS), s,
-                     fits, items, n_items, all);
-  KMD_LAUNCH_CHECK();
+  if (shape == 1)
+    launch_assign_nt<KS, assign_threads(KS) / 2>(fits, items, n_items, all, s, mixed);
+  else
+    launch_assign_nt<KS, assign_threads(KS)>(fits, items, n_items, all, s, mixed);
 }
 
 inline int assign_bucket(int d) { return ks_bucket(d) == 8 ? 0 : (ks_bucket(d) == 16 ? 1 : 2); }
@@ -1002,7 +1346,7 @@ inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 // Byte layout of one fit's workspace arena.
 struct Arena {
   size_t st, xr, perm, dist, sums, scale, hn, sizes, list, totals, cursor, start,
-      citems, nonfinite, maxbits, total;
+      citems, nonfinite, maxbits, estat, total;
   Arena(int64_t n, int64_t d, int64_t k) {
     size_t o = 0;
     auto take = [&](size_t b) {
@@ -1024,7 +1368,8 @@ struct Arena {
     start = take(sizeof(int) * (size_t)(k + 1));
     citems = take(sizeof(int) * (size_t)(k + 1));
     nonfinite = take(sizeof(int));
-    maxbits = take(sizeof(unsigned) * (size_t)(d + 1));
+    maxbits = take(sizeof(unsigned) * (size_t)range_maxbits_count(d));
+    estat = take(sizeof(u64) * (size_t)range_estat_count(d));
     total = o;
   }
 };
@@ -1068,10 +1413,24 @@ struct BatchEngine {
   std::vector<int64_t> take_finished();
   int64_t resident_bytes() const;
   void rebuild();
-  void iteration(hipStream_t q);
+  // ev_a0 / ev_a1 (may be null): recorded around the assignment launches
+  void iteration(hipStream_t q, hipEvent_t ev_a0 = nullptr, hipEvent_t ev_a1 = nullptr);
   void epilogue(const std::vector<Slot *> &done);
   void check_inputs();
   void drop_graph();
+  // fits whose data range needs floating-point sums (fixed_point.hpp flag 2):
+  // set aside for the single-fit driver when route_wide_ is set, else add()
+  // / step() raise ValueError
+  bool route_wide_ = false;
+  std::vector<BatchedFit> wide_;
+  // assignment with the BF16 cross term + certified FP32 re-score
+  bool mixed_ = false;
+  // launch shape of the assignment kernels (launch_assign_bucket); with
+  // tune_shape_ set, the first two iterations of step() run shapes 0 and 1
+  // (ungraphed, their assignment launches timed) and the faster is kept
+  int assign_shape_ = 0;
+  bool tune_shape_ = false;
+  float shape_ms_[2] = {0.f, 0.f};
 
   hipStream_t user_ = nullptr, s_ = nullptr;
   hipEvent_t ev_ = nullptr;
@@ -1105,6 +1464,7 @@ BatchEngine::BatchEngine(hipStream_t user_stream) : user_(user_stream) {
   (void)assign_grid<8>();
   (void)assign_grid<16>();
   (void)assign_grid<32>();
+
 }
 
 BatchEngine::~BatchEngine() {
@@ -1180,6 +1540,7 @@ int64_t BatchEngine::add(const BatchedFit &f, bool check_now, bool defer_prep,
   F.citems = reinterpret_cast<int *>(b + A.citems);
   F.nonfinite = reinterpret_cast<int *>(b + A.nonfinite);
   F.maxbits = reinterpret_cast<unsigned *>(b + A.maxbits);
+  F.estat = reinterpret_cast<u64 *>(b + A.estat);
   if (defer_prep) {
     pending_prep_.push_back(slot.get());
   } else {
@@ -1264,8 +1625,14 @@ void BatchEngine::check_inputs() {
     KMD_HIP_CHECK(hipStreamSynchronize(s_));
   }
   std::vector<Slot *> failed;
+  bool nonfinite = false, wide = false;
   for (size_t i = 0; i < unchecked_.size(); ++i)
-    if (bad[i]) failed.push_back(unchecked_[i]);
+    if (bad[i]) {
+      failed.push_back(unchecked_[i]);
+      nonfinite |= bad[i] == 1;
+      wide |= bad[i] == 2;
+      if (bad[i] == 2 && route_wide_) wide_.push_back(unchecked_[i]->desc);
+    }
   unchecked_.clear();
   if (!failed.empty()) {
     for (Slot *f : failed)
@@ -1275,7 +1642,10 @@ void BatchEngine::check_inputs() {
                                   }),
                    slots_.end());
     dirty_ = true;
-    throw ValueError("batched fit: X or sample_weight holds inf/NaN");
+    if (nonfinite) throw ValueError("batched fit: X or sample_weight holds inf/NaN");
+    if (wide && !route_wide_)
+      throw ValueError("batched fit: the range of X or sample_weight needs "
+                       "floating-point sums; use the single-fit driver");
   }
 }
 
@@ -1312,16 +1682,21 @@ void BatchEngine::rebuild() {
   dirty_ = false;
 }
 
-void BatchEngine::iteration(hipStream_t q) {
+void BatchEngine::iteration(hipStream_t q, hipEvent_t ev_a0, hipEvent_t ev_a1) {
   const BFit *dF = fits_d_->as<const BFit>();
   const int nf = (int)host_fits_.size();
   const int nci = (int)ci_.size();
   if (nf == 0) return;
   hipLaunchKernelGGL(bt_hn_krn, dim3(nf), dim3(kFB), 0, q, dF, 0);
   KMD_LAUNCH_CHECK();
-  launch_assign_bucket<8>(dF, ai_d_[0]->as<AItem>(), (int)ai_[0].size(), 0, q);
-  launch_assign_bucket<16>(dF, ai_d_[1]->as<AItem>(), (int)ai_[1].size(), 0, q);
-  launch_assign_bucket<32>(dF, ai_d_[2]->as<AItem>(), (int)ai_[2].size(), 0, q);
+  if (ev_a0 != nullptr) KMD_HIP_CHECK(hipEventRecord(ev_a0, q));
+  launch_assign_bucket<8>(dF, ai_d_[0]->as<AItem>(), (int)ai_[0].size(), 0, q, mixed_,
+                          assign_shape_);
+  launch_assign_bucket<16>(dF, ai_d_[1]->as<AItem>(), (int)ai_[1].size(), 0, q, mixed_,
+                           assign_shape_);
+  launch_assign_bucket<32>(dF, ai_d_[2]->as<AItem>(), (int)ai_[2].size(), 0, q, mixed_,
+                           assign_shape_);
+  if (ev_a1 != nullptr) KMD_HIP_CHECK(hipEventRecord(ev_a1, q));
   hipLaunchKernelGGL(bt_zero_krn, dim3(nf), dim3(kFB), 0, q, dF);
   KMD_LAUNCH_CHECK();
   hipLaunchKernelGGL(bt_hist_krn, dim3(nci), dim3(256), 0, q, dF, ci_d_->as<CItem>());
@@ -1387,9 +1762,12 @@ void BatchEngine::epilogue(const std::vector<Slot *> &done) {
   const BFit *F = dF->as<const BFit>();
   hipLaunchKernelGGL(bt_hn_krn, dim3(nf), dim3(kFB), 0, s_, F, 1);
   KMD_LAUNCH_CHECK();
-  launch_assign_bucket<8>(F, dai[0]->as<AItem>(), (int)ai[0].size(), 1, s_);
-  launch_assign_bucket<16>(F, dai[1]->as<AItem>(), (int)ai[1].size(), 1, s_);
-  launch_assign_bucket<32>(F, dai[2]->as<AItem>(), (int)ai[2].size(), 1, s_);
+  launch_assign_bucket<8>(F, dai[0]->as<AItem>(), (int)ai[0].size(), 1, s_, mixed_,
+                          assign_shape_);
+  launch_assign_bucket<16>(F, dai[1]->as<AItem>(), (int)ai[1].size(), 1, s_, mixed_,
+                           assign_shape_);
+  launch_assign_bucket<32>(F, dai[2]->as<AItem>(), (int)ai[2].size(), 1, s_, mixed_,
+                           assign_shape_);
   hipLaunchKernelGGL(bt_inertia_krn, dim3((unsigned)ci.size()), dim3(256), 0, s_, F,
                      dci->as<CItem>());
   KMD_LAUNCH_CHECK();
@@ -1430,6 +1808,31 @@ int64_t BatchEngine::step(int iterations, bool use_graph) {
   check_inputs();
   if (dirty_) rebuild();
   if (host_fits_.empty()) return 0;
+  // the launch-shape trial: one timed iteration per shape (each a real
+  // iteration of every active fit; the shapes give the same results)
+  int trial = 0;
+  if (tune_shape_) {
+    hipEvent_t e0, e1;
+    KMD_HIP_CHECK(hipEventCreate(&e0));
+    KMD_HIP_CHECK(hipEventCreate(&e1));
+    for (int sh = 0; sh < 2 && trial < iterations; ++sh, ++trial) {
+      assign_shape_ = sh;
+      iteration(s_, e0, e1);
+      KMD_HIP_CHECK(hipEventSynchronize(e1));
+      KMD_HIP_CHECK(hipEventElapsedTime(&shape_ms_[sh], e0, e1));
+    }
+    KMD_HIP_CHECK(hipEventDestroy(e0));
+    KMD_HIP_CHECK(hipEventDestroy(e1));
+    if (trial == 2) {
+      tune_shape_ = false;
+      assign_shape_ = shape_ms_[1] < shape_ms_[0] ? 1 : 0;
+      drop_graph();
+    } else {
+      assign_shape_ = 0;
+    }
+    iterations -= trial;
+    launches_ += trial;
+  }
   if (use_graph && !exec_) {
     KMD_HIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
     iteration(s_);
@@ -1500,9 +1903,15 @@ std::vector<int64_t> batch_engine_take_finished(BatchEngine *e) {
 }
 int64_t batch_engine_resident_bytes(const BatchEngine *e) { return e->resident_bytes(); }
 int64_t batch_engine_launches(const BatchEngine *e) { return e->launches_; }
+void batch_engine_set_mixed(BatchEngine *e, bool mixed) {
+  if (e->mixed_ == mixed) return;
+  e->mixed_ = mixed;
+  e->drop_graph();    // the captured iteration launches the other kernels
+  e->dirty_ = true;
+}
 
 void batched_lloyd(const std::vector<BatchedFit> &in, int graph_replays,
-                   hipStream_t user_stream, BatchedStats *stats) {
+                   hipStream_t user_stream, BatchedStats *stats, bool mixed) {
   if (in.empty()) return;
   // one allocation for all fits' workspaces (declared before the engine, so
   // it is released after the engine has drained its stream)
@@ -1525,6 +1934,9 @@ void batched_lloyd(const std::vector<BatchedFit> &in, int graph_replays,
   int64_t replays = 0;
   int64_t in_set = (int64_t)in.size();
   eng.defer_epilogue_ = true;
+  eng.route_wide_ = true;
+  eng.mixed_ = mixed;
+  eng.tune_shape_ = true;
   // a fit with max_iter 0 (or converged) finishes at the first check
   for (;;) {
     const int64_t r = std::min<int64_t>(R, std::max<int64_t>(0, max_iter - replays));
@@ -1541,7 +1953,14 @@ void batched_lloyd(const std::vector<BatchedFit> &in, int graph_replays,
     }
   }
   eng.flush_stopped();
-  if (stats) stats->graph_replays = replays;
+  // fits whose data range needs floating-point sums: the single-fit driver
+  for (const BatchedFit &f : eng.wide_) driver_lloyd_single(f, user_stream);
+  if (stats) {
+    stats->graph_replays = replays;
+    stats->assign_shape = eng.assign_shape_;
+    stats->shape_ms[0] = eng.shape_ms_[0];
+    stats->shape_ms[1] = eng.shape_ms_[1];
+  }
   (void)hipStreamSynchronize(user_stream);
 }
 

@@ -89,6 +89,16 @@ inline void check_ptr(const void *p, int64_t count, const char *name) {
   if (count > 0 && p == nullptr)
     throw ValueError(std::string("null pointer for non-empty array ") + name);
 }
+// The five arrays of one Lloyd fit (driver, batched fits, engine, autotuner).
+inline void check_fit_arrays(int64_t n, int64_t d, int64_t k, const void *X_t,
+                             const void *w, const void *init,
+                             const void *labels, const void *res) {
+  check_ptr(X_t, n * d, "X_t");
+  check_ptr(w, n, "sample_weight");
+  check_ptr(init, d * k, "init_centroids_t");
+  check_ptr(labels, n, "assignments_id");
+  check_ptr(res, d * k, "res_centroids_t");
+}
 
 // Calls fn.template operator()<T>() for the runtime dtype.
 template <typename F>
@@ -246,6 +256,13 @@ int kmd_reduce_centroids_data(int dtype, int itype, int64_t n_copies,
     require(n_copies >= 0 && n_features >= 0 && n_clusters >= 0,
             "negative array dimension");
     (void)work_group_size;
+    check_ptr(cluster_sizes_private_copies, n_copies * n_clusters,
+              "cluster_sizes_private_copies");
+    check_ptr(centroids_t_private_copies, n_copies * n_features * n_clusters,
+              "centroids_t_private_copies");
+    check_ptr(out_cluster_sizes, n_clusters, "out_cluster_sizes");
+    check_ptr(out_centroids_t, n_features * n_clusters, "out_centroids_t");
+    check_ptr(out_empty_clusters_list, n_clusters, "out_empty_clusters_list");
     check_ptr(out_n_empty_clusters, 1, "out_n_empty_clusters");
     dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
       using T = std::remove_pointer_t<decltype(tp)>;
@@ -320,6 +337,15 @@ int kmd_relocate_empty_clusters(
     require(n_samples > 0 && n_features > 0 && n_clusters > 0,
             "Input dimensions are inconsistent");
     (void)work_group_size;
+    check_ptr(X_t, n_samples * n_features, "X_t");
+    check_ptr(sample_weights, n_samples, "sample_weights");
+    check_ptr(assignment_id, n_samples, "assignment_id");
+    check_ptr(empty_clusters_list, n_empty_clusters, "empty_clusters_list");
+    check_ptr(sq_dist_to_nearest_centroid, n_samples,
+              "sq_dist_to_nearest_centroid");
+    check_ptr(centroid_t, n_features * n_clusters, "centroid_t");
+    check_ptr(cluster_sizes, n_clusters, "cluster_sizes");
+    check_ptr(per_sample_inertia, n_samples, "per_sample_inertia");
     dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
       using T = std::remove_pointer_t<decltype(tp)>;
       using I = std::remove_pointer_t<decltype(ip)>;
@@ -343,6 +369,9 @@ int kmd_compute_centroid_shifts_squared(int dtype, int64_t n_features,
   return guarded([&] {
     check_dtype(dtype);
     require(n_features >= 0 && n_clusters >= 0, "negative array dimension");
+    check_ptr(old_centroid_t, n_features * n_clusters, "old_centroid_t");
+    check_ptr(new_centroid_t, n_features * n_clusters, "new_centroid_t");
+    check_ptr(out_centroid_shifts, n_clusters, "out_centroid_shifts");
     dispatch_t(dtype, [&](auto *tp) {
       using T = std::remove_pointer_t<decltype(tp)>;
       launch_centroid_shifts<T>(n_features, n_clusters,
@@ -363,6 +392,10 @@ int kmd_compute_centroid_to_sample_distances(
             "negative array dimension");
     require(work_group_size > 0 && centroids_window_height > 0,
             "work_group_size and centroids_window_height must be positive");
+    check_ptr(X_t, n_samples * n_features, "X_t");
+    check_ptr(centroid_t, n_features * n_clusters, "centroid_t");
+    check_ptr(euclidean_distances_t, n_clusters * n_samples,
+              "euclidean_distances_t");
     if (n_features == 0 || n_clusters == 0 || n_samples == 0) {
       if (n_features == 0 && n_clusters > 0 && n_samples > 0) {
         KMD_HIP_CHECK(hipMemsetAsync(euclidean_distances_t, 0,
@@ -395,6 +428,10 @@ int kmd_assignment(int dtype, int itype, int64_t n_samples, int64_t n_features,
             "work_group_size and centroids_window_height must be positive");
     require(n_clusters <= INT32_MAX && n_features <= INT32_MAX,
             "n_clusters and n_features must fit in 32 bits");
+    check_ptr(X_t, n_samples * n_features, "X_t");
+    check_ptr(centroids_t, n_features * n_clusters, "centroids_t");
+    check_ptr(centroids_half_l2_norm, n_clusters, "centroids_half_l2_norm");
+    check_ptr(assignment_id, n_samples, "assignment_id");
     dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
       using T = std::remove_pointer_t<decltype(tp)>;
       using I = std::remove_pointer_t<decltype(ip)>;
@@ -418,6 +455,10 @@ int kmd_compute_inertia(int dtype, int itype, int64_t n_samples,
     require(n_samples >= 0 && n_features >= 0 && n_clusters >= 0,
             "Array dimensions are not consistent");
     (void)work_group_size;
+    check_ptr(X_t, n_samples * n_features, "X_t");
+    check_ptr(centroid_t, n_features * n_clusters, "centroid_t");
+    check_ptr(assignment_id, n_samples, "assignment_id");
+    check_ptr(out_per_sample_inertia, n_samples, "out_per_sample_inertia");
     dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
       using T = std::remove_pointer_t<decltype(tp)>;
       using I = std::remove_pointer_t<decltype(ip)>;
@@ -435,6 +476,7 @@ int kmd_reduce_vector_blocking(int dtype, int64_t n, const void *data,
     check_dtype(dtype);
     require(n >= 0, "negative size");
     require(result != nullptr, "result must not be null");
+    check_ptr(data, n, "data");
     hipStream_t s = as_stream(stream);
     TempBuffer dev(sizeof(double), s);
     dispatch_t(dtype, [&](auto *tp) {
@@ -469,6 +511,15 @@ int kmd_fused_lloyd_single_step(
             "work_group_size and centroids_window_height must be positive");
     require(n_clusters <= INT32_MAX && n_features <= INT32_MAX,
             "n_clusters and n_features must fit in 32 bits");
+    check_ptr(X_t, n_samples * n_features, "X_t");
+    check_ptr(sample_weight, n_samples, "sample_weight");
+    check_ptr(centroids_t, n_features * n_clusters, "centroids_t");
+    check_ptr(centroids_half_l2_norm, n_clusters, "centroids_half_l2_norm");
+    check_ptr(assignments_idx, n_samples, "assignments_idx");
+    check_ptr(new_centroids_t_private_copies, n_copies * n_features * n_clusters,
+              "new_centroids_t_private_copies");
+    check_ptr(cluster_sizes_private_copies, n_copies * n_clusters,
+              "cluster_sizes_private_copies");
     dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
       using T = std::remove_pointer_t<decltype(tp)>;
       using I = std::remove_pointer_t<decltype(ip)>;
@@ -532,6 +583,8 @@ int kmd_kmeans_lloyd_driver(
             "n_clusters and n_features must fit in 32 bits");
     require(n_iters != nullptr && total_inertia != nullptr,
             "output scalars must not be null");
+    check_fit_arrays(n_samples, n_features, n_clusters, X_t, sample_weight,
+                     init_centroids_t, assignments_id, res_centroids_t);
     dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
       using T = std::remove_pointer_t<decltype(tp)>;
       using I = std::remove_pointer_t<decltype(ip)>;
@@ -550,6 +603,8 @@ int kmd_iterative_merge_sort(int dtype, int64_t n, const void *input,
   return guarded([&] {
     check_dtype(dtype);
     require(n >= 0, "negative size");
+    check_ptr(input, n, "input");
+    check_ptr(output, n, "output");
     dispatch_t(dtype, [&](auto *tp) {
       using T = std::remove_pointer_t<decltype(tp)>;
       launch_merge_sort<T>(n, (const T *)input, (T *)output, as_stream(stream));
@@ -578,6 +633,8 @@ int kmd_kmeans_lloyd_driver_ex(
             "n_clusters and n_features must fit in 32 bits");
     require(n_iters != nullptr && total_inertia != nullptr,
             "output scalars must not be null");
+    check_fit_arrays(n_samples, n_features, n_clusters, X_t, sample_weight,
+                     init_centroids_t, assignments_id, res_centroids_t);
     DriverStats st;
     dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
       using T = std::remove_pointer_t<decltype(tp)>;
@@ -598,12 +655,13 @@ int kmd_kmeans_lloyd_driver_ex(
 
 int kmd_kmeans_lloyd_batched(int64_t n_fits, kmd_fit *fits, int precision,
                              int graph_replays, int64_t *graph_launches,
+                             int *assign_shape,
                              kmd_stream_t stream) {
   return guarded([&] {
     require(n_fits >= 0, "n_fits must be non-negative");
     require(n_fits == 0 || fits != nullptr, "fits must not be null");
-    require(precision == KMD_PREC_FP32,
-            "batched fits support precision KMD_PREC_FP32 only");
+    require(precision == KMD_PREC_FP32 || precision == KMD_PREC_BF16,
+            "batched fits support precision KMD_PREC_FP32 or KMD_PREC_BF16");
     std::vector<BatchedFit> batch;
     std::vector<int64_t> idx;
     for (int64_t i = 0; i < n_fits; ++i) {
@@ -612,6 +670,9 @@ int kmd_kmeans_lloyd_batched(int64_t n_fits, kmd_fit *fits, int precision,
               "Array dimensions are not consistent");
       require(f.tol >= 0.0, "Tolerance must be non-negative");
       require(f.max_iter >= 0, "max_iter must be non-negative");
+      check_fit_arrays(f.n_samples, f.n_features, f.n_clusters, f.X_t,
+                       f.sample_weight, f.init_centroids_t, f.assignments_id,
+                       f.res_centroids_t);
       if (batched_fit_supported(f.n_samples, f.n_features, f.n_clusters)) {
         BatchedFit b;
         b.n = f.n_samples;
@@ -634,7 +695,8 @@ int kmd_kmeans_lloyd_batched(int64_t n_fits, kmd_fit *fits, int precision,
     BatchedStats st;
     const hipStream_t s = as_stream(stream);
     if (!batch.empty())
-      batched_lloyd(batch, graph_replays > 0 ? graph_replays : 4, s, &st);
+      batched_lloyd(batch, graph_replays > 0 ? graph_replays : 4, s, &st,
+                    precision == KMD_PREC_BF16);
     for (int64_t i : idx) {
       kmd_fit &f = fits[i];
       driver_lloyd<float, int32_t>(
@@ -645,6 +707,7 @@ int kmd_kmeans_lloyd_batched(int64_t n_fits, kmd_fit *fits, int precision,
           nullptr);
     }
     if (graph_launches) *graph_launches = st.graph_replays;
+    if (assign_shape) *assign_shape = st.assign_shape;
   });
 }
 
@@ -664,6 +727,15 @@ int kmd_engine_create(kmd_engine_t *engine, kmd_stream_t stream) {
       throw;
     }
     *engine = e;
+  });
+}
+
+int kmd_engine_set_precision(kmd_engine_t engine, int precision) {
+  return guarded([&] {
+    require(engine != nullptr, "engine must not be null");
+    require(precision == KMD_PREC_FP32 || precision == KMD_PREC_BF16,
+            "engine precision must be KMD_PREC_FP32 or KMD_PREC_BF16");
+    batch_engine_set_mixed(engine->eng, precision == KMD_PREC_BF16);
   });
 }
 
@@ -694,6 +766,9 @@ int kmd_engine_add(kmd_engine_t engine, kmd_fit *fit, int64_t *id) {
     require(batched_fit_supported(fit->n_samples, fit->n_features, fit->n_clusters),
             "engine fits need 1 <= n_features <= 64 and 1 <= n_clusters <= "
             "min(256, n_samples)");
+    check_fit_arrays(fit->n_samples, fit->n_features, fit->n_clusters, fit->X_t,
+                     fit->sample_weight, fit->init_centroids_t,
+                     fit->assignments_id, fit->res_centroids_t);
     BatchedFit b;
     b.n = fit->n_samples;
     b.d = fit->n_features;
@@ -747,9 +822,9 @@ int kmd_autotune(int dtype, int itype, int precision, int64_t n_samples,
                  int64_t n_features, int64_t n_clusters, const void *X_t,
                  const void *sample_weight, const void *init_centroids_t,
                  int iters, int *best_path, int *best_blocks_per_cu,
-                 double *best_ms, int *cand_path, int *cand_blocks_per_cu,
-                 double *cand_ms, int64_t capacity, int64_t *n_candidates,
-                 kmd_stream_t stream) {
+                 int *best_shape, double *best_ms, int *cand_path,
+                 int *cand_blocks_per_cu, int *cand_shape, double *cand_ms,
+                 int64_t capacity, int64_t *n_candidates, kmd_stream_t stream) {
   return guarded([&] {
     check_types(dtype, itype);
     require(precision == KMD_PREC_FP32 || precision == KMD_PREC_BF16X3 ||
@@ -759,6 +834,9 @@ int kmd_autotune(int dtype, int itype, int precision, int64_t n_samples,
             "Array dimensions are not consistent");
     require(iters >= 1, "iters must be positive");
     require(capacity >= 0, "capacity must be non-negative");
+    check_ptr(X_t, n_samples * n_features, "X_t");
+    check_ptr(sample_weight, n_samples, "sample_weight");
+    check_ptr(init_centroids_t, n_features * n_clusters, "init_centroids_t");
     std::vector<TuneResult> res;
     dispatch_ti(dtype, itype, [&](auto *tp, auto *ip) {
       using T = std::remove_pointer_t<decltype(tp)>;
@@ -773,11 +851,13 @@ int kmd_autotune(int dtype, int itype, int precision, int64_t n_samples,
       if (r.ms > 0.0 && (best->ms <= 0.0 || r.ms < best->ms)) best = &r;
     if (best_path) *best_path = best ? best->path : 0;
     if (best_blocks_per_cu) *best_blocks_per_cu = best ? best->blocks_per_cu : 0;
+    if (best_shape) *best_shape = best ? best->shape : 0;
     if (best_ms) *best_ms = best ? best->ms : 0.0;
     const int64_t m = std::min<int64_t>(capacity, (int64_t)res.size());
     for (int64_t i = 0; i < m; ++i) {
       if (cand_path) cand_path[i] = res[i].path;
       if (cand_blocks_per_cu) cand_blocks_per_cu[i] = res[i].blocks_per_cu;
+      if (cand_shape) cand_shape[i] = res[i].shape;
       if (cand_ms) cand_ms[i] = res[i].ms;
     }
     if (n_candidates) *n_candidates = (int64_t)res.size();
@@ -875,10 +955,11 @@ int kmd_kmeans_plusplus(int dtype, int64_t n_samples, int64_t n_features,
   });
 }
 
-int kmd_set_tuning(int path, int blocks_per_cu) {
+int kmd_set_tuning(int path, int blocks_per_cu, int shape) {
   return guarded([&] {
     require(path >= -1 && path <= 2, "path must be -1, 0, 1 or 2");
-    set_tuning_override(path, blocks_per_cu);
+    require(shape >= 0 && shape <= 2, "shape must be 0, 1 or 2");
+    set_tuning_override(path, blocks_per_cu, shape);
   });
 }
 

@@ -62,6 +62,9 @@ int64_t device_l2_bytes(int device);
 // autotuner (driver.hip) or kmd_set_tuning; read when a launch is planned.
 int tuned_blocks_per_cu();
 int tuned_path();
+// Launch shape of the LDS-resident fixed-point step (0 = 16 waves x one
+// 32-sample column block per wave, 1 = 12 waves x two, 2 = 8 waves x two).
+int tuned_shape();
 inline int apply_blocks_cap(int per_cu) {
   const int cap = tuned_blocks_per_cu();
   return (cap > 0 && per_cu > cap) ? cap : per_cu;
@@ -223,10 +226,16 @@ void launch_assignment(int64_t n, int64_t d, int64_t k, int64_t h,
 template <typename T>
 void launch_distances(int64_t n, int64_t d, int64_t k, int64_t h,
                       const T *X, const T *C, T *D, hipStream_t s);
+// The reference's fused step (accumulate-into private copies).  scratch:
+// fused_step_scratch_bytes<T>(d, k) bytes of device memory for the launch's
+// working data (scales or centroid tiles), or NULL to allocate per call.
+template <typename T>
+size_t fused_step_scratch_bytes(int64_t d, int64_t k);
 template <typename T, typename I>
 void launch_fused_step_copies(int64_t n, int64_t d, int64_t k,
                               int64_t n_copies, int64_t h, const T *X,
                               const T *w, const T *C, const T *hn, I *labels,
-                              T *copies, T *sizes_copies, hipStream_t s);
+                              T *copies, T *sizes_copies, hipStream_t s,
+                              void *scratch = nullptr);
 
 }  // namespace kmd

@@ -187,18 +187,22 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_tiled_workspace(
 namespace {
 thread_local int t_blocks_cap = 0;
 thread_local int t_path = 0;
+thread_local int t_shape = 0;
 struct TuneChoice {
-  int path = 0, blocks_per_cu = 0;
+  int path = 0, blocks_per_cu = 0, shape = 0;
 };
 struct TuneGuard {
-  int pb, pp;
-  explicit TuneGuard(const TuneChoice &c) : pb(t_blocks_cap), pp(t_path) {
+  int pb, pp, ps;
+  explicit TuneGuard(const TuneChoice &c)
+      : pb(t_blocks_cap), pp(t_path), ps(t_shape) {
     t_blocks_cap = c.blocks_per_cu;
     t_path = c.path;
+    t_shape = c.shape;
   }
   ~TuneGuard() {
     t_blocks_cap = pb;
     t_path = pp;
+    t_shape = ps;
   }
 };
 std::mutex g_tune_mu;
@@ -222,12 +226,14 @@ TuneChoice choose_tuning(int64_t n, int64_t d, int64_t k, int precision) {
 
 int tuned_blocks_per_cu() { return t_blocks_cap; }
 int tuned_path() { return t_path; }
+int tuned_shape() { return t_shape; }
 
-void set_tuning_override(int path, int blocks_per_cu) {
+void set_tuning_override(int path, int blocks_per_cu, int shape) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_override = path >= 0;
   g_override_choice.path = path < 0 ? 0 : path;
   g_override_choice.blocks_per_cu = blocks_per_cu < 0 ? 0 : blocks_per_cu;
+  g_override_choice.shape = shape < 0 ? 0 : shape;
 }
 
 // Test hook: KMD_FORCE_TILED=1 selects the tiled path for every shape it
@@ -518,11 +524,23 @@ std::vector<TuneResult> autotune_lloyd(int64_t n, int64_t d, int64_t k,
   if (paths.empty()) paths.push_back(0);
   TempBuffer c0(sizeof(T) * d * k, s), c1(sizeof(T) * d * k, s),
       lab(sizeof(I) * n, s);
+  struct Cand {
+    int path, cap, shape;
+  };
+  std::vector<Cand> cands;
   for (int path : paths) {
-    for (int cap : {0, 1, 2, 4}) {
+    // launch shapes (waves per block x column blocks per wave) of the
+    // LDS-resident fixed-point step, then blocks per CU
+    const int n_shapes = (path == 1 && std::is_same<T, float>::value) ? 3 : 1;
+    for (int shape = 0; shape < n_shapes; ++shape)
+      for (int cap : {0, 1, 2, 4}) cands.push_back(Cand{path, cap, shape});
+  }
+  for (const Cand &cd : cands) {
+    {
+      const int path = cd.path, cap = cd.cap, shape = cd.shape;
       KMD_HIP_CHECK(hipMemcpyAsync(c0.as<T>(), init, sizeof(T) * d * k,
                                    hipMemcpyDeviceToDevice, s));
-      TuneGuard g(TuneChoice{path, cap});
+      TuneGuard g(TuneChoice{path, cap, shape});
       DriverStats st;
       int64_t it = 0;
       double inertia = 0.0;
@@ -532,6 +550,7 @@ std::vector<TuneResult> autotune_lloyd(int64_t n, int64_t d, int64_t k,
       TuneResult r;
       r.path = path;
       r.blocks_per_cu = cap;
+      r.shape = shape;
       r.ms = st.fused_launches > 0 ? st.fused_kernel_ms / st.fused_launches : 0.0;
       out.push_back(r);
     }
@@ -542,7 +561,7 @@ std::vector<TuneResult> autotune_lloyd(int64_t n, int64_t d, int64_t k,
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[std::make_tuple(d, k, precision, log2_bucket(n))] =
-        TuneChoice{best->path, best->blocks_per_cu};
+        TuneChoice{best->path, best->blocks_per_cu, best->shape};
   }
   KMD_HIP_CHECK(hipStreamSynchronize(s));
   return out;
@@ -759,5 +778,12 @@ KMD_INST_DRIVER(float, int32_t)
 KMD_INST_DRIVER(float, int64_t)
 KMD_INST_DRIVER(double, int32_t)
 KMD_INST_DRIVER(double, int64_t)
+
+void driver_lloyd_single(const BatchedFit &f, hipStream_t s) {
+  driver_lloyd<float, int32_t>(f.n, f.d, f.k, f.X, f.w, f.init, f.labels, f.res,
+                               f.tol, false, f.max_iter, f.out_iters,
+                               f.out_inertia, nullptr, nullptr, s, KMD_PREC_FP32,
+                               nullptr);
+}
 
 }  // namespace kmd

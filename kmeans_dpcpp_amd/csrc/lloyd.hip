@@ -179,17 +179,35 @@ void launch_fused_generic(int64_t n, int64_t d, int64_t k, int64_t n_copies,
   KMD_LAUNCH_CHECK();
 }
 
+template <typename T>
+size_t fused_step_scratch_bytes(int64_t d, int64_t k) {
+  if (fast_fused_supported<T>(d, k)) return fast_fused_scratch_bytes(d);
+  if (std::is_same<T, float>::value && tiled_supported(d, k))
+    return tiled_tiles_bytes(d, k);
+  return 0;  // the generic kernel works in LDS and the caller's copies only
+}
+
 template <typename T, typename I>
 void launch_fused_step_copies(int64_t n, int64_t d, int64_t k,
                               int64_t n_copies, int64_t h, const T *X,
                               const T *w, const T *C, const T *hn, I *labels,
-                              T *copies, T *sizes_copies, hipStream_t s) {
+                              T *copies, T *sizes_copies, hipStream_t s,
+                              void *scratch) {
   (void)h;
   if (n == 0) return;
   if (fast_fused_supported<T>(d, k)) {
     launch_fast_fused<T, I>(n, d, k, n_copies, /*store_flush=*/false, X, w, C,
-                            hn, labels, copies, sizes_copies, s);
+                            hn, labels, copies, sizes_copies, s, scratch);
     return;
+  }
+  if constexpr (std::is_same<T, float>::value) {
+    // centroids or sums beyond LDS (d <= 128): MFMA assignment with centroid
+    // tiles streamed through LDS, then the update from those labels
+    if (tiled_supported(d, k)) {
+      launch_tiled_assign<I>(n, d, k, X, C, hn, labels, s, nullptr, scratch);
+      launch_label_update<I>(n, d, k, n_copies, X, w, labels, copies, sizes_copies, s);
+      return;
+    }
   }
   FusedPlan p = plan_generic_fused<T>(n, d, k);
   launch_fused_generic<T, I>(n, d, k, n_copies, p, false, X, w, C, hn, labels,
@@ -240,7 +258,7 @@ void launch_distances(int64_t n, int64_t d, int64_t k, int64_t h, const T *X,
       const T *, const T *, const T *, I *, T *, T *, hipStream_t);            \
   template void launch_fused_step_copies<T, I>(                                \
       int64_t, int64_t, int64_t, int64_t, int64_t, const T *, const T *,       \
-      const T *, const T *, I *, T *, T *, hipStream_t);                       \
+      const T *, const T *, I *, T *, T *, hipStream_t, void *);               \
   template void launch_assignment<T, I>(int64_t, int64_t, int64_t, int64_t,   \
                                         const T *, const T *, const T *, I *,  \
                                         hipStream_t);
@@ -248,6 +266,8 @@ KMD_INST_LLOYD(float, int32_t)
 KMD_INST_LLOYD(float, int64_t)
 KMD_INST_LLOYD(double, int32_t)
 KMD_INST_LLOYD(double, int64_t)
+template size_t fused_step_scratch_bytes<float>(int64_t, int64_t);
+template size_t fused_step_scratch_bytes<double>(int64_t, int64_t);
 template FusedPlan plan_generic_fused<float>(int64_t, int64_t, int64_t);
 template FusedPlan plan_generic_fused<double>(int64_t, int64_t, int64_t);
 template void launch_distances<float>(int64_t, int64_t, int64_t, int64_t,

@@ -442,49 +442,79 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
 
 // ---- fixed-point scales ------------------------------------------------------
 
-// maxbits[f] = max over samples of the bit pattern of |X[f, s]| (for
-// non-negative floats the bit pattern orders like the value; any NaN is
-// above +inf); maxbits[d] the same for the weights.
+// The range statistics of fixed_point.hpp for row f = blockIdx.y of X (f ==
+// d: the weights): maximum bit pattern of |value| (for non-negative floats
+// the bit pattern orders like the value; any NaN is above +inf), exponent
+// sum and count of the normal values, and for the weights the complement of
+// the smallest non-zero |w|.
 __global__ void __launch_bounds__(256)
     absmax_bits_krn(int64_t n, int d, const float *__restrict__ X,
-                    const float *__restrict__ w, unsigned *maxbits) {
+                    const float *__restrict__ w, unsigned *maxbits,
+                    unsigned long long *estat) {
   const int f = blockIdx.y;  // 0..d (d = weights)
   const float *row = (f < d) ? X + (int64_t)f * n : w;
-  unsigned m = 0u;
+  const bool weights = f == d;
+  unsigned m = 0u, esum = 0u, ecnt = 0u, wmin_inv = 0u;
+  auto acc = [&](unsigned v) {
+    range_accumulate(v, m, esum, ecnt);
+    const unsigned b = v & 0x7fffffffu;
+    if (weights && b != 0u) wmin_inv = max(wmin_inv, ~b);
+  };
   if ((reinterpret_cast<uintptr_t>(row) & 15u) == 0) {
-    // 16-byte loads, two in flight per thread; the tail element by element
+    // 16-byte loads, four in flight per thread; the tail element by element
     const int64_t n4 = n / 4;
     const uint4 *r4 = reinterpret_cast<const uint4 *>(row);
     const int64_t stride = (int64_t)gridDim.x * 256;
     int64_t i = blockIdx.x * 256ll + threadIdx.x;
-    for (; i + stride < n4; i += 2 * stride) {
-      const uint4 a = r4[i], b = r4[i + stride];
-      m = max(m, max(max(a.x & 0x7fffffffu, a.y & 0x7fffffffu),
-                     max(a.z & 0x7fffffffu, a.w & 0x7fffffffu)));
-      m = max(m, max(max(b.x & 0x7fffffffu, b.y & 0x7fffffffu),
-                     max(b.z & 0x7fffffffu, b.w & 0x7fffffffu)));
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+      const uint4 a = r4[i], b = r4[i + stride], c = r4[i + 2 * stride],
+                  e = r4[i + 3 * stride];
+      acc(a.x), acc(a.y), acc(a.z), acc(a.w);
+      acc(b.x), acc(b.y), acc(b.z), acc(b.w);
+      acc(c.x), acc(c.y), acc(c.z), acc(c.w);
+      acc(e.x), acc(e.y), acc(e.z), acc(e.w);
     }
     for (; i < n4; i += stride) {
       const uint4 a = r4[i];
-      m = max(m, max(max(a.x & 0x7fffffffu, a.y & 0x7fffffffu),
-                     max(a.z & 0x7fffffffu, a.w & 0x7fffffffu)));
+      acc(a.x), acc(a.y), acc(a.z), acc(a.w);
     }
     for (int64_t j = 4 * n4 + blockIdx.x * 256ll + threadIdx.x; j < n; j += stride)
-      m = max(m, __float_as_uint(row[j]) & 0x7fffffffu);
+      acc(__float_as_uint(row[j]));
   } else {
     for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * 256)
-      m = max(m, __float_as_uint(row[i]) & 0x7fffffffu);
+      acc(__float_as_uint(row[i]));
   }
-  for (int o = kWave / 2; o > 0; o >>= 1)
+  unsigned long long es = esum, ec = ecnt;
+  for (int o = kWave / 2; o > 0; o >>= 1) {
     m = max(m, (unsigned)__shfl_xor((int)m, o, kWave));
-  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(&maxbits[f], m);
+    wmin_inv = max(wmin_inv, (unsigned)__shfl_xor((int)wmin_inv, o, kWave));
+    es += __shfl_xor(es, o, kWave);
+    ec += __shfl_xor(ec, o, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicMax(&maxbits[f], m);
+    if (weights && wmin_inv != 0u) atomicMax(&maxbits[d + 1], wmin_inv);
+    if (ec != 0) {
+      atomicAdd(&estat[2 * f], es);
+      atomicAdd(&estat[2 * f + 1], ec);
+    }
+  }
 }
 
-// Scales from the maxima (fixed_point.hpp), one thread.
+// words[0 : count] = 0 (a kernel rather than a memset node, so that callers
+// may capture the launches in a graph)
+__global__ void zero_words_krn(unsigned *words, int64_t count) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * 256)
+    words[i] = 0u;
+}
+
+// Scales and flag from the statistics (fixed_point.hpp), one thread.
 __global__ void scales_krn(int64_t n, int d, const unsigned *maxbits,
-                           float *scale, int *nonfinite) {
-  fixed_point_scales_from_bits(n, d, maxbits, scale, nonfinite);
+                           const unsigned long long *estat, float *scale,
+                           int *flag) {
+  fixed_point_scales_from_bits(n, d, maxbits, estat, scale, flag);
 }
 
 // ---- launch planning -----------------------------------------------------
@@ -520,14 +550,14 @@ inline int sum_copies(int64_t d, int64_t k) {
 
 using KernFn = void (*)(FastArgs);
 
-template <int NT, typename I, bool UPDATE, bool STORE>
+template <int NT, typename I, bool UPDATE, bool STORE, int S = kS>
 KernFn pick_nt(int64_t d) {
   const int ks = ks_bucket(d);
   const bool full = (d == 2 * ks);
 #define KMD_KS_CASE(KS)                                                   \
   case KS:                                                                \
-    return full ? &mfma_lloyd_krn<NT, KS, kS, true, I, UPDATE, STORE>     \
-                : &mfma_lloyd_krn<NT, KS, kS, false, I, UPDATE, STORE>;
+    return full ? &mfma_lloyd_krn<NT, KS, S, true, I, UPDATE, STORE>      \
+                : &mfma_lloyd_krn<NT, KS, S, false, I, UPDATE, STORE>;
   switch (ks) {
     KMD_KS_CASE(2)
     KMD_KS_CASE(4)
@@ -546,15 +576,36 @@ KernFn pick_nt(int64_t d) {
 // centroids and of the per-block sums in LDS.
 constexpr int kThreads = 1024;
 
+// Launch shapes of the fixed-point fused step that the autotuner chooses
+// from (tuned_shape()): threads per block x 32-sample column blocks per wave.
+// Every shape computes the same per-pair arithmetic, so the same labels and
+// sums; they differ in how many waves share a block's LDS and how many
+// samples each wave keeps in registers.
+struct FastShape {
+  int threads, s;
+};
+constexpr FastShape kFastShapes[] = {{1024, 1}, {768, 2}, {512, 2}};
+constexpr int kNumFastShapes = 3;
+
 // A resolved launch: kernel variant, block size, dynamic LDS and the
 // persistent grid (occupancy-limited, never more blocks than work).
 struct MfmaLaunch {
   KernFn fn = nullptr;
   int threads = kThreads;
+  int s = kS;      // 32-sample column blocks per wave
   int blocks = 0;
   size_t lds = 0;
   int copies = 1;  // copies of the LDS sums (UPDATE)
 };
+
+template <typename I, bool UPDATE, bool STORE>
+KernFn pick_shape(int64_t d, int shape) {
+  if constexpr (UPDATE && STORE) {
+    if (shape == 1) return pick_nt<768, I, true, true, 2>(d);
+    if (shape == 2) return pick_nt<512, I, true, true, 2>(d);
+  }
+  return pick_nt<kThreads, I, UPDATE, STORE>(d);
+}
 
 template <typename I, bool UPDATE, bool STORE>
 MfmaLaunch plan_launch(int64_t n, int64_t d, int64_t k) {
@@ -563,8 +614,11 @@ MfmaLaunch plan_launch(int64_t n, int64_t d, int64_t k) {
   L.copies = UPDATE ? sum_copies(d, k) : 1;
   L.lds = lds_bytes(d, k, UPDATE, L.copies);
   if (L.lds > kLdsPerCu) return L;
-  L.threads = kThreads;
-  L.fn = pick_nt<kThreads, I, UPDATE, STORE>(d);
+  int shape = (UPDATE && STORE) ? tuned_shape() : 0;
+  if (shape < 0 || shape >= kNumFastShapes) shape = 0;
+  L.threads = kFastShapes[shape].threads;
+  L.s = kFastShapes[shape].s;
+  L.fn = pick_shape<I, UPDATE, STORE>(d, shape);
   if (L.fn == nullptr) return L;
   static thread_local KernFn attr_done[128] = {};
   bool seen = false;
@@ -588,7 +642,7 @@ MfmaLaunch plan_launch(int64_t n, int64_t d, int64_t k) {
     return L;
   }
   const int64_t waves_per_block = L.threads / kWave;
-  const int64_t n_super = ceil_div<int64_t>(n > 0 ? n : 1, 32 * kS);
+  const int64_t n_super = ceil_div<int64_t>(n > 0 ? n : 1, 32 * L.s);
   int64_t blocks = (int64_t)device_cu_count() * per_cu;
   const int64_t need = ceil_div<int64_t>(n_super, waves_per_block);
   if (blocks > need) blocks = need;
@@ -602,8 +656,7 @@ inline void run(const MfmaLaunch &L, const FastArgs &a, hipStream_t s) {
   KMD_LAUNCH_CHECK();
 }
 
-inline bool vec_ok(const float *X, int64_t n) {
-  const int64_t S = kS;
+inline bool vec_ok(const float *X, int64_t n, int64_t S = kS) {
   return (n % S == 0) && ((reinterpret_cast<uintptr_t>(X) & (4 * S - 1)) == 0);
 }
 
@@ -626,22 +679,35 @@ inline FastArgs make_args(int64_t n, int64_t d, int64_t k, const float *X,
 
 }  // namespace
 
+size_t fixed_point_stats_bytes(int64_t d) {
+  const size_t est_off = (sizeof(unsigned) * range_maxbits_count(d) + 15) & ~size_t(15);
+  return est_off + sizeof(unsigned long long) * range_estat_count(d);
+}
+
 void launch_fixed_point_scales(int64_t n, int64_t d, const float *X,
                                const float *w, float *scale, int *nonfinite,
-                               hipStream_t s) {
-  TempBuffer bits(sizeof(unsigned) * (d + 1), s);
-  KMD_HIP_CHECK(hipMemsetAsync(bits.as<unsigned>(), 0, bits.bytes(), s));
+                               hipStream_t s, void *stats_scratch) {
+  const size_t est_off = (sizeof(unsigned) * range_maxbits_count(d) + 15) & ~size_t(15);
+  const size_t bytes = fixed_point_stats_bytes(d);
+  TempBuffer own(stats_scratch ? 0 : bytes, s);
+  char *stats = stats_scratch ? static_cast<char *>(stats_scratch) : own.as<char>();
+  hipLaunchKernelGGL(zero_words_krn, dim3((unsigned)ceil_div<int64_t>(bytes / 4, 256)),
+                     dim3(256), 0, s, reinterpret_cast<unsigned *>(stats),
+                     (int64_t)(bytes / 4));
+  KMD_LAUNCH_CHECK();
+  unsigned *maxbits = reinterpret_cast<unsigned *>(stats);
+  unsigned long long *estat = reinterpret_cast<unsigned long long *>(stats + est_off);
   int gx = (int)ceil_div<int64_t>(n > 0 ? n : 1, 256 * 32);
   const int cap = 8192 / (int)(d + 1) + 1;
   if (gx > cap) gx = cap;
   if (gx < 1) gx = 1;
   if (n > 0) {
     hipLaunchKernelGGL(absmax_bits_krn, dim3(gx, (unsigned)(d + 1)), dim3(256),
-                       0, s, n, (int)d, X, w, bits.as<unsigned>());
+                       0, s, n, (int)d, X, w, maxbits, estat);
     KMD_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(scales_krn, dim3(1), dim3(1), 0, s, n, (int)d,
-                     bits.as<const unsigned>(), scale, nonfinite);
+  hipLaunchKernelGGL(scales_krn, dim3(1), dim3(1), 0, s, n, (int)d, maxbits, estat,
+                     scale, nonfinite);
   KMD_LAUNCH_CHECK();
 }
 
@@ -672,6 +738,7 @@ void launch_fast_fused_fixed(int64_t n, int64_t d, int64_t k, const T *X,
     if (n_partials < L.blocks)
       throw RuntimeError("fused step needs one partial slot per block");
     FastArgs a = make_args(n, d, k, X, w, C, hn, labels);
+    a.vec = vec_ok(X, n, L.s) ? 1 : 0;
     a.scale = scale;
     a.nonfinite = nonfinite;
     a.copies = partials;
@@ -685,23 +752,31 @@ void launch_fast_fused_fixed(int64_t n, int64_t d, int64_t k, const T *X,
   }
 }
 
+size_t fast_fused_scratch_bytes(int64_t d) {
+  return ((sizeof(float) * (d + 1) + sizeof(int) + 255) & ~size_t(255)) +
+         fixed_point_stats_bytes(d);
+}
+
 template <typename T, typename I>
 void launch_fast_fused(int64_t n, int64_t d, int64_t k, int64_t n_copies,
                        bool store_flush, const T *X, const T *w, const T *C,
                        const T *hn, I *labels, T *copies, T *sizes_copies,
-                       hipStream_t s) {
+                       hipStream_t s, void *scratch) {
   if constexpr (std::is_same<T, float>::value) {
     if (n <= 0) return;
     if (store_flush)
       throw RuntimeError("store-flush partials go through the fixed-point path");
     const MfmaLaunch L = plan_launch<I, true, false>(n, d, k);
     if (L.fn == nullptr) throw RuntimeError("MFMA fused step: no launch plan");
-    TempBuffer sc(sizeof(float) * (d + 1) + 16, s);
-    int *flag =
-        reinterpret_cast<int *>(sc.as<char>() + sizeof(float) * (d + 1));
-    launch_fixed_point_scales(n, d, X, w, sc.as<float>(), flag, s);
+    // scales (d + 1 floats) and flag, then the statistics of the pre-pass
+    const size_t sc_bytes = (sizeof(float) * (d + 1) + sizeof(int) + 255) & ~size_t(255);
+    TempBuffer own(scratch ? 0 : fast_fused_scratch_bytes(d), s);
+    char *base = scratch ? static_cast<char *>(scratch) : own.as<char>();
+    float *scale = reinterpret_cast<float *>(base);
+    int *flag = reinterpret_cast<int *>(base + sizeof(float) * (d + 1));
+    launch_fixed_point_scales(n, d, X, w, scale, flag, s, base + sc_bytes);
     FastArgs a = make_args(n, d, k, X, w, C, hn, labels);
-    a.scale = sc.as<const float>();
+    a.scale = scale;
     a.nonfinite = flag;
     a.copies = copies;
     a.sizes_copies = sizes_copies;
@@ -834,7 +909,7 @@ template bool fast_fused_supported<double>(int64_t, int64_t);
   template void launch_fast_fused<T, I>(int64_t, int64_t, int64_t, int64_t,   \
                                         bool, const T *, const T *,           \
                                         const T *, const T *, I *, T *, T *,  \
-                                        hipStream_t);                         \
+                                        hipStream_t, void *);                 \
   template void launch_fast_fused_fixed<T, I>(                                \
       int64_t, int64_t, int64_t, const T *, const T *, const T *, const T *,  \
       I *, const float *, const int *, void *, int64_t, hipStream_t,         \

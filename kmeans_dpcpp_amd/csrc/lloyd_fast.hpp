@@ -18,6 +18,7 @@ struct DriverStats {
 // persistent blocks per CU (0 = occupancy limit), fused-step ms per launch.
 struct TuneResult {
   int path = 0, blocks_per_cu = 0;
+  int shape = 0;   // launch shape of the LDS-resident step (tuned_shape())
   double ms = 0.0;
 };
 template <typename T, typename I>
@@ -25,7 +26,7 @@ std::vector<TuneResult> autotune_lloyd(int64_t n, int64_t d, int64_t k,
                                        const T *X, const T *w, const T *init,
                                        int precision, int iters, hipStream_t s);
 // Process-wide override of the tuning choice (path < 0 clears it).
-void set_tuning_override(int path, int blocks_per_cu);
+void set_tuning_override(int path, int blocks_per_cu, int shape = 0);
 
 // Launch plan of a persistent fused-step kernel: grid size, dynamic LDS and
 // whether centroid sums are privatised in LDS.
@@ -53,11 +54,14 @@ bool fast_fused_supported(int64_t d, int64_t k);
 template <typename I>
 int fast_fused_blocks(int64_t n, int64_t d, int64_t k);
 
+// scratch (fast_fused_scratch_bytes(d) bytes, or NULL: allocated per call)
+// holds the fixed-point scales and the statistics of their pre-pass.
+size_t fast_fused_scratch_bytes(int64_t d);
 template <typename T, typename I>
 void launch_fast_fused(int64_t n, int64_t d, int64_t k, int64_t n_copies,
                        bool store_flush, const T *X, const T *w, const T *C,
                        const T *hn, I *labels, T *copies, T *sizes_copies,
-                       hipStream_t s);
+                       hipStream_t s, void *scratch = nullptr);
 
 template <typename T, typename I>
 void launch_fast_assign(int64_t n, int64_t d, int64_t k, const T *X,
@@ -66,10 +70,15 @@ void launch_fast_assign(int64_t n, int64_t d, int64_t k, const T *X,
 // Fixed-point scales of the MFMA update: scale[f] = 2^E_f (f < d) and
 // scale[d] = 2^E_w such that max|w| max|x_f| 2^E_f < 2^30 (one element fits
 // an int32) and n max|w| max|x_f| 2^E_f < 2^61 (the sum fits an int64);
-// *nonfinite = 1 when X or w holds inf/NaN (the kernel then sums in FP32).
+// *nonfinite = 1 when X or w holds inf/NaN, 2 when their range would leave
+// typical values coarsely quantised (the kernels then sum in FP32).
+// stats_scratch: fixed_point_stats_bytes(d) bytes for the pre-pass's
+// statistics, or NULL (allocated per call).  No memset nodes: the statistics
+// are cleared by a kernel.
+size_t fixed_point_stats_bytes(int64_t d);
 void launch_fixed_point_scales(int64_t n, int64_t d, const float *X,
                                const float *w, float *scale, int *nonfinite,
-                               hipStream_t s);
+                               hipStream_t s, void *stats_scratch = nullptr);
 
 // MFMA fused step writing one raw partial per block into
 // partials[block][d*k + k] (int64 fixed point, or FP32 when *nonfinite).
@@ -137,11 +146,20 @@ bool tiled_supported(int64_t d, int64_t k);
 // X_r (n, d) = transpose of X_t (d, n)
 void launch_transpose(int64_t n, int64_t d, const float *X_t, float *X_r,
                       hipStream_t s);
-// FP32 MFMA assignment (labels only)
+// FP32 MFMA assignment (labels only); tiles_scratch (tiled_tiles_bytes(d, k)
+// bytes, or NULL: allocated per call) holds the prepared centroid tiles
+size_t tiled_tiles_bytes(int64_t d, int64_t k);
 template <typename I>
 void launch_tiled_assign(int64_t n, int64_t d, int64_t k, const float *X_t,
                          const float *C, const float *hn, I *labels,
-                         hipStream_t s, const int *skip = nullptr);
+                         hipStream_t s, const int *skip = nullptr,
+                         void *tiles_scratch = nullptr);
+// The reference fused step's accumulate-into update from given labels
+// (FP32 sums, LDS-privatised per feature group), for the tiled shapes
+template <typename I>
+void launch_label_update(int64_t n, int64_t d, int64_t k, int64_t n_copies,
+                         const float *X_t, const float *w, const I *labels,
+                         float *copies, float *sizes_copies, hipStream_t s);
 // FP32 re-score of the samples in the queue segments (same arithmetic as
 // launch_tiled_assign); samples read from X_r; *n_rescored = queue length
 template <typename I>
@@ -205,12 +223,20 @@ struct BatchedFit {
 };
 struct BatchedStats {
   int64_t graph_replays = 0;
+  // launch shape of the assignment kernels the call's trial chose (0: the
+  // default threads per block, 1: half of them) and the trial's times
+  int assign_shape = 0;
+  float shape_ms[2] = {0.f, 0.f};
 };
 // Shapes the batched kernels serve: float data, d <= 64, k <= 256, k <= n.
 bool batched_fit_supported(int64_t n, int64_t d, int64_t k);
+// One fit through the single-fit FP32 driver (driver_lloyd<float, int32_t>).
+void driver_lloyd_single(const BatchedFit &f, hipStream_t s);
 // Runs all fits; results equal driver_lloyd<float, int32_t> fit by fit.
+// mixed: assignment with the BF16 cross term and the certified FP32
+// re-score (labels, and so the fits, identical to the FP32 kernels')
 void batched_lloyd(const std::vector<BatchedFit> &fits, int graph_replays,
-                   hipStream_t s, BatchedStats *stats);
+                   hipStream_t s, BatchedStats *stats, bool mixed = false);
 // Device bytes the batch engine holds for one fit while it runs.
 int64_t batch_fit_footprint(int64_t n, int64_t d, int64_t k);
 // A running set of fits (continuous batching): add() joins a fit (its
@@ -227,5 +253,7 @@ int64_t batch_engine_step(BatchEngine *e, int iterations, bool use_graph);
 std::vector<int64_t> batch_engine_take_finished(BatchEngine *e);
 int64_t batch_engine_resident_bytes(const BatchEngine *e);
 int64_t batch_engine_launches(const BatchEngine *e);
+// Selects the BF16 + certified re-score assignment (true) or the FP32 one.
+void batch_engine_set_mixed(BatchEngine *e, bool mixed);
 
 }  // namespace kmd

@@ -1137,6 +1137,15 @@ TiledFn pick_fp32(int64_t d) {
   }
 }
 
+}  // namespace
+
+size_t tiled_tiles_bytes(int64_t d, int64_t k) {
+  const int TF = 2 * ks_of(d) * 32 + 32;
+  return sizeof(float) * (size_t)(ceil_div<int64_t>(k, 32) * TF) + 1024;
+}
+
+namespace {
+
 size_t fp32_lds_bytes(int64_t d, bool gather) {
   const int KS = ks_of(d);
   const int TF = 2 * KS * 32 + 32;
@@ -1146,29 +1155,35 @@ size_t fp32_lds_bytes(int64_t d, bool gather) {
   return 2 * buf + (gather ? sizeof(int) * (kMaxSegments + 1) : 0);
 }
 
-// FP32 centroid tiles in the kernel's LDS layout (+1 KB read-ahead pad).
-TempBuffer prep_fp32_tiles(int64_t d, int64_t k, const float *C, const float *hn,
-                           hipStream_t s) {
+// FP32 centroid tiles in the kernels' LDS layout into t
+// (tiled_tiles_bytes(d, k) bytes: the tiles + a 1 KB read-ahead pad).
+void prep_fp32_tiles_into(int64_t d, int64_t k, const float *C, const float *hn,
+                          float *t, hipStream_t s) {
   const int KS = ks_of(d);
   const int TF = 2 * KS * 32 + 32;
   const int n_ct = (int)ceil_div<int64_t>(k, 32);
   const int64_t total = (int64_t)n_ct * TF;
-  TempBuffer t(sizeof(float) * (size_t)total + 1024, s);
   const dim3 grid((unsigned)ceil_div<int64_t>(total, 256));
   switch (KS) {
     case 16:
       hipLaunchKernelGGL(tiled_fp32_prep_krn<16>, grid, dim3(256), 0, s, (int)d,
-                         (int)k, n_ct, C, hn, t.as<float>());
+                         (int)k, n_ct, C, hn, t);
       break;
     case 32:
       hipLaunchKernelGGL(tiled_fp32_prep_krn<32>, grid, dim3(256), 0, s, (int)d,
-                         (int)k, n_ct, C, hn, t.as<float>());
+                         (int)k, n_ct, C, hn, t);
       break;
     default:
       hipLaunchKernelGGL(tiled_fp32_prep_krn<64>, grid, dim3(256), 0, s, (int)d,
-                         (int)k, n_ct, C, hn, t.as<float>());
+                         (int)k, n_ct, C, hn, t);
   }
   KMD_LAUNCH_CHECK();
+}
+
+TempBuffer prep_fp32_tiles(int64_t d, int64_t k, const float *C, const float *hn,
+                           hipStream_t s) {
+  TempBuffer t(tiled_tiles_bytes(d, k), s);
+  prep_fp32_tiles_into(d, k, C, hn, t.as<float>(), s);
   return t;
 }
 
@@ -1240,7 +1255,7 @@ void launch_transpose(int64_t n, int64_t d, const float *X_t, float *X_r,
 template <typename I>
 void launch_tiled_assign(int64_t n, int64_t d, int64_t k, const float *X_t,
                          const float *C, const float *hn, I *labels,
-                         hipStream_t s, const int *skip) {
+                         hipStream_t s, const int *skip, void *tiles_scratch) {
   if (n <= 0) return;
   require(tiled_supported(d, k), "tiled assignment: unsupported shape");
   TiledFn fn = pick_fp32<I, false>(d);
@@ -1250,14 +1265,16 @@ void launch_tiled_assign(int64_t n, int64_t d, int64_t k, const float *X_t,
   const int S = ks_of(d) <= 32 ? 2 : 1;
   const int64_t need = ceil_div<int64_t>(ceil_div<int64_t>(n, 32 * S), kTW);
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(grid_max, need));
-  TempBuffer tiles = prep_fp32_tiles(d, k, C, hn, s);
+  TempBuffer own(tiles_scratch ? 0 : tiled_tiles_bytes(d, k), s);
+  float *tiles = tiles_scratch ? static_cast<float *>(tiles_scratch) : own.as<float>();
+  prep_fp32_tiles_into(d, k, C, hn, tiles, s);
   TiledArgs a{};
   a.n = n;
   a.d = (int)d;
   a.k = (int)k;
   a.n_ct = (int)ceil_div<int64_t>(k, 32);
   a.X = X_t;
-  a.tiles = tiles.as<const float>();
+  a.tiles = tiles;
   a.labels = labels;
   a.skip = skip;
   hipLaunchKernelGGL(fn, dim3(blocks), dim3(kTNT), lds, s, a);
@@ -1510,11 +1527,94 @@ void launch_tiled_update(int64_t n, int64_t d, int64_t k, const float *X_r,
   KMD_LAUNCH_CHECK();
 }
 
+// ---- the reference fused step's update on the tiled shapes -----------------
+
+// copies[b % n_copies][f][l_s] += w_s x_{f,s}, sizes_copies[b % n_copies][l_s]
+// += w_s (src/lloyd_single_step.hpp:170-204, the accumulate-into contract).
+// Block (b, g) takes samples [b chunk, (b + 1) chunk) and feature group g of
+// F features (g == n_groups - 1: the counts); the group's totals live in LDS
+// as FP32 (ds_add_f32, order-dependent like the reference's float atomics)
+// and go to the copy with one global atomic per non-zero cell.  X rows are
+// read coalesced; labels and weights once per group.
+template <typename I>
+__global__ void __launch_bounds__(512)
+    label_update_krn(int64_t n, int d, int k, int F, int64_t chunk,
+                     const float *__restrict__ X, const float *__restrict__ w,
+                     const I *__restrict__ labels, float *copies,
+                     float *sizes_copies, int n_copies) {
+  extern __shared__ float acc[];
+  const int g = blockIdx.y;
+  const bool counts = g == (int)gridDim.y - 1;
+  const int f0 = g * F;
+  const int nf = counts ? 1 : min(F, d - f0);
+  const int cells = nf * k;
+  for (int i = threadIdx.x; i < cells; i += blockDim.x) acc[i] = 0.f;
+  __syncthreads();
+  const int64_t s0 = (int64_t)blockIdx.x * chunk;
+  const int64_t s1 = min(n, s0 + chunk);
+  constexpr int kF = 8;   // feature loads in flight per thread
+  for (int64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+    const int l = (int)labels[s];
+    const float ws = w[s];
+    if (counts) {
+      atomicAdd(&acc[l], ws);
+      continue;
+    }
+    const float *col = X + (int64_t)f0 * n + s;
+    for (int f = 0; f < nf; f += kF) {
+      float x[kF];
+#pragma unroll
+      for (int j = 0; j < kF; ++j) x[j] = f + j < nf ? col[(int64_t)(f + j) * n] : 0.f;
+#pragma unroll
+      for (int j = 0; j < kF; ++j)
+        if (f + j < nf) atomicAdd(&acc[(f + j) * k + l], ws * x[j]);
+    }
+  }
+  __syncthreads();
+  const int copy = (int)(blockIdx.x % (unsigned)n_copies);
+  float *dst = counts ? sizes_copies + (int64_t)copy * k
+                      : copies + ((int64_t)copy * d + f0) * k;
+  for (int i = threadIdx.x; i < cells; i += blockDim.x) {
+    const float v = acc[i];
+    if (v != 0.f) atomicAdd(&dst[i], v);
+  }
+}
+
+template <typename I>
+void launch_label_update(int64_t n, int64_t d, int64_t k, int64_t n_copies,
+                         const float *X_t, const float *w, const I *labels,
+                         float *copies, float *sizes_copies, hipStream_t s) {
+  if (n <= 0) return;
+  require(k <= 40960, "label update: too many clusters for one LDS row");
+  // F features per group: F k floats <= 64 KB (two blocks per CU)
+  const int F = (int)std::max<int64_t>(1, std::min<int64_t>(d, 16384 / k));
+  const int groups = (int)ceil_div<int64_t>(d, F) + 1;
+  const size_t lds = sizeof(float) * (size_t)F * k;
+  const int per_cu = lds <= 80 * 1024 ? 2 : 1;
+  const int64_t target = (int64_t)device_cu_count() * per_cu * 2;
+  const int64_t b = std::max<int64_t>(
+      1, std::min<int64_t>(ceil_div<int64_t>(target, groups), ceil_div<int64_t>(n, 512)));
+  const int64_t chunk = ceil_div<int64_t>(n, b);
+  if (lds > 64 * 1024)
+    KMD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&label_update_krn<I>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+  hipLaunchKernelGGL(label_update_krn<I>, dim3((unsigned)ceil_div<int64_t>(n, chunk),
+                                              (unsigned)groups),
+                     dim3(512), lds, s, n, (int)d, (int)k, F, chunk, X_t, w, labels,
+                     copies, sizes_copies, (int)n_copies);
+  KMD_LAUNCH_CHECK();
+}
+
 #define KMD_INST_TILED(I)                                                      \
   template void launch_tiled_assign<I>(int64_t, int64_t, int64_t,              \
                                        const float *, const float *,           \
                                        const float *, I *, hipStream_t,        \
-                                       const int *);                           \
+                                       const int *, void *);                   \
+  template void launch_label_update<I>(int64_t, int64_t, int64_t, int64_t,     \
+                                       const float *, const float *,           \
+                                       const I *, float *, float *,            \
+                                       hipStream_t);                           \
   template void launch_tiled_rescore<I>(int64_t, int64_t, int64_t,             \
                                         const float *, const float *,          \
                                         const float *, I *, const int *,       \

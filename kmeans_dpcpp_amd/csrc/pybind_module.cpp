@@ -867,8 +867,9 @@ std::pair<Event, Event> py_kmeans_plusplus(py::object X_, py::object w_,
 py::tuple py_kmeans_lloyd_batched(py::list fits, const std::string &precision,
                                   py::object q, int graph_replays,
                                   py::list depends) {
-  if (precision != "fp32")
-    throw py::value_error("batched fits support precision 'fp32' only");
+  if (precision != "fp32" && precision != "bf16")
+    throw py::value_error("batched fits support precision 'fp32' or 'bf16'");
+  const int prec = precision == "bf16" ? KMD_PREC_BF16 : KMD_PREC_FP32;
   kmd_stream_t s = resolve_stream(q);
   std::vector<kmd_fit> v;
   std::vector<Arr> keep;
@@ -921,26 +922,37 @@ py::tuple py_kmeans_lloyd_batched(py::list fits, const std::string &precision,
   }
   wait_depends(s, depends);
   int64_t launches = 0;
+  int shape = 0;
   int rc;
   {
     py::gil_scoped_release nogil;
-    rc = kmd_kmeans_lloyd_batched((int64_t)v.size(), v.data(), KMD_PREC_FP32,
-                                  graph_replays, &launches, s);
+    rc = kmd_kmeans_lloyd_batched((int64_t)v.size(), v.data(), prec,
+                                  graph_replays, &launches, &shape, s);
   }
   check_rc(rc);
   py::list out;
   for (const kmd_fit &f : v) out.append(py::make_tuple((size_t)f.n_iters, f.inertia));
   py::dict stats;
   stats["graph_launches"] = launches;
+  stats["assign_shape"] = shape;
   return py::make_tuple(out, stats);
 }
 
 // Continuous-batching engine: fits join a running set and advance together.
 class Engine {
  public:
-  explicit Engine(py::object q) : q_(q) {
+  explicit Engine(py::object q, const std::string &precision) : q_(q) {
+    if (precision != "fp32" && precision != "bf16")
+      throw py::value_error("engine precision must be 'fp32' or 'bf16'");
     s_ = resolve_stream(q);
     check_rc(kmd_engine_create(&e_, s_));
+    const int rc = kmd_engine_set_precision(
+        e_, precision == "bf16" ? KMD_PREC_BF16 : KMD_PREC_FP32);
+    if (rc != KMD_OK) {
+      kmd_engine_destroy(e_);
+      e_ = nullptr;
+      check_rc(rc);
+    }
   }
   ~Engine() {
     if (e_) kmd_engine_destroy(e_);
@@ -1062,25 +1074,27 @@ py::dict py_autotune(py::object X_, py::object w_, py::object init_,
              : precision == "bf16" ? KMD_PREC_BF16 : -1;
   if (prec < 0) throw py::value_error("precision must be 'fp32', 'bf16x3' or 'bf16'");
   use_device(X);
-  int bp = 0, bb = 0;
+  constexpr int kCap = 64;
+  int bp = 0, bb = 0, bs = 0;
   double bms = 0.0;
-  int cp[16], cb[16];
-  double cm[16];
+  int cp[kCap], cb[kCap], cs[kCap];
+  double cm[kCap];
   int64_t nc = 0;
   int rc;
   {
     py::gil_scoped_release nogil;
     rc = kmd_autotune(X.dtype(), KMD_I32, prec, n, d, k, X.data, w.data, init.data,
-                      iters, &bp, &bb, &bms, cp, cb, cm, 16, &nc, s);
+                      iters, &bp, &bb, &bs, &bms, cp, cb, cs, cm, kCap, &nc, s);
   }
   check_rc(rc);
   static const char *kPath[] = {"generic", "lds-resident", "tiled"};
   py::list cands;
-  for (int64_t i = 0; i < std::min<int64_t>(nc, 16); ++i)
-    cands.append(py::make_tuple(kPath[cp[i]], cb[i], cm[i]));
+  for (int64_t i = 0; i < std::min<int64_t>(nc, kCap); ++i)
+    cands.append(py::make_tuple(kPath[cp[i]], cb[i], cs[i], cm[i]));
   py::dict out;
   out["path"] = kPath[bp];
   out["blocks_per_cu"] = bb;
+  out["shape"] = bs;
   out["ms"] = bms;
   out["candidates"] = cands;
   return out;
@@ -1226,8 +1240,10 @@ PYBIND11_MODULE(_kmeans_dpcpp_amd, m) {
                      "fit to the running set, step() advances every active fit, "
                      "take_finished() returns [(id, n_iters, inertia)] of fits that "
                      "stopped.  Each fit's results equal kmeans_lloyd_driver's for it "
-                     "alone.")
-      .def(py::init<py::object>(), py::arg("sycl_queue") = py::none())
+                     "alone.  precision 'bf16': the assignment takes the cross term in "
+                     "BF16 with a certified FP32 re-score (the same results).")
+      .def(py::init<py::object, const std::string &>(), py::arg("sycl_queue") = py::none(),
+           py::arg("precision") = "fp32")
       .def("add", &Engine::add, py::arg("X_t"), py::arg("sample_weight"),
            py::arg("init_centroid_t"), py::arg("assignments_id"),
            py::arg("res_centroids_t"), py::arg("tol"), py::arg("max_iter"))
@@ -1240,18 +1256,22 @@ PYBIND11_MODULE(_kmeans_dpcpp_amd, m) {
                   py::arg("n_features"), py::arg("n_clusters"));
   m.def("autotune", &py_autotune,
         "Time the driver's launch plans (LDS-resident or tiled centroids x "
-        "persistent blocks per CU) for this problem and keep the fastest for "
-        "later fits of its shape; results never depend on the plan.  Returns "
-        "{'path', 'blocks_per_cu', 'ms', 'candidates'}.",
+        "launch shape of the LDS-resident step x persistent blocks per CU) for "
+        "this problem and keep the fastest for later fits of its shape; results "
+        "never depend on the plan.  Returns {'path', 'blocks_per_cu', 'shape', "
+        "'ms', 'candidates': [(path, blocks_per_cu, shape, ms)]}.",
         py::arg("X_t"), py::arg("sample_weight"), py::arg("init_centroid_t"),
         py::arg("precision") = "fp32", py::arg("iters") = 3,
         py::arg("sycl_queue") = py::none());
   m.def("set_tuning",
-        [](int path, int blocks_per_cu) { check_rc(kmd_set_tuning(path, blocks_per_cu)); },
+        [](int path, int blocks_per_cu, int shape) {
+          check_rc(kmd_set_tuning(path, blocks_per_cu, shape));
+        },
         "Override the launch plan of driver calls: path 0 automatic, 1 "
         "LDS-resident, 2 tiled (-1 clears the override); blocks_per_cu 0 = "
-        "occupancy limit.",
-        py::arg("path"), py::arg("blocks_per_cu") = 0);
+        "occupancy limit; shape 0 (16 waves x 1 column block per wave), 1 (12 x "
+        "2) or 2 (8 x 2) for the LDS-resident step.",
+        py::arg("path"), py::arg("blocks_per_cu") = 0, py::arg("shape") = 0);
   m.def("kmeans_lloyd_batched", &py_kmeans_lloyd_batched,
         "Many independent Lloyd fits at once: fits is a list of (X_t, "
         "sample_weight, init_centroid_t, assignments_id, res_centroids_t, tol, "

@@ -121,8 +121,10 @@ class KMeans:
 
     Parameters follow sklearn.cluster.KMeans: n_clusters, init
     ("k-means++", "random", an array (n_clusters, n_features) or a callable
-    (X, n_clusters, random_state) -> array), n_init ("auto": 1 for
-    k-means++ / array / callable, 10 for random), max_iter, tol (scaled by
+    (X, n_clusters, random_state) -> array, called with the centred X and
+    its result taken in centred coordinates, as scikit-learn does), n_init
+    ("auto": 1 for k-means++ / array, 10 for random / callable), max_iter,
+    tol (scaled by
     the mean feature variance), verbose, random_state, copy_x (X is never
     modified here).  Extra: precision ("fp32", "bf16x3", "bf16": the cross
     term of the assignment; labels equal the fp32 ones in every mode),
@@ -166,7 +168,10 @@ class KMeans:
         if isinstance(self.init, str) and self.init not in ("k-means++", "random"):
             raise ValueError("init must be 'k-means++', 'random', an array or a callable")
         if self.n_init == "auto":
-            n_init = 10 if (isinstance(self.init, str) and self.init == "random") else 1
+            # scikit-learn 1.7's rule: 10 runs for 'random' and for a callable,
+            # one for 'k-means++' and for an explicit array
+            n_init = 10 if ((isinstance(self.init, str) and self.init == "random")
+                            or callable(self.init)) else 1
         else:
             n_init = int(self.n_init)
             if n_init < 1:
@@ -198,10 +203,20 @@ class KMeans:
             _native.gather_columns(X_t, idx, C, q)
             return C
         if callable(self.init):
-            init = self.init(self._X_user, k, random_state=rs)
-        else:
-            init = self.init
-        C0, _ = _as_device(init, X_t.device, X_t.dtype)
+            # as scikit-learn: the callable sees the centred data and its
+            # result is taken in centred coordinates
+            Xc = X_t.t()
+            Xc = Xc.cpu().numpy() if self._host else Xc.contiguous()
+            C0, _ = _as_device(self.init(Xc, k, random_state=rs), X_t.device,
+                               X_t.dtype)
+            if tuple(C0.shape) != (k, d):
+                raise ValueError(f"The shape of the initial centers {tuple(C0.shape)}"
+                                 f" does not match the number of clusters {k} / "
+                                 f"features {d}.")
+            C = torch.empty((d, k), dtype=X_t.dtype, device=X_t.device)
+            _native.transpose(C0.contiguous(), C, None, q)
+            return C
+        C0, _ = _as_device(self.init, X_t.device, X_t.dtype)
         if tuple(C0.shape) != (k, d):
             raise ValueError(f"The shape of the initial centers {tuple(C0.shape)} does"
                              f" not match the number of clusters {k} / features {d}.")
@@ -226,7 +241,6 @@ class KMeans:
         n, d = Xd.shape
         n_init = self._check_params(n)
         self._host = host
-        self._X_user = X
         w = (torch.ones(n, dtype=Xd.dtype, device=dev) if sample_weight is None
              else _as_device(sample_weight, dev, Xd.dtype)[0])
         if w.shape != (n,):
@@ -241,10 +255,7 @@ class KMeans:
         self._tol = tol
         rs = _check_random_state(self.random_state)
         k = self.n_clusters
-        try:
-            inits = [self._init_centers(X_t, Xd, w, mean, rs, q) for _ in range(n_init)]
-        finally:
-            self._X_user = None    # only a callable init needs the caller's X
+        inits = [self._init_centers(X_t, Xd, w, mean, rs, q) for _ in range(n_init)]
         if self.verbose:
             print("Initialization complete")
         runs = self._run_lloyd(X_t, w, inits, tol, q)
@@ -273,7 +284,7 @@ class KMeans:
         d, n = X_t.shape
         k = self.n_clusters
         out = []
-        batched = (len(inits) > 1 and self.precision == "fp32" and
+        batched = (len(inits) > 1 and self.precision in ("fp32", "bf16") and
                    X_t.dtype == torch.float32 and not self.verbose)
         if batched:
             fits = []
@@ -282,7 +293,7 @@ class KMeans:
                 res = torch.empty_like(C)
                 fits.append((X_t, w, C, lab, res, tol, self.max_iter))
             # shapes outside the batched kernels run fit by fit in the driver
-            results, _ = _native.kmeans_lloyd_batched(fits, "fp32", q)
+            results, _ = _native.kmeans_lloyd_batched(fits, self.precision, q)
             for (_, _, _, lab, res, _, _), (it, inertia) in zip(fits, results):
                 out.append((lab, res, it, float(inertia)))
             return out

@@ -75,7 +75,9 @@ class FitRequest:
 
 
 def engine_serves(n, d, k) -> bool:
-    return 1 <= d <= 64 and 1 <= k <= 256 and k <= n
+    """The shapes the batch engine's kernels take (the native
+    batched-fit check: d <= 64, k <= 256, k <= n < 2^30)."""
+    return 1 <= d <= 64 and 1 <= k <= 256 and k <= n < (1 << 30)
 
 
 def request_footprint(n, d, k) -> int:
@@ -95,13 +97,18 @@ class Server:
     (first come first served, by footprint against ``hbm_budget_bytes``),
     ticks the batch engine and hands large fits to ``large_workers`` worker
     threads.
+    ``small_precision`` is the batch engine's assignment ("fp32", or "bf16":
+    the cross term in BF16 with a certified FP32 re-score, the same
+    results); ``large_precision`` that of the single-fit driver runs.
     ``engine_factory`` / ``large_runner`` replace the GPU engine and the
     single-fit runner (for host-only tests of the loop's logic)."""
 
     def __init__(self, hbm_budget_bytes, iterations_per_tick=1, use_graph=False,
                  large_precision="fp32", engine_factory=None, large_runner=None,
-                 footprint=request_footprint, serves=engine_serves, large_workers=2):
+                 footprint=request_footprint, serves=engine_serves, large_workers=2,
+                 small_precision="fp32"):
         self.budget = int(hbm_budget_bytes)
+        self.small_precision = small_precision
         self.iterations_per_tick = int(iterations_per_tick)
         self.use_graph = bool(use_graph)
         self.large_precision = large_precision
@@ -116,6 +123,7 @@ class Server:
         self.peak_resident = 0
         self.ticks = 0
         self._running = True
+        self._dead = None           # the error that ended the loop thread
         self._inflight = {}
         self._local = threading.local()
         self._loop = threading.Thread(target=self._run_loop, name="kmd-serve",
@@ -139,6 +147,11 @@ class Server:
             raise ValueError("request needs more device memory than the server's "
                              "budget")
         with self._lock:
+            if self._dead is not None:
+                raise RuntimeError("the server's loop has stopped on an error") \
+                    from self._dead
+            if not self._running:
+                raise RuntimeError("the server is closed")
             self._pending.append(req)
             self._lock.notify_all()
         return req
@@ -173,9 +186,18 @@ class Server:
             if req.batched:
                 try:
                     self._add_to_engine(engine, req)
+                    continue
+                except ValueError as e:
+                    if "single-fit driver" not in str(e):
+                        self._finish(req, error=e)
+                        continue
+                    # the data's range needs floating-point centroid sums,
+                    # which only the single-fit driver has
+                    req.batched = False
                 except Exception as e:  # noqa: BLE001 - reported on the request
                     self._finish(req, error=e)
-            else:
+                    continue
+            if not req.batched:
                 with self._lock:
                     self._large.append(req)
                     self._lock.notify_all()
@@ -207,7 +229,7 @@ class Server:
             if self._engine_factory is not None:
                 engine = self._engine_factory()
             else:
-                engine = _native.Engine()
+                engine = _native.Engine(None, self.small_precision)
             while True:
                 with self._lock:
                     while self._running and not self._pending and not self._inflight:
@@ -226,8 +248,27 @@ class Server:
                         if self._pending and not self._inflight:
                             # waiting for memory held by large fits
                             self._lock.wait(0.005)
+        except BaseException as e:  # noqa: BLE001 - every waiting request gets it
+            self._fail_all(e)
         finally:
             del engine
+
+    def _fail_all(self, error):
+        """The loop thread cannot go on: finish every request it holds or
+        would admit with `error` and refuse new submissions."""
+        with self._lock:
+            self._dead = error
+            self._running = False
+            stranded = [req for req, _ in self._inflight.values()]
+            self._inflight.clear()
+            stranded += list(self._pending)
+            self._pending.clear()
+            self._lock.notify_all()
+        for req in stranded:
+            if not req.done():
+                self._finish(req, error=RuntimeError(
+                    f"fit request abandoned: the server's loop stopped on "
+                    f"{type(error).__name__}: {error}"))
 
     def _run_worker(self):
         while True:

@@ -142,3 +142,51 @@ def test_default_event_is_complete(kd):
     ev = kd.Event()
     assert ev.is_complete
     ev.wait()
+
+
+def test_c_abi_rejects_null_arrays(lib):
+    """A NULL device array with a non-zero element count is KMD_EVALUE, not a
+    device fault (the checks run before any HIP call)."""
+    i64, p, dbl = ctypes.c_int64, ctypes.c_void_p, ctypes.c_double
+    fake = p(0x1000)          # never dereferenced: a NULL elsewhere fails first
+    n_it, inert = ctypes.c_int64(0), ctypes.c_double(0)
+    calls = {
+        "assignment": lambda: lib.kmd_assignment(0, 0, i64(8), i64(3), i64(2), i64(8),
+                                                 i64(128), p(0), fake, fake, fake, p(0)),
+        "compute_inertia": lambda: lib.kmd_compute_inertia(
+            0, 0, i64(8), i64(3), i64(2), i64(128), fake, p(0), fake, p(0), fake, p(0)),
+        "fused_lloyd_single_step": lambda: lib.kmd_fused_lloyd_single_step(
+            0, 0, i64(8), i64(3), i64(2), i64(1), i64(8), i64(128), fake, p(0), fake,
+            fake, fake, fake, fake, p(0)),
+        "kmeans_lloyd_driver": lambda: lib.kmd_kmeans_lloyd_driver(
+            0, 0, i64(8), i64(3), i64(2), fake, p(0), fake, fake, fake, dbl(0.0), 0,
+            i64(10), i64(8), i64(128), dbl(0.5), ctypes.byref(n_it), ctypes.byref(inert),
+            p(0), p(0), p(0)),
+        "kmeans_lloyd_driver_ex": lambda: lib.kmd_kmeans_lloyd_driver_ex(
+            0, 0, 0, i64(8), i64(3), i64(2), p(0), fake, fake, fake, fake, dbl(0.0), 0,
+            i64(10), ctypes.byref(n_it), ctypes.byref(inert), p(0), p(0), p(0), p(0),
+            p(0), p(0), p(0)),
+        "relocate_empty_clusters": lambda: lib.kmd_relocate_empty_clusters(
+            0, 0, i64(8), i64(3), i64(2), i64(1), i64(32), fake, fake, fake, fake, fake,
+            p(0), fake, fake, p(0)),
+        "reduce_centroids_data": lambda: lib.kmd_reduce_centroids_data(
+            0, 0, i64(2), i64(3), i64(4), i64(32), fake, p(0), fake, fake, fake, fake,
+            p(0)),
+        "compute_centroid_shifts_squared": lambda: lib.kmd_compute_centroid_shifts_squared(
+            0, i64(3), i64(4), fake, fake, p(0), p(0)),
+        "compute_centroid_to_sample_distances":
+            lambda: lib.kmd_compute_centroid_to_sample_distances(
+                0, i64(8), i64(3), i64(2), i64(128), i64(8), fake, p(0), fake, p(0)),
+        "reduce_vector_blocking": lambda: lib.kmd_reduce_vector_blocking(
+            0, i64(8), p(0), ctypes.byref(inert), p(0)),
+        "iterative_merge_sort": lambda: lib.kmd_iterative_merge_sort(
+            0, i64(8), fake, p(0), p(0)),
+    }
+    for name, call in calls.items():
+        rc = call()
+        assert rc == 1, name
+        assert "null pointer" in _err(lib), (name, _err(lib))
+    # a NULL weight is the documented unweighted inertia (not an error), and
+    # empty arrays may be NULL
+    assert lib.kmd_compute_centroid_shifts_squared(0, i64(0), i64(0), p(0), p(0), p(0),
+                                                   p(0)) == 0

@@ -35,7 +35,7 @@ def single(kd, Xt, w, init, tol, max_iter):
     return to_np(lab_d), to_np(res_d), int(n_iter), float(inertia)
 
 
-def batched(kd, problems, replays):
+def batched(kd, problems, replays, precision="fp32"):
     dev = torch.device("cuda", 0)
     fits, outs = [], []
     for Xt, w, init, tol, max_iter in problems:
@@ -44,7 +44,7 @@ def batched(kd, problems, replays):
         lab_d = torch.empty(Xt.shape[1], dtype=torch.int32, device=dev)
         fits.append((to_dev(Xt), to_dev(w), init_d, lab_d, res_d, tol, max_iter))
         outs.append((lab_d, res_d))
-    results, stats = kd.kmeans_lloyd_batched(fits, "fp32", torch.cuda.current_stream(),
+    results, stats = kd.kmeans_lloyd_batched(fits, precision, torch.cuda.current_stream(),
                                              replays)
     return [(to_np(l), to_np(r), int(it), float(inr))
             for (l, r), (it, inr) in zip(outs, results)], stats
@@ -63,11 +63,13 @@ def problem_mix(seed):
     return probs
 
 
-@pytest.mark.parametrize("replays", [1, 4])
-def test_batched_fits_equal_single_fits(kd, torch_dev, replays):
+@pytest.mark.parametrize("replays,precision", [(1, "fp32"), (4, "fp32"), (4, "bf16")])
+def test_batched_fits_equal_single_fits(kd, torch_dev, replays, precision):
+    """FP32 and BF16 (+ certified FP32 re-score) batched assignment: every fit
+    bitwise equal to the FP32 single-fit driver."""
     probs = problem_mix(seed=1)
     refs = [single(kd, Xt, w, init, tol, mi) for Xt, w, init, tol, mi in probs]
-    got, stats = batched(kd, probs, replays)
+    got, stats = batched(kd, probs, replays, precision)
     assert stats["graph_launches"] >= max(r[2] for r in refs)
     for i, (g, r) in enumerate(zip(got, refs)):
         assert g[2] == r[2], (i, g[2], r[2])
@@ -76,7 +78,8 @@ def test_batched_fits_equal_single_fits(kd, torch_dev, replays):
         assert g[3] == r[3], (i, g[3], r[3])
 
 
-def test_batched_relocates_like_the_driver(kd, torch_dev):
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_batched_relocates_like_the_driver(kd, torch_dev, precision):
     """Duplicate initial centroids leave clusters empty after the first step;
     the batched finalize relocates them to the same farthest samples."""
     rng = np.random.default_rng(9)
@@ -90,7 +93,7 @@ def test_batched_relocates_like_the_driver(kd, torch_dev):
             init[:, :] = init[:, [0]]    # all equal: k - 1 empty clusters
         probs.append((Xt, w, init, 0.0, 5))
     refs = [single(kd, Xt, w, init, tol, mi) for Xt, w, init, tol, mi in probs]
-    got, _ = batched(kd, probs, 2)
+    got, _ = batched(kd, probs, 2, precision)
     for g, r in zip(got, refs):
         assert g[2] == r[2]
         assert_array_equal(g[0], r[0])
@@ -112,8 +115,8 @@ def test_batched_mixes_in_unserved_shapes(kd, torch_dev):
         assert_array_equal(g[1].view(np.uint32), r[1].view(np.uint32))
 
 
-@pytest.mark.parametrize("replays", [1, 3])
-def test_batched_edge_shapes_equal_single_fits(kd, torch_dev, replays):
+@pytest.mark.parametrize("replays,precision", [(1, "fp32"), (3, "fp32"), (3, "bf16")])
+def test_batched_edge_shapes_equal_single_fits(kd, torch_dev, replays, precision):
     """Shapes at the segment sums' and assignment's edges: d = 1 (128 row
     groups of two columns), d = 63 (four groups of 64), d = 64 (three groups
     of 65), k = 1 (one cluster split over many 256-member items), n below one
@@ -126,9 +129,61 @@ def test_batched_edge_shapes_equal_single_fits(kd, torch_dev, replays):
         Xt, w, init = make_fit(rng, n, d, k, seed=900 + i)
         probs.append((Xt, w, init, 1e-4, 30))
     refs = [single(kd, Xt, w, init, tol, mi) for Xt, w, init, tol, mi in probs]
-    got, _ = batched(kd, probs, replays)
+    got, _ = batched(kd, probs, replays, precision)
     for i, (g, r) in enumerate(zip(got, refs)):
         assert g[2] == r[2], (shapes[i], g[2], r[2])
         assert_array_equal(g[0], r[0])
         assert_array_equal(g[1].view(np.uint32), r[1].view(np.uint32))
         assert g[3] == r[3], (shapes[i], g[3], r[3])
+
+
+def test_wide_range_fit_in_a_batch_goes_to_the_single_fit_driver(kd, torch_dev):
+    """A fit whose data range needs floating-point sums (one outlier of 1e6
+    beside a cluster of values below 1e-3) is taken out of the batched
+    kernels and fitted by the single-fit driver; the other fits stay
+    bitwise equal to their single-fit runs."""
+    from tests.test_gpu_driver import _wide_range_data
+    Xw, initw, groups = _wide_range_data(seed=23)
+    probs = problem_mix(seed=4)[:4]
+    probs.insert(2, (Xw, np.ones(Xw.shape[1], np.float32), initw, 0.0, 8))
+    refs = [single(kd, Xt, w, init, tol, mi) for Xt, w, init, tol, mi in probs]
+    got, _ = batched(kd, probs, 4)
+    for i, (g, r) in enumerate(zip(got, refs)):
+        assert g[2] == r[2], i
+        assert_array_equal(g[0], r[0])
+        if i != 2:
+            assert_array_equal(g[1].view(np.uint32), r[1].view(np.uint32))
+    lab, C = got[2][0], got[2][1]
+    assert_array_equal(lab, groups)
+    for c in range(6):
+        mean = Xw[:, lab == c].astype(np.float64).mean(axis=1)
+        assert np.abs(C[:, c] - mean).max() <= 1e-5 * np.abs(mean).max()
+    # the continuous-batching engine refuses it with a message that the
+    # request loop uses to route it to the single-fit driver
+    eng = kd.Engine()
+    dev = torch.device("cuda", 0)
+    with pytest.raises(ValueError, match="single-fit driver"):
+        eng.add(to_dev(Xw), to_dev(np.ones(Xw.shape[1], np.float32)), to_dev(initw),
+                torch.empty(Xw.shape[1], dtype=torch.int32, device=dev),
+                torch.empty(initw.shape, device=dev), 0.0, 8)
+    del eng
+
+
+def test_batched_bf16_with_near_ties_equals_fp32(kd, torch_dev):
+    """Duplicate and near-duplicate initial centroids make every sample of
+    their blobs a near-tie: the BF16 pass leaves those to the FP32 re-score,
+    and the fits stay bitwise equal to the FP32 single-fit driver."""
+    rng = np.random.default_rng(13)
+    probs = []
+    for i, (n, d, k) in enumerate([(6000, 16, 32), (9000, 32, 64), (7000, 64, 128)]):
+        Xt, w, init = make_fit(rng, n, d, k, seed=1300 + i)
+        init = init.copy()
+        init[:, 1::2] = init[:, 0::2] + np.float32(1e-4)   # pairs 1e-4 apart
+        probs.append((Xt, w, init, 1e-4, 25))
+    refs = [single(kd, Xt, w, init, tol, mi) for Xt, w, init, tol, mi in probs]
+    got, _ = batched(kd, probs, 4, "bf16")
+    for g, r in zip(got, refs):
+        assert g[2] == r[2]
+        assert_array_equal(g[0], r[0])
+        assert_array_equal(g[1].view(np.uint32), r[1].view(np.uint32))
+        assert g[3] == r[3]

@@ -10,7 +10,7 @@ from numpy.testing import assert_allclose, assert_array_equal
 
 from oracle import c_oracle as co
 from oracle import lloyd_oracle as lo
-from tests.conftest import CORNERS, to_dev, to_np
+from tests.conftest import CORNERS, check_label_mismatches, fp64_update, to_dev, to_np
 
 pytestmark = pytest.mark.gpu
 
@@ -248,10 +248,14 @@ def test_mixed_precision_matches_fp32_bit_exactly(kd, torch_dev, shape, seed, st
         assert got[3] == ref[3]
         rescored[prec] = got[4]["n_rescored"]
     assert rescored["bf16x3"] <= rescored["bf16"]
-    # and the fit is within 1e-5 (relative to the centroid scale) of FP64
-    l64, C64, it64, in64 = lo.kmeans_lloyd(Xt, w, init, 1e-4, 25, dtype=np.float64)
-    if it64 == ref[2] and (l64 == ref[0]).mean() > 0.9999:
-        assert np.abs(ref[1] - C64).max() <= 1e-5 * np.abs(C64).max()
+    # the last update is within 1e-5 (relative to the centroid scale) of an
+    # FP64 update from the labels of the centroids one iteration earlier, and
+    # the returned labels are those of the returned centroids
+    prev = run_driver_ex(kd, Xt, w, init, 1e-4, ref[2] - 1, "bf16")
+    C64, cnt = fp64_update(Xt, w, prev[0], k)
+    ok = cnt > 0
+    assert np.abs(ref[1][:, ok] - C64[:, ok]).max() <= 1e-5 * np.abs(C64[:, ok]).max()
+    check_label_mismatches(Xt, ref[1], ref[0], co.assign(Xt, ref[1]), limit=max(1, n // 10000))
 
 
 def test_mixed_precision_falls_back_on_nonfinite(kd, torch_dev):
@@ -344,21 +348,27 @@ def test_assignment_wide_shape_matches_c_restatement(kd, torch_dev):
 
 def test_launch_plans_do_not_change_results(kd, torch_dev):
     """Every plan the autotuner can pick -- LDS-resident or tiled centroids,
-    any cap on persistent blocks per CU -- gives the same fit bit for bit."""
-    Xt, init = lo.gaussian_blobs(50_000, 32, 64, seed=61, std=1.5)
-    w = np.random.default_rng(2).uniform(0.5, 2.0, 50_000).astype(np.float32)
-    try:
-        for prec in ("fp32", "bf16x3"):
-            runs = []
-            for path, cap in ((1, 0), (1, 1), (2, 0), (2, 2)):
-                kd.set_tuning(path, cap)
-                runs.append(run_driver_ex(kd, Xt, w, init, 1e-4, 12, prec))
-            assert runs[0][4]["path"].startswith(("fp32", "bf16"))
-            assert runs[2][4]["path"].startswith("tiled")
-            for r in runs[1:]:
-                same_fit(r, runs[0])
-    finally:
-        kd.set_tuning(-1)
+    each launch shape of the LDS-resident step (waves per block x column
+    blocks per wave), any cap on persistent blocks per CU -- gives the same
+    fit bit for bit, on an even and an odd sample count (the two-column
+    shapes load pairs of samples when n is even)."""
+    for n, d in ((50_000, 32), (30_001, 14)):
+        Xt, init = lo.gaussian_blobs(n, d, 64, seed=61, std=1.5)
+        w = np.random.default_rng(2).uniform(0.5, 2.0, n).astype(np.float32)
+        try:
+            for prec in ("fp32", "bf16x3"):
+                runs = []
+                plans = ((1, 0, 0), (1, 1, 0), (1, 0, 1), (1, 0, 2), (1, 2, 2),
+                         (2, 0, 0), (2, 2, 0))
+                for path, cap, shape in plans:
+                    kd.set_tuning(path, cap, shape)
+                    runs.append(run_driver_ex(kd, Xt, w, init, 1e-4, 12, prec))
+                assert runs[0][4]["path"].startswith(("fp32", "bf16"))
+                assert runs[5][4]["path"].startswith("tiled")
+                for r in runs[1:]:
+                    same_fit(r, runs[0])
+        finally:
+            kd.set_tuning(-1)
 
 
 def test_autotune_records_a_plan_the_driver_uses(kd, torch_dev):
@@ -366,7 +376,77 @@ def test_autotune_records_a_plan_the_driver_uses(kd, torch_dev):
     w = np.ones(40_000, np.float32)
     r = kd.autotune(to_dev(Xt), to_dev(w), to_dev(init), "fp32", 2, q())
     assert r["path"] in ("lds-resident", "tiled")
-    assert len(r["candidates"]) == 8 and all(c[2] > 0 for c in r["candidates"])
-    assert r["ms"] == min(c[2] for c in r["candidates"])
+    # LDS-resident: 3 launch shapes x 4 block caps; tiled: 4 block caps
+    assert len(r["candidates"]) == 16 and all(c[3] > 0 for c in r["candidates"])
+    assert {c[2] for c in r["candidates"] if c[0] == "lds-resident"} == {0, 1, 2}
+    assert r["ms"] == min(c[3] for c in r["candidates"])
     got = run_driver_ex(kd, Xt, w, init, 1e-4, 5, "fp32")
     assert got[4]["path"] == ("fp32-mfma" if r["path"] == "lds-resident" else "tiled-fp32")
+
+
+def _wide_range_data(seed=21):
+    """Four ordinary blobs, a cluster of tiny values U(0, 1e-3) and one outlier
+    sample at 1e6 in feature 0 (its own cluster); init = one member of each
+    group, so every fit keeps the groups apart."""
+    rng = np.random.default_rng(seed)
+    d = 4
+    centers = np.array([[10, 0, 0, 0], [0, 10, 0, 0], [0, 0, 10, 0], [0, 0, 0, 10]],
+                       np.float64)
+    big = centers[rng.integers(0, 4, 8000)] + rng.normal(size=(8000, d))
+    tiny = rng.uniform(0, 1e-3, size=(2000, d))
+    X = np.concatenate([big, tiny, [[1e6, 0.5, 0.5, 0.5]]]).astype(np.float32)
+    perm = rng.permutation(len(X))
+    X = X[perm]
+    groups = np.concatenate([np.argmax(big, axis=1), np.full(2000, 4), [5]])[perm]
+    init_idx = [int(np.flatnonzero(groups == g)[0]) for g in range(6)]
+    return np.ascontiguousarray(X.T), np.ascontiguousarray(X[init_idx].T), groups
+
+
+def test_wide_range_data_keeps_small_clusters_accurate(kd, torch_dev):
+    """One outlier of 1e6 in a feature would make fixed-point steps of 2^-10
+    there; the pre-pass sees the range and the sums go through floating point,
+    so the tiny cluster's centre is accurate against its own magnitude."""
+    Xt, init, groups = _wide_range_data()
+    w = np.ones(Xt.shape[1], np.float32)
+    for name in ("fp32", "bf16"):
+        dev = torch.device("cuda", 0)
+        lab_d = torch.empty(Xt.shape[1], dtype=torch.int32, device=dev)
+        init_d, res_d = to_dev(init), torch.empty(init.shape, device=dev)
+        n_iter, inertia, _ = kd.kmeans_lloyd_driver_ex(to_dev(Xt), to_dev(w), init_d,
+                                                       lab_d, res_d, 0.0, False, 6, name,
+                                                       q())
+        lab, C = to_np(lab_d), to_np(res_d)
+        assert_array_equal(lab, groups)
+        for c in range(6):
+            mean = Xt[:, lab == c].astype(np.float64).mean(axis=1)
+            assert np.abs(C[:, c] - mean).max() <= 1e-5 * np.abs(mean).max(), (name, c)
+
+
+def test_tiny_weight_cluster_is_not_empty(kd, torch_dev):
+    """A cluster whose only member has weight 1e-10 next to unit weights: its
+    count must not round to zero (which would relocate it as empty)."""
+    Xt, init, groups = _wide_range_data(seed=22)
+    Xt = Xt.copy()
+    Xt[0, groups == 5] = 60.0            # no value outlier: only the weight is extreme
+    init[0, 5] = 60.0
+    w = np.ones(Xt.shape[1], np.float32)
+    w[groups == 5] = 1e-10
+    lab, C, n_iter, inertia = run_driver(kd, Xt, w, init, 0.0, 5)
+    assert_array_equal(lab, groups)
+    assert_allclose(C[:, 5], Xt[:, groups == 5][:, 0], rtol=1e-6)
+    for c in range(5):
+        mean = Xt[:, lab == c].astype(np.float64).mean(axis=1)
+        assert np.abs(C[:, c] - mean).max() <= 1e-5 * np.abs(mean).max()
+
+
+def test_ordinary_data_keeps_exact_sums(kd, torch_dev):
+    """C1-like blobs (centres in [-20, 20], no outliers) stay on the exact
+    fixed-point sums: two runs are bitwise identical and equal the mixed
+    path's (which uses the same integers)."""
+    Xt, init = lo.gaussian_blobs(60_000, 32, 64, seed=31, center_box=20.0,
+                                 std_range=(0.5, 2.0))
+    w = np.ones(Xt.shape[1], np.float32)
+    a = run_driver(kd, Xt, w, init, 0.0, 4)
+    b = run_driver(kd, Xt, w, init, 0.0, 4)
+    assert_array_equal(a[0], b[0])
+    assert a[1].tobytes() == b[1].tobytes()

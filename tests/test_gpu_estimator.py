@@ -148,3 +148,23 @@ def test_bad_parameters(kd):
         est = kd.KMeans(2, init=np.array([[0.0, 0.0], [4.0, 4.0]]), n_init=3).fit(
             np.arange(10, dtype=float).reshape(5, 2))
     assert est.n_iter_ >= 1
+
+
+def test_callable_init_matches_sklearn(kd):
+    """A callable init sees the centred X and its result is used in centred
+    coordinates (scikit-learn's rule); n_init='auto' runs it 10 times.  The
+    callable halves the chosen samples, so it does not commute with the
+    centring: passing uncentred X would give other seeds."""
+    X = _blobs(11, 12000, 6, 9, np.float64) + 50.0
+
+    def half_samples(Xc, k, random_state):
+        return np.asarray(Xc)[random_state.choice(len(Xc), k, replace=False)] * 0.5
+
+    for n_init in (1, "auto"):
+        ours = kd.KMeans(9, init=half_samples, n_init=n_init, max_iter=100,
+                         tol=1e-6, random_state=5).fit(X)
+        ref = skc.KMeans(9, init=half_samples, n_init=n_init, max_iter=100,
+                         tol=1e-6, random_state=5, algorithm="lloyd").fit(X)
+        assert_array_equal(ours.labels_, ref.labels_)
+        assert _rel(ours.cluster_centers_, ref.cluster_centers_) < 1e-9
+        assert ours.inertia_ == pytest.approx(ref.inertia_, rel=1e-9)

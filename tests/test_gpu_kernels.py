@@ -13,7 +13,7 @@ from scipy.spatial.distance import cdist
 
 from oracle import c_oracle as co
 from oracle import lloyd_oracle as lo
-from tests.conftest import CORNERS, scal, to_dev, to_np
+from tests.conftest import CORNERS, check_label_mismatches, scal, to_dev, to_np
 
 pytestmark = pytest.mark.gpu
 
@@ -433,8 +433,7 @@ def test_fused_step_matches_oracle(kd, torch_dev, dt, shape, n_copies):
     kd.fused_lloyd_single_step(to_dev(Xt), to_dev(w), to_dev(init), to_dev(hn), ids, cp, scp,
                                8, 256, q())[0].wait()
     lab = to_np(ids).astype(np.int64)
-    exp = co.assign(Xt, init, hn)
-    assert (lab != exp).sum() <= max(1, n // 10000)
+    check_label_mismatches(Xt, init, lab, co.assign(Xt, init, hn), limit=max(1, n // 10000))
     sizes = to_np(scp).sum(axis=0) - 2.0 * n_copies
     assert_array_equal(sizes, np.bincount(lab, minlength=k))
     sums = to_np(cp).astype(np.float64).sum(axis=0) - 0.5 * n_copies

@@ -41,10 +41,11 @@ def same(got, ref):
     assert got[3] == ref[3]
 
 
-def test_engine_fits_joining_mid_run_equal_alone(kd, torch_dev):
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_engine_fits_joining_mid_run_equal_alone(kd, torch_dev, precision):
     probs = [problem(3000 + 500 * i, [16, 32, 64][i % 3], [8, 32, 64][i % 3], 40 + i)
              for i in range(6)]
-    eng = kd.Engine(torch.cuda.current_stream())
+    eng = kd.Engine(torch.cuda.current_stream(), precision)
     outs = {}
 
     def add(i):
@@ -70,7 +71,8 @@ def test_engine_fits_joining_mid_run_equal_alone(kd, torch_dev):
         same(results[i], alone(kd, X, w, init, 1e-4, 40))
 
 
-def test_server_results_equal_alone_under_a_budget(kd, torch_dev):
+@pytest.mark.parametrize("small_precision", ["fp32", "bf16"])
+def test_server_results_equal_alone_under_a_budget(kd, torch_dev, small_precision):
     from kmeans_dpcpp_amd import serving
     probs = [problem(2000 + 700 * i, [16, 32, 64, 8][i % 4], [8, 32, 64, 16][i % 4],
                      70 + i) for i in range(10)]
@@ -78,7 +80,8 @@ def test_server_results_equal_alone_under_a_budget(kd, torch_dev):
     foot = max(serving.request_footprint(X.shape[1], X.shape[0], init.shape[1])
                for X, _, init in probs)
     budget = 3 * foot + serving.request_footprint(20_000, 100, 40)
-    with serving.Server(budget, iterations_per_tick=2) as srv:
+    with serving.Server(budget, iterations_per_tick=2,
+                        small_precision=small_precision) as srv:
         reqs = []
         for i, (X, w, init) in enumerate(probs):
             reqs.append(srv.submit(X, w, init, tol=1e-4, max_iter=50))

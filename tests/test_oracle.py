@@ -258,27 +258,69 @@ def test_number_of_private_copies_formula():
     assert lo.compute_number_of_private_copies(1024, 8, 10**6, 128, 1024, 0.1, 256) == 1
 
 
-# ---- planted mistakes: each must be caught by the checks above ----------
+# ---- planted mistakes: each is planted IN THE ORACLE (monkeypatched on the
+# CPU) and must make the pins above fail ---------------------------------
 
-def test_planted_sign_error_is_caught(golden):
-    Xt = np.ascontiguousarray(golden["assign_X_32"].T)
-    Ct = np.ascontiguousarray(CORNERS.T)
-    hn = lo.half_l2_norm_squared(Ct)
-    wrong = lo.assignment_from_pseudo(hn[:, None] + lo.dot_products(Xt, Ct))  # + instead of -
-    assert not np.array_equal(wrong, np.repeat(np.arange(8), 32))
-
-
-def test_planted_transposed_operand_is_caught(golden):
-    sc = golden["rcde_sizes_copies"]
-    wrong_sizes = sc.sum(axis=1)     # summing over clusters instead of copies
-    assert wrong_sizes.shape != (4,) or not np.allclose(wrong_sizes, [4, 4, 6, 0])
-    sizes, *_ = lo.reduce_centroids_data(sc, golden["rcde_cent_copies"], np.full(4, 10), 0)
-    assert_allclose(sizes, [4, 4, 6, 0])
+def _pins_fail(fn, *args):
+    try:
+        fn(*args)
+    except AssertionError:
+        return True
+    return False
 
 
-def test_planted_dropped_term_is_caught(golden):
-    X = golden["dist_X"]
-    Xt = np.ascontiguousarray(X.T)
-    Ct = np.ascontiguousarray(CORNERS.T)
-    wrong = np.sqrt(np.sum((Ct[:2, :, None] - Xt[:2, None, :]) ** 2, axis=0))  # drops feature 2
-    assert not np.allclose(wrong, cdist(CORNERS, X), rtol=1e-6)
+def test_planted_sign_error_is_caught(golden, monkeypatch):
+    good = lo.dot_products
+    monkeypatch.setattr(lo, "dot_products", lambda X_t, C_t: -good(X_t, C_t))
+    assert _pins_fail(test_assignment_kat_and_brute_force, golden, np.float64)
+
+
+def test_planted_transposed_operand_is_caught(golden, monkeypatch):
+    good = lo.reduce_centroids_data
+
+    def transposed(sizes_copies, cent_copies, lst, n_empty):
+        # each copy's (d, k) table read as if it were stored (k, d)
+        cc = np.asarray(cent_copies)
+        c, d, k = cc.shape
+        return good(sizes_copies, cc.reshape(c, k, d).transpose(0, 2, 1), lst, n_empty)
+    monkeypatch.setattr(lo, "reduce_centroids_data", transposed)
+    assert _pins_fail(test_reduce_centroids_kat, golden, np.float64)
+
+
+def test_planted_transposed_dot_is_caught(golden, monkeypatch):
+    def transposed(X_t, C_t):
+        # features of the centroid table taken in reverse order
+        return np.asarray(C_t)[::-1].T @ np.asarray(X_t)
+    monkeypatch.setattr(lo, "dot_products", transposed)
+    assert _pins_fail(test_assignment_kat_and_brute_force, golden, np.float64)
+
+
+def test_planted_dropped_term_is_caught(golden, monkeypatch):
+    good = lo.compute_distances
+    monkeypatch.setattr(lo, "compute_distances",
+                        lambda X_t, C_t: good(np.asarray(X_t)[:-1], np.asarray(C_t)[:-1]))
+    assert _pins_fail(test_distances_vs_cdist, golden, np.float64)
+
+
+def test_planted_missing_half_is_caught(golden, monkeypatch):
+    good = lo.half_l2_norm_squared
+    monkeypatch.setattr(lo, "half_l2_norm_squared", lambda C_t: 2 * good(C_t))
+    assert _pins_fail(test_half_l2_norm_kat, golden, np.float64)
+
+
+def test_planted_weight_before_sum_dropped_is_caught(golden, monkeypatch):
+    good = lo.relocate_empty_clusters
+
+    def no_weight(n_empty, X_t, w, *rest):
+        return good(n_empty, X_t, np.ones_like(w) * 2, *rest)
+    monkeypatch.setattr(lo, "relocate_empty_clusters", no_weight)
+    assert _pins_fail(test_relocate_kat, golden, np.float64)
+
+
+def test_planted_tie_rule_is_caught(monkeypatch):
+    def last_index_wins(pseudo):
+        pseudo = np.asarray(pseudo)
+        k = pseudo.shape[0]
+        return (k - 1 - np.argmin(pseudo[::-1], axis=0)).astype(np.int64)
+    monkeypatch.setattr(lo, "assignment_from_pseudo", last_index_wins)
+    assert _pins_fail(test_assignment_tie_and_nan_rules)

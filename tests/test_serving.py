@@ -130,3 +130,83 @@ def test_latency_summary():
     s = serving.latency_summary([R(x / 100) for x in range(1, 101)])
     assert s["count"] == 100
     assert abs(s["p50"] - 0.5) <= 0.011 and abs(s["p99"] - 0.99) <= 0.011
+
+
+class FailingEngine(FakeEngine):
+    """Steps fine until `fail` is set, then raises (as a HIP error would)."""
+
+    def __init__(self, fail):
+        super().__init__()
+        self.fail = fail
+
+    def step(self, iterations, use_graph):
+        if self.fail.is_set():
+            raise RuntimeError("hipErrorLaunchFailure (stand-in)")
+        return super().step(iterations, use_graph)
+
+
+def test_engine_error_finishes_every_request_and_closes_submit():
+    fp = lambda n, d, k: n * d  # noqa: E731
+    fail = threading.Event()
+    srv = serving.Server(10**9, engine_factory=lambda: FailingEngine(fail),
+                         footprint=fp, serves=lambda n, d, k: True)
+    reqs = [srv.submit(*make(64, 4, 3, seed=i), max_iter=10**6) for i in range(6)]
+    fail.set()
+    for r in reqs:
+        with pytest.raises(RuntimeError):
+            r.result(timeout=10)
+        assert r.done()
+    with pytest.raises(RuntimeError):
+        srv.submit(*make(64, 4, 3), max_iter=1)
+    srv.close()
+
+
+def test_engine_creation_error_fails_requests():
+    def boom():
+        raise RuntimeError("no device (stand-in)")
+    srv = serving.Server(10**9, engine_factory=boom, footprint=lambda n, d, k: 1,
+                         serves=lambda n, d, k: True)
+    srv._loop.join(timeout=10)
+    with pytest.raises(RuntimeError):
+        srv.submit(*make(64, 4, 3), max_iter=1)
+    srv.close()
+
+
+def test_engine_serves_matches_native_limits():
+    assert serving.engine_serves(1000, 64, 256)
+    assert not serving.engine_serves(1000, 65, 8)
+    assert not serving.engine_serves(1000, 16, 257)
+    assert not serving.engine_serves(100, 16, 101)
+    assert not serving.engine_serves(1 << 30, 16, 8)
+
+
+class WideRangeEngine(FakeEngine):
+    """Refuses fits of d == 5 as the native engine refuses data whose range
+    needs floating-point sums."""
+
+    def add(self, X, w, init, labels, res, tol, max_iter):
+        if X.shape[0] == 5:
+            raise ValueError("batched fit: the range of X or sample_weight needs "
+                             "floating-point sums; use the single-fit driver")
+        return super().add(X, w, init, labels, res, tol, max_iter)
+
+
+def test_fits_the_engine_refuses_for_range_go_to_the_single_fit_driver():
+    ran = []
+
+    def runner(req):
+        ran.append(req.shape)
+        req.labels = torch.zeros(req.shape[0], dtype=torch.int32)
+        req.centroids = req.init_centroids_t.clone()
+        return 4, 2.5
+
+    eng = WideRangeEngine()
+    with serving.Server(1 << 30, engine_factory=lambda: eng, large_runner=runner,
+                        footprint=lambda n, d, k: 1,
+                        serves=lambda n, d, k: True) as srv:
+        wide = srv.submit(*make(200, 5, 3), max_iter=3)
+        plain = srv.submit(*make(200, 6, 3), max_iter=3)
+        assert wide.result(timeout=10)[2:] == (4, 2.5)
+        plain.result(timeout=10)
+    assert ran == [(200, 5, 3)]
+    assert not wide.batched and plain.batched
