@@ -199,6 +199,25 @@ int kmd_fused_lloyd_single_step(
     void *assignments_idx, void *new_centroids_t_private_copies,
     void *cluster_sizes_private_copies, kmd_stream_t stream);
 
+/* kmd_fused_lloyd_single_step without any device allocation.  No
+ * counterpart in the reference: for callers that cannot allocate -- code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Same arguments, kernels and results as
+ * kmd_fused_lloyd_single_step; its working data (fixed-point scales and
+ * their pre-pass statistics, or the prepared centroid tiles of the shapes
+ * beyond LDS) lives in one 1 MiB device array per GPU, held until the
+ * process ends.  Shapes that need more than 1 MiB (float32 with d <= 128 and
+ * a centroid table beyond LDS: about 4 k (d + 1) bytes, e.g. d = 128 with
+ * k > 2016) return KMD_EVALUE.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams or threads) race. */
+int kmd_fused_lloyd_single_step_noalloc(
+    int dtype, int itype, int64_t n_samples, int64_t n_features,
+    int64_t n_clusters, int64_t n_copies, int64_t centroids_window_height,
+    int64_t work_group_size, const void *X_t, const void *sample_weight,
+    const void *centroids_t, const void *centroids_half_l2_norm,
+    void *assignments_idx, void *new_centroids_t_private_copies,
+    void *cluster_sizes_private_copies, kmd_stream_t stream);
+
 /* *out = max(1, min(floor(cache * occupancy / (sizeof(T) k (d + 1))),
  * ceil(n / wg) * wg / 8)) with cache = L2 size of `device`.  Replaces
  * py_compute_number_of_private_copies (python_api/_kmeans_lloyd.cpp:1111-1147)

@@ -21,7 +21,7 @@ BUILD = os.path.join(HERE, "_build")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 HIP_SOURCES = ["util.hip", "sort.hip", "lloyd.hip", "lloyd_fast.hip",
-               "lloyd_mixed.hip", "lloyd_tiled.hip", "batched.hip", "seeding.hip",
+               "lloyd_mixed.hip", "lloyd_tiled.hip", "batched.hip", "seeding.hip", "noalloc.hip",
                "driver.hip", "capi.hip"]
 LIB_NAME = "libkmeans_dpcpp_amd.so"
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
