@@ -27,6 +27,7 @@ METRIC = "Lloyd iters/s + % HBM/MFMA roofline; served fits/s with p50/p99 latenc
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
+FP64_PEAK_TFLOPS = 78.6
 
 N, D, K = 10_000_000, 32, 256
 SEED = 1001
@@ -37,7 +38,10 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "bf16"])
+    p.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "bf16", "fp8"])
+    p.add_argument("--dtype", default="f32", choices=["f32", "f64"],
+                   help="c1: element type of X, weights and centroids (f64: "
+                        "the reference's double-precision entry points)")
     p.add_argument("--n", type=int, default=N)
     p.add_argument("--d", type=int, default=D)
     p.add_argument("--k", type=int, default=K)
@@ -69,6 +73,9 @@ def parse():
                         "FP32 figure")
     p.add_argument("--extra-requests", type=int, default=200,
                    help="c1 extras: requests of the served stream")
+    p.add_argument("--saturated-rate", type=float, default=2000.0,
+                   help="c1 extras: arrival rate of the second served stream "
+                        "(above what either form serves)")
     p.add_argument("--requests", type=int, default=400, help="c4 requests")
     p.add_argument("--fits", type=int, default=512)
     p.add_argument("--copy-probe-mib", type=int, default=0,
@@ -152,6 +159,38 @@ def fp64_update(X_t, w, labels, k):
     return (sums / cnt[:, None]).T, cnt
 
 
+def fp64_inertia(X_t, w, labels, C_t):
+    """FP64 sum of w * ||x - c_label||^2 for the given labels and centroids."""
+    import torch
+    C64 = C_t.double()
+    lab = labels.long()
+    tot = torch.zeros((), dtype=torch.float64, device=X_t.device)
+    chunk = 1 << 20
+    for s in range(0, X_t.shape[1], chunk):
+        e = min(X_t.shape[1], s + chunk)
+        diff = X_t[:, s:e].double() - C64[:, lab[s:e]]
+        tot += (w[s:e].double() * (diff * diff).sum(0)).sum()
+    return float(tot)
+
+
+def fp64_errors(X_t, w, labels_prev, centroids, labels, inertia):
+    """Error of one fit's result against FP64: the centroids against one FP64
+    update from the labels of the iteration before the last (clusters empty
+    in those labels left out: they were relocated), the inertia against the
+    FP64 inertia of the final labels and centroids."""
+    k = centroids.shape[1]
+    c64, cnt = fp64_update(X_t, w, labels_prev, k)
+    ok = cnt > 0
+    c_err = 0.0
+    if bool(ok.any()):
+        c_err = float((centroids.double() - c64)[:, ok].abs().max()
+                      / c64[:, ok].abs().max().clamp_min(1e-300))
+    i64 = fp64_inertia(X_t, w, labels, centroids)
+    i_err = abs(float(inertia) - i64) / max(abs(i64), 1e-300)
+    return {"centroid_rel_err": c_err, "inertia_rel_err": i_err,
+            "empty_before_last_update": int((~ok).sum())}
+
+
 def bench_c2(args):
     """Compute-bound single fit: the mixed-precision driver against plain FP32
     on the same data, with an FP64 check of the last centroid update."""
@@ -187,10 +226,7 @@ def bench_c2(args):
     # FP64 check: centroids after `steps` iterations against one FP64 update
     # from the labels the mixed form gives for its centroids after steps - 1
     _, lab_prev, _, _, _ = run(prec, steps - 1, False)
-    c64, cnt = fp64_update(X_t, w, lab_prev, k)
-    ok = cnt > 0
-    diff = (cen_m.double() - c64)[:, ok].abs()
-    rel = float(diff.max() / c64[:, ok].abs().max())
+    err = fp64_errors(X_t, w, lab_prev, cen_m, lab_m, inr_m)
 
     kern_ms = st_m["fused_kernel_ms"] / max(1, st_m["fused_launches"])
     flops = 2.0 * n * k * d + 2.0 * n * d
@@ -238,8 +274,7 @@ def bench_c2(args):
         "centroids_identical_to_fp32": bool(torch.equal(cen_m.view(torch.int32),
                                                         cen_p.view(torch.int32))),
         "inertia": [inr_m, inr_p],
-        "centroid_rel_err_vs_fp64_update": rel,
-        "empty_clusters_at_check": int((~ok).sum()),
+        "error_vs_fp64": err,
     }
     print(json.dumps(line), flush=True)
 
@@ -712,7 +747,11 @@ def main():
     dev = torch.device("cuda", 0)
     n, d, k = args.n, args.d, args.k
     X_t, init_t = make_data(n, d, k, SEED, dev)
-    w = torch.ones(n, dtype=torch.float32, device=dev)
+    if args.dtype == "f64":
+        if args.precision != "fp32":
+            sys.exit("--dtype f64 runs the FP64 arithmetic only (--precision fp32)")
+        X_t, init_t = X_t.double(), init_t.double()
+    w = torch.ones(n, dtype=X_t.dtype, device=dev)
     labels = torch.empty(n, dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream()
     copy_probe(args.copy_probe_mib, dev)
@@ -736,14 +775,26 @@ def main():
     el = time.perf_counter() - t0
     assert n_iter == args.steps, (n_iter, args.steps)
 
+    # error against FP64 (untimed): the last update against an FP64 update of
+    # the labels one iteration earlier, the inertia against FP64
+    lab_prev = torch.empty_like(labels)
+    c_p, c_q = init_t.clone(), torch.empty_like(init_t)
+    kd.kmeans_lloyd_driver_ex(X_t, w, c_p, lab_prev, c_q, 0.0, False, args.steps - 1,
+                              args.precision, stream)
+    err = fp64_errors(X_t, w, lab_prev, c_b, labels, inertia)
+    del lab_prev, c_p, c_q
+
     iters_per_s = n_iter / el
     ms_step = 1e3 * el / n_iter
     kern_ms = stats["fused_kernel_ms"] / max(1, stats["fused_launches"])
-    bytes_per_launch = n * d * 4 + n * 4 + n * 4      # X, weights, labels
+    esz = X_t.element_size()
+    bytes_per_launch = n * d * esz + n * esz + n * 4      # X, weights, labels
     flops_per_launch = 2.0 * n * k * d + 2.0 * n * d
     hbm_gbs = bytes_per_launch / (kern_ms * 1e-3) / 1e9
     tflops = flops_per_launch / (kern_ms * 1e-3) / 1e12
-    if args.precision == "fp32":
+    if args.dtype == "f64":
+        compute_peak = FP64_PEAK_TFLOPS
+    elif args.precision == "fp32":
         compute_peak = FP32_PEAK_TFLOPS
     else:
         compute_peak = BF16_PEAK_TFLOPS
@@ -754,7 +805,8 @@ def main():
     # when the shape is the one profiled)
     traffic = (traffic_from_profile(os.path.join(here, "profiles",
                                                  "fused_step_n10M_d32_k256_pmc.md"))
-               if (n, d, k, args.precision) == (N, D, K, "fp32") else None)
+               if (n, d, k, args.precision, args.dtype) == (N, D, K, "fp32", "f32")
+               else None)
     if t_cmp >= t_hbm:
         roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": compute_peak,
                 "unit": "TFLOP/s", "frac": round(tflops / compute_peak, 4)}
@@ -769,7 +821,7 @@ def main():
     roof["algorithmic_flops"] = flops_per_launch
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.dtype == "f32":
         cpu = cpu_baseline(X_t, init_t, n, d, k, args.cpu_seconds)
 
     line = {
@@ -783,7 +835,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": (args.dtype if args.precision == "fp32"
+                  else f"{args.precision} cross term + f32 re-score"),
         "data": "synthetic",
         "config": {"workload": f"plain Lloyd, n={n:,} d={d} k={k} Gaussian blobs, "
                                f"tol=0, one iteration per step",
@@ -792,15 +845,21 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "inertia": inertia,
+        "error_vs_fp64": err,
         "path": stats.get("path"),
         "rescored_per_iter": stats.get("n_rescored", 0) / max(1, n_iter),
     }
-    if not args.no_extras and args.precision == "fp32":
+    if not args.no_extras and args.precision == "fp32" and args.dtype == "f32":
         # beside the plain FP32 figure: the same fit with the BF16 cross term
         # (HBM roofline), the batched many-fit call and the served stream
         line["mixed_bf16"] = run_c1_mixed(args, X_t, w, init_t, labels, c_b, inertia)
+        line["reference_api_fused_step"] = run_c1_fused_step(args, X_t, w, init_t)
         line["batched_c3"] = run_c3(args.fits, "bf16")
         line["served_c4"] = run_c4(args.extra_requests, args.rate, "bf16", "bf16")
+        # arrivals faster than either form serves them: capacity, not the
+        # arrival rate, bounds both fits/s figures
+        line["served_c4_saturated"] = run_c4(args.extra_requests, args.saturated_rate,
+                                             "bf16", "bf16")
     print(json.dumps(line), flush=True)
 
 
@@ -841,6 +900,48 @@ def run_c1_mixed(args, X_t, w, init_t, labels_fp32, cent_fp32, inertia_fp32):
         "centroids_identical_to_fp32": bool(torch.equal(c_b.view(torch.int32),
                                                         cent_fp32.view(torch.int32))),
         "inertia_identical_to_fp32": inertia == inertia_fp32,
+    }
+
+
+def run_c1_fused_step(args, X_t, w, init_t):
+    """The reference's own entry point on the same data: fused_lloyd_single_step
+    (labels + private-copy sums from the seeded centroids), timed per call
+    with HIP events on the stream it runs on; its labels are checked against
+    the assignment entry point's."""
+    import torch
+    import kmeans_dpcpp_amd as kd
+    stream = torch.cuda.current_stream()
+    d, n = X_t.shape
+    k = init_t.shape[1]
+    hn = torch.empty(k, dtype=torch.float32, device=X_t.device)
+    kd.half_l2_norm_squared(init_t, hn, 256, stream)
+    n_copies = 4
+    ids = torch.empty(n, dtype=torch.int32, device=X_t.device)
+    cp = torch.zeros((n_copies, d, k), dtype=torch.float32, device=X_t.device)
+    scp = torch.zeros((n_copies, k), dtype=torch.float32, device=X_t.device)
+    calls = max(1, args.steps)
+    for _ in range(2):
+        kd.fused_lloyd_single_step(X_t, w, init_t, hn, ids, cp, scp, 8, 128, stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(calls):
+        kd.fused_lloyd_single_step(X_t, w, init_t, hn, ids, cp, scp, 8, 128, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / calls
+    ref = torch.empty_like(ids)
+    kd.assignment(X_t, init_t, hn, ref, 8, 128, stream)
+    torch.cuda.synchronize()
+    counts = scp.double().sum(0) / (calls + 2)
+    flops = 2.0 * n * k * d + 2.0 * n * d
+    return {
+        "ms_per_call": round(ms, 4),
+        "n_copies": n_copies,
+        "frac_fp32_peak": round(flops / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+        "labels_equal_assignment": bool(torch.equal(ids, ref)),
+        "counts_equal_bincount": bool(torch.equal(
+            counts, torch.bincount(ref.long(), minlength=k).double())),
     }
 
 

@@ -42,8 +42,13 @@ enum { KMD_I32 = 0, KMD_I64 = 1 };
  *                    exact FP32 rescore of every sample whose best/second-best
  *                    margin is below the split's error bound.
  * KMD_PREC_BF16    : one-term BF16 on MFMA + the same bounded FP32 rescore.
- * Labels are those of the FP32 form in all three modes. */
-enum { KMD_PREC_FP32 = 0, KMD_PREC_BF16X3 = 1, KMD_PREC_BF16 = 2 };
+ * KMD_PREC_FP8     : FP8 e4m3 on MFMA with a power-of-two scale per
+ *                    32-centroid tile (and one for the samples), FP32
+ *                    accumulate, + the bounded FP32 rescore (e4m3's error
+ *                    bound; float32 data with n_features <= 128 on the
+ *                    tiled kernels, else the FP32 form).
+ * Labels are those of the FP32 form in every mode. */
+enum { KMD_PREC_FP32 = 0, KMD_PREC_BF16X3 = 1, KMD_PREC_BF16 = 2, KMD_PREC_FP8 = 3 };
 
 typedef void *kmd_stream_t;
 typedef void *kmd_event_t;
@@ -64,6 +69,10 @@ int kmd_stream_wait_event(kmd_stream_t stream, kmd_event_t event);
 int kmd_stream_synchronize(kmd_stream_t stream);
 int kmd_stream_create(kmd_stream_t *stream);
 int kmd_stream_destroy(kmd_stream_t stream);
+/* out[i] = in[i] rounded to FP8 e4m3 and back, with the conversion the
+ * KMD_PREC_FP8 kernels use (device arrays of n floats): lets a test pin the
+ * rounding their error bound assumes. */
+int kmd_fp8_roundtrip(int64_t n, const float *in, float *out, kmd_stream_t stream);
 
 /* ---- the 14 reference entry points ------------------------------------ */
 
@@ -203,9 +212,9 @@ int kmd_fused_lloyd_single_step(
  * counterpart in the reference: for callers that cannot allocate -- code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc.  Same arguments, kernels and results as
- * kmd_fused_lloyd_single_step; its working data (fixed-point scales and
- * their pre-pass statistics, or the prepared centroid tiles of the shapes
- * beyond LDS) lives in one 1 MiB device array per GPU, held until the
+ * kmd_fused_lloyd_single_step; its working data (the prepared centroid
+ * tiles of the shapes beyond LDS; the LDS-resident shapes need none) lives
+ * in one 1 MiB device array per GPU, held until the
  * process ends.  Shapes that need more than 1 MiB (float32 with d <= 128 and
  * a centroid table beyond LDS: about 4 k (d + 1) bytes, e.g. d = 128 with
  * k > 2016) return KMD_EVALUE.  Every call on one GPU shares that GPU's
@@ -260,7 +269,7 @@ int kmd_iterative_merge_sort(int dtype, int64_t n, const void *input,
  * kernel launches (in the mixed modes: BF16 kernel + FP32 re-score),
  * *fused_launches = their number; *fused_path = 0 generic kernel, 1 FP32
  * MFMA, 2 BF16 + re-score, 3 BF16x3 + re-score, 4/5/6 the same three on the
- * tiled path (d <= 128 with centroids or sums beyond LDS: centroid tiles
+ * tiled path, 7 FP8 + re-score on the tiled path (d <= 128 with centroids or sums beyond LDS: centroid tiles
  * streamed through LDS, update as a separate exact pass); a mixed mode falls
  * back to FP32 for shapes it does not serve or non-finite inputs. */
 int kmd_kmeans_lloyd_driver_ex(

@@ -34,6 +34,7 @@ from ._kmeans_dpcpp_amd import (  # noqa: E402
     compute_inertia,
     compute_number_of_private_copies,
     compute_threshold,
+    fp8_roundtrip,
     fused_lloyd_single_step,
     gather_columns,
     half_l2_norm_squared,

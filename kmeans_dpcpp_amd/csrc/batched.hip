@@ -1293,11 +1293,15 @@ void launch_assign_nt(const BFit *fits, const AItem *items, int n_items, int all
 }
 
 // mixed: the BF16 cross term with the certified FP32 re-score (the same
-// labels as the FP32 kernel); shape: 0 or 1 (assign_grid_nt)
+// labels as the FP32 kernel) for the d <= 32 buckets; the d <= 64 bucket
+// runs FP32 either way (its BF16 kernel measured slower: 430 against
+// 385 us per launch on the 512-fit workload).  shape: 0 or 1
+// (assign_grid_nt)
 template <int KS>
 void launch_assign_bucket(const BFit *fits, const AItem *items, int n_items,
                           int all, hipStream_t s, bool mixed, int shape) {
   if (n_items <= 0) return;
+  mixed = mixed && KS <= 16;
   if (shape == 1)
     launch_assign_nt<KS, assign_threads(KS) / 2>(fits, items, n_items, all, s, mixed);
   else

@@ -623,8 +623,9 @@ int kmd_kmeans_lloyd_driver_ex(
   return guarded([&] {
     check_types(dtype, itype);
     require(precision == KMD_PREC_FP32 || precision == KMD_PREC_BF16X3 ||
-                precision == KMD_PREC_BF16,
-            "precision must be KMD_PREC_FP32, KMD_PREC_BF16X3 or KMD_PREC_BF16");
+                precision == KMD_PREC_BF16 || precision == KMD_PREC_FP8,
+            "precision must be KMD_PREC_FP32, KMD_PREC_BF16X3, KMD_PREC_BF16 or "
+            "KMD_PREC_FP8");
     require(tol >= 0.0, "Tolerance must be non-negative");
     require(max_iter >= 0, "max_iter must be non-negative");
     require(n_samples >= 0 && n_features >= 1 && n_clusters >= 1,
@@ -828,8 +829,9 @@ int kmd_autotune(int dtype, int itype, int precision, int64_t n_samples,
   return guarded([&] {
     check_types(dtype, itype);
     require(precision == KMD_PREC_FP32 || precision == KMD_PREC_BF16X3 ||
-                precision == KMD_PREC_BF16,
-            "precision must be KMD_PREC_FP32, KMD_PREC_BF16X3 or KMD_PREC_BF16");
+                precision == KMD_PREC_BF16 || precision == KMD_PREC_FP8,
+            "precision must be KMD_PREC_FP32, KMD_PREC_BF16X3, KMD_PREC_BF16 or "
+            "KMD_PREC_FP8");
     require(n_samples >= 1 && n_features >= 1 && n_clusters >= 1,
             "Array dimensions are not consistent");
     require(iters >= 1, "iters must be positive");
@@ -952,6 +954,15 @@ int kmd_kmeans_plusplus(int dtype, int64_t n_samples, int64_t n_features,
                                 (const T *)sample_weight, u_dev.as<double>(),
                                 (T *)centers_t, indices, s);
     });
+  });
+}
+
+int kmd_fp8_roundtrip(int64_t n, const float *in, float *out, kmd_stream_t stream) {
+  return guarded([&] {
+    require(n >= 0, "negative size");
+    check_ptr(in, n, "in");
+    check_ptr(out, n, "out");
+    launch_f8_roundtrip(n, in, out, as_stream(stream));
   });
 }
 

@@ -47,9 +47,12 @@ struct LloydWorkspace {
   int blocks = 1;          // private copies / partial slots
   bool fast = false;       // MFMA fast path (fixed-point partials)
   bool store_flush = false;
-  int mixed_terms = 0;     // BF16 split terms (1 or 3) of the mixed path, 0: FP32
+  int mixed_terms = 0;     // BF16 split terms (1 or 3) or 8 (FP8) of the mixed path, 0: FP32
   bool tiled = false;      // tiled path: centroids streamed, separate update
   int blocks_b = 0;        // partial slots of the FP32 re-score (mixed path)
+  // LDS-resident mixed path: the re-score restricted to candidate tiles,
+  // from the sample-major copy xr, and the queue's separate exact update
+  bool mixed_cand = false;
   FusedPlan plan;
   TempBuffer hn, sizes, shifts, per_sample, copies, sizes_copies, list,
       scalars, scales;
@@ -93,6 +96,18 @@ struct LloydWorkspace {
   }
 };
 
+// KMD_MIXED_CAND=1 switches the LDS-resident mixed path's re-score from the
+// gather of every queued sample over all tiles (the default: one pass that
+// also adds the re-scored samples to the sums) to the candidate-tile
+// re-score from a sample-major copy of X plus a separate exact update of the
+// queue.  The candidate form moves fewer bytes per queued sample but costs
+// four more launches per iteration; at n = 10M, d = 32, k = 256 it measured
+// 630 iters/s against the gather's 795, so it is not the default.
+static bool mixed_cand_requested() {
+  const char *e = std::getenv("KMD_MIXED_CAND");
+  return e != nullptr && e[0] == '1';
+}
+
 // Mixed-precision workspace: BF16 kernel partials, then FP32 re-score
 // partials; returns nullptr when the shape is not served or the inputs hold
 // non-finite values (the FP32 path handles those).
@@ -102,7 +117,12 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_mixed_workspace(
     hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     if (n <= 0 || !fast_fused_supported<T>(d, k)) return nullptr;
-    const int ba = mixed_blocks<I>(n, d, k, terms);
+    // candidate tiles: the re-score of an undecided sample covers only the
+    // 32-centroid tiles whose BF16 best is within the bound of its best,
+    // reading the sample from a sample-major copy of X (made once per fit)
+    const bool cand = tiled_cand_supported(k) && tiled_supported(d, k) &&
+                      mixed_cand_fits(d, k, terms) && mixed_cand_requested();
+    const int ba = mixed_blocks<I>(n, d, k, terms, cand);
     const int bb = fast_rescore_blocks<I>(d, k, ba);
     if (ba < 1 || bb < 1) return nullptr;
     auto ws = std::make_unique<LloydWorkspace<T, I>>(
@@ -121,6 +141,13 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_mixed_workspace(
                                  hipMemcpyDeviceToHost, s));
     KMD_HIP_CHECK(hipStreamSynchronize(s));
     if (bad) return nullptr;
+    if (cand) {
+      ws->mixed_cand = true;
+      ws->qmask = TempBuffer(sizeof(unsigned) * (size_t)ba * ws->seg_cap, s);
+      ws->cand = TempBuffer(tiled_cand_scratch_bytes(n, k), s);
+      ws->xr = TempBuffer(sizeof(float) * (size_t)n * d, s);
+      launch_transpose(n, d, X, ws->xr.template as<float>(), s);
+    }
     return ws;
   } else {
     (void)n, (void)d, (void)k, (void)terms, (void)X, (void)w, (void)s;
@@ -151,25 +178,33 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_tiled_workspace(
     launch_transpose(n, d, X, ws->xr.template as<float>(), s);
     ws->upd = TempBuffer(tiled_update_scratch_bytes(n, k), s);
     if (precision != KMD_PREC_FP32) {
-      const int terms = precision == KMD_PREC_BF16X3 ? 3 : 1;
+      // BF16 terms, or 8: the FP8 (e4m3) kernel
+      const int terms = precision == KMD_PREC_BF16X3 ? 3
+                        : precision == KMD_PREC_FP8  ? 8
+                                                     : 1;
       const int ba = tiled_mixed_blocks<I>(n, d, k, terms);
       if (ba >= 1) {
         ws->mixed_terms = terms;
         ws->blocks_b = ba;
         ws->seg_cap = tiled_mixed_seg_cap(n, d, terms, ba);
         ws->aop = TempBuffer(tiled_aop_bytes(d, k, terms), s);
-        ws->bound = TempBuffer(2 * sizeof(float), s);
+        ws->bound = TempBuffer(sizeof(float) * tiled_bound_floats(), s);
         ws->queue = TempBuffer(sizeof(int) * (size_t)ba * ws->seg_cap, s);
         ws->queue_counts = TempBuffer(sizeof(int) * (size_t)ba, s);
         if (tiled_cand_supported(k)) {
           ws->qmask = TempBuffer(sizeof(unsigned) * (size_t)ba * ws->seg_cap, s);
           ws->cand = TempBuffer(tiled_cand_scratch_bytes(n, k), s);
         }
-        if (terms == 1) {
+        if (terms == 1 || terms == 8) {
           ws->xb = TempBuffer(tiled_xb_bytes(n, d), s);
           ws->xnb = TempBuffer(sizeof(float) * (size_t)n, s);
-          launch_tiled_xb_prep(n, d, X, ws->xb.template as<void>(),
-                               ws->xnb.template as<float>(), s);
+          if (terms == 1)
+            launch_tiled_xb_prep(n, d, X, ws->xb.template as<void>(),
+                                 ws->xnb.template as<float>(), s);
+          else
+            launch_tiled_x8_prep(n, d, X, ws->xb.template as<void>(),
+                                 ws->xnb.template as<float>(),
+                                 ws->bound.template as<float>(), s);
         }
       }
     }
@@ -251,8 +286,11 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_workspace(int64_t n,
                                                             const T *w,
                                                             int precision,
                                                             hipStream_t s) {
-  const bool tiled_first = force_tiled() || tuned_path() == 2;
-  if (precision != KMD_PREC_FP32 && !tiled_first) {
+  // the FP8 cross term is built on the tiled kernels only
+  const bool tiled_first = force_tiled() || tuned_path() == 2 ||
+                           (precision == KMD_PREC_FP8 && std::is_same<T, float>::value &&
+                            tiled_supported(d, k));
+  if (precision != KMD_PREC_FP32 && precision != KMD_PREC_FP8 && !tiled_first) {
     auto ws = make_mixed_workspace<T, I>(
         n, d, k, precision == KMD_PREC_BF16X3 ? 3 : 1, X, w, s);
     if (ws) return ws;
@@ -438,13 +476,32 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
                            ws.bound.template as<const float>(), ws.scale(),
                            labels, ws.copies.template as<void>(),
                            ws.queue.template as<int>(),
-                           ws.queue_counts.template as<int>(), s, skip);
-      launch_fast_rescore<I>(n, d, k, X, w, cur, ws.hn.template as<const T>(),
-                             labels, ws.scale(), ws.nonfinite(),
-                             ws.copies.template as<void>(), ws.blocks,
-                             ws.queue.template as<const int>(),
-                             ws.queue_counts.template as<const int>(),
-                             ws.blocks, ws.seg_cap, ws.queue_count(), s);
+                           ws.queue_counts.template as<int>(),
+                           ws.mixed_cand ? ws.qmask.template as<unsigned>() : nullptr, s,
+                           skip);
+      if (ws.mixed_cand) {
+        // FP32 re-score of the candidate tiles (labels), then the exact sums
+        // of the re-scored samples into the re-score partial slots
+        launch_tiled_cand_rescore<I>(
+            n, d, k, ws.xr.template as<const float>(), cur, ws.hn.template as<const T>(),
+            labels, ws.queue.template as<const int>(),
+            ws.queue_counts.template as<const int>(),
+            ws.qmask.template as<const unsigned>(), ws.blocks, ws.seg_cap,
+            ws.cand.template as<void>(), ws.queue_count(), s);
+        launch_queue_update<I>(d, k, ws.blocks_b, ws.blocks, ws.seg_cap,
+                               ws.queue.template as<const int>(),
+                               ws.queue_counts.template as<const int>(),
+                               ws.xr.template as<const float>(), w, labels, ws.scale(),
+                               ws.copies.template as<int64_t>() + (size_t)ws.blocks * (d * k + k),
+                               s, skip);
+      } else {
+        launch_fast_rescore<I>(n, d, k, X, w, cur, ws.hn.template as<const T>(),
+                               labels, ws.scale(), ws.nonfinite(),
+                               ws.copies.template as<void>(), ws.blocks,
+                               ws.queue.template as<const int>(),
+                               ws.queue_counts.template as<const int>(),
+                               ws.blocks, ws.seg_cap, ws.queue_count(), s);
+      }
       timer.end(s);
       launch_reduce_fixed_partials<T, I>(
           ws.blocks + ws.blocks_b, d, k, ws.copies.template as<const void>(),
@@ -760,6 +817,7 @@ static void driver_impl(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
     int path = ws.mixed_terms > 0 ? (ws.mixed_terms == 3 ? 3 : 2)
                                   : (ws.fast ? 1 : 0);
     if (ws.tiled) path += 3;  // 4 tiled FP32, 5 tiled BF16, 6 tiled BF16x3
+    if (ws.tiled && ws.mixed_terms == 8) path = 7;  // tiled FP8
     stats->fused_path = path;
     stats->n_rescored = rescored;
   }

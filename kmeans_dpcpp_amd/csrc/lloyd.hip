@@ -181,7 +181,7 @@ void launch_fused_generic(int64_t n, int64_t d, int64_t k, int64_t n_copies,
 
 template <typename T>
 size_t fused_step_scratch_bytes(int64_t d, int64_t k) {
-  if (fast_fused_supported<T>(d, k)) return fast_fused_scratch_bytes(d);
+  if (fast_fused_supported<T>(d, k)) return 0;  // FP32 sums in LDS, no scales
   if (std::is_same<T, float>::value && tiled_supported(d, k))
     return tiled_tiles_bytes(d, k);
   return 0;  // the generic kernel works in LDS and the caller's copies only
@@ -196,8 +196,8 @@ void launch_fused_step_copies(int64_t n, int64_t d, int64_t k,
   (void)h;
   if (n == 0) return;
   if (fast_fused_supported<T>(d, k)) {
-    launch_fast_fused<T, I>(n, d, k, n_copies, /*store_flush=*/false, X, w, C,
-                            hn, labels, copies, sizes_copies, s, scratch);
+    launch_fast_fused<T, I>(n, d, k, n_copies, X, w, C, hn, labels, copies,
+                            sizes_copies, s);
     return;
   }
   if constexpr (std::is_same<T, float>::value) {

@@ -168,6 +168,75 @@ __device__ inline void tile_argmin(int ct, const f32x16 (&acc)[S],
   }
 }
 
+// Exact binade-window sums (the reference-API fused step, which runs no
+// pass over X for scales).  A finite FP32 value v = m 2^(e - 150) (m the
+// 24-bit significand, e the biased exponent) whose binade e lies in
+// [lo, lo + W] is the integer m << (e - lo) in units of 2^(lo - 150), with
+// no rounding: those values go into the block's int64 LDS sums exactly
+// (|m << W| < 2^(24 + W) leaves 2^(39 - W) additions of headroom).  Values
+// outside the window (zeros aside), infinities and NaNs are added straight
+// into the caller's copy with an FP32 atomic, as the reference adds every
+// value.  lo is set per feature so that the window ends kWinMargin binades
+// above the largest centroid coordinate of that feature: samples lie near
+// their centroids, so nearly every value falls inside.
+constexpr int kWinMargin = 4;
+constexpr int kWinTopW = 20;   // weights: window up to 2^20
+
+__device__ inline void window_add(u64 *sum, float *fallback, int64_t off,
+                                  float v, int lo, int W) {
+  const unsigned b = __float_as_uint(v);
+  const int sh = (int)((b >> 23) & 0xffu) - lo;
+  if ((unsigned)sh <= (unsigned)W) {
+    const u64 m = (u64)((b & 0x7fffffu) | 0x800000u) << sh;
+    atomicAdd(sum + off, (b >> 31) ? (u64)0 - m : m);
+  } else if (v != 0.f) {
+    atomic_add(fallback + off, v);
+  }
+}
+
+// v_max3_f32, written out so that no canonicalisation is put in front of
+// it: NaN operands are passed over (IEEE mode), so a NaN score never wins.
+__device__ inline float vmax3(float a, float b, float c) {
+  float r;
+  asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// The same first minimum in (tile, row) order as tile_argmin, with less
+// vector work per tile: the tile's best (largest acc = smallest score) by a
+// max3 chain, and on a strict improvement the tile index and a copy of its
+// 16 accumulators; the row is resolved once per sample by best_row.
+template <int S>
+__device__ inline void tile_track_best(int ct, const f32x16 (&acc)[S],
+                                       float (&bestm)[S], int (&bestt)[S],
+                                       f32x16 (&besta)[S]) {
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    // a tree of depth 3 rather than a chain of 8
+    const float t0 = vmax3(acc[q][0], acc[q][1], acc[q][2]);
+    const float t1 = vmax3(acc[q][3], acc[q][4], acc[q][5]);
+    const float t2 = vmax3(acc[q][6], acc[q][7], acc[q][8]);
+    const float t3 = vmax3(acc[q][9], acc[q][10], acc[q][11]);
+    const float t4 = vmax3(acc[q][12], acc[q][13], acc[q][14]);
+    const float u0 = vmax3(t0, t1, t2);
+    const float u1 = vmax3(t3, t4, acc[q][15]);
+    const float m = vmax3(u0, u1, u1);
+    const bool upd = m > bestm[q];   // ties keep the earlier tile
+    bestm[q] = upd ? m : bestm[q];
+    bestt[q] = upd ? ct : bestt[q];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) besta[q][r] = upd ? acc[q][r] : besta[q][r];
+  }
+}
+
+// Lowest row of the kept tile that holds the best value.
+__device__ inline int best_row(const f32x16 &besta, float bestm) {
+  int r0 = 0;
+#pragma unroll
+  for (int r = 15; r >= 0; --r) r0 = (besta[r] == bestm) ? r : r0;
+  return r0;
+}
+
 __device__ inline int best_index(int t, int r, int half) {
   return t * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
 }
@@ -178,7 +247,10 @@ struct FastArgs {
   int d, k, kpad, vec;
   const float *X, *w, *C, *hn;
   const float *scale;       // [d + 1] fixed-point scales (2^E_f, 2^E_w)
-  const int *nonfinite;     // 1: X or w has a non-finite value
+  const int *nonfinite;     // non-zero: FP32 sums (X or w non-finite, or a
+                            // range the scales cannot hold)
+  int window;               // > 0: no scales; exact binade-window sums with
+                            // this many binades (the reference-API step)
   void *labels;
   void *copies;             // u64 [blocks][d*k + k] or float [n_copies][d*k]
   float *sizes_copies;      // float [n_copies][k] (atomic flush only)
@@ -205,6 +277,11 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
   // an iteration enqueued ahead of a relocation: nothing to compute
   if (a.skip != nullptr && *a.skip != 0) return;
   constexpr int kWavesPerBlock = NT / kWave;
+  // The best tile's accumulators are kept in registers (tile_track_best)
+  // where the sample operands leave room for 16 more without spilling (not
+  // beside the binade-window sums of the atomic-flush form).
+  constexpr bool kKeepTile =
+      KS * S <= 16 && (FULL || KS * S <= 12) && !(UPDATE && !STORE_FLUSH);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int64_t n = a.n;
   const int d = a.d, k = a.k, kpad = a.kpad;
@@ -220,7 +297,11 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
   const int tid = threadIdx.x;
   const int64_t dk = (int64_t)d * k;
   const int R = (UPDATE && a.n_priv > 1) ? a.n_priv : 1;
-  const bool fixed = UPDATE ? (a.nonfinite[0] == 0) : false;
+  // The atomic-flush form (the reference-API step) always sums in binade
+  // windows; the store-flush form (the driver) in fixed point or FP32.
+  constexpr bool window = UPDATE && !STORE_FLUSH;
+  const bool fixed = UPDATE && !window && a.nonfinite[0] == 0;
+  int *Ew = reinterpret_cast<int *>(Sc);   // window mode: lowest binade per row
   int *seg_off = reinterpret_cast<int *>(S64 + R * (dk + k));  // gather: [nseg+1]
   if constexpr (GATHER) {
     // exclusive prefix sum of the segment counts (one wave, 64 per step)
@@ -254,10 +335,27 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
   for (int c = tid; c < kpad; c += NT)
     Hl[c] = (c < k) ? -a.hn[c] : -__builtin_huge_valf();   // -hn, -inf padded
   if (UPDATE) {
-    for (int f = tid; f <= d; f += NT) Sc[f] = a.scale[f];
+    if (fixed)
+      for (int f = tid; f <= d; f += NT) Sc[f] = a.scale[f];
+    if constexpr (window)
+      for (int f = tid; f <= d; f += NT) Ew[f] = 0;
     for (int64_t i = tid; i < R * (dk + k); i += NT) S64[i] = 0ull;
   }
   __syncthreads();
+  if constexpr (window) {
+    // the window of feature f ends kWinMargin binades above the largest
+    // centroid coordinate of f (that of the weights at 2^kWinTopW)
+    for (int64_t i = tid; i < dk; i += NT) {
+      const int e = (int)((__float_as_uint(a.C[i]) >> 23) & 0xffu);
+      if (e != 0xff) atomicMax(&Ew[i / k], e);
+    }
+    __syncthreads();
+    for (int f = tid; f <= d; f += NT) {
+      const int top = f < d ? Ew[f] + kWinMargin : 127 + kWinTopW;
+      Ew[f] = min(max(top, a.window + 1), 254) - a.window;
+    }
+    __syncthreads();
+  }
 
   const int lane = tid & (kWave - 1);
   const int wave = tid / kWave;
@@ -330,25 +428,49 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
     }
 
     float bestv[S];
-    int bestt[S], bestr[S], besti[S];
-#pragma unroll
-    for (int q = 0; q < S; ++q) {
-      bestv[q] = __builtin_huge_valf();
-      bestt[q] = 0;
-      bestr[q] = 0;
-    }
+    int bestt[S], besti[S];
     const float *Cw = Cl + lane;
-    for (int ct = 0; ct < n_ct; ++ct) {
-      f32x16 acc[S];
-      tile_mfma<KS, S>(Cw + ct * (KS * 64), Hl + ct * 32 + 4 * half, xr, acc);
-      tile_argmin<S>(ct, acc, bestv, bestt, bestr);
-    }
-    // no row of any tile below +inf: centroid 0, as the scan would give
+    if constexpr (kKeepTile) {
+      float bestm[S];
+      f32x16 besta[S];
 #pragma unroll
-    for (int q = 0; q < S; ++q)
-      besti[q] = (bestv[q] < __builtin_huge_valf())
-                     ? best_index(bestt[q], bestr[q], half)
-                     : 0;
+      for (int q = 0; q < S; ++q) {
+        bestm[q] = -__builtin_huge_valf();
+        bestt[q] = 0;
+        besta[q] = f32x16{};
+      }
+      for (int ct = 0; ct < n_ct; ++ct) {
+        f32x16 acc[S];
+        tile_mfma<KS, S>(Cw + ct * (KS * 64), Hl + ct * 32 + 4 * half, xr, acc);
+        tile_track_best<S>(ct, acc, bestm, bestt, besta);
+      }
+      // no row of any tile below +inf: centroid 0, as the scan would give
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        bestv[q] = -bestm[q];
+        besti[q] = (bestv[q] < __builtin_huge_valf())
+                       ? best_index(bestt[q], best_row(besta[q], bestm[q]), half)
+                       : 0;
+      }
+    } else {
+      int bestr[S];
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        bestv[q] = __builtin_huge_valf();
+        bestt[q] = 0;
+        bestr[q] = 0;
+      }
+      for (int ct = 0; ct < n_ct; ++ct) {
+        f32x16 acc[S];
+        tile_mfma<KS, S>(Cw + ct * (KS * 64), Hl + ct * 32 + 4 * half, xr, acc);
+        tile_argmin<S>(ct, acc, bestv, bestt, bestr);
+      }
+#pragma unroll
+      for (int q = 0; q < S; ++q)
+        besti[q] = (bestv[q] < __builtin_huge_valf())
+                       ? best_index(bestt[q], bestr[q], half)
+                       : 0;
+    }
     // merge the two halves of the column (lanes l and l ^ 32)
 #pragma unroll
     for (int q = 0; q < S; ++q) {
@@ -384,14 +506,30 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
             if (half == 0)
               atomicAdd(&Sp[dk + lab],
                         (u64)(long long)__float2int_rn(ws * Sc[d]));
+          } else if constexpr (window) {
+            u64 *Sp = S64 + (int64_t)(lane & (R - 1)) * (dk + k);
+            const int64_t copy = blockIdx.x % a.n_copies;
+            float *cg = reinterpret_cast<float *>(a.copies) + copy * dk;
+#pragma unroll 4
+            for (int ks = 0; ks < KS; ++ks) {
+              const int f = 2 * ks + half;
+              if (FULL || f < d)
+                window_add(Sp, cg, (int64_t)f * k + lab, xr[ks][q] * ws, Ew[f],
+                           a.window);
+            }
+            if (half == 0)
+              window_add(Sp + dk, a.sizes_copies + copy * k, lab, ws, Ew[d],
+                         a.window);
           } else {
+            // FP32 sums, spread over the R copies like the integer ones
+            float *Sp = S32 + (int64_t)(lane & (R - 1)) * (dk + k);
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
               const int f = 2 * ks + half;
               if (FULL || f < d)
-                atomic_add(&S32[(int64_t)f * k + lab], xr[ks][q] * ws);
+                atomic_add(&Sp[(int64_t)f * k + lab], xr[ks][q] * ws);
             }
-            if (half == 0) atomic_add(&S32[dk + lab], ws);
+            if (half == 0) atomic_add(&Sp[dk + lab], ws);
           }
         }
       }
@@ -413,7 +551,11 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
       } else {
         float *cp = reinterpret_cast<float *>(a.copies) +
                     (a.slot0 + blockIdx.x) * (dk + k);
-        for (int64_t i = tid; i < dk + k; i += NT) cp[i] = S32[i];
+        for (int64_t i = tid; i < dk + k; i += NT) {
+          float t = S32[i];
+          for (int r = 1; r < R; ++r) t += S32[r * (dk + k) + i];
+          cp[i] = t;
+        }
       }
     } else {
       const int64_t copy = blockIdx.x % a.n_copies;
@@ -426,8 +568,14 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
           u64 t = S64[i];
           for (int r = 1; r < R; ++r) t += S64[r * (dk + k) + i];
           v = (float)((double)(long long)t / (double)Sc[f]);
+        } else if constexpr (window) {
+          const int f = (int)(i / k);  // f == d for the counts
+          u64 t = S64[i];
+          for (int r = 1; r < R; ++r) t += S64[r * (dk + k) + i];
+          v = (float)ldexp((double)(long long)t, Ew[f] - 150);
         } else {
           v = S32[i];
+          for (int r = 1; r < R; ++r) v += S32[r * (dk + k) + i];
         }
         if (v != 0.f) {
           if (i < dk)
@@ -752,32 +900,28 @@ void launch_fast_fused_fixed(int64_t n, int64_t d, int64_t k, const T *X,
   }
 }
 
-size_t fast_fused_scratch_bytes(int64_t d) {
-  return ((sizeof(float) * (d + 1) + sizeof(int) + 255) & ~size_t(255)) +
-         fixed_point_stats_bytes(d);
-}
-
+// The reference's fused step (private copies, FP32 atomics): no fixed-point
+// scales and so no pre-pass over X; the block sums are FP32 in LDS and reach
+// the copies through FP32 atomics, order-dependent like the reference's.
 template <typename T, typename I>
 void launch_fast_fused(int64_t n, int64_t d, int64_t k, int64_t n_copies,
-                       bool store_flush, const T *X, const T *w, const T *C,
-                       const T *hn, I *labels, T *copies, T *sizes_copies,
-                       hipStream_t s, void *scratch) {
+                       const T *X, const T *w, const T *C, const T *hn,
+                       I *labels, T *copies, T *sizes_copies, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     if (n <= 0) return;
-    if (store_flush)
-      throw RuntimeError("store-flush partials go through the fixed-point path");
     const MfmaLaunch L = plan_launch<I, true, false>(n, d, k);
     if (L.fn == nullptr) throw RuntimeError("MFMA fused step: no launch plan");
-    // scales (d + 1 floats) and flag, then the statistics of the pre-pass
-    const size_t sc_bytes = (sizeof(float) * (d + 1) + sizeof(int) + 255) & ~size_t(255);
-    TempBuffer own(scratch ? 0 : fast_fused_scratch_bytes(d), s);
-    char *base = scratch ? static_cast<char *>(scratch) : own.as<char>();
-    float *scale = reinterpret_cast<float *>(base);
-    int *flag = reinterpret_cast<int *>(base + sizeof(float) * (d + 1));
-    launch_fixed_point_scales(n, d, X, w, scale, flag, s, base + sc_bytes);
     FastArgs a = make_args(n, d, k, X, w, C, hn, labels);
-    a.scale = scale;
-    a.nonfinite = flag;
+    a.vec = vec_ok(X, n, L.s) ? 1 : 0;
+    a.scale = nullptr;
+    a.nonfinite = nullptr;
+    // window width from the most samples one block adds into one sum:
+    // 2^(24 + W) * count < 2^63
+    const int64_t per_block =
+        ceil_div<int64_t>(ceil_div<int64_t>(n, 32 * L.s), L.blocks) * 32 * L.s;
+    int bits = 0;
+    while (bits < 40 && (int64_t(1) << bits) < per_block) ++bits;
+    a.window = std::max(1, std::min(30, 38 - bits));
     a.copies = copies;
     a.sizes_copies = sizes_copies;
     a.n_copies = n_copies;
@@ -907,9 +1051,9 @@ template bool fast_fused_supported<float>(int64_t, int64_t);
 template bool fast_fused_supported<double>(int64_t, int64_t);
 #define KMD_INST_FAST(T, I)                                                   \
   template void launch_fast_fused<T, I>(int64_t, int64_t, int64_t, int64_t,   \
-                                        bool, const T *, const T *,           \
-                                        const T *, const T *, I *, T *, T *,  \
-                                        hipStream_t, void *);                 \
+                                        const T *, const T *, const T *,      \
+                                        const T *, I *, T *, T *,             \
+                                        hipStream_t);                         \
   template void launch_fast_fused_fixed<T, I>(                                \
       int64_t, int64_t, int64_t, const T *, const T *, const T *, const T *,  \
       I *, const float *, const int *, void *, int64_t, hipStream_t,         \

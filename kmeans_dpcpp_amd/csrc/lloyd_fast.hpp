@@ -54,14 +54,13 @@ bool fast_fused_supported(int64_t d, int64_t k);
 template <typename I>
 int fast_fused_blocks(int64_t n, int64_t d, int64_t k);
 
-// scratch (fast_fused_scratch_bytes(d) bytes, or NULL: allocated per call)
-// holds the fixed-point scales and the statistics of their pre-pass.
-size_t fast_fused_scratch_bytes(int64_t d);
+// The reference's fused step on the MFMA kernel: FP32 block sums in LDS,
+// flushed into the caller's private copies with FP32 atomics (no scales, no
+// pre-pass, no working memory).
 template <typename T, typename I>
 void launch_fast_fused(int64_t n, int64_t d, int64_t k, int64_t n_copies,
-                       bool store_flush, const T *X, const T *w, const T *C,
-                       const T *hn, I *labels, T *copies, T *sizes_copies,
-                       hipStream_t s, void *scratch = nullptr);
+                       const T *X, const T *w, const T *C, const T *hn,
+                       I *labels, T *copies, T *sizes_copies, hipStream_t s);
 
 template <typename T, typename I>
 void launch_fast_assign(int64_t n, int64_t d, int64_t k, const T *X,
@@ -110,8 +109,11 @@ int mixed_idx_bits(int64_t k);
 // Bytes of the BF16 A-operand array for T split terms (1 or 3).
 size_t mixed_aop_bytes(int64_t d, int64_t k, int T);
 // Blocks of the BF16 kernel for this shape; 0 when the shape is not served.
+// cand: with candidate-tile tracking (launch_mixed_step's qmask).
 template <typename I>
-int mixed_blocks(int64_t n, int64_t d, int64_t k, int T);
+int mixed_blocks(int64_t n, int64_t d, int64_t k, int T, bool cand = false);
+// Whether the candidate tracking fits this shape (k <= 1024, LDS).
+bool mixed_cand_fits(int64_t d, int64_t k, int T);
 // Builds the BF16 A operands (centroids + split half-norms) and the error
 // bound inputs bound[0] = max ||c||, bound[1] = max hn.
 void launch_mixed_prep(int64_t d, int64_t k, int T, const float *C,
@@ -121,12 +123,23 @@ void launch_mixed_prep(int64_t d, int64_t k, int T, const float *C,
 int64_t mixed_seg_cap(int64_t n, int64_t d, int blocks);
 // BF16 assignment + fixed-point update; block b appends its undecided samples
 // to queue[b * seg_cap + 0 : queue_counts[b]].  Writes `blocks` partial slots.
+// qmask (may be null): per queued sample the bit mask of the 32-centroid
+// tiles whose best BF16 score is within the certified bound of the sample's
+// best (launch with mixed_blocks(..., cand = true)).
 template <typename I>
 void launch_mixed_step(int64_t n, int64_t d, int64_t k, int T, int blocks,
                        const float *X, const float *w, const void *aop,
                        const float *bound, const float *scale, I *labels,
                        void *partials, int *queue, int *queue_counts,
-                       hipStream_t s, const int *skip = nullptr);
+                       unsigned *qmask, hipStream_t s, const int *skip = nullptr);
+// int64 fixed-point sums of the queued samples (labels already final), read
+// from the sample-major X_r: `blocks` partial slots from `partials`.
+template <typename I>
+void launch_queue_update(int64_t d, int64_t k, int blocks, int n_segments,
+                         int64_t seg_cap, const int *queue, const int *queue_counts,
+                         const float *X_r, const float *w, const I *labels,
+                         const float *scale, void *partials, hipStream_t s,
+                         const int *skip = nullptr);
 // FP32 MFMA re-score of the queued samples (the FP32 path's arithmetic):
 // reads the n_segments segments of the queue, writes
 // fast_rescore_blocks<I>(d, k, n_segments) partial slots from slot0 and the
@@ -189,6 +202,14 @@ void launch_tiled_mixed_assign(int64_t n, int64_t d, int64_t k, int T,
 size_t tiled_xb_bytes(int64_t n, int64_t d);
 void launch_tiled_xb_prep(int64_t n, int64_t d, const float *X_t, void *xb,
                           float *xnb, hipStream_t s);
+// FP8 (T = 8 in the tiled mixed launchers): e4m3 samples x8 (n bf16_d bytes,
+// within tiled_xb_bytes), norm bounds and the fit's sample scale in bound
+// (tiled_bound_floats() floats)
+size_t tiled_bound_floats();
+void launch_tiled_x8_prep(int64_t n, int64_t d, const float *X_t, void *x8,
+                          float *xnb, float *bound, hipStream_t s);
+// out[i] = e4m3 encode + decode of in[i] with the FP8 kernels' conversion
+void launch_f8_roundtrip(int64_t n, const float *in, float *out, hipStream_t s);
 // Re-score of the queued samples restricted to their candidate tiles (the
 // BF16 kernel's qmask; needs k <= 1024), with the FP32 kernels' arithmetic;
 // falls back to the full gather re-score when the (sample, tile) pairs

@@ -60,6 +60,8 @@ struct MixedArgs {
   u64 *partials;         // [slot][d*k + k]
   int *queue;            // [block][seg_cap] samples left for the FP32 re-score
   int *queue_counts;     // [block] entries used in each segment
+  unsigned *qmask;       // [block][seg_cap] candidate tiles of each queued
+                         // sample (k <= 1024), or null: no candidate tracking
   int64_t seg_cap;
   float pinf;            // +inf, passed at run time so that med3(b, v, pinf)
                          // stays one v_med3 (no max pattern, no canonicalise)
@@ -154,6 +156,11 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
   float *Sc = reinterpret_cast<float *>(Al + (int64_t)n_ct * KK * 64);
   u64 *S64 = reinterpret_cast<u64 *>(Sc + kScPad);             // [d*k + k]
   int &q_used = *reinterpret_cast<int *>(S64 + dk + k);  // queue entries used
+  // candidate tracking: each lane's best score per tile, rounded up to bf16,
+  // [wave][tile][lane]
+  unsigned short *tml = reinterpret_cast<unsigned short *>(
+      reinterpret_cast<unsigned char *>(S64 + dk + k) + 16);
+  const bool cand = a.qmask != nullptr;
   const int tid = threadIdx.x;
   for (int64_t i = tid; i < (int64_t)n_ct * KK * 64; i += NT) Al[i] = a.aop[i];
   for (int f = tid; f <= d; f += NT) Sc[f] = a.scale[f];
@@ -288,6 +295,9 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
     // is max(x, y) without the NaN canonicalisation of a max; the values are
     // finite here or the sample is re-scored.
     auto tile_track = [&](int ct, const f32x16(&acc)[kS]) {
+      float tmax[kS];
+#pragma unroll
+      for (int q = 0; q < kS; ++q) tmax[q] = -__builtin_huge_valf();
 #pragma unroll
       for (int r = 0; r < 16; r += 2) {
         const unsigned i0 = (unsigned)(ct * 32 + 8 * (r / 4) + 4 * half + (r % 4));
@@ -300,7 +310,15 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
           const float m = __builtin_amdgcn_fmed3f(b1[q], u, v);
           b1[q] = max3f(b1[q], u, v);
           b2[q] = __builtin_amdgcn_fmed3f(b2[q], m, pinf);
+          if (cand) tmax[q] = max3f(tmax[q], u, v);
         }
+      }
+      // the tile's best, rounded toward +inf to bf16 (never below it)
+      if (cand) {
+        static_assert(kS == 1, "one tile maximum per lane and tile");
+        const unsigned b = __float_as_uint(tmax[0]);
+        const unsigned up = (b & 0x80000000u) ? (b >> 16) : ((b + 0xffffu) >> 16);
+        tml[(wave * n_ct + ct) * kWave + lane] = (unsigned short)up;
       }
     };
     if constexpr (D <= 16) {
@@ -352,6 +370,22 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
       const bool valid = s < n;
       if (half == 0 && valid && !amb[q]) labels[s] = (I)lab[q];
       const bool enq = (half == 0) && valid && amb[q];
+      // candidate tiles against the final best: a centroid can be the FP32
+      // argmin only if its score is within 2 (E1 + E2) of the best; with the
+      // packing error of both values this is 2 e_part + 6 pack M
+      unsigned tiles_any = 0u;
+      if (cand) {
+        const float big = 1.01f * (xnorm[q] * cmax + hnmax);
+        const float thr = b1[q] - (2.f * (xnorm[q] * cmax * alpha + hnmax * beta) +
+                                   6.f * pack_err * big);
+        unsigned bits = 0u;
+        for (int t = 0; t < n_ct; ++t) {
+          const float v = __uint_as_float(
+              (unsigned)tml[(wave * n_ct + t) * kWave + lane] << 16);
+          if (v >= thr) bits |= 1u << t;
+        }
+        tiles_any = bits | (unsigned)__shfl_xor((int)bits, 32, kWave);
+      }
       const uint64_t mask = __ballot(enq);
       if (mask != 0ull) {
         const int cnt = __popcll(mask);
@@ -362,6 +396,7 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
         if (enq) {
           const int rank = __popcll(mask & ((1ull << lane) - 1ull));
           qseg[base + rank] = (int)s;
+          if (cand) a.qmask[(int64_t)blockIdx.x * a.seg_cap + base + rank] = tiles_any;
         }
       }
     }
@@ -407,11 +442,15 @@ inline int d_bucket(int64_t d) {
 
 inline int kk_of(int D, int T) { return (T * D + 3 + 15) / 16; }
 
-inline size_t mixed_lds_bytes(int64_t d, int64_t k, int T) {
+// cand: with the per-tile maxima of every lane (candidate tiles, k <= 1024)
+inline size_t mixed_lds_bytes(int64_t d, int64_t k, int T, bool cand = false) {
   const int D = d_bucket(d);
   const int64_t kpad = ceil_div<int64_t>(k, 32) * 32;
+  const size_t tml = cand ? sizeof(unsigned short) * (size_t)(nt_of(D) / kWave) *
+                                (size_t)(kpad / 32) * kWave
+                          : 0;
   return (size_t)(kpad / 32) * kk_of(D, T) * 64 * 16 + sizeof(float) * kScPad +
-         sizeof(u64) * (size_t)(d + 1) * k + 16;
+         sizeof(u64) * (size_t)(d + 1) * k + 16 + tml;
 }
 
 using MixedFn = void (*)(MixedArgs);
@@ -458,11 +497,16 @@ int64_t mixed_seg_cap(int64_t n, int64_t d, int blocks) {
   return ceil_div<int64_t>(n_super, waves) * (nt / kWave) * 32 * kS;
 }
 
+bool mixed_cand_fits(int64_t d, int64_t k, int T) {
+  return k <= 1024 && mixed_lds_bytes(d, k, T, true) <= kLdsPerCu;
+}
+
 template <typename I>
-int mixed_blocks(int64_t n, int64_t d, int64_t k, int T) {
+int mixed_blocks(int64_t n, int64_t d, int64_t k, int T, bool cand) {
   if (d < 1 || d > 64 || k < 1 || n >= (int64_t(1) << 31)) return 0;
   if (mixed_idx_bits(k) > 12) return 0;  // index packing: <= 4096 centroids
-  const size_t lds = mixed_lds_bytes(d, k, T);
+  if (cand && !mixed_cand_fits(d, k, T)) return 0;
+  const size_t lds = mixed_lds_bytes(d, k, T, cand);
   if (lds > kLdsPerCu) return 0;
   MixedFn fn = pick_mixed<I>(d, T);
   if (fn == nullptr) return 0;
@@ -524,7 +568,7 @@ void launch_mixed_step(int64_t n, int64_t d, int64_t k, int T, int blocks,
                        const float *X, const float *w, const void *aop,
                        const float *bound, const float *scale, I *labels,
                        void *partials, int *queue, int *queue_counts,
-                       hipStream_t s, const int *skip) {
+                       unsigned *qmask, hipStream_t s, const int *skip) {
   MixedFn fn = pick_mixed<I>(d, T);
   if (fn == nullptr || blocks < 1)
     throw RuntimeError("mixed-precision step: unsupported shape");
@@ -543,28 +587,99 @@ void launch_mixed_step(int64_t n, int64_t d, int64_t k, int T, int blocks,
   a.partials = reinterpret_cast<u64 *>(partials);
   a.queue = queue;
   a.queue_counts = queue_counts;
+  a.qmask = qmask;
   a.seg_cap = mixed_seg_cap(n, d, blocks);
   a.pinf = std::numeric_limits<float>::infinity();
   a.skip = skip;
   hipLaunchKernelGGL(fn, dim3(blocks), dim3(nt_of(d_bucket(d))),
-                     mixed_lds_bytes(d, k, T), s,
+                     mixed_lds_bytes(d, k, T, qmask != nullptr), s,
                      a);
   KMD_LAUNCH_CHECK();
 }
 
-template int mixed_blocks<int32_t>(int64_t, int64_t, int64_t, int);
-template int mixed_blocks<int64_t>(int64_t, int64_t, int64_t, int);
+namespace {
+// int64 fixed-point update of the queued (re-scored) samples: block b takes
+// segments b, b + grid, ... of the queue, adds rint((x w) scale_f) of each
+// sample (x from the sample-major copy X_r) under its label into LDS, and
+// writes its partial slot [d k + k] (the FP32 kernels' element arithmetic,
+// so the integers are the ones the FP32 path adds for these samples).
+template <typename I>
+__global__ void __launch_bounds__(256)
+    queue_update_krn(int d, int k, int n_segments, int64_t seg_cap,
+                     const int *__restrict__ queue, const int *__restrict__ queue_counts,
+                     const float *__restrict__ X_r, const float *__restrict__ w,
+                     const I *__restrict__ labels, const float *__restrict__ scale,
+                     u64 *partials, const int *skip) {
+  if (skip != nullptr && *skip != 0) return;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  u64 *S64 = reinterpret_cast<u64 *>(lds_raw);   // [d k + k]
+  float *Sc = reinterpret_cast<float *>(S64 + (int64_t)(d + 1) * k);
+  const int64_t dk = (int64_t)d * k;
+  for (int64_t i = threadIdx.x; i < dk + k; i += 256) S64[i] = 0ull;
+  for (int f = threadIdx.x; f <= d; f += 256) Sc[f] = scale[f];
+  __syncthreads();
+  for (int seg = blockIdx.x; seg < n_segments; seg += gridDim.x) {
+    const int m = queue_counts[seg];
+    const int *qs = queue + (int64_t)seg * seg_cap;
+    for (int j = threadIdx.x; j < m; j += 256) {
+      const int64_t s = qs[j];
+      const int l = (int)labels[s];
+      const float ws = w[s];
+      const float *row = X_r + s * d;
+      for (int f = 0; f < d; ++f) {
+        const int qv = __float2int_rn((row[f] * ws) * Sc[f]);
+        atomicAdd(&S64[(int64_t)f * k + l], (u64)(long long)qv);
+      }
+      atomicAdd(&S64[dk + l], (u64)(long long)__float2int_rn(ws * Sc[d]));
+    }
+  }
+  __syncthreads();
+  u64 *cp = partials + (int64_t)blockIdx.x * (dk + k);
+  for (int64_t i = threadIdx.x; i < dk + k; i += 256) cp[i] = S64[i];
+}
+}  // namespace
+
+template <typename I>
+void launch_queue_update(int64_t d, int64_t k, int blocks, int n_segments,
+                         int64_t seg_cap, const int *queue, const int *queue_counts,
+                         const float *X_r, const float *w, const I *labels,
+                         const float *scale, void *partials, hipStream_t s,
+                         const int *skip) {
+  const size_t lds = sizeof(u64) * (size_t)(d + 1) * k + sizeof(float) * (d + 1);
+  require(lds <= kLdsPerCu, "queue update: sums beyond LDS");
+  if (lds > 64 * 1024)
+    KMD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&queue_update_krn<I>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+  hipLaunchKernelGGL(queue_update_krn<I>, dim3((unsigned)std::max(1, blocks)), dim3(256),
+                     lds, s, (int)d, (int)k, n_segments, seg_cap, queue, queue_counts,
+                     X_r, w, labels, scale, reinterpret_cast<u64 *>(partials), skip);
+  KMD_LAUNCH_CHECK();
+}
+
+template int mixed_blocks<int32_t>(int64_t, int64_t, int64_t, int, bool);
+template int mixed_blocks<int64_t>(int64_t, int64_t, int64_t, int, bool);
 template void launch_mixed_step<int32_t>(int64_t, int64_t, int64_t, int, int,
                                          const float *, const float *,
                                          const void *, const float *,
                                          const float *, int32_t *, void *,
-                                         int *, int *, hipStream_t,
+                                         int *, int *, unsigned *, hipStream_t,
                                          const int *);
 template void launch_mixed_step<int64_t>(int64_t, int64_t, int64_t, int, int,
                                          const float *, const float *,
                                          const void *, const float *,
                                          const float *, int64_t *, void *,
-                                         int *, int *, hipStream_t,
+                                         int *, int *, unsigned *, hipStream_t,
                                          const int *);
+template void launch_queue_update<int32_t>(int64_t, int64_t, int, int, int64_t,
+                                           const int *, const int *, const float *,
+                                           const float *, const int32_t *,
+                                           const float *, void *, hipStream_t,
+                                           const int *);
+template void launch_queue_update<int64_t>(int64_t, int64_t, int, int, int64_t,
+                                           const int *, const int *, const float *,
+                                           const float *, const int64_t *,
+                                           const float *, void *, hipStream_t,
+                                           const int *);
 
 }  // namespace kmd

@@ -851,6 +851,376 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
   if (tid == 0) a.queue_counts[blockIdx.x] = q_used;
 }
 
+// ---- FP8 (e4m3) assignment with per-tile scales and certified margin ----------
+//
+// The cross term on the matrix core in FP8 (v_mfma_f32_32x32x16_fp8_fp8, FP32
+// accumulation): centroid tile t enters as e4m3(c s_t), its own power-of-two
+// scale s_t keeping |c s_t| <= kF8Target, the samples as e4m3(x s_x) with one
+// power-of-two scale for the fit.  The accumulator starts from
+// -hn s_t s_x (FP32) and the result is unscaled exactly by 1 / (s_t s_x), so
+// D = sum_f x~_f c~_f - hn, argmax D = argmin (hn - c.x).  The decision and
+// the FP32 re-score of undecided samples are the BF16 kernel's; the error
+// bound is e4m3's: per operand |e4m3(v) - v| <= u |v| + eta (u = 2^-4 for
+// round-to-nearest with 3 mantissa bits; eta = 2^-10, half the smallest
+// subnormal spacing), so
+//   |D~ - D| <= ||x|| cmax (2u + u^2) + sqrt(D) (eta_x cmax + eta_c ||x||)(1 + u)
+//               + D eta_x eta_c + FP32 accumulation,
+// with eta_x = eta / s_x and eta_c = eta / min_t s_t in data units.
+constexpr int kF8 = 8;                    // the "terms" value of the FP8 kernels
+constexpr float kF8Target = 224.f;        // scaled magnitudes (both encodings' range)
+constexpr float kF8U = 0.0625f;           // e4m3 unit roundoff
+constexpr float kF8Eta = 1.0f / 1024.f;   // half the smallest subnormal spacing
+typedef long long2v __attribute__((ext_vector_type(2)));
+
+constexpr int f8_slots(int D) { return D / 32 + 1; }   // 1 KB A-operand slots per tile
+constexpr int f8_s() { return 2; }                     // sample blocks per wave
+constexpr int f8_tml_bytes() {
+  return kTW * f8_s() * 32 * kWave * (int)sizeof(unsigned short);
+}
+constexpr int f8_cpc(int D) {
+  const int cap = (int)kLdsPerCu - f8_tml_bytes() - 1024;
+  const int chunk = kChunkBytes < cap / 2 ? kChunkBytes : cap / 2;
+  return chunk / (f8_slots(D) * 1024) < 1 ? 1 : chunk / (f8_slots(D) * 1024);
+}
+
+// Largest power of two p with m p <= kF8Target (1 for m = 0 or non-finite m).
+__device__ inline float f8_scale(float m) {
+  if (!(m > 0.f) || !(m < 1e30f)) return 1.f;
+  int e;
+  frexpf(kF8Target / m, &e);   // kF8Target / m in [2^(e-1), 2^e)
+  return ldexpf(1.f, e - 1);
+}
+
+// e4m3 of eight values, element j in byte j
+__device__ inline long f8_pack8(const float (&v)[8]) {
+  int lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+  lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], lo, true);
+  int hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4], v[5], 0, false);
+  hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6], v[7], hi, true);
+  return (long)(unsigned)lo | ((long)(unsigned)hi << 32);
+}
+
+template <int D, typename I>
+__global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    tiled_fp8_krn(TiledMixedArgs a) {
+  if (a.skip != nullptr && *a.skip != 0) return;
+  constexpr int KD = D / 16, KH = D / 32, KK = f8_slots(D);
+  constexpr int S = f8_s();
+  constexpr int CPC = f8_cpc(D);
+  constexpr int kBuf = CPC * KK * 1024;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  int &q_used = *reinterpret_cast<int *>(lds_raw + 2 * kBuf);
+  unsigned short *tml = reinterpret_cast<unsigned short *>(lds_raw + 2 * kBuf + 16);
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1), wave = tid / kWave;
+  const int col = lane & 31, half = lane >> 5;
+  const int64_t n = a.n;
+  const int n_ct = a.n_ct;
+  I *labels = reinterpret_cast<I *>(a.labels);
+  int *qseg = a.queue + (int64_t)blockIdx.x * a.seg_cap;
+  if (tid == 0) q_used = 0;
+
+  const unsigned idx_mask = (1u << a.idx_bits) - 1u;
+  // bound: [0] max ||c||, [1] max hn, [2] eta / min s_t, [3] s_x, [4] eta / s_x,
+  // [5] max s_t
+  const float cmax = a.bound[0], hnmax = a.bound[1];
+  const float eta_c = a.bound[2], sx = a.bound[3], eta_x = a.bound[4];
+  const float smax = a.bound[5];
+  const bool finite_c = hnmax < 1e30f && (cmax + hnmax) * smax * sx < 1e30f;
+  const float pinf = a.pinf;
+  const float acc_err = (float)(D + 2 + a.fp32_len + 8) * 5.9604645e-8f;
+  const float alpha = 2.f * kF8U + kF8U * kF8U + acc_err * (1.f + kF8U) * (1.f + kF8U);
+  const float beta = acc_err;
+  const float rootd = sqrtf((float)D);
+  const float pack_err = ldexpf(1.0f, a.idx_bits - 23) * 1.01f;
+
+  constexpr int kSuper = 32 * S;
+  const int64_t n_super = ceil_div<int64_t>(n, kSuper);
+  const int64_t per_round = (int64_t)gridDim.x * kTW;
+  const int64_t rounds = ceil_div<int64_t>(n_super, per_round);
+  const int n_chunks = (n_ct + CPC - 1) / CPC;
+  const int64_t total_chunks = rounds * n_chunks;
+  const unsigned char *aop = reinterpret_cast<const unsigned char *>(a.aop);
+  auto chunk_bytes = [&](int c) { return min(CPC, n_ct - c * CPC) * KK * 1024; };
+  if (total_chunks > 0) lds_dma(aop, lds_raw, chunk_bytes(0), wave, lane);
+  int64_t g = 0;
+  // streamed FP8 samples: lane (col, half) of sample s reads the D / 2 bytes
+  // of its half (k-steps 2 j and 2 j + 1 in one 16-byte load)
+  const unsigned char *x8 = reinterpret_cast<const unsigned char *>(a.xb);
+  long2v nb[KH][S];
+  float nxn[S];
+  auto load_x8 = [&](int64_t rr) {
+    const int64_t sbr = rr * per_round + (int64_t)blockIdx.x * kTW + wave;
+    const int64_t sr = sbr * kSuper + (int64_t)S * col;
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      const bool ok = sbr < n_super && sr + q < n;
+      const long2v *row = reinterpret_cast<const long2v *>(x8 + (sr + q) * D + half * (D / 2));
+#pragma unroll
+      for (int j = 0; j < KH; ++j) {
+        if (ok) {
+          nb[j][q] = row[j];
+        } else {
+          nb[j][q][0] = 0;
+          nb[j][q][1] = 0;
+        }
+      }
+      nxn[q] = ok ? a.xnb[sr + q] : 0.f;
+    }
+  };
+  if (rounds > 0) load_x8(0);
+  for (int64_t r = 0; r < rounds; ++r) {
+    const int64_t sb = r * per_round + (int64_t)blockIdx.x * kTW + wave;
+    const bool active = sb < n_super;
+    const int64_t s0 = sb * kSuper + (int64_t)S * col;
+    float xnorm[S];
+    long2v bx[KH][S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      xnorm[q] = nxn[q];
+#pragma unroll
+      for (int j = 0; j < KH; ++j) bx[j][q] = nb[j][q];
+    }
+    if (r + 1 < rounds) load_x8(r + 1);
+    float b1[S], b2[S];
+    unsigned cmask[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      b1[q] = -__builtin_huge_valf();
+      b2[q] = -__builtin_huge_valf();
+      cmask[q] = 0u;
+    }
+    for (int ci = 0; ci < n_chunks; ++ci, ++g) {
+      const int c0 = ci * CPC;
+      const int cnt = min(CPC, n_ct - c0);
+      wait_dma_and_sync();
+      if (g + 1 < total_chunks) {
+        const int cn = (int)((g + 1) % n_chunks);
+        lds_dma(aop + (size_t)cn * CPC * KK * 1024, lds_raw + ((g + 1) & 1) * kBuf,
+                chunk_bytes(cn), wave, lane);
+      }
+      const unsigned char *Al = lds_raw + (g & 1) * kBuf;
+      if (active) {
+        for (int t = 0; t < cnt; ++t) {
+          const unsigned char *At = Al + (size_t)t * KK * 1024;
+          const float *Ht = reinterpret_cast<const float *>(At + KH * 1024);
+          long2v av[KH];
+#pragma unroll
+          for (int j = 0; j < KH; ++j)
+            av[j] = *reinterpret_cast<const long2v *>(At + j * 1024 + lane * 16);
+          f32x16 init;
+#pragma unroll
+          for (int gg = 0; gg < 4; ++gg) {
+            const f32x4 h4 = *reinterpret_cast<const f32x4 *>(Ht + 8 * gg + 4 * half);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) init[4 * gg + e] = h4[e];
+          }
+          const float inv = Ht[32];
+          f32x16 acc[S];
+#pragma unroll
+          for (int q = 0; q < S; ++q) {
+            acc[q] = init;
+#pragma unroll
+            for (int kk = 0; kk < KD; ++kk)
+              acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(
+                  av[kk / 2][kk % 2], bx[kk / 2][q][kk % 2], acc[q], 0, 0, 0);
+          }
+          const int ct = c0 + t;
+          float tmax[S];
+#pragma unroll
+          for (int q = 0; q < S; ++q) tmax[q] = -__builtin_huge_valf();
+#pragma unroll
+          for (int rr = 0; rr < 16; rr += 2) {
+            const unsigned i0 =
+                (unsigned)(ct * 32 + 8 * (rr / 4) + 4 * half + (rr % 4));
+#pragma unroll
+            for (int q = 0; q < S; ++q) {
+              const float uu =
+                  __uint_as_float((__float_as_uint(acc[q][rr] * inv) & ~idx_mask) | i0);
+              const float vv = __uint_as_float(
+                  (__float_as_uint(acc[q][rr + 1] * inv) & ~idx_mask) | (i0 + 1u));
+              const float m = __builtin_amdgcn_fmed3f(b1[q], uu, vv);
+              b1[q] = max3f(b1[q], uu, vv);
+              b2[q] = __builtin_amdgcn_fmed3f(b2[q], m, pinf);
+              tmax[q] = max3f(tmax[q], uu, vv);
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < S; ++q)
+            if (a.qmask != nullptr && ct < 32) {
+              const unsigned b = __float_as_uint(tmax[q]);
+              const unsigned up = (b & 0x80000000u) ? (b >> 16) : ((b + 0xffffu) >> 16);
+              tml[((wave * S + q) * 32 + ct) * kWave + lane] = (unsigned short)up;
+            }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      const float o1 = __shfl_xor(b1[q], 32, kWave);
+      const float o2 = __shfl_xor(b2[q], 32, kWave);
+      const float n2 = fmaxf(fminf(b1[q], o1), fmaxf(b2[q], o2));
+      b1[q] = fmaxf(b1[q], o1);
+      b2[q] = n2;
+      const float e_part = xnorm[q] * cmax * alpha + hnmax * beta +
+                           rootd * (eta_x * cmax + eta_c * xnorm[q]) * (1.f + kF8U) +
+                           (float)D * eta_x * eta_c;
+      const float e = e_part + pack_err * (fabsf(b1[q]) + fabsf(b2[q]));
+      const unsigned lab = __float_as_uint(b1[q]) & idx_mask;
+      const bool amb = !(b1[q] - b2[q] > 2.f * e) || !(xnorm[q] * cmax < 1e30f) ||
+                       !finite_c || lab >= (unsigned)a.k;
+      const int64_t s = s0 + q;
+      const bool valid = active && s < n;
+      if (half == 0 && valid && !amb) labels[s] = (I)lab;
+      const bool enq = (half == 0) && valid && amb;
+      if (a.qmask != nullptr) {
+        const float big = 1.01f * (xnorm[q] * cmax + hnmax);
+        const float thr = b1[q] - (2.f * e_part + 6.f * pack_err * big);
+        unsigned bits = 0u;
+        const int nt = min(n_ct, 32);
+        for (int t = 0; t < nt; ++t) {
+          const float v = __uint_as_float(
+              (unsigned)tml[((wave * S + q) * 32 + t) * kWave + lane] << 16);
+          if (v >= thr) bits |= 1u << t;
+        }
+        cmask[q] = bits;
+      }
+      const unsigned tiles_any = cmask[q] | (unsigned)__shfl_xor((int)cmask[q], 32, kWave);
+      const uint64_t mask = __ballot(enq);
+      if (mask != 0ull) {
+        const int c = __popcll(mask);
+        const int leader = __ffsll((long long)mask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&q_used, c);
+        base = __shfl(base, leader, kWave);
+        if (enq) {
+          const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+          qseg[base + rank] = (int)s;
+          if (a.qmask) a.qmask[(int64_t)blockIdx.x * a.seg_cap + base + rank] = tiles_any;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) a.queue_counts[blockIdx.x] = q_used;
+}
+
+// FP8 A operands of tile blockIdx.x (one block per tile): slot j < D / 32 holds,
+// for lane l (centroid ct 32 + l % 32), e4m3(c s_t) of k-steps 2 j and 2 j + 1
+// (K positions 16 kk + 8 (l / 32) + 0..7 in bytes 8 (kk % 2) + 0..7); slot
+// D / 32 holds the chain's start -hn s_t s_x for the 32 rows, then 1 / (s_t s_x).
+// bound[2] and bound[5] gather eta / s_t and s_t over the tiles (the launcher
+// clears them); bound[3] = s_x.
+__global__ void __launch_bounds__(256)
+    tiled_fp8_prep_krn(int d, int k, int D, const float *__restrict__ C,
+                       const float *__restrict__ hn, unsigned char *__restrict__ aop,
+                       float *bound) {
+  __shared__ float red[256 / kWave];
+  __shared__ float s_tile;
+  const int ct = blockIdx.x, tid = threadIdx.x;
+  const int KH = D / 32, KK = KH + 1;
+  float m = 0.f;
+  for (int i = tid; i < 32 * d; i += 256) {
+    const int f = i >> 5, c = ct * 32 + (i & 31);
+    if (c < k) m = fmaxf(m, fabsf(C[(int64_t)f * k + c]));
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, kWave));
+  if ((tid & (kWave - 1)) == 0) red[tid / kWave] = m;
+  __syncthreads();
+  if (tid == 0) {
+    float t = 0.f;
+    for (int i = 0; i < 256 / kWave; ++i) t = fmaxf(t, red[i]);
+    s_tile = f8_scale(t);
+    atomicMax(reinterpret_cast<unsigned *>(&bound[2]), __float_as_uint(kF8Eta / s_tile));
+    atomicMax(reinterpret_cast<unsigned *>(&bound[5]), __float_as_uint(s_tile));
+  }
+  __syncthreads();
+  const float st = s_tile, sx = bound[3];
+  unsigned char *tile = aop + (size_t)ct * KK * 1024;
+  for (int gi = tid; gi < KH * 64; gi += 256) {
+    const int j = gi >> 6, l = gi & 63;
+    const int c = ct * 32 + (l & 31), hf = l >> 5;
+    long2v out;
+#pragma unroll
+    for (int h2 = 0; h2 < 2; ++h2) {
+      const int kk = 2 * j + h2;
+      float v[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const int f = 16 * kk + 8 * hf + jj;
+        v[jj] = (f < d && c < k) ? C[(int64_t)f * k + c] * st : 0.f;
+      }
+      out[h2] = f8_pack8(v);
+    }
+    *reinterpret_cast<long2v *>(tile + j * 1024 + l * 16) = out;
+  }
+  float *hs = reinterpret_cast<float *>(tile + KH * 1024);
+  const float sc = st * sx;
+  for (int i = tid; i < 256; i += 256) {
+    float v = 0.f;
+    if (i < 32) {
+      const int c = ct * 32 + i;
+      // padding rows: far below every real score once unscaled (>= -3e38)
+      v = c < k ? -hn[c] * sc : fmaxf(-3.0e38f, -1.0e30f * sc);
+    } else if (i == 32) {
+      v = 1.f / sc;
+    }
+    hs[i] = v;
+  }
+}
+
+// max |x| over X_t as a bit pattern (non-negative floats order like their bits)
+__global__ void __launch_bounds__(256)
+    tiled_x8_max_krn(int64_t count, const float *__restrict__ X_t, unsigned *maxbits) {
+  unsigned m = 0u;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * 256)
+    m = max(m, __float_as_uint(X_t[i]) & 0x7fffffffu);
+  for (int o = kWave / 2; o > 0; o >>= 1)
+    m = max(m, (unsigned)__shfl_xor((int)m, o, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(maxbits, m);
+}
+
+// x8[s] = e4m3(x_s s_x) laid out per half of the lane pair (bytes
+// h D/2 + 8 kk + j = feature 16 kk + 8 h + j), xnb[s] = 1.0001 ||x_s||;
+// block 0 writes bound[3] = s_x and bound[4] = eta / s_x.
+__global__ void __launch_bounds__(256)
+    tiled_x8_prep_krn(int64_t n, int d, int D, const float *__restrict__ X_t,
+                      const unsigned *maxbits, unsigned char *__restrict__ x8,
+                      float *__restrict__ xnb, float *bound) {
+  const float sx = f8_scale(__uint_as_float(*maxbits));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    bound[3] = sx;
+    bound[4] = kF8Eta / sx;
+  }
+  const int64_t s = blockIdx.x * 256ll + threadIdx.x;
+  if (s >= n) return;
+  float ss = 0.f;
+  for (int hf = 0; hf < 2; ++hf)
+    for (int kk = 0; kk < D / 16; ++kk) {
+      float v[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const int f = 16 * kk + 8 * hf + jj;
+        const float x = (f < d) ? X_t[(int64_t)f * n + s] : 0.f;
+        ss = fmaf(x, x, ss);
+        v[jj] = x * sx;
+      }
+      *reinterpret_cast<long *>(x8 + s * D + hf * (D / 2) + 8 * kk) = f8_pack8(v);
+    }
+  xnb[s] = sqrtf(ss) * 1.0001f;
+}
+
+// e4m3 encode + decode of in[i] (the FP8 kernels' conversion), for tests of
+// the rounding the error bound assumes
+__global__ void __launch_bounds__(256)
+    f8_roundtrip_krn(int64_t n, const float *__restrict__ in, float *__restrict__ out) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= n) return;
+  const int w = __builtin_amdgcn_cvt_pk_fp8_f32(in[i], 0.f, 0, false);
+  out[i] = __builtin_amdgcn_cvt_f32_fp8(w, 0);
+}
+
 // A operands of the BF16 kernel for a runtime bucket D: element j of lane l
 // for (tile ct, k-step kk) is centroid ct*32 + l%32 at K position
 // p = 16 kk + 8 (l / 32) + j: p < T*D the split centroid features, then the
@@ -1187,12 +1557,14 @@ TempBuffer prep_fp32_tiles(int64_t d, int64_t k, const float *C, const float *hn
   return t;
 }
 
+// T: BF16 terms (1, 3) or kF8 (the FP8 kernel)
 template <typename I>
 TiledMixedFn pick_mixed_tiled(int64_t d, int T) {
   const int D = bf16_d_of(d);
 #define KMD_TM_CASE(DD)                                                     \
   if (D == DD)                                                              \
-    return T == 3 ? &tiled_mixed_krn<DD, 3, I> : &tiled_mixed_krn<DD, 1, I>;
+    return T == kF8 ? &tiled_fp8_krn<DD, I>                                 \
+                    : (T == 3 ? &tiled_mixed_krn<DD, 3, I> : &tiled_mixed_krn<DD, 1, I>);
   KMD_TM_CASE(32)
   KMD_TM_CASE(64)
   KMD_TM_CASE(96)
@@ -1203,8 +1575,33 @@ TiledMixedFn pick_mixed_tiled(int64_t d, int T) {
 
 size_t mixed_tiled_lds_bytes(int64_t d, int T) {
   const int D = bf16_d_of(d);
+  if (T == kF8)
+    return 2 * (size_t)f8_cpc(D) * f8_slots(D) * 1024 + 16 + (size_t)f8_tml_bytes();
   return 2 * (size_t)mixed_cpc(D, T) * kk_of(D, T) * 64 * 16 + 16 +
          (size_t)mixed_tml_bytes(D, T);
+}
+
+// 32-sample blocks per wave of the BF16 (T = 1, 3) or FP8 (T = kF8) kernel
+int mixed_s_of(int D, int T) { return T == kF8 ? f8_s() : bf16_s(D, T); }
+
+// bound[0], bound[1] as tiled_bound_krn; the FP8 kernels' per-tile gathers
+// bound[2] (eta / min s_t) and bound[5] (max s_t) start from 0
+__global__ void __launch_bounds__(256)
+    tiled_f8_bound_krn(int k, const float *__restrict__ hn, float *bound) {
+  __shared__ float red[256 / kWave];
+  float m = 0.f;
+  for (int c = threadIdx.x; c < k; c += 256) m = fmaxf(m, hn[c]);
+  for (int o = kWave / 2; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.f;
+    for (int i = 0; i < 256 / kWave; ++i) t = fmaxf(t, red[i]);
+    bound[0] = sqrtf(2.f * t);
+    bound[1] = t;
+    bound[2] = 0.f;
+    bound[5] = 0.f;
+  }
 }
 
 template <typename F>
@@ -1400,7 +1797,7 @@ int tiled_mixed_blocks(int64_t n, int64_t d, int64_t k, int T) {
   if (fn == nullptr) return 0;
   const int grid = occupancy_grid(fn, mixed_tiled_lds_bytes(d, T));
   if (grid < 1) return 0;
-  const int S = bf16_s(bf16_d_of(d), T);
+  const int S = mixed_s_of(bf16_d_of(d), T);
   const int64_t need = ceil_div<int64_t>(ceil_div<int64_t>(n, 32 * S), kTW);
   int64_t blocks = std::min<int64_t>(grid, need);
   if (blocks > kMaxSegments) blocks = kMaxSegments;
@@ -1408,7 +1805,7 @@ int tiled_mixed_blocks(int64_t n, int64_t d, int64_t k, int T) {
 }
 
 int64_t tiled_mixed_seg_cap(int64_t n, int64_t d, int T, int blocks) {
-  const int S = bf16_s(bf16_d_of(d), T);
+  const int S = mixed_s_of(bf16_d_of(d), T);
   const int64_t n_super = ceil_div<int64_t>(n > 0 ? n : 1, 32 * S);
   const int64_t rounds = ceil_div<int64_t>(n_super, (int64_t)blocks * kTW);
   return rounds * kTW * 32 * S;
@@ -1429,15 +1826,53 @@ void launch_tiled_xb_prep(int64_t n, int64_t d, const float *X_t, void *xb,
 
 size_t tiled_aop_bytes(int64_t d, int64_t k, int T) {
   // + 1 KB: the chunk copies read whole 1 KB pieces
+  if (T == kF8)
+    return (size_t)ceil_div<int64_t>(k, 32) * f8_slots(bf16_d_of(d)) * 1024 + 1024;
   return (size_t)ceil_div<int64_t>(k, 32) * kk_of(bf16_d_of(d), T) * 64 * 16 + 1024;
+}
+
+size_t tiled_bound_floats() { return 8; }
+
+void launch_tiled_x8_prep(int64_t n, int64_t d, const float *X_t, void *x8,
+                          float *xnb, float *bound, hipStream_t s) {
+  if (n <= 0) return;
+  TempBuffer mb(sizeof(unsigned), s);
+  KMD_HIP_CHECK(hipMemsetAsync(mb.as<unsigned>(), 0, sizeof(unsigned), s));
+  const int64_t count = n * d;
+  const int grid = (int)std::max<int64_t>(
+      1, std::min<int64_t>(ceil_div<int64_t>(count, 256 * 16), 4 * device_cu_count()));
+  hipLaunchKernelGGL(tiled_x8_max_krn, dim3(grid), dim3(256), 0, s, count, X_t,
+                     mb.as<unsigned>());
+  KMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(tiled_x8_prep_krn, dim3((unsigned)ceil_div<int64_t>(n, 256)),
+                     dim3(256), 0, s, n, (int)d, bf16_d_of(d), X_t,
+                     mb.as<const unsigned>(), reinterpret_cast<unsigned char *>(x8), xnb,
+                     bound);
+  KMD_LAUNCH_CHECK();
+}
+
+void launch_f8_roundtrip(int64_t n, const float *in, float *out, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(f8_roundtrip_krn, dim3((unsigned)ceil_div<int64_t>(n, 256)),
+                     dim3(256), 0, s, n, in, out);
+  KMD_LAUNCH_CHECK();
 }
 
 void launch_tiled_mixed_prep(int64_t d, int64_t k, int T, const float *C,
                              const float *hn, void *aop, float *bound,
                              hipStream_t s) {
   const int D = bf16_d_of(d);
-  const int KK = kk_of(D, T);
   const int n_ct = (int)ceil_div<int64_t>(k, 32);
+  if (T == kF8) {
+    hipLaunchKernelGGL(tiled_f8_bound_krn, dim3(1), dim3(256), 0, s, (int)k, hn, bound);
+    KMD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tiled_fp8_prep_krn, dim3((unsigned)n_ct), dim3(256), 0, s,
+                       (int)d, (int)k, D, C, hn, reinterpret_cast<unsigned char *>(aop),
+                       bound);
+    KMD_LAUNCH_CHECK();
+    return;
+  }
+  const int KK = kk_of(D, T);
   const int64_t total = (int64_t)n_ct * KK * 64;
   hipLaunchKernelGGL(tiled_prep_krn, dim3((unsigned)ceil_div<int64_t>(total, 256)),
                      dim3(256), 0, s, (int)d, (int)k, D, T, KK, n_ct, C, hn,
@@ -1473,10 +1908,10 @@ void launch_tiled_mixed_assign(int64_t n, int64_t d, int64_t k, int T,
   a.seg_cap = tiled_mixed_seg_cap(n, d, T, blocks);
   a.pinf = std::numeric_limits<float>::infinity();
   a.skip = skip;
-  if (T == 1 && (xb == nullptr || xnb == nullptr))
-    throw RuntimeError("tiled BF16 assignment: the BF16 sample copy is required");
-  a.xb = (T == 1) ? reinterpret_cast<const bf16x8 *>(xb) : nullptr;
-  a.xnb = (T == 1) ? xnb : nullptr;
+  if ((T == 1 || T == kF8) && (xb == nullptr || xnb == nullptr))
+    throw RuntimeError("tiled BF16 / FP8 assignment: the sample copy is required");
+  a.xb = (T == 1 || T == kF8) ? reinterpret_cast<const bf16x8 *>(xb) : nullptr;
+  a.xnb = (T == 1 || T == kF8) ? xnb : nullptr;
   hipLaunchKernelGGL(fn, dim3(blocks), dim3(kTNT), mixed_tiled_lds_bytes(d, T), s,
                      a);
   KMD_LAUNCH_CHECK();

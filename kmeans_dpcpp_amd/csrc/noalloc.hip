@@ -1,11 +1,11 @@
 // kmd_fused_lloyd_single_step_noalloc: the reference's fused Lloyd step
 // (kmd_fused_lloyd_single_step) for callers that cannot allocate device
 // memory -- code under HIP graph capture, tight serving loops, harnesses that
-// forbid hipMalloc.  The launch's working data (the fixed-point scales and
-// the statistics of their pre-pass, or the prepared centroid tiles of the
-// tiled shapes) lives in one file-scope device array of kScratchBytes per
-// GPU instead of a per-call stream-ordered allocation; the kernels are the
-// ones kmd_fused_lloyd_single_step launches.
+// forbid hipMalloc.  The launch's working data (the prepared centroid tiles
+// of the tiled shapes; the LDS-resident shapes need none) lives in one
+// file-scope device array of kScratchBytes per GPU instead of a per-call
+// stream-ordered allocation; the kernels are the ones
+// kmd_fused_lloyd_single_step launches.
 //
 // Every call on one GPU shares that GPU's array: calls that overlap on one
 // GPU (two streams, two threads) race on it.  Each call clears what must

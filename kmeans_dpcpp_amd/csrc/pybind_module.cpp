@@ -712,8 +712,10 @@ py::tuple py_kmeans_lloyd_driver_ex(py::object X_, py::object w_,
     prec = KMD_PREC_BF16X3;
   else if (precision == "bf16")
     prec = KMD_PREC_BF16;
+  else if (precision == "fp8")
+    prec = KMD_PREC_FP8;
   else
-    throw py::value_error("precision must be 'fp32', 'bf16x3' or 'bf16'");
+    throw py::value_error("precision must be 'fp32', 'bf16x3', 'bf16' or 'fp8'");
   use_device(X);
   wait_depends(s, depends);
   int64_t n_iters = 0, n_rescored = 0, launches = 0;
@@ -736,8 +738,8 @@ py::tuple py_kmeans_lloyd_driver_ex(py::object X_, py::object w_,
   static const char *kPaths[] = {"generic",        "fp32-mfma",
                                  "bf16+rescore",   "bf16x3+rescore",
                                  "tiled-fp32",     "tiled-bf16+rescore",
-                                 "tiled-bf16x3+rescore"};
-  stats["path"] = kPaths[(path >= 0 && path < 7) ? path : 0];
+                                 "tiled-bf16x3+rescore", "tiled-fp8+rescore"};
+  stats["path"] = kPaths[(path >= 0 && path < 8) ? path : 0];
   return py::make_tuple((size_t)n_iters, inertia, stats);
 }
 
@@ -1071,8 +1073,10 @@ py::dict py_autotune(py::object X_, py::object w_, py::object init_,
     throw py::value_error("Array dimensions are not consistent");
   int prec = precision == "fp32" ? KMD_PREC_FP32
              : precision == "bf16x3" ? KMD_PREC_BF16X3
-             : precision == "bf16" ? KMD_PREC_BF16 : -1;
-  if (prec < 0) throw py::value_error("precision must be 'fp32', 'bf16x3' or 'bf16'");
+             : precision == "bf16" ? KMD_PREC_BF16
+             : precision == "fp8" ? KMD_PREC_FP8 : -1;
+  if (prec < 0)
+    throw py::value_error("precision must be 'fp32', 'bf16x3', 'bf16' or 'fp8'");
   use_device(X);
   constexpr int kCap = 64;
   int bp = 0, bb = 0, bs = 0;
@@ -1263,6 +1267,19 @@ PYBIND11_MODULE(_kmeans_dpcpp_amd, m) {
         py::arg("X_t"), py::arg("sample_weight"), py::arg("init_centroid_t"),
         py::arg("precision") = "fp32", py::arg("iters") = 3,
         py::arg("sycl_queue") = py::none());
+  m.def("fp8_roundtrip",
+        [](py::object in_, py::object out_, py::object q) {
+          kmd_stream_t s = resolve_stream(q);
+          Arr in = to_arr(in_, s, "in"), out = to_arr(out_, s, "out");
+          if (in.ndim != 1 || out.ndim != 1 || in.dim(0) != out.dim(0) ||
+              !in.is_float() || in.dtype() != KMD_F32 || !in.same_type(out))
+            throw py::value_error("in and out must be float32 vectors of one length");
+          use_device(in);
+          check_rc(kmd_fp8_roundtrip(in.dim(0), static_cast<const float *>(in.data),
+                                     static_cast<float *>(out.data), s));
+        },
+        "out = in rounded to FP8 e4m3 and back with the FP8 kernels' conversion.",
+        py::arg("in"), py::arg("out"), py::arg("sycl_queue") = py::none());
   m.def("set_tuning",
         [](int path, int blocks_per_cu, int shape) {
           check_rc(kmd_set_tuning(path, blocks_per_cu, shape));

@@ -126,7 +126,7 @@ class KMeans:
     ("auto": 1 for k-means++ / array, 10 for random / callable), max_iter,
     tol (scaled by
     the mean feature variance), verbose, random_state, copy_x (X is never
-    modified here).  Extra: precision ("fp32", "bf16x3", "bf16": the cross
+    modified here).  Extra: precision ("fp32", "bf16x3", "bf16", "fp8": the cross
     term of the assignment; labels equal the fp32 ones in every mode),
     device (default: the current ROCm device).
     """
@@ -158,8 +158,8 @@ class KMeans:
             raise ValueError(f"n_samples={n} should be >= n_clusters={self.n_clusters}.")
         if self.algorithm != "lloyd":
             raise ValueError("only algorithm='lloyd' is implemented")
-        if self.precision not in ("fp32", "bf16x3", "bf16"):
-            raise ValueError("precision must be 'fp32', 'bf16x3' or 'bf16'")
+        if self.precision not in ("fp32", "bf16x3", "bf16", "fp8"):
+            raise ValueError("precision must be 'fp32', 'bf16x3', 'bf16' or 'fp8'")
         if not (isinstance(self.max_iter, numbers.Integral) and self.max_iter >= 1):
             raise ValueError("max_iter must be a positive integer")
         if self.tol < 0:

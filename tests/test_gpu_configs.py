@@ -82,7 +82,7 @@ def test_c1_plain_fp32_full_size(kd, torch_dev):
     assert abs(inr3 - fp64_inertia(Xt, w, C3n, L3)) <= 1e-5 * abs(inr3)
 
 
-@pytest.mark.parametrize("precision", ["bf16", "bf16x3"])
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3", "fp8"])
 def test_c2_mixed_precision_full_size(kd, torch_dev, precision):
     n, d, k = 2_000_000, 128, 1024
     X_d, init_d = bench.make_c2_data(n, d, k, 2001, torch_dev)

@@ -229,10 +229,17 @@ def test_relocation_with_next_iteration_enqueued_ahead(kd, torch_dev, monkeypatc
                                             ((300_001, 32, 256), 7, 2.0),
                                             ((100_000, 64, 100), 3, 1.5),
                                             ((65_537, 5, 33), 9, 3.0)])
-def test_mixed_precision_matches_fp32_bit_exactly(kd, torch_dev, shape, seed, std):
+@pytest.mark.parametrize("rescore", ["candidate-tiles", "gather"])
+def test_mixed_precision_matches_fp32_bit_exactly(kd, torch_dev, monkeypatch, shape, seed,
+                                                  std, rescore):
     """BF16 (+ certified-margin FP32 re-score) and BF16x3 fits give the FP32
     path's labels; with exact fixed-point sums the centroids are then
-    bit-identical too, iteration for iteration."""
+    bit-identical too, iteration for iteration.  The re-score either covers
+    only each queued sample's candidate tiles (read from a sample-major copy)
+    with a separate exact update of the queue, or gathers every queued
+    sample over all tiles."""
+    if rescore == "candidate-tiles":
+        monkeypatch.setenv("KMD_MIXED_CAND", "1")
     n, d, k = shape
     Xt, init = lo.gaussian_blobs(n, d, k, seed=seed, std=std)
     w = np.ones(n, np.float32)
@@ -312,14 +319,15 @@ def test_tiled_fp32_wide_shape_vs_oracle(kd, torch_dev):
                                             ((40_000, 96, 512), 47, 1.0),
                                             ((25_001, 70, 200), 49, 2.0)])
 def test_tiled_mixed_matches_tiled_fp32_bit_exactly(kd, torch_dev, shape, seed, std):
-    """BF16 and BF16x3 on the tiled path, with the certified margin and the
-    FP32 gather re-score, give the tiled FP32 fit bit for bit."""
+    """BF16, BF16x3 and FP8 (e4m3, per-tile scales) on the tiled path, with
+    the certified margin and the FP32 re-score, give the tiled FP32 fit bit
+    for bit."""
     n, d, k = shape
     Xt, init = lo.gaussian_blobs(n, d, k, seed=seed, std=std)
     w = np.ones(n, np.float32)
     ref = run_driver_ex(kd, Xt, w, init, 1e-4, 8, "fp32")
     assert ref[4]["path"] == "tiled-fp32"
-    for prec in ("bf16", "bf16x3"):
+    for prec in ("bf16", "bf16x3", "fp8"):
         got = run_driver_ex(kd, Xt, w, init, 1e-4, 8, prec)
         assert got[4]["path"] == f"tiled-{prec}+rescore"
         same_fit(got, ref)
@@ -450,3 +458,31 @@ def test_ordinary_data_keeps_exact_sums(kd, torch_dev):
     b = run_driver(kd, Xt, w, init, 0.0, 4)
     assert_array_equal(a[0], b[0])
     assert a[1].tobytes() == b[1].tobytes()
+
+
+def test_fp8_conversion_rounds_to_nearest_even(kd, torch_dev):
+    """The e4m3 conversion the FP8 kernels use rounds to nearest (ties to
+    even) with 3 mantissa bits: |e4m3(v) - v| <= 2^-4 |v| for normal values
+    and <= 2^-10 below them -- the error bound's assumptions."""
+    rng = np.random.default_rng(17)
+    v = np.concatenate([
+        np.exp(rng.uniform(np.log(2.0 ** -14), np.log(224.0), 200_000)) *
+        rng.choice([-1.0, 1.0], 200_000),
+        [0.0, 1.0, 1.0625, 1.1875, 1.0625 + 2.0 ** -10, 0.5, 224.0, -224.0, 2.0 ** -6,
+         2.0 ** -9, 3 * 2.0 ** -10]]).astype(np.float32)
+    out = torch.empty(v.shape[0], dtype=torch.float32, device="cuda")
+    kd.fp8_roundtrip(to_dev(v), out, q())
+    r = to_np(out).astype(np.float64)
+    x = v.astype(np.float64)
+    err = np.abs(r - x)
+    normal = np.abs(x) >= 2.0 ** -6
+    assert np.all(err[normal] <= 2.0 ** -4 * np.abs(x[normal]))
+    assert np.all(err[~normal] <= 2.0 ** -10)
+    # ties go to the even mantissa; exact values stay exact
+    tail = r[-11:]
+    assert tail[0] == 0.0 and tail[1] == 1.0
+    assert tail[2] == 1.0           # 1 + 2^-4: halfway between 1 and 1.125
+    assert tail[3] == 1.25          # 1 + 3 2^-4: halfway between 1.125 and 1.25
+    assert tail[4] == 1.125         # just above the tie
+    assert tail[5] == 0.5 and tail[6] == 224.0 and tail[7] == -224.0
+    assert tail[8] == 2.0 ** -6 and tail[9] == 2.0 ** -9

@@ -140,7 +140,7 @@ def test_bad_parameters(kd):
     with pytest.raises(ValueError):
         kd.KMeans(2, init="nope").fit(X)
     with pytest.raises(ValueError):
-        kd.KMeans(2, precision="fp8").fit(X)
+        kd.KMeans(2, precision="fp4").fit(X)
     with pytest.raises(ValueError):
         kd.KMeans(2, init=np.zeros((3, 2))).fit(X + np.arange(5)[:, None])
     with warnings.catch_warnings(record=True):

@@ -2,10 +2,10 @@
 that cannot allocate (its working data lives in a 1 MiB device array per
 GPU).  Called through the C ABI (ctypes) and compared with
 kmd_fused_lloyd_single_step on the same inputs, two calls in a row, the
-second smaller than the first, for the LDS-resident shapes (fixed-point
-scales in the array), the tiled shapes (centroid tiles in the array) and a
-float64 shape (no working data); shapes whose working data exceed the array
-are refused before any device work.
+second smaller than the first, for the tiled shapes (centroid tiles in the
+array), the LDS-resident shapes and a float64 shape (no working data);
+shapes whose working data exceed the array are refused before any device
+work.
 
 Bar: labels and unit-weight counts bit-exact; centroid sums within 1e-6 of
 their scale (the per-block flush into the copies is a float atomic add, so
@@ -78,7 +78,7 @@ def one_case(lib, n, d, k, n_copies, dtype, seed):
 
 
 def test_noalloc_fused_step_equals_the_allocating_one(lib, torch_dev):
-    # LDS-resident shapes: a large call, then a smaller one on the same array
+    # LDS-resident shapes (no working data): a large call, then a smaller one
     one_case(lib, 300_000, 32, 256, 8, np.float32, seed=1)
     one_case(lib, 50_001, 14, 127, 3, np.float32, seed=2)
     # tiled shapes (centroid tiles in the array): large, then smaller
