@@ -144,18 +144,18 @@ def make_c2_data(n, d, k, seed, device):
 
 
 def fp64_update(X_t, w, labels, k):
-    """One FP64 centroid update from given labels (empty clusters: NaN)."""
+    """One FP64 centroid update from given labels (empty clusters: NaN):
+    samples sorted by label, then per-cluster FP64 segment sums (no atomics,
+    so its cost does not grow with few, crowded clusters)."""
     import torch
-    d = X_t.shape[0]
-    sums = torch.zeros((k, d), dtype=torch.float64, device=X_t.device)
-    cnt = torch.zeros(k, dtype=torch.float64, device=X_t.device)
     lab = labels.long()
-    chunk = 1 << 20
-    for s in range(0, X_t.shape[1], chunk):
-        e = min(X_t.shape[1], s + chunk)
-        ws = w[s:e].double()
-        sums.index_add_(0, lab[s:e], X_t[:, s:e].T.double() * ws[:, None])
-        cnt.index_add_(0, lab[s:e], ws)
+    order = torch.argsort(lab, stable=True)
+    lens = torch.bincount(lab, minlength=k)
+    ws = w[order].double()
+    cnt = torch.segment_reduce(ws, "sum", lengths=lens, axis=0)
+    xs = X_t.T[order].double() * ws[:, None]
+    sums = torch.segment_reduce(xs, "sum", lengths=lens, axis=0)
+    del xs
     return (sums / cnt[:, None]).T, cnt
 
 

@@ -20,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-HIP_SOURCES = ["util.hip", "sort.hip", "lloyd.hip", "lloyd_fast.hip",
+HIP_SOURCES = ["util.hip", "sort.hip", "lloyd.hip", "lloyd_fast.hip", "lloyd_f64.hip",
                "lloyd_mixed.hip", "lloyd_tiled.hip", "batched.hip", "seeding.hip", "noalloc.hip",
                "driver.hip", "capi.hip"]
 LIB_NAME = "libkmeans_dpcpp_amd.so"

@@ -53,6 +53,8 @@ struct LloydWorkspace {
   // LDS-resident mixed path: the re-score restricted to candidate tiles,
   // from the sample-major copy xr, and the queue's separate exact update
   bool mixed_cand = false;
+  // LDS-resident mixed path: undecided samples re-scored inside the kernel
+  bool mixed_ink = false;
   FusedPlan plan;
   TempBuffer hn, sizes, shifts, per_sample, copies, sizes_copies, list,
       scalars, scales;
@@ -96,17 +98,21 @@ struct LloydWorkspace {
   }
 };
 
-// KMD_MIXED_CAND=1 switches the LDS-resident mixed path's re-score from the
-// gather of every queued sample over all tiles (the default: one pass that
-// also adds the re-scored samples to the sums) to the candidate-tile
-// re-score from a sample-major copy of X plus a separate exact update of the
-// queue.  The candidate form moves fewer bytes per queued sample but costs
-// four more launches per iteration; at n = 10M, d = 32, k = 256 it measured
-// 630 iters/s against the gather's 795, so it is not the default.
-static bool mixed_cand_requested() {
-  const char *e = std::getenv("KMD_MIXED_CAND");
+// Re-score forms of the LDS-resident mixed path.  By default undecided
+// samples are re-scored inside the BF16 kernel (when the FP32 centroid tiles
+// fit LDS beside its own data), else by a second kernel that gathers every
+// queued sample over all tiles and adds it to the sums.
+// KMD_MIXED_GATHER=1 keeps the second kernel; KMD_MIXED_CAND=1 switches to
+// the candidate-tile re-score from a sample-major copy of X plus a separate
+// exact update of the queue (fewer bytes per queued sample, four more
+// launches per iteration; at n = 10M, d = 32, k = 256: 630 iters/s against
+// the gather's 795).
+static bool env_is_1(const char *name) {
+  const char *e = std::getenv(name);
   return e != nullptr && e[0] == '1';
 }
+static bool mixed_cand_requested() { return env_is_1("KMD_MIXED_CAND"); }
+static bool mixed_gather_requested() { return env_is_1("KMD_MIXED_GATHER"); }
 
 // Mixed-precision workspace: BF16 kernel partials, then FP32 re-score
 // partials; returns nullptr when the shape is not served or the inputs hold
@@ -122,17 +128,19 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_mixed_workspace(
     // reading the sample from a sample-major copy of X (made once per fit)
     const bool cand = tiled_cand_supported(k) && tiled_supported(d, k) &&
                       mixed_cand_fits(d, k, terms) && mixed_cand_requested();
-    const int ba = mixed_blocks<I>(n, d, k, terms, cand);
-    const int bb = fast_rescore_blocks<I>(d, k, ba);
-    if (ba < 1 || bb < 1) return nullptr;
+    const bool ink = !cand && !mixed_gather_requested() && mixed_ink_fits(d, k, terms);
+    const int ba = mixed_blocks<I>(n, d, k, terms, cand, ink);
+    const int bb = ink ? 0 : fast_rescore_blocks<I>(d, k, ba);
+    if (ba < 1 || (!ink && bb < 1)) return nullptr;
     auto ws = std::make_unique<LloydWorkspace<T, I>>(
         n, d, k, ba, true, true, FusedPlan{},
         sizeof(int64_t) * (size_t)(ba + bb) * (d * k + k), 0, s);
     ws->mixed_terms = terms;
+    ws->mixed_ink = ink;
     ws->blocks_b = bb;
     ws->aop = TempBuffer(mixed_aop_bytes(d, k, terms), s);
     ws->bound = TempBuffer(2 * sizeof(float), s);
-    ws->seg_cap = mixed_seg_cap(n, d, ba);
+    ws->seg_cap = ink ? 0 : mixed_seg_cap(n, d, ba);
     ws->queue = TempBuffer(sizeof(int) * (size_t)ba * ws->seg_cap, s);
     ws->queue_counts = TempBuffer(sizeof(int) * (size_t)ba, s);
     launch_fixed_point_scales(n, d, X, w, ws->scale(), ws->nonfinite(), s);
@@ -478,8 +486,12 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
                            ws.queue.template as<int>(),
                            ws.queue_counts.template as<int>(),
                            ws.mixed_cand ? ws.qmask.template as<unsigned>() : nullptr, s,
-                           skip);
-      if (ws.mixed_cand) {
+                           skip, ws.mixed_ink ? cur : nullptr,
+                           ws.mixed_ink ? ws.hn.template as<const T>() : nullptr,
+                           ws.mixed_ink ? ws.queue_count() : nullptr);
+      if (ws.mixed_ink) {
+        // re-scored inside the kernel: nothing more to launch
+      } else if (ws.mixed_cand) {
         // FP32 re-score of the candidate tiles (labels), then the exact sums
         // of the re-scored samples into the re-score partial slots
         launch_tiled_cand_rescore<I>(

@@ -150,6 +150,9 @@ size_t window_lds_bytes() {
 // private-copy buffers it allocates match the launch.
 template <typename T>
 FusedPlan plan_generic_fused(int64_t n, int64_t d, int64_t k) {
+  // float64 shapes whose centroids and sums fit LDS: the FP64 MFMA kernel
+  if (std::is_same<T, double>::value && fast64_supported(d, k, true))
+    return plan_fast64_fused(n, d, k);
   FusedPlan p;
   const int64_t priv_bytes = (int64_t)sizeof(T) * (d + 1) * k;
   p.priv = priv_bytes <= kPrivMaxBytes;
@@ -172,6 +175,13 @@ void launch_fused_generic(int64_t n, int64_t d, int64_t k, int64_t n_copies,
                           const T *w, const T *C, const T *hn, I *labels,
                           T *copies, T *sizes_copies, hipStream_t s) {
   if (n == 0) return;
+  if constexpr (std::is_same<T, double>::value) {
+    if (fast64_supported(d, k, true)) {
+      launch_fast64_fused<I>(n, d, k, n_copies, p, store_flush, X, w, C, hn, labels,
+                             copies, sizes_copies, s);
+      return;
+    }
+  }
   hipLaunchKernelGGL((lloyd_tile_krn<T, I, MODE_FUSED>), dim3(p.blocks),
                      dim3(kBlock), p.lds_bytes, s, n, (int)d, (int)k, X, w, C,
                      hn, labels, copies, sizes_copies, n_copies, (int)p.priv,
@@ -227,6 +237,11 @@ void launch_assignment(int64_t n, int64_t d, int64_t k, int64_t h,
   if constexpr (std::is_same<T, float>::value) {
     if (tiled_supported(d, k)) {
       launch_tiled_assign<I>(n, d, k, X, C, hn, labels, s);
+      return;
+    }
+  } else {
+    if (fast64_supported(d, k, false)) {
+      launch_fast64_assign<I>(n, d, k, X, C, hn, labels, s);
       return;
     }
   }

@@ -45,6 +45,21 @@ void launch_fused_generic(int64_t n, int64_t d, int64_t k, int64_t n_copies,
                           const T *w, const T *C, const T *hn, I *labels,
                           T *copies, T *sizes_copies, hipStream_t s);
 
+// FP64 MFMA path (v_mfma_f64_16x16x4f64; csrc/lloyd_f64.hip): true when the
+// centroid operands (and, with update, the FP64 sums) fit LDS, d <= 64.
+bool fast64_supported(int64_t d, int64_t k, bool update);
+FusedPlan plan_fast64_fused(int64_t n, int64_t d, int64_t k);
+template <typename I>
+void launch_fast64_fused(int64_t n, int64_t d, int64_t k, int64_t n_copies,
+                         const FusedPlan &p, bool store_flush, const double *X,
+                         const double *w, const double *C, const double *hn,
+                         I *labels, double *copies, double *sizes_copies,
+                         hipStream_t s);
+template <typename I>
+void launch_fast64_assign(int64_t n, int64_t d, int64_t k, const double *X,
+                          const double *C, const double *hn, I *labels,
+                          hipStream_t s);
+
 // FP32 MFMA path: true when the (d, k) shape is served by it.
 template <typename T>
 bool fast_fused_supported(int64_t d, int64_t k);
@@ -109,11 +124,15 @@ int mixed_idx_bits(int64_t k);
 // Bytes of the BF16 A-operand array for T split terms (1 or 3).
 size_t mixed_aop_bytes(int64_t d, int64_t k, int T);
 // Blocks of the BF16 kernel for this shape; 0 when the shape is not served.
-// cand: with candidate-tile tracking (launch_mixed_step's qmask).
+// cand: with candidate-tile tracking (launch_mixed_step's qmask); ink: with
+// the in-kernel FP32 re-score (launch_mixed_step's C, hn).
 template <typename I>
-int mixed_blocks(int64_t n, int64_t d, int64_t k, int T, bool cand = false);
+int mixed_blocks(int64_t n, int64_t d, int64_t k, int T, bool cand = false,
+                 bool ink = false);
 // Whether the candidate tracking fits this shape (k <= 1024, LDS).
 bool mixed_cand_fits(int64_t d, int64_t k, int T);
+// Whether the in-kernel re-score (FP32 centroid tiles in LDS) fits.
+bool mixed_ink_fits(int64_t d, int64_t k, int T);
 // Builds the BF16 A operands (centroids + split half-norms) and the error
 // bound inputs bound[0] = max ||c||, bound[1] = max hn.
 void launch_mixed_prep(int64_t d, int64_t k, int T, const float *C,
@@ -125,13 +144,19 @@ int64_t mixed_seg_cap(int64_t n, int64_t d, int blocks);
 // to queue[b * seg_cap + 0 : queue_counts[b]].  Writes `blocks` partial slots.
 // qmask (may be null): per queued sample the bit mask of the 32-centroid
 // tiles whose best BF16 score is within the certified bound of the sample's
-// best (launch with mixed_blocks(..., cand = true)).
+// best (launch with mixed_blocks(..., cand = true)).  With C and hn (FP32
+// centroids and half norms; launch with mixed_blocks(..., ink = true)) the
+// undecided samples are re-scored inside the kernel in the FP32 path's
+// arithmetic and added to the same partial slots; queue_counts[b] is then
+// block b's re-scored count and *rescored_total their sum.
 template <typename I>
 void launch_mixed_step(int64_t n, int64_t d, int64_t k, int T, int blocks,
                        const float *X, const float *w, const void *aop,
                        const float *bound, const float *scale, I *labels,
                        void *partials, int *queue, int *queue_counts,
-                       unsigned *qmask, hipStream_t s, const int *skip = nullptr);
+                       unsigned *qmask, hipStream_t s, const int *skip = nullptr,
+                       const float *C = nullptr, const float *hn = nullptr,
+                       int *rescored_total = nullptr);
 // int64 fixed-point sums of the queued samples (labels already final), read
 // from the sample-major X_r: `blocks` partial slots from `partials`.
 template <typename I>

@@ -24,6 +24,7 @@
 // pair), so every label equals the FP32 path's label.  Both kernels add into
 // exact int64 partial sums, so which kernel handled a sample does not change
 // the sums.
+#include "fp32_tile.hpp"
 #include "lloyd_fast.hpp"
 
 namespace kmd {
@@ -37,7 +38,10 @@ typedef unsigned long long u64;
 // threads per block: 16 waves (4 per SIMD) where the registers allow it
 // (d <= 32 at one 32-sample column block per wave: <= 128 VGPRs; measured
 // faster than 12 waves x 2 column blocks at n = 10 M, d = 32, k = 256)
-constexpr int nt_of(int D) { return D <= 32 ? 1024 : 512; }
+// (the in-kernel re-score's FP32 registers on top: 12 waves, <= 168 VGPRs)
+constexpr int nt_of(int D, bool ink = false) {
+  return D <= 32 ? (ink ? 768 : 1024) : 512;
+}
 constexpr int kS = 1;                  // 32-sample column blocks per wave
 constexpr size_t kLdsPerCu = 160 * 1024;
 constexpr int kScPad = 66;             // scale slots in LDS (d + 1 <= 65)
@@ -62,6 +66,8 @@ struct MixedArgs {
   int *queue_counts;     // [block] entries used in each segment
   unsigned *qmask;       // [block][seg_cap] candidate tiles of each queued
                          // sample (k <= 1024), or null: no candidate tracking
+  const float *C;        // in-kernel re-score: FP32 centroids [d][k]
+  const float *hn;       //   and half norms [k]
   int64_t seg_cap;
   float pinf;            // +inf, passed at run time so that med3(b, v, pinf)
                          // stays one v_med3 (no max pattern, no canonicalise)
@@ -141,7 +147,13 @@ __global__ void __launch_bounds__(256)
 // FULL: d == D (no feature beyond d in the lane layout).  NT threads per
 // block: one block per CU (the LDS sums fill most of it), so NT sets the
 // waves per SIMD that hide each other's load and matrix-core latency.
-template <int D, int T, bool FULL, typename I, int NT>
+// INK: undecided samples are re-scored inside this kernel, in the FP32
+// path's arithmetic (fp32_tile.hpp over FP32 centroid tiles in LDS): each
+// wave keeps its undecided samples in an LDS list and scores 32 of them at
+// a time, reading their features again (L2-resident: the wave has just
+// read them), then adds them to the block's sums; no queue leaves the
+// kernel.  Otherwise the queue segments go to the separate re-score.
+template <int D, int T, bool FULL, typename I, int NT, bool INK>
 __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
   // an iteration enqueued ahead of a relocation: nothing to compute
   if (a.skip != nullptr && *a.skip != 0) return;
@@ -160,13 +172,29 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
   // [wave][tile][lane]
   unsigned short *tml = reinterpret_cast<unsigned short *>(
       reinterpret_cast<unsigned char *>(S64 + dk + k) + 16);
-  const bool cand = a.qmask != nullptr;
+  const bool cand = !INK && a.qmask != nullptr;
   const int tid = threadIdx.x;
+  // in-kernel re-score: FP32 tiles Cf[((ct KSF + ks) 2 + half) 32 + col] =
+  // C[2 ks + half][32 ct + col], -hn (-inf padded), per-wave lists [wave][64]
+  constexpr int KSF = D / 2;
+  float *Cf = reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(S64 + dk + k) + 16);
+  float *Hf = Cf + 2 * KSF * a.kpad;
+  int *wq = reinterpret_cast<int *>(Hf + a.kpad);
   for (int64_t i = tid; i < (int64_t)n_ct * KK * 64; i += NT) Al[i] = a.aop[i];
   for (int f = tid; f <= d; f += NT) Sc[f] = a.scale[f];
   for (int64_t i = tid; i < dk + k; i += NT) S64[i] = 0ull;
+  if constexpr (INK) {
+    for (int i = tid; i < 2 * KSF * a.kpad; i += NT) {
+      const int cl = i & 31, hf = (i >> 5) & 1, blk = i >> 6;
+      const int ks = blk % KSF, t = blk / KSF;
+      const int f = 2 * ks + hf, c = 32 * t + cl;
+      Cf[i] = (f < d && c < k) ? a.C[(int64_t)f * k + c] : 0.f;
+    }
+    for (int c = tid; c < a.kpad; c += NT)
+      Hf[c] = (c < k) ? -a.hn[c] : -__builtin_huge_valf();
+  }
   if (tid == 0) q_used = 0;
-  int *qseg = a.queue + (int64_t)blockIdx.x * a.seg_cap;
+  int *qseg = INK ? nullptr : a.queue + (int64_t)blockIdx.x * a.seg_cap;
   __syncthreads();
 
   const int lane = tid & (kWave - 1);
@@ -191,6 +219,50 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
 
   constexpr int kSuper = 32 * kS;
   const int64_t n_super = ceil_div<int64_t>(n, kSuper);
+  // INK: the wave's list of undecided samples and its length (wave-uniform)
+  static_assert(!INK || kS == 1, "one 32-sample block per wave and iteration");
+  int *wl = wq + wave * kWave;
+  int wl_len = 0, wl_done = 0;
+  // orders the list's LDS stores and loads across the lanes of the wave
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  };
+  auto rescore_list = [&](int cnt) {
+    // lanes l and l ^ 32 take list entry l % 32 (entries cnt.. are padding)
+    const int64_t s = col < cnt ? (int64_t)wl[col] : -1;
+    float xq[KSF][1];
+#pragma unroll
+    for (int ks = 0; ks < KSF; ++ks) {
+      const int f = 2 * ks + half;
+      xq[ks][0] = (s >= 0 && (FULL || f < d)) ? X[(int64_t)f * n + s] : 0.f;
+    }
+    float bv[1] = {__builtin_huge_valf()};
+    int bt[1] = {0}, br[1] = {0};
+    for (int ct = 0; ct < n_ct; ++ct) {
+      f32x16 acc[1];
+      tile_mfma<KSF, 1>(Cf + lane + ct * (KSF * 64), Hf + ct * 32 + 4 * half, xq, acc);
+      tile_argmin<1>(ct, acc, bv, bt, br);
+    }
+    int bi = (bv[0] < __builtin_huge_valf()) ? best_index(bt[0], br[0], half) : 0;
+    const float ov = __shfl_xor(bv[0], 32, kWave);
+    const int oi = __shfl_xor(bi, 32, kWave);
+    if ((ov < bv[0]) || (ov == bv[0] && oi < bi)) bi = oi;
+    if (s >= 0) {
+      if (half == 0) labels[s] = (I)bi;
+      const float ws = w[s];
+      u64 *row = S64 + (int64_t)bi * (d + 1);
+#pragma unroll
+      for (int ks = 0; ks < KSF; ++ks) {
+        const int f = 2 * ks + half;
+        if (FULL || f < d)
+          atomicAdd(row + f, (u64)(long long)__float2int_rn((xq[ks][0] * ws) * Sc[f]));
+      }
+      if (half == 0)
+        atomicAdd(row + d, (u64)(long long)__float2int_rn(ws * Sc[d]));
+    }
+    wl_done += cnt;
+  };
   // lane features f(i) = 16 (i / 8) + 8 half + i % 8
   auto load = [&](int64_t s0, float (&dst)[XH][kS]) {
     const bool whole = vec && (s0 + kS <= n);
@@ -387,7 +459,11 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
         tiles_any = bits | (unsigned)__shfl_xor((int)bits, 32, kWave);
       }
       const uint64_t mask = __ballot(enq);
-      if (mask != 0ull) {
+      if (INK) {
+        // append to the wave's list (< 32 entries before, <= 32 added)
+        if (enq) wl[wl_len + __popcll(mask & ((1ull << lane) - 1ull))] = (int)s;
+        wl_len += __popcll(mask);
+      } else if (mask != 0ull) {
         const int cnt = __popcll(mask);
         const int leader = __ffsll((long long)mask) - 1;
         int base = 0;
@@ -422,6 +498,24 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
       if (half == 0)
         atomicAdd(row + d, (u64)(long long)__float2int_rn(ws * Sc[d]));
     }
+    if (INK && wl_len >= 32) {
+      wave_sync();
+      rescore_list(32);
+      // move the rest (< 32 entries) to the front
+      const int rest = wl_len - 32;
+      const int v = lane < rest ? wl[32 + lane] : 0;
+      wave_sync();
+      if (lane < rest) wl[lane] = v;
+      wave_sync();
+      wl_len = rest;
+    }
+  }
+  if (INK) {
+    if (wl_len > 0) {
+      wave_sync();
+      rescore_list(wl_len);
+    }
+    if (lane == 0 && wl_done > 0) atomicAdd(&q_used, wl_done);
   }
   __syncthreads();
   // partial slot layout: [f][c] sums, then [c] counts
@@ -442,30 +536,36 @@ inline int d_bucket(int64_t d) {
 
 inline int kk_of(int D, int T) { return (T * D + 3 + 15) / 16; }
 
-// cand: with the per-tile maxima of every lane (candidate tiles, k <= 1024)
-inline size_t mixed_lds_bytes(int64_t d, int64_t k, int T, bool cand = false) {
+// cand: with the per-tile maxima of every lane (candidate tiles, k <= 1024);
+// ink: with the FP32 tiles, -hn and the per-wave lists of the in-kernel
+// re-score
+inline size_t mixed_lds_bytes(int64_t d, int64_t k, int T, bool cand = false,
+                              bool ink = false) {
   const int D = d_bucket(d);
   const int64_t kpad = ceil_div<int64_t>(k, 32) * 32;
-  const size_t tml = cand ? sizeof(unsigned short) * (size_t)(nt_of(D) / kWave) *
+  const size_t tml = cand ? sizeof(unsigned short) * (size_t)(nt_of(D, false) / kWave) *
                                 (size_t)(kpad / 32) * kWave
                           : 0;
+  const size_t inks = ink ? sizeof(float) * (size_t)(D + 1) * kpad +
+                                sizeof(int) * (size_t)nt_of(D, true)
+                          : 0;
   return (size_t)(kpad / 32) * kk_of(D, T) * 64 * 16 + sizeof(float) * kScPad +
-         sizeof(u64) * (size_t)(d + 1) * k + 16 + tml;
+         sizeof(u64) * (size_t)(d + 1) * k + 16 + std::max(tml, inks);
 }
 
 using MixedFn = void (*)(MixedArgs);
 
-template <typename I>
-MixedFn pick_mixed(int64_t d, int T) {
+template <typename I, bool INK>
+MixedFn pick_mixed_ink(int64_t d, int T) {
   const int D = d_bucket(d);
 #define KMD_MIXED_CASE(DD)                                                  \
   if (D == DD) {                                                            \
-    constexpr int NT = nt_of(DD);                                           \
+    constexpr int NT = nt_of(DD, INK);                                      \
     if (d == DD)                                                            \
-      return T == 3 ? &mixed_lloyd_krn<DD, 3, true, I, NT>                  \
-                    : &mixed_lloyd_krn<DD, 1, true, I, NT>;                 \
-    return T == 3 ? &mixed_lloyd_krn<DD, 3, false, I, NT>                   \
-                  : &mixed_lloyd_krn<DD, 1, false, I, NT>;                  \
+      return T == 3 ? &mixed_lloyd_krn<DD, 3, true, I, NT, INK>             \
+                    : &mixed_lloyd_krn<DD, 1, true, I, NT, INK>;            \
+    return T == 3 ? &mixed_lloyd_krn<DD, 3, false, I, NT, INK>              \
+                  : &mixed_lloyd_krn<DD, 1, false, I, NT, INK>;             \
   }
   KMD_MIXED_CASE(16)
   KMD_MIXED_CASE(32)
@@ -473,6 +573,11 @@ MixedFn pick_mixed(int64_t d, int T) {
   KMD_MIXED_CASE(64)
 #undef KMD_MIXED_CASE
   return nullptr;
+}
+
+template <typename I>
+MixedFn pick_mixed(int64_t d, int T, bool ink = false) {
+  return ink ? pick_mixed_ink<I, true>(d, T) : pick_mixed_ink<I, false>(d, T);
 }
 
 }  // namespace
@@ -501,14 +606,21 @@ bool mixed_cand_fits(int64_t d, int64_t k, int T) {
   return k <= 1024 && mixed_lds_bytes(d, k, T, true) <= kLdsPerCu;
 }
 
+bool mixed_ink_fits(int64_t d, int64_t k, int T) {
+  // d <= 48: the FP32 re-score's registers fit beside the BF16 kernel's
+  // without spilling (not at d = 64)
+  return d <= 48 && mixed_lds_bytes(d, k, T, false, true) <= kLdsPerCu;
+}
+
 template <typename I>
-int mixed_blocks(int64_t n, int64_t d, int64_t k, int T, bool cand) {
+int mixed_blocks(int64_t n, int64_t d, int64_t k, int T, bool cand, bool ink) {
   if (d < 1 || d > 64 || k < 1 || n >= (int64_t(1) << 31)) return 0;
   if (mixed_idx_bits(k) > 12) return 0;  // index packing: <= 4096 centroids
+  if (cand && ink) return 0;
   if (cand && !mixed_cand_fits(d, k, T)) return 0;
-  const size_t lds = mixed_lds_bytes(d, k, T, cand);
+  const size_t lds = mixed_lds_bytes(d, k, T, cand, ink);
   if (lds > kLdsPerCu) return 0;
-  MixedFn fn = pick_mixed<I>(d, T);
+  MixedFn fn = pick_mixed<I>(d, T, ink);
   if (fn == nullptr) return 0;
   static thread_local MixedFn attr_done[64] = {};
   bool seen = false;
@@ -525,12 +637,12 @@ int mixed_blocks(int64_t n, int64_t d, int64_t k, int T, bool cand) {
   }
   int per_cu = 0;
   KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void *>(fn), nt_of(d_bucket(d)), lds));
+      &per_cu, reinterpret_cast<const void *>(fn), nt_of(d_bucket(d), ink), lds));
   per_cu = apply_blocks_cap(per_cu);
   if (per_cu < 1) return 0;
   const int64_t need =
       ceil_div<int64_t>(ceil_div<int64_t>(n > 0 ? n : 1, 32 * kS),
-                        nt_of(d_bucket(d)) / kWave);
+                        nt_of(d_bucket(d), ink) / kWave);
   int64_t blocks = (int64_t)device_cu_count() * per_cu;
   if (blocks > need) blocks = need;
   if (blocks > 1024) blocks = 1024;  // queue segments the re-score reads
@@ -563,13 +675,30 @@ void launch_mixed_prep(int64_t d, int64_t k, int T, const float *C,
   KMD_LAUNCH_CHECK();
 }
 
+namespace {
+// total of the per-block counts of the in-kernel re-score (one wave)
+__global__ void count_total_krn(int blocks, const int *counts, int *total,
+                                const int *skip) {
+  if (skip != nullptr && *skip != 0) return;
+  int t = 0;
+  for (int b = threadIdx.x; b < blocks; b += kWave) t += counts[b];
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);
+  if (threadIdx.x == 0) total[0] = t;
+}
+}  // namespace
+
 template <typename I>
 void launch_mixed_step(int64_t n, int64_t d, int64_t k, int T, int blocks,
                        const float *X, const float *w, const void *aop,
                        const float *bound, const float *scale, I *labels,
                        void *partials, int *queue, int *queue_counts,
-                       unsigned *qmask, hipStream_t s, const int *skip) {
-  MixedFn fn = pick_mixed<I>(d, T);
+                       unsigned *qmask, hipStream_t s, const int *skip,
+                       const float *C, const float *hn, int *rescored_total) {
+  const bool ink = C != nullptr;
+  if (ink && (qmask != nullptr || hn == nullptr || rescored_total == nullptr))
+    throw RuntimeError("mixed-precision step: in-kernel re-score arguments");
+  MixedFn fn = pick_mixed<I>(d, T, ink);
   if (fn == nullptr || blocks < 1)
     throw RuntimeError("mixed-precision step: unsupported shape");
   MixedArgs a{};
@@ -591,10 +720,17 @@ void launch_mixed_step(int64_t n, int64_t d, int64_t k, int T, int blocks,
   a.seg_cap = mixed_seg_cap(n, d, blocks);
   a.pinf = std::numeric_limits<float>::infinity();
   a.skip = skip;
-  hipLaunchKernelGGL(fn, dim3(blocks), dim3(nt_of(d_bucket(d))),
-                     mixed_lds_bytes(d, k, T, qmask != nullptr), s,
+  a.C = C;
+  a.hn = hn;
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(nt_of(d_bucket(d), ink)),
+                     mixed_lds_bytes(d, k, T, qmask != nullptr, ink), s,
                      a);
   KMD_LAUNCH_CHECK();
+  if (ink) {
+    hipLaunchKernelGGL(count_total_krn, dim3(1), dim3(kWave), 0, s, blocks,
+                       (const int *)queue_counts, rescored_total, skip);
+    KMD_LAUNCH_CHECK();
+  }
 }
 
 namespace {
@@ -657,20 +793,22 @@ void launch_queue_update(int64_t d, int64_t k, int blocks, int n_segments,
   KMD_LAUNCH_CHECK();
 }
 
-template int mixed_blocks<int32_t>(int64_t, int64_t, int64_t, int, bool);
-template int mixed_blocks<int64_t>(int64_t, int64_t, int64_t, int, bool);
+template int mixed_blocks<int32_t>(int64_t, int64_t, int64_t, int, bool, bool);
+template int mixed_blocks<int64_t>(int64_t, int64_t, int64_t, int, bool, bool);
 template void launch_mixed_step<int32_t>(int64_t, int64_t, int64_t, int, int,
                                          const float *, const float *,
                                          const void *, const float *,
                                          const float *, int32_t *, void *,
                                          int *, int *, unsigned *, hipStream_t,
-                                         const int *);
+                                         const int *, const float *, const float *,
+                                         int *);
 template void launch_mixed_step<int64_t>(int64_t, int64_t, int64_t, int, int,
                                          const float *, const float *,
                                          const void *, const float *,
                                          const float *, int64_t *, void *,
                                          int *, int *, unsigned *, hipStream_t,
-                                         const int *);
+                                         const int *, const float *, const float *,
+                                         int *);
 template void launch_queue_update<int32_t>(int64_t, int64_t, int, int, int64_t,
                                            const int *, const int *, const float *,
                                            const float *, const int32_t *,

@@ -135,6 +135,25 @@ def test_driver_ragged_and_multiwindow_shapes(kd, torch_dev):
         assert_allclose(C, C64, rtol=1e-9, atol=1e-9)
 
 
+@pytest.mark.parametrize("shape", [(100_003, 32, 256), (60_000, 64, 100), (30_001, 16, 33),
+                                   (10_000, 5, 7), (20_000, 100, 40)])
+def test_float64_fit_matches_fp64_oracle(kd, torch_dev, shape):
+    """Float64 fits (the FP64 matrix-core kernels where the centroids and
+    FP64 sums fit LDS, d <= 64; the generic kernel beyond) against the FP64
+    oracle: same labels and iteration count, centroids and inertia to a few
+    FP64 ulps of accumulated rounding."""
+    n, d, k = shape
+    print(f"float64 fit n={n} d={d} k={k}", flush=True)
+    Xt, init = lo.gaussian_blobs(n, d, k, seed=d * k, dtype=np.float64)
+    w = np.random.default_rng(k).uniform(0.5, 2.0, n)
+    lab, C, n_iter, inertia = run_driver(kd, Xt, w, init, 1e-10, 12)
+    l64, C64, it64, in64 = lo.kmeans_lloyd(Xt, w, init, 1e-10, 12, dtype=np.float64)
+    assert n_iter == it64
+    assert_array_equal(lab, l64)
+    assert np.abs(C - C64).max() <= 1e-12 * np.abs(C64).max()
+    assert abs(inertia - in64) <= 1e-12 * abs(in64)
+
+
 def separated_blobs(n, d, k, seed):
     """Blobs far apart relative to their spread, init near the true centers:
     no sample is within rounding of a tie, so FP32 and FP64 fits follow the
@@ -229,17 +248,20 @@ def test_relocation_with_next_iteration_enqueued_ahead(kd, torch_dev, monkeypatc
                                             ((300_001, 32, 256), 7, 2.0),
                                             ((100_000, 64, 100), 3, 1.5),
                                             ((65_537, 5, 33), 9, 3.0)])
-@pytest.mark.parametrize("rescore", ["candidate-tiles", "gather"])
+@pytest.mark.parametrize("rescore", ["in-kernel", "candidate-tiles", "gather"])
 def test_mixed_precision_matches_fp32_bit_exactly(kd, torch_dev, monkeypatch, shape, seed,
                                                   std, rescore):
     """BF16 (+ certified-margin FP32 re-score) and BF16x3 fits give the FP32
     path's labels; with exact fixed-point sums the centroids are then
-    bit-identical too, iteration for iteration.  The re-score either covers
-    only each queued sample's candidate tiles (read from a sample-major copy)
-    with a separate exact update of the queue, or gathers every queued
-    sample over all tiles."""
+    bit-identical too, iteration for iteration.  The undecided samples are
+    re-scored inside the BF16 kernel (the default where it fits), or by a
+    second kernel over each queued sample's candidate tiles (read from a
+    sample-major copy) with a separate exact update of the queue, or by a
+    second kernel that gathers every queued sample over all tiles."""
     if rescore == "candidate-tiles":
         monkeypatch.setenv("KMD_MIXED_CAND", "1")
+    elif rescore == "gather":
+        monkeypatch.setenv("KMD_MIXED_GATHER", "1")
     n, d, k = shape
     Xt, init = lo.gaussian_blobs(n, d, k, seed=seed, std=std)
     w = np.ones(n, np.float32)
