@@ -422,6 +422,56 @@ struct IterEvents {
   IterEvents &operator=(const IterEvents &) = delete;
 };
 
+// LDS-resident mixed path: BF16 step (operands already prepared for `cur`)
+// and the re-score of its undecided samples, in the workspace's form:
+// labels (the FP32 path's) and the int64 partial slots.
+template <typename T, typename I>
+static void mixed_labels_and_partials(LloydWorkspace<T, I> &ws, const T *X,
+                                      const T *w, const T *cur, I *labels,
+                                      hipStream_t s, const int *skip) {
+  if constexpr (std::is_same<T, float>::value) {
+    const int64_t n = ws.n, d = ws.d, k = ws.k;
+    launch_mixed_step<I>(n, d, k, ws.mixed_terms, ws.blocks, X, w,
+                         ws.aop.template as<const void>(),
+                         ws.bound.template as<const float>(), ws.scale(),
+                         labels, ws.copies.template as<void>(),
+                         ws.queue.template as<int>(),
+                         ws.queue_counts.template as<int>(),
+                         ws.mixed_cand ? ws.qmask.template as<unsigned>() : nullptr, s,
+                         skip, ws.mixed_ink ? cur : nullptr,
+                         ws.mixed_ink ? ws.hn.template as<const T>() : nullptr,
+                         ws.mixed_ink ? ws.queue_count() : nullptr);
+    if (ws.mixed_ink) {
+      // re-scored inside the kernel: nothing more to launch
+    } else if (ws.mixed_cand) {
+      // FP32 re-score of the candidate tiles (labels), then the exact sums
+      // of the re-scored samples into the re-score partial slots
+      launch_tiled_cand_rescore<I>(
+          n, d, k, ws.xr.template as<const float>(), cur, ws.hn.template as<const T>(),
+          labels, ws.queue.template as<const int>(),
+          ws.queue_counts.template as<const int>(),
+          ws.qmask.template as<const unsigned>(), ws.blocks, ws.seg_cap,
+          ws.cand.template as<void>(), ws.queue_count(), s);
+      launch_queue_update<I>(d, k, ws.blocks_b, ws.blocks, ws.seg_cap,
+                             ws.queue.template as<const int>(),
+                             ws.queue_counts.template as<const int>(),
+                             ws.xr.template as<const float>(), w, labels, ws.scale(),
+                             ws.copies.template as<int64_t>() + (size_t)ws.blocks * (d * k + k),
+                             s, skip);
+    } else {
+      launch_fast_rescore<I>(n, d, k, X, w, cur, ws.hn.template as<const T>(),
+                             labels, ws.scale(), ws.nonfinite(),
+                             ws.copies.template as<void>(), ws.blocks,
+                             ws.queue.template as<const int>(),
+                             ws.queue_counts.template as<const int>(),
+                             ws.blocks, ws.seg_cap, ws.queue_count(), s);
+    }
+  } else {
+    (void)ws, (void)X, (void)w, (void)cur, (void)labels, (void)s, (void)skip;
+    throw RuntimeError("mixed-precision step: float32 only");
+  }
+}
+
 // hn (unless the previous iteration's finalize computed it), fused step and
 // reduction for centroids `cur`, sums into `next`; the empty count is
 // written, not accumulated.
@@ -479,41 +529,7 @@ static void lloyd_front(LloydWorkspace<T, I> &ws, const T *X, const T *w,
                         ws.aop.template as<void>(), ws.bound.template as<float>(),
                         s);
       timer.begin(s);
-      launch_mixed_step<I>(n, d, k, ws.mixed_terms, ws.blocks, X, w,
-                           ws.aop.template as<const void>(),
-                           ws.bound.template as<const float>(), ws.scale(),
-                           labels, ws.copies.template as<void>(),
-                           ws.queue.template as<int>(),
-                           ws.queue_counts.template as<int>(),
-                           ws.mixed_cand ? ws.qmask.template as<unsigned>() : nullptr, s,
-                           skip, ws.mixed_ink ? cur : nullptr,
-                           ws.mixed_ink ? ws.hn.template as<const T>() : nullptr,
-                           ws.mixed_ink ? ws.queue_count() : nullptr);
-      if (ws.mixed_ink) {
-        // re-scored inside the kernel: nothing more to launch
-      } else if (ws.mixed_cand) {
-        // FP32 re-score of the candidate tiles (labels), then the exact sums
-        // of the re-scored samples into the re-score partial slots
-        launch_tiled_cand_rescore<I>(
-            n, d, k, ws.xr.template as<const float>(), cur, ws.hn.template as<const T>(),
-            labels, ws.queue.template as<const int>(),
-            ws.queue_counts.template as<const int>(),
-            ws.qmask.template as<const unsigned>(), ws.blocks, ws.seg_cap,
-            ws.cand.template as<void>(), ws.queue_count(), s);
-        launch_queue_update<I>(d, k, ws.blocks_b, ws.blocks, ws.seg_cap,
-                               ws.queue.template as<const int>(),
-                               ws.queue_counts.template as<const int>(),
-                               ws.xr.template as<const float>(), w, labels, ws.scale(),
-                               ws.copies.template as<int64_t>() + (size_t)ws.blocks * (d * k + k),
-                               s, skip);
-      } else {
-        launch_fast_rescore<I>(n, d, k, X, w, cur, ws.hn.template as<const T>(),
-                               labels, ws.scale(), ws.nonfinite(),
-                               ws.copies.template as<void>(), ws.blocks,
-                               ws.queue.template as<const int>(),
-                               ws.queue_counts.template as<const int>(),
-                               ws.blocks, ws.seg_cap, ws.queue_count(), s);
-      }
+      mixed_labels_and_partials(ws, X, w, cur, labels, s, skip);
       timer.end(s);
       launch_reduce_fixed_partials<T, I>(
           ws.blocks + ws.blocks_b, d, k, ws.copies.template as<const void>(),
@@ -807,6 +823,14 @@ static void driver_impl(int64_t n, int64_t d, int64_t k, const T *X, const T *w,
                                 labels, ws.queue.template as<const int>(),
                                 ws.queue_counts.template as<const int>(),
                                 ws.blocks_b, ws.seg_cap, ws.queue_count(), s);
+      labelled = true;
+    } else if (ws.mixed_terms > 0 && n > 0) {
+      // the BF16 step and its exact re-score: the FP32 labels (its sums go
+      // to the partial slots unread)
+      launch_mixed_prep(d, k, ws.mixed_terms, cur, ws.hn.template as<const T>(),
+                        ws.aop.template as<void>(), ws.bound.template as<float>(),
+                        s);
+      mixed_labels_and_partials(ws, X, w, cur, labels, s, nullptr);
       labelled = true;
     }
   }
