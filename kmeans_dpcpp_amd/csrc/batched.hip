@@ -326,8 +326,9 @@ __global__ void __launch_bounds__(NT)
   constexpr int TF = tile_floats<KS>();
   constexpr int D = 2 * KS, KD = D / 16, KK = mixed_kk<KS>(), XH = D / 2;
   constexpr int kNT = NT, kW = kNT / kWave;
-  constexpr int kIdxBits = 8;                      // k <= kMaxK = 256
-  constexpr unsigned kIdxMask = (1u << kIdxBits) - 1u;
+  // a lane's row r (0..15) of a tile is packed into the 4 low mantissa bits
+  constexpr int kRowBits = 4;
+  constexpr unsigned kRowMask = (1u << kRowBits) - 1u;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   float *Cl = reinterpret_cast<float *>(lds_raw);  // FP32 tiles [kMaxK / 32][TF]
   bf16x8 *Al = reinterpret_cast<bf16x8 *>(lds_raw + mixed_aop_offset<KS>());
@@ -343,7 +344,7 @@ __global__ void __launch_bounds__(NT)
   constexpr float acc_err = (float)(D + 3 + D + 8) * 5.9604645e-8f;
   constexpr float alpha = 2.f * u + u * u + acc_err;
   constexpr float beta = acc_err;
-  const float pack_err = ldexpf(1.0f, kIdxBits - 23) * 1.01f;
+  const float pack_err = ldexpf(1.0f, kRowBits - 23) * 1.01f;
   for (int base = blockIdx.x; base < n_items; base += kWave * gridDim.x) {
     __syncthreads();
     if (wave == 0) {
@@ -474,6 +475,7 @@ __global__ void __launch_bounds__(NT)
 #pragma unroll
           for (int jj = 0; jj < 8; ++jj) bhi[kd][jj] = (__bf16)xr[8 * kd + jj];
         float b1 = -__builtin_huge_valf(), b2 = -__builtin_huge_valf();
+        int bt = 0;
         for (int ct = 0; ct < n_ct; ++ct) {
           const bf16x8 *At = Al + ct * KK * 64 + lane;
           f32x16 acc = {};
@@ -482,29 +484,36 @@ __global__ void __launch_bounds__(NT)
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(At[kk * 64],
                                                           kk < KD ? bhi[kk < KD ? kk : 0] : bh,
                                                           acc, 0, 0, 0);
-          // rows m = 8 (r / 4) + 4 half + r % 4 of the tile; top-2 of
-          // {b1, b2, u, v}: b1' = max3(b1, u, v), b2' = max(b2, med3(b1, u, v))
+          // rows m = 8 (r / 4) + 4 half + r % 4 of the tile, r packed into
+          // the 4 low bits; top-2 of {b1, b2, u, v}: b1' = max3(b1, u, v),
+          // b2' = max(b2, med3(b1, u, v)); the best's tile is the last tile
+          // in which b1 rose
+          const float b1_in = b1;
 #pragma unroll
           for (int r = 0; r < 16; r += 2) {
-            const unsigned i0 = (unsigned)(ct * 32 + 8 * (r / 4) + 4 * half + (r % 4));
-            const float uu = __uint_as_float((__float_as_uint(acc[r]) & ~kIdxMask) | i0);
-            const float vv =
-                __uint_as_float((__float_as_uint(acc[r + 1]) & ~kIdxMask) | (i0 + 1u));
+            const float uu =
+                __uint_as_float((__float_as_uint(acc[r]) & ~kRowMask) | (unsigned)r);
+            const float vv = __uint_as_float((__float_as_uint(acc[r + 1]) & ~kRowMask) |
+                                             (unsigned)(r + 1));
             const float mm = __builtin_amdgcn_fmed3f(b1, uu, vv);
             b1 = max3f(b1, uu, vv);
             b2 = __builtin_amdgcn_fmed3f(b2, mm, pinf);
           }
+          bt = (b1 != b1_in) ? ct : bt;
         }
+        const int rb = (int)(__float_as_uint(b1) & kRowMask);
+        int lab = bt * 32 + 8 * (rb >> 2) + 4 * half + (rb & 3);
         {
           const float o1 = __shfl_xor(b1, 32, kWave);
           const float o2 = __shfl_xor(b2, 32, kWave);
+          const int ol = __shfl_xor(lab, 32, kWave);
           const float n2 = fmaxf(fminf(b1, o1), fmaxf(b2, o2));
+          lab = (o1 > b1) ? ol : lab;
           b1 = fmaxf(b1, o1);
           b2 = n2;
         }
         const float e = xnorm * cmax * alpha + hnmax * beta +
                         pack_err * (fabsf(b1) + fabsf(b2));
-        const int lab = (int)(__float_as_uint(b1) & kIdxMask);
         const bool amb = !(b1 - b2 > 2.f * e) || !(xnorm * cmax < 1e36f) || !finite_c ||
                          lab >= k;
         const bool valid = half == 0 && p < end;

@@ -10,9 +10,10 @@
 // terms, so hn is folded into the matrix product (the samples' side of those
 // K columns is -1).  The argmin of hn - dot is the argmax of D.
 //
-// Decision.  Each lane keeps the best and second-best D of its sample (the
-// centroid index packed into the low mantissa bits of the value: one
-// v_and_or and two v_med3 per candidate).  Let E1 bound the error
+// Decision.  Each lane keeps the best and second-best D of its sample (its
+// row within the 32-centroid tile packed into the 4 low mantissa bits of
+// the value, the tile of the best tracked beside it: one v_and_or per
+// candidate and a v_max3 and two v_med3 per pair of candidates).  Let E1 bound the error
 // of D against exact arithmetic (bf16 rounding of the operands, FP32
 // accumulation, index packing) and E2 the error of the FP32 path's
 // pseudo-inertia (FP32 MFMA accumulation and the final subtraction), both
@@ -201,7 +202,9 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
   const int wave = tid / kWave;
   const int col = lane & 31;
   const int half = lane >> 5;
-  const unsigned idx_mask = (1u << a.idx_bits) - 1u;
+  // the lane's row r (0..15) of a tile is packed into the low 4 mantissa
+  // bits of each value; the tile of the running best is tracked beside it
+  constexpr unsigned kRowMask = 15u;
   // error bound E1 + E2 = ||x|| cmax alpha + hnmax beta + pack (|b1| + |b2|)
   const float cmax = a.bound[0], hnmax = a.bound[1];
   // the split of hn into bf16 terms and the bound need a finite, moderate hn
@@ -211,7 +214,7 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
   constexpr float acc_err = (float)(T * D + 3 + D + 8) * 5.9604645e-8f;
   const float alpha = ((T == 1) ? (2.f * u + u * u) : (3.2f * u * u)) + acc_err;
   const float beta = acc_err;
-  const float pack_err = ldexpf(1.0f, a.idx_bits - 23) * 1.01f;
+  const float pack_err = ldexpf(1.0f, 4 - 23) * 1.01f;
   const float *X = a.X;
   const float *w = a.w;
   I *labels = reinterpret_cast<I *>(a.labels);
@@ -327,10 +330,12 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
       bh[j] = to_bf16((half == 0 && j < 3) ? -1.f : 0.f);
 
     float b1[kS], b2[kS];
+    int bt[kS];
 #pragma unroll
     for (int q = 0; q < kS; ++q) {
       b1[q] = -__builtin_huge_valf();
       b2[q] = -__builtin_huge_valf();
+      bt[q] = 0;
     }
     // products of tile ct into acc (K steps: x high parts, then for T = 3
     // the two cross terms, then the half-norm columns)
@@ -360,31 +365,38 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
         }
       }
     };
-    // top-2 update with the 16 rows of tile ct this lane holds:
-    // m = 8 (r / 4) + 4 half + (r % 4)
+    // top-2 update with the 16 rows of tile ct this lane holds (row r is
+    // centroid 32 ct + 8 (r / 4) + 4 half + (r % 4)), r packed into the low
+    // bits (one v_and_or with an inline constant per value).
     // Two candidates u, v at a time: top-2 of {b1, b2, u, v} (b1 >= b2) is
     // b1' = max3(b1, u, v), b2' = max(b2, med3(b1, u, v)).  med3(x, y, +inf)
     // is max(x, y) without the NaN canonicalisation of a max; the values are
-    // finite here or the sample is re-scored.
+    // finite here or the sample is re-scored.  The best's tile is the last
+    // tile in which b1 rose (a tie with an earlier tile's value leaves a gap
+    // of 0: such a sample is re-scored).
     auto tile_track = [&](int ct, const f32x16(&acc)[kS]) {
-      float tmax[kS];
+      float tmax[kS], b1_in[kS];
 #pragma unroll
-      for (int q = 0; q < kS; ++q) tmax[q] = -__builtin_huge_valf();
+      for (int q = 0; q < kS; ++q) {
+        tmax[q] = -__builtin_huge_valf();
+        b1_in[q] = b1[q];
+      }
 #pragma unroll
       for (int r = 0; r < 16; r += 2) {
-        const unsigned i0 = (unsigned)(ct * 32 + 8 * (r / 4) + 4 * half + (r % 4));
 #pragma unroll
         for (int q = 0; q < kS; ++q) {
-          const float u =
-              __uint_as_float((__float_as_uint(acc[q][r]) & ~idx_mask) | i0);
+          const float u = __uint_as_float(
+              (__float_as_uint(acc[q][r]) & ~kRowMask) | (unsigned)r);
           const float v = __uint_as_float(
-              (__float_as_uint(acc[q][r + 1]) & ~idx_mask) | (i0 + 1u));
+              (__float_as_uint(acc[q][r + 1]) & ~kRowMask) | (unsigned)(r + 1));
           const float m = __builtin_amdgcn_fmed3f(b1[q], u, v);
           b1[q] = max3f(b1[q], u, v);
           b2[q] = __builtin_amdgcn_fmed3f(b2[q], m, pinf);
           if (cand) tmax[q] = max3f(tmax[q], u, v);
         }
       }
+#pragma unroll
+      for (int q = 0; q < kS; ++q) bt[q] = (b1[q] != b1_in[q]) ? ct : bt[q];
       // the tile's best, rounded toward +inf to bf16 (never below it)
       if (cand) {
         static_assert(kS == 1, "one tile maximum per lane and tile");
@@ -415,23 +427,25 @@ __global__ void __launch_bounds__(NT) mixed_lloyd_krn(MixedArgs a) {
         tile_track(ct, acc);
       }
     }
+    int lab[kS];
 #pragma unroll
     for (int q = 0; q < kS; ++q) {
+      lab[q] = best_index(bt[q], (int)(__float_as_uint(b1[q]) & kRowMask), half);
       const float o1 = __shfl_xor(b1[q], 32, kWave);
       const float o2 = __shfl_xor(b2[q], 32, kWave);
+      const int ol = __shfl_xor(lab[q], 32, kWave);
       const float n2 = fmaxf(fminf(b1[q], o1), fmaxf(b2[q], o2));
+      lab[q] = (o1 > b1[q]) ? ol : lab[q];
       b1[q] = fmaxf(b1[q], o1);
       b2[q] = n2;
     }
     bool amb[kS];
-    int lab[kS];
 #pragma unroll
     for (int q = 0; q < kS; ++q) {
       const float e = xnorm[q] * cmax * alpha + hnmax * beta +
                       pack_err * (fabsf(b1[q]) + fabsf(b2[q]));
       const float gap = b1[q] - b2[q];
       // far from overflow |D| <= ||x|| ||c|| (1 + 2^-8)^2 + hn stays finite
-      lab[q] = (int)(__float_as_uint(b1[q]) & idx_mask);
       amb[q] = !(gap > 2.f * e) || !(xnorm[q] * cmax < 1e36f) || !finite_c ||
                lab[q] >= k;
     }

@@ -589,7 +589,9 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
   int *qseg = a.queue + (int64_t)blockIdx.x * a.seg_cap;
   if (tid == 0) q_used = 0;
 
-  const unsigned idx_mask = (1u << a.idx_bits) - 1u;
+  // a lane's row r (0..15) of a tile is packed into the 4 low mantissa
+  // bits; the tile of the running best is tracked beside it
+  constexpr unsigned kRowMask = 15u;
   const float cmax = a.bound[0], hnmax = a.bound[1];
   const bool finite_c = hnmax < 1e36f;
   const float pinf = a.pinf;
@@ -597,7 +599,7 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
   const float acc_err = (float)(T * D + 3 + a.fp32_len + 8) * 5.9604645e-8f;
   const float alpha = ((T == 1) ? (2.f * u + u * u) : (3.2f * u * u)) + acc_err;
   const float beta = acc_err;
-  const float pack_err = ldexpf(1.0f, a.idx_bits - 23) * 1.01f;
+  const float pack_err = ldexpf(1.0f, 4 - 23) * 1.01f;
   const bool vec = S == 2 && ((reinterpret_cast<uintptr_t>(a.X) & 7u) == 0) &&
                    (n % 2 == 0);
 
@@ -702,6 +704,7 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
     for (int j = 0; j < 8; ++j) bh[j] = (__bf16)((half == 0 && j < 3) ? -1.f : 0.f);
 
     float b1[S], b2[S];
+    int bt[S];
     // candidate tiles of each sample (bit t: tile t), from the per-tile
     // maxima once the sample's best is known
     unsigned cmask[S];
@@ -709,6 +712,7 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
     for (int q = 0; q < S; ++q) {
       b1[q] = -__builtin_huge_valf();
       b2[q] = -__builtin_huge_valf();
+      bt[q] = 0;
       cmask[q] = 0u;
     }
     for (int ci = 0; ci < n_chunks; ++ci, ++g) {
@@ -767,25 +771,28 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
             }
           }
           const int ct = c0 + t;
-          float tmax[S];
+          float tmax[S], b1_in[S];
 #pragma unroll
-          for (int q = 0; q < S; ++q) tmax[q] = -__builtin_huge_valf();
+          for (int q = 0; q < S; ++q) {
+            tmax[q] = -__builtin_huge_valf();
+            b1_in[q] = b1[q];
+          }
 #pragma unroll
           for (int rr = 0; rr < 16; rr += 2) {
-            const unsigned i0 =
-                (unsigned)(ct * 32 + 8 * (rr / 4) + 4 * half + (rr % 4));
 #pragma unroll
             for (int q = 0; q < S; ++q) {
-              const float uu =
-                  __uint_as_float((__float_as_uint(acc[q][rr]) & ~idx_mask) | i0);
+              const float uu = __uint_as_float(
+                  (__float_as_uint(acc[q][rr]) & ~kRowMask) | (unsigned)rr);
               const float vv = __uint_as_float(
-                  (__float_as_uint(acc[q][rr + 1]) & ~idx_mask) | (i0 + 1u));
+                  (__float_as_uint(acc[q][rr + 1]) & ~kRowMask) | (unsigned)(rr + 1));
               const float m = __builtin_amdgcn_fmed3f(b1[q], uu, vv);
               b1[q] = max3f(b1[q], uu, vv);
               b2[q] = __builtin_amdgcn_fmed3f(b2[q], m, pinf);
               tmax[q] = max3f(tmax[q], uu, vv);
             }
           }
+#pragma unroll
+          for (int q = 0; q < S; ++q) bt[q] = (b1[q] != b1_in[q]) ? ct : bt[q];
           // the tile's best, rounded toward +inf to bf16 (never below it)
 #pragma unroll
           for (int q = 0; q < S; ++q)
@@ -799,14 +806,17 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
     }
 #pragma unroll
     for (int q = 0; q < S; ++q) {
+      const unsigned rb = __float_as_uint(b1[q]) & kRowMask;
+      unsigned lab = (unsigned)(bt[q] * 32) + 8u * (rb >> 2) + 4u * half + (rb & 3u);
       const float o1 = __shfl_xor(b1[q], 32, kWave);
       const float o2 = __shfl_xor(b2[q], 32, kWave);
+      const unsigned ol = (unsigned)__shfl_xor((int)lab, 32, kWave);
       const float n2 = fmaxf(fminf(b1[q], o1), fmaxf(b2[q], o2));
+      lab = (o1 > b1[q]) ? ol : lab;
       b1[q] = fmaxf(b1[q], o1);
       b2[q] = n2;
       const float e = xnorm[q] * cmax * alpha + hnmax * beta +
                       pack_err * (fabsf(b1[q]) + fabsf(b2[q]));
-      const unsigned lab = __float_as_uint(b1[q]) & idx_mask;
       const bool amb = !(b1[q] - b2[q] > 2.f * e) || !(xnorm[q] * cmax < 1e36f) ||
                        !finite_c || lab >= (unsigned)a.k;
       const int64_t s = s0 + q;
@@ -920,7 +930,9 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
   int *qseg = a.queue + (int64_t)blockIdx.x * a.seg_cap;
   if (tid == 0) q_used = 0;
 
-  const unsigned idx_mask = (1u << a.idx_bits) - 1u;
+  // a lane's row r (0..15) of a tile is packed into the 4 low mantissa
+  // bits; the tile of the running best is tracked beside it
+  constexpr unsigned kRowMask = 15u;
   // bound: [0] max ||c||, [1] max hn, [2] eta / min s_t, [3] s_x, [4] eta / s_x,
   // [5] max s_t
   const float cmax = a.bound[0], hnmax = a.bound[1];
@@ -932,7 +944,7 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
   const float alpha = 2.f * kF8U + kF8U * kF8U + acc_err * (1.f + kF8U) * (1.f + kF8U);
   const float beta = acc_err;
   const float rootd = sqrtf((float)D);
-  const float pack_err = ldexpf(1.0f, a.idx_bits - 23) * 1.01f;
+  const float pack_err = ldexpf(1.0f, 4 - 23) * 1.01f;
 
   constexpr int kSuper = 32 * S;
   const int64_t n_super = ceil_div<int64_t>(n, kSuper);
@@ -983,11 +995,13 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
     }
     if (r + 1 < rounds) load_x8(r + 1);
     float b1[S], b2[S];
+    int bt[S];
     unsigned cmask[S];
 #pragma unroll
     for (int q = 0; q < S; ++q) {
       b1[q] = -__builtin_huge_valf();
       b2[q] = -__builtin_huge_valf();
+      bt[q] = 0;
       cmask[q] = 0u;
     }
     for (int ci = 0; ci < n_chunks; ++ci, ++g) {
@@ -1026,25 +1040,28 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
                   av[kk / 2][kk % 2], bx[kk / 2][q][kk % 2], acc[q], 0, 0, 0);
           }
           const int ct = c0 + t;
-          float tmax[S];
+          float tmax[S], b1_in[S];
 #pragma unroll
-          for (int q = 0; q < S; ++q) tmax[q] = -__builtin_huge_valf();
+          for (int q = 0; q < S; ++q) {
+            tmax[q] = -__builtin_huge_valf();
+            b1_in[q] = b1[q];
+          }
 #pragma unroll
           for (int rr = 0; rr < 16; rr += 2) {
-            const unsigned i0 =
-                (unsigned)(ct * 32 + 8 * (rr / 4) + 4 * half + (rr % 4));
 #pragma unroll
             for (int q = 0; q < S; ++q) {
-              const float uu =
-                  __uint_as_float((__float_as_uint(acc[q][rr] * inv) & ~idx_mask) | i0);
+              const float uu = __uint_as_float(
+                  (__float_as_uint(acc[q][rr] * inv) & ~kRowMask) | (unsigned)rr);
               const float vv = __uint_as_float(
-                  (__float_as_uint(acc[q][rr + 1] * inv) & ~idx_mask) | (i0 + 1u));
+                  (__float_as_uint(acc[q][rr + 1] * inv) & ~kRowMask) | (unsigned)(rr + 1));
               const float m = __builtin_amdgcn_fmed3f(b1[q], uu, vv);
               b1[q] = max3f(b1[q], uu, vv);
               b2[q] = __builtin_amdgcn_fmed3f(b2[q], m, pinf);
               tmax[q] = max3f(tmax[q], uu, vv);
             }
           }
+#pragma unroll
+          for (int q = 0; q < S; ++q) bt[q] = (b1[q] != b1_in[q]) ? ct : bt[q];
 #pragma unroll
           for (int q = 0; q < S; ++q)
             if (a.qmask != nullptr && ct < 32) {
@@ -1057,16 +1074,19 @@ __global__ void __launch_bounds__(kTNT) __attribute__((amdgpu_waves_per_eu(2, 2)
     }
 #pragma unroll
     for (int q = 0; q < S; ++q) {
+      const unsigned rb = __float_as_uint(b1[q]) & kRowMask;
+      unsigned lab = (unsigned)(bt[q] * 32) + 8u * (rb >> 2) + 4u * half + (rb & 3u);
       const float o1 = __shfl_xor(b1[q], 32, kWave);
       const float o2 = __shfl_xor(b2[q], 32, kWave);
+      const unsigned ol = (unsigned)__shfl_xor((int)lab, 32, kWave);
       const float n2 = fmaxf(fminf(b1[q], o1), fmaxf(b2[q], o2));
+      lab = (o1 > b1[q]) ? ol : lab;
       b1[q] = fmaxf(b1[q], o1);
       b2[q] = n2;
       const float e_part = xnorm[q] * cmax * alpha + hnmax * beta +
                            rootd * (eta_x * cmax + eta_c * xnorm[q]) * (1.f + kF8U) +
                            (float)D * eta_x * eta_c;
       const float e = e_part + pack_err * (fabsf(b1[q]) + fabsf(b2[q]));
-      const unsigned lab = __float_as_uint(b1[q]) & idx_mask;
       const bool amb = !(b1[q] - b2[q] > 2.f * e) || !(xnorm[q] * cmax < 1e30f) ||
                        !finite_c || lab >= (unsigned)a.k;
       const int64_t s = s0 + q;
