@@ -34,6 +34,17 @@ int device_cu_count() {
   return cache[dev];
 }
 
+void allow_full_lds(const void *kernel) {
+  static std::mutex mu;
+  static std::vector<const void *> done;
+  std::lock_guard<std::mutex> lock(mu);
+  for (const void *f : done)
+    if (f == kernel) return;
+  KMD_HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    160 * 1024));
+  done.push_back(kernel);
+}
+
 int64_t device_l2_bytes(int device) {
   int bytes = 0;
   KMD_HIP_CHECK(

@@ -52,6 +52,11 @@ __host__ __device__ inline T ceil_div(T n, T m) {
 
 // Number of compute units of the current device (cached per device).
 int device_cu_count();
+
+// Lets `kernel` launch with up to 160 KB of dynamic LDS (the MI355X CU's
+// LDS); the attribute is set once per kernel (setting it costs about a
+// millisecond per call).
+void allow_full_lds(const void *kernel);
 // L2 cache size in bytes of `device` (the reference uses SYCL's
 // global_mem_cache_size, src/lloyd_single_step.hpp:23).
 int64_t device_l2_bytes(int device);

@@ -210,9 +210,7 @@ int grid64(int64_t n, int64_t d, int64_t k) {
   const int ks4 = ks4_bucket(d);
   Kern64 fn = pick64<I, UPDATE>(ks4);
   const size_t lds = lds64_bytes(ks4, d, k, UPDATE);
-  KMD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds));
+  allow_full_lds(reinterpret_cast<const void *>(fn));
   int per_cu = 0;
   KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
       &per_cu, reinterpret_cast<const void *>(fn), kNT64, lds));

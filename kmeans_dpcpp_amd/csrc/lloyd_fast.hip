@@ -874,9 +874,7 @@ MfmaLaunch plan_rescore(int64_t d, int64_t k) {
   if (L.lds > kLdsPerCu) return L;
   L.fn = pick_rescore<I>(d);
   if (L.fn == nullptr) return L;
-  KMD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(L.fn),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)kLdsPerCu));
+  allow_full_lds(reinterpret_cast<const void *>(L.fn));
   int per_cu = 0;
   KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
       &per_cu, reinterpret_cast<const void *>(L.fn), L.threads, L.lds));

@@ -797,10 +797,7 @@ void launch_queue_update(int64_t d, int64_t k, int blocks, int n_segments,
                          const int *skip) {
   const size_t lds = sizeof(u64) * (size_t)(d + 1) * k + sizeof(float) * (d + 1);
   require(lds <= kLdsPerCu, "queue update: sums beyond LDS");
-  if (lds > 64 * 1024)
-    KMD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&queue_update_krn<I>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
+  if (lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void *>(&queue_update_krn<I>));
   hipLaunchKernelGGL(queue_update_krn<I>, dim3((unsigned)std::max(1, blocks)), dim3(256),
                      lds, s, (int)d, (int)k, n_segments, seg_cap, queue, queue_counts,
                      X_r, w, labels, scale, reinterpret_cast<u64 *>(partials), skip);

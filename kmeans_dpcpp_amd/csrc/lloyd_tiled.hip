@@ -2050,10 +2050,7 @@ void launch_label_update(int64_t n, int64_t d, int64_t k, int64_t n_copies,
   const int64_t b = std::max<int64_t>(
       1, std::min<int64_t>(ceil_div<int64_t>(target, groups), ceil_div<int64_t>(n, 512)));
   const int64_t chunk = ceil_div<int64_t>(n, b);
-  if (lds > 64 * 1024)
-    KMD_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&label_update_krn<I>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
+  if (lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void *>(&label_update_krn<I>));
   hipLaunchKernelGGL(label_update_krn<I>, dim3((unsigned)ceil_div<int64_t>(n, chunk),
                                               (unsigned)groups),
                      dim3(512), lds, s, n, (int)d, (int)k, F, chunk, X_t, w, labels,

@@ -442,6 +442,56 @@ def test_fused_step_matches_oracle(kd, torch_dev, dt, shape, n_copies):
     assert np.abs(sums - exp_sums).max() <= 1e-5 * scale
 
 
+def test_fused_step_values_outside_the_sum_window(kd, torch_dev):
+    """The float32 fused step sums exactly in binade windows set from the
+    centroids and adds values outside them (outliers far above every
+    centroid coordinate, tiny values, infinities and NaNs) with FP32 atomics
+    as the reference does: every cluster's sums stay within FP32 rounding of
+    an FP64 sum over the GPU labels, and a NaN reaches its cluster's sum."""
+    n, d, k = 60_000, 8, 16
+    Xt, init = lo.gaussian_blobs(n, d, k, seed=11, dtype=np.float32)
+    rng = np.random.default_rng(5)
+    Xt[0, rng.choice(n, 40, replace=False)] = 3e6        # far above the window
+    Xt[1, rng.choice(n, 300, replace=False)] = 1e-12      # below it
+    Xt[2, rng.choice(n, 300, replace=False)] *= 1e-7      # partly below it
+    Xt[5] = rng.uniform(-1e-12, 1e-12, n)                 # a whole feature below it
+    w = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    w[rng.choice(n, 20, replace=False)] = 3e7             # weights beyond the window
+    hn = co.half_norm(init)
+    n_copies = 4
+    ids = to_dev(np.empty(n, dtype=np.int32))
+    cp = to_dev(np.zeros((n_copies, d, k), np.float32))
+    scp = to_dev(np.zeros((n_copies, k), np.float32))
+    kd.fused_lloyd_single_step(to_dev(Xt), to_dev(w), to_dev(init), to_dev(hn), ids, cp, scp,
+                               8, 128, q())[0].wait()
+    lab = to_np(ids).astype(np.int64)
+    ref_ids = to_dev(np.empty(n, dtype=np.int32))
+    kd.assignment(to_dev(Xt), to_dev(init), to_dev(hn), ref_ids, 8, 128, q())
+    sync()
+    assert_array_equal(lab, to_np(ref_ids))
+    sums = to_np(cp).astype(np.float64).sum(axis=0)
+    sizes = to_np(scp).astype(np.float64).sum(axis=0)
+    wx = Xt.astype(np.float64) * w.astype(np.float64)
+    exp = np.stack([np.bincount(lab, wx[f], minlength=k) for f in range(d)])
+    mag = np.stack([np.bincount(lab, np.abs(wx[f]), minlength=k) for f in range(d)])
+    assert np.all(np.abs(sums - exp) <= 1e-5 * mag + 1e-30)
+    exp_w = np.bincount(lab, w.astype(np.float64), minlength=k)
+    assert np.all(np.abs(sizes - exp_w) <= 1e-5 * exp_w)
+    # non-finite values reach their cluster's sums
+    Xt2 = Xt.copy()
+    Xt2[3, 7] = np.nan
+    Xt2[4, 9] = np.inf
+    cp2 = to_dev(np.zeros((n_copies, d, k), np.float32))
+    scp2 = to_dev(np.zeros((n_copies, k), np.float32))
+    ids2 = to_dev(np.empty(n, dtype=np.int32))
+    kd.fused_lloyd_single_step(to_dev(Xt2), to_dev(w), to_dev(init), to_dev(hn), ids2, cp2,
+                               scp2, 8, 128, q())[0].wait()
+    lab2 = to_np(ids2)
+    s2 = to_np(cp2).astype(np.float64).sum(axis=0)
+    assert np.isnan(s2[3, lab2[7]])
+    assert np.isinf(s2[4, lab2[9]]) or np.isnan(s2[4, lab2[9]])
+
+
 def test_compute_number_of_private_copies(kd, torch_dev):
     arr = torch.empty(1, device=torch_dev)
     l2 = torch.cuda.get_device_properties(0).L2_cache_size
