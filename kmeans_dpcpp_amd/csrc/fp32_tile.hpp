@@ -93,7 +93,10 @@ __device__ inline float vmax3(float a, float b, float c) {
 }
 
 // Largest of a lane's 16 accumulators (NaNs passed over): a max3 tree of
-// depth 3 rather than a chain of 8.
+// depth 3 rather than a chain of 8.  The tree's last operand is -inf, so a
+// lane whose 16 rows are all NaN reports -inf (no candidate) rather than
+// NaN: the scan of the reference skips NaN scores, and a NaN here would
+// otherwise win the merge with the other half of the column.
 __device__ inline float tile_max(const f32x16 &acc) {
   const float t0 = vmax3(acc[0], acc[1], acc[2]);
   const float t1 = vmax3(acc[3], acc[4], acc[5]);
@@ -102,7 +105,7 @@ __device__ inline float tile_max(const f32x16 &acc) {
   const float t4 = vmax3(acc[12], acc[13], acc[14]);
   const float u0 = vmax3(t0, t1, t2);
   const float u1 = vmax3(t3, t4, acc[15]);
-  return vmax3(u0, u1, u1);
+  return vmax3(u0, u1, -__builtin_huge_valf());
 }
 
 // The same first minimum in (tile, row) order as tile_argmin, with less

@@ -50,6 +50,7 @@ namespace kmd {
 namespace {
 
 typedef unsigned long long u64;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // 32-sample MFMA column blocks per wave; each A operand read from LDS feeds
 // S MFMAs.  One block per wave keeps a wave at ~110 VGPRs, so 16 waves (4
@@ -482,6 +483,221 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
   }
 }
 
+// ---- few centroids (k <= 16) -----------------------------------------------
+//
+// With k <= 16 the 32-row tile above computes 2-8 x the products it needs,
+// and its per-block sums take most of a wave's 64 atomic adds into a handful
+// of addresses.  This kernel gives each lane one sample and all of its
+// centroids instead:
+//
+// Cross term.  v_mfma_f32_4x4x1f32 runs 16 independent 4 x 4 products with
+// K = 1: block b = lane / 4 multiplies the 4-vector C[f][4 g .. 4 g + 3]
+// (the same in every block) by X[f][s0 + 4 b .. 4 b + 3], so lane l ends
+// with the scores of sample s0 + l against centroids 4 g .. 4 g + 3 in its
+// 4 accumulators, for each of G = ceil(k / 4) groups.  A chain of FP32 MFMAs
+// (any shape) rounds exactly like fmaf(c, x, acc) applied feature after
+// feature in ascending order, so the chain started from -hn gives the same
+// bits as the 32x32x2 tile chain of every other FP32 path: same scores,
+// same labels.  The argmin over the lane's 4 G scores needs no exchange
+// between lanes.
+//
+// Update.  The int64 fixed-point sums live in LDS as [label][feature][copy]
+// with the copy index (lane % R) fastest: in one ds_add_u64 the 64 lanes
+// add into words whose offsets differ by their copy index, whatever their
+// labels, so with R = 32 no two lanes of an instruction share an address
+// except lanes l and l + 32, and the atomics run at the LDS's full width
+// instead of serialising on the few labels.
+constexpr int kSmallK = 16;
+
+// Threads per block and copies of the sums (R) per feature-row bucket DB:
+// R k (d + 1) int64 sums must fit the LDS beside the centroids, and the
+// offsets f R of a lane's adds are then compile-time immediates.
+template <int DB>
+constexpr int smallk_threads() { return DB <= 32 ? 1024 : 512; }
+template <int DB>
+constexpr int smallk_copies() { return DB <= 32 ? 32 : 16; }
+// Word 3 of a raw buffer descriptor on gfx9-family devices (untyped dword
+// access, no swizzle).
+constexpr int kRsrcWord3 = 0x00020000;
+
+template <int DB, int G, typename I, bool UPDATE>
+__global__ void __launch_bounds__(smallk_threads<DB>())
+    smallk_lloyd_krn(FastArgs a) {
+  if (a.skip != nullptr && *a.skip != 0) return;
+  constexpr int NT = smallk_threads<DB>();
+  constexpr int R = smallk_copies<DB>();
+  constexpr int kWavesPerBlock = NT / kWave;
+  constexpr int KP = 4 * G;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int64_t n = a.n;
+  const int d = a.d, k = a.k;
+  // Ca[(4 g + j) DB + f] = C[f][4 g + j] (0 outside the matrix): one 16-byte
+  // read gives a lane its A operands of four consecutive features
+  float *Ca = reinterpret_cast<float *>(lds_raw);
+  float *Hl = Ca + KP * DB;                          // [KP] -hn, -inf padded
+  float *Sc = Hl + KP;                               // [DB + 4] scales
+  u64 *S64 = reinterpret_cast<u64 *>(Sc + DB + 4);   // [k][DB + 1][R]
+  float *S32 = reinterpret_cast<float *>(S64);       // non-finite path
+  const int tid = threadIdx.x;
+  const int n_sums = UPDATE ? k * (DB + 1) * R : 0;
+  const bool fixed = UPDATE && a.nonfinite[0] == 0;
+
+  for (int i = tid; i < KP * DB; i += NT) {
+    const int c = i / DB, f = i % DB;
+    Ca[i] = (f < d && c < k) ? a.C[(int64_t)f * k + c] : 0.f;
+  }
+  for (int c = tid; c < KP; c += NT)
+    Hl[c] = (c < k) ? -a.hn[c] : -__builtin_huge_valf();
+  if (UPDATE) {
+    // Sc[f] = the scale of feature f < d, 0 for padded rows; Sc[DB] = the
+    // weights' scale
+    for (int f = tid; f < DB + 4; f += NT)
+      Sc[f] = !fixed ? 0.f : f < d ? a.scale[f] : f == DB ? a.scale[d] : 0.f;
+    for (int i = tid; i < n_sums; i += NT) S64[i] = 0ull;
+  }
+  __syncthreads();
+
+  const int lane = tid & (kWave - 1);
+  // uniform in the wave (the compiler cannot see that of tid / 64), so the
+  // per-row buffer descriptors below stay in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+  const int j4 = lane & 3;
+  const int64_t n_super = ceil_div<int64_t>(n, kWave);
+  const int64_t sb_stride = (int64_t)gridDim.x * kWavesPerBlock;
+  I *labels = reinterpret_cast<I *>(a.labels);
+
+  // Samples of super-block b: row f is read through a buffer descriptor
+  // that covers the block's valid samples of that row and nothing else, so
+  // lanes past the last sample and rows f >= d read 0 (out-of-range buffer
+  // loads return zero) with no branch: the padded rows then add 0 x 0 to
+  // the chains (their C entries are 0 too) and 0 to sum slots that are
+  // never flushed.
+  auto load = [&](int64_t b, float (&x)[DB], float &wv) {
+    // d and n re-enter through an empty asm each time, so that the compiler
+    // walks the rows with one running address instead of hoisting DB row
+    // bases and row tests out of the loop (scalar registers run out)
+    int dd = d;
+    int64_t nn = n;
+    asm volatile("" : "+s"(dd), "+s"(nn));
+    const int64_t s0 = b * kWave;
+    const int bytes = (int)(min<int64_t>(kWave, nn - s0) * 4);
+    const float *p = a.X + s0;
+#pragma unroll
+    for (int f = 0; f < DB; ++f) {
+      const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+          const_cast<float *>(p), (short)0, f < dd ? bytes : 0, kRsrcWord3);
+      x[f] = __uint_as_float(
+          __builtin_amdgcn_raw_buffer_load_b32(r, lane * 4, 0, 0));
+      if (f + 1 < dd) p += nn;
+    }
+    if (UPDATE) {
+      const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+          const_cast<float *>(a.w + s0), (short)0, bytes, kRsrcWord3);
+      wv = __uint_as_float(
+          __builtin_amdgcn_raw_buffer_load_b32(r, lane * 4, 0, 0));
+    }
+  };
+
+  auto body = [&](int64_t sb, const float (&xr)[DB], float ws) {
+    const int64_t s = sb * kWave + lane;
+    const bool valid = s < n;
+    f32x4 acc[G];   // the chains start from -hn (read from LDS, not held)
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      acc[g] = *reinterpret_cast<const f32x4 *>(Hl + 4 * g);
+#pragma unroll
+    for (int f4 = 0; f4 < DB; f4 += 4) {
+      f32x4 av[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        av[g] = *reinterpret_cast<const f32x4 *>(Ca + (4 * g + j4) * DB + f4);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+          acc[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(av[g][e], xr[f4 + e],
+                                                      acc[g], 0, 0, 0);
+      }
+    }
+    // largest acc (smallest score hn - c.x) over the KP rows, NaNs passed
+    // over, -inf when there is none; then its lowest centroid
+    float m = -__builtin_huge_valf();
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      m = vmax3(m, acc[g][0], acc[g][1]);
+      m = vmax3(m, acc[g][2], acc[g][3]);
+    }
+    int lab = 0;
+#pragma unroll
+    for (int c = KP - 1; c >= 0; --c) lab = (acc[c / 4][c % 4] == m) ? c : lab;
+    lab = (-m < __builtin_huge_valf()) ? lab : 0;
+    if (valid) labels[s] = (I)lab;
+
+    if (UPDATE && valid) {
+      const int base = lab * (DB + 1) * R + (lane & (R - 1));
+      if (fixed) {
+        u64 *Sp = S64 + base;
+#pragma unroll
+        for (int f4 = 0; f4 < DB; f4 += 4) {
+          const f32x4 sc = *reinterpret_cast<const f32x4 *>(Sc + f4);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int qv = __float2int_rn((xr[f4 + e] * ws) * sc[e]);
+            atomicAdd(Sp + (f4 + e) * R, (u64)(long long)qv);
+          }
+        }
+        atomicAdd(Sp + DB * R, (u64)(long long)__float2int_rn(ws * Sc[DB]));
+      } else {
+        float *Sp = S32 + base;
+#pragma unroll
+        for (int f = 0; f < DB; ++f) atomic_add(Sp + f * R, xr[f] * ws);
+        atomic_add(Sp + DB * R, ws);
+      }
+    }
+  };
+
+  // two register buffers, alternating: the next super-block's samples are
+  // in flight while this one is scored
+  float xa[DB], xb[DB], wa = 0.f, wb = 0.f;
+  int64_t sb = (int64_t)blockIdx.x * kWavesPerBlock + wave;
+  if (sb < n_super) load(sb, xa, wa);
+  for (; sb < n_super; sb += 2 * sb_stride) {
+    if (sb + sb_stride < n_super) load(sb + sb_stride, xb, wb);
+    body(sb, xa, wa);
+    if (sb + sb_stride >= n_super) break;
+    if (sb + 2 * sb_stride < n_super) load(sb + 2 * sb_stride, xa, wa);
+    body(sb + sb_stride, xb, wb);
+  }
+
+  if (UPDATE) {
+    // raw partials into this block's slot, in the [f][label] layout of the
+    // other fused-step kernels (f == d: the weight sums)
+    __syncthreads();
+    const int dk1 = (d + 1) * k;
+    if (fixed) {
+      u64 *cp = reinterpret_cast<u64 *>(a.copies) + (a.slot0 + blockIdx.x) * dk1;
+      for (int i = tid; i < dk1; i += NT) {
+        const int f = i / k, c = i % k;
+        const u64 *src = S64 + (c * (DB + 1) + (f < d ? f : DB)) * R;
+        u64 t = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) t += src[r];
+        cp[i] = t;
+      }
+    } else {
+      float *cp = reinterpret_cast<float *>(a.copies) + (a.slot0 + blockIdx.x) * dk1;
+      for (int i = tid; i < dk1; i += NT) {
+        const int f = i / k, c = i % k;
+        const float *src = S32 + (c * (DB + 1) + (f < d ? f : DB)) * R;
+        float t = src[0];
+#pragma unroll
+        for (int r = 1; r < R; ++r) t += src[r];
+        cp[i] = t;
+      }
+    }
+  }
+}
+
 // ---- fixed-point scales ------------------------------------------------------
 
 // The range statistics of fixed_point.hpp for row f = blockIdx.y of X (f ==
@@ -649,10 +865,94 @@ KernFn pick_shape(int64_t d, int shape) {
   return pick_nt<kThreads, I, UPDATE, STORE>(d);
 }
 
+// ---- few-centroid launches (smallk_lloyd_krn) ----
+
+// Feature rows a lane holds: d rounded up to 8, 16, 32 or 64.
+inline int smallk_db(int64_t d) {
+  return d <= 8 ? 8 : d <= 16 ? 16 : d <= 32 ? 32 : 64;
+}
+
+// Groups of 4 centroids the kernel computes (its template G): 1, 2 or 4.
+inline int smallk_groups(int64_t k) { return k <= 4 ? 1 : k <= 8 ? 2 : 4; }
+
+inline size_t smallk_lds(int64_t d, int64_t k, bool update, int copies) {
+  const int db = smallk_db(d);
+  const int kp = 4 * smallk_groups(k);   // the kernel's KP rows
+  size_t b = sizeof(float) * ((size_t)kp * db + kp + db + 4);
+  if (update) b += sizeof(u64) * (size_t)k * (db + 1) * copies;
+  return b;
+}
+
+// KMD_NO_SMALLK=1 sends k <= 16 to the 32-row tile kernel (the two compute
+// the same labels and sums; the switch exists to compare them).
+inline bool smallk_enabled() {
+  const char *e = std::getenv("KMD_NO_SMALLK");
+  return !(e != nullptr && e[0] == '1' && e[1] == '\0');
+}
+
+template <typename I, bool UPDATE>
+KernFn pick_smallk(int64_t d, int64_t k) {
+  const int g = smallk_groups(k);
+  const int db = smallk_db(d);
+#define KMD_SK_CASE(DB)                                        \
+  case DB:                                                     \
+    return g == 1   ? &smallk_lloyd_krn<DB, 1, I, UPDATE>      \
+           : g == 2 ? &smallk_lloyd_krn<DB, 2, I, UPDATE>      \
+                    : &smallk_lloyd_krn<DB, 4, I, UPDATE>;
+  switch (db) {
+    KMD_SK_CASE(8)
+    KMD_SK_CASE(16)
+    KMD_SK_CASE(32)
+    KMD_SK_CASE(64)
+    default:
+      return nullptr;
+  }
+#undef KMD_SK_CASE
+}
+
+template <typename I, bool UPDATE>
+MfmaLaunch plan_smallk(int64_t n, int64_t d, int64_t k) {
+  MfmaLaunch L;
+  // the kernel's lane-keyed copies of the sums (fixed per bucket)
+  L.copies = UPDATE ? (smallk_db(d) <= 32 ? smallk_copies<32>()
+                                         : smallk_copies<64>())
+                    : 1;
+  L.lds = smallk_lds(d, k, UPDATE, L.copies);
+  if (L.lds > kLdsPerCu) return L;
+  L.threads = smallk_db(d) <= 32 ? 1024 : 512;
+  L.s = 1;
+  L.fn = pick_smallk<I, UPDATE>(d, k);
+  if (L.fn == nullptr) return L;
+  allow_full_lds(reinterpret_cast<const void *>(L.fn));
+  int per_cu = 0;
+  KMD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void *>(L.fn), L.threads, L.lds));
+  per_cu = apply_blocks_cap(per_cu);
+  if (per_cu < 1) {
+    L.fn = nullptr;
+    return L;
+  }
+  const int64_t need = ceil_div<int64_t>(
+      ceil_div<int64_t>(n > 0 ? n : 1, kWave), L.threads / kWave);
+  int64_t blocks = (int64_t)device_cu_count() * per_cu;
+  if (blocks > need) blocks = need;
+  L.blocks = (int)(blocks < 1 ? 1 : blocks);
+  return L;
+}
+
 template <typename I, bool UPDATE, bool STORE>
 MfmaLaunch plan_launch(int64_t n, int64_t d, int64_t k) {
   MfmaLaunch L;
   if (d < 1 || d > kMaxD || k < 1 || k > (1 << 20)) return L;
+  // the driver's step (store flush) and the assignment; the reference-API
+  // step (atomic flush, binade windows) stays on the tile kernel
+  if constexpr (!UPDATE || STORE) {
+    if (k <= kSmallK && smallk_enabled()) {
+      L = plan_smallk<I, UPDATE>(n, d, k);
+      if (L.fn != nullptr) return L;
+      L = MfmaLaunch{};
+    }
+  }
   L.copies = UPDATE ? sum_copies(d, k) : 1;
   L.lds = lds_bytes(d, k, UPDATE, L.copies);
   if (L.lds > kLdsPerCu) return L;

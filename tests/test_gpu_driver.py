@@ -304,7 +304,8 @@ def same_fit(a, b):
     assert a[3] == b[3]
 
 
-@pytest.mark.parametrize("shape", [(30_000, 24, 40), (20_001, 64, 100), (8_193, 7, 33)])
+@pytest.mark.parametrize("shape", [(30_000, 24, 40), (20_001, 64, 100), (8_193, 7, 33),
+                                   (30_000, 24, 8)])
 def test_tiled_fp32_equals_lds_resident_fp32(kd, torch_dev, monkeypatch, shape):
     """Forced onto the tiled path, a shape the LDS-resident kernel serves
     gives the same labels, centroids (bit for bit), iteration count and
@@ -318,6 +319,73 @@ def test_tiled_fp32_equals_lds_resident_fp32(kd, torch_dev, monkeypatch, shape):
     got = run_driver_ex(kd, Xt, w, init, 1e-4, 20, "fp32")
     assert got[4]["path"] == "tiled-fp32"
     same_fit(got, ref)
+
+
+# ---- few centroids: the one-sample-per-lane 4x4x1 kernel
+
+def assign_labels(kd, Xt, C, itype=torch.int32):
+    hn = lo.half_l2_norm_squared(C)
+    lab_d = torch.empty(Xt.shape[1], dtype=itype, device="cuda")
+    _, ev = kd.assignment(to_dev(Xt), to_dev(C), to_dev(hn), lab_d, 8, 128, q())
+    ev.wait()
+    return to_np(lab_d).astype(np.int64)
+
+
+@pytest.mark.parametrize("shape", [(100_003, 32, 8), (70_001, 7, 3), (50_000, 16, 16),
+                                   (40_001, 33, 5), (30_000, 64, 13), (20_011, 1, 1),
+                                   (9_999, 48, 9), (65, 2, 4), (3, 5, 2), (20_000, 32, 10),
+                                   (12_345, 40, 12), (8_000, 64, 11)])
+def test_small_k_kernel_equals_tile_kernel(kd, torch_dev, monkeypatch, shape):
+    """k <= 16 runs on the kernel that gives each lane one sample and all its
+    centroids (v_mfma_f32_4x4x1f32); KMD_NO_SMALLK=1 runs the same fit on
+    the 32-row tile kernel.  Both chains of FP32 MFMAs round like fmaf over
+    ascending features from -hn, the argmin rule is the same and the sums
+    are the same int64 fixed-point integers, so fits and assignments are
+    equal bit for bit -- on blobs and on a grid full of exact ties between
+    duplicated centroids (lowest index wins in both)."""
+    n, d, k = shape
+    Xt, init = lo.gaussian_blobs(n, d, k, seed=17, std=1.5)
+    w = np.random.default_rng(6).uniform(0.5, 2.0, n).astype(np.float32)
+    rng = np.random.default_rng(7)
+    Xg = rng.integers(0, 3, size=(d, n)).astype(np.float32)
+    Cg = rng.integers(0, 3, size=(d, k)).astype(np.float32)
+    if k > 2:
+        Cg[:, k - 1] = Cg[:, 1]
+    got = run_driver_ex(kd, Xt, w, init, 1e-4, 15, "fp32")
+    got_a = [assign_labels(kd, Xg, Cg, it) for it in (torch.int32, torch.int64)]
+    monkeypatch.setenv("KMD_NO_SMALLK", "1")
+    ref = run_driver_ex(kd, Xt, w, init, 1e-4, 15, "fp32")
+    ref_a = [assign_labels(kd, Xg, Cg, it) for it in (torch.int32, torch.int64)]
+    same_fit(got, ref)
+    for g, r in zip(got_a, ref_a):
+        assert_array_equal(g, r)
+    assert_array_equal(got_a[0], co.assign(Xg, Cg))   # integer data: exact scores
+
+
+@pytest.mark.parametrize("k,nan_cols", [(8, [0, 1, 2, 3]),
+                                        (20, [0, 1, 2, 3, 8, 9, 10, 11, 16, 17, 18, 19])])
+@pytest.mark.parametrize("smallk", ["1", "0"])
+def test_nan_centroids_are_passed_over(kd, torch_dev, monkeypatch, k, nan_cols, smallk):
+    """Centroids with NaN coordinates score NaN for every sample; the
+    reference's scan (strict <) never takes a NaN, so each sample goes to its
+    best finite centroid.  k = 20 puts the NaN centroids on every row that
+    one half of the 32-row tile holds (rows 0-3, 8-11, 16-19), the case in
+    which that half has no candidate at all; k = 8 does the same for the
+    few-centroid kernel and, with KMD_NO_SMALLK=1, for the tile kernel."""
+    monkeypatch.setenv("KMD_NO_SMALLK", "0" if smallk == "1" else "1")
+    rng = np.random.default_rng(3)
+    d, n = 6, 4096
+    Xt = rng.standard_normal((d, n)).astype(np.float32)
+    C = (2 * rng.standard_normal((d, k))).astype(np.float32)
+    C[:, nan_cols] = np.nan
+    got = assign_labels(kd, Xt, C)
+    ref = co.assign(Xt, C)
+    finite = np.setdiff1d(np.arange(k), nan_cols)
+    assert np.isin(ref, finite).all()
+    margin = lo.pseudo_margin(Xt.astype(np.float64), C[:, finite].astype(np.float64))
+    bad = got != ref
+    assert (margin[bad] < 1e-4).all(), margin[bad]
+    assert bad.sum() <= 2
 
 
 def test_tiled_fp32_wide_shape_vs_oracle(kd, torch_dev):
