@@ -509,30 +509,48 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
 // instead of serialising on the few labels.
 constexpr int kSmallK = 16;
 
-// Threads per block and copies of the sums (R) per feature-row bucket DB:
-// R k (d + 1) int64 sums must fit the LDS beside the centroids, and the
-// offsets f R of a lane's adds are then compile-time immediates.
+// Per feature-row bucket DB and centroid groups G: threads per block,
+// copies of the sums (R k (DB + 1) int64 beside the centroids and the label
+// stage must fit the LDS; the offsets f R of a lane's adds are then
+// compile-time immediates) and super-blocks of labels a wave stages in LDS.
 template <int DB>
 constexpr int smallk_threads() { return DB <= 32 ? 1024 : 512; }
-template <int DB>
-constexpr int smallk_copies() { return DB <= 32 ? 32 : 16; }
+template <int DB, int G>
+constexpr int smallk_copies() { return DB <= 32 ? (G == 4 ? 16 : 32) : (G == 4 ? 8 : 16); }
+// Labels are held in LDS, one byte per sample (k <= 16), and written out
+// when kLabelStage super-blocks have been scored and at the end: HBM writes
+// interleaved with the stream of reads cost far more than their bytes (at
+// n = 10 M, d = 32 the label stores of every super-block added 15-40 us to
+// a 0.25 ms step), so a wave writes its labels in one burst after its last
+// load wherever n allows.
+constexpr int kLabelStage = 64;
 // Word 3 of a raw buffer descriptor on gfx9-family devices (untyped dword
 // access, no swizzle).
 constexpr int kRsrcWord3 = 0x00020000;
+
+// acc += C[f][4 g + r] x (row r of every 4x4 block): CBSZ = 4 makes the 16
+// blocks one broadcast group and ABID = g hands them block g's A values,
+// i.e. lanes 4 g .. 4 g + 3, which hold centroids 4 g .. 4 g + 3.
+template <int Gi>
+__device__ inline f32x4 mfma4_group(float a, float x, f32x4 acc) {
+  return __builtin_amdgcn_mfma_f32_4x4x1f32(a, x, acc, 4, Gi, 0);
+}
 
 template <int DB, int G, typename I, bool UPDATE>
 __global__ void __launch_bounds__(smallk_threads<DB>())
     smallk_lloyd_krn(FastArgs a) {
   if (a.skip != nullptr && *a.skip != 0) return;
   constexpr int NT = smallk_threads<DB>();
-  constexpr int R = smallk_copies<DB>();
+  constexpr int R = smallk_copies<DB, G>();
   constexpr int kWavesPerBlock = NT / kWave;
   constexpr int KP = 4 * G;
+  // scales and -hn in scalar registers where they fit
+  constexpr bool kScalarConsts = DB <= 32;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int64_t n = a.n;
   const int d = a.d, k = a.k;
-  // Ca[(4 g + j) DB + f] = C[f][4 g + j] (0 outside the matrix): one 16-byte
-  // read gives a lane its A operands of four consecutive features
+  // Ca[c DB + f] = C[f][c] (0 outside the matrix): lane c < KP reads its
+  // centroid's A operands of four consecutive features with one 16-byte read
   float *Ca = reinterpret_cast<float *>(lds_raw);
   float *Hl = Ca + KP * DB;                          // [KP] -hn, -inf padded
   float *Sc = Hl + KP;                               // [DB + 4] scales
@@ -540,6 +558,8 @@ __global__ void __launch_bounds__(smallk_threads<DB>())
   float *S32 = reinterpret_cast<float *>(S64);       // non-finite path
   const int tid = threadIdx.x;
   const int n_sums = UPDATE ? k * (DB + 1) * R : 0;
+  // [wave][kLabelStage][64] labels of the wave's staged super-blocks
+  unsigned char *stage_all = reinterpret_cast<unsigned char *>(S64 + n_sums);
   const bool fixed = UPDATE && a.nonfinite[0] == 0;
 
   for (int i = tid; i < KP * DB; i += NT) {
@@ -561,10 +581,22 @@ __global__ void __launch_bounds__(smallk_threads<DB>())
   // uniform in the wave (the compiler cannot see that of tid / 64), so the
   // per-row buffer descriptors below stay in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
-  const int j4 = lane & 3;
   const int64_t n_super = ceil_div<int64_t>(n, kWave);
   const int64_t sb_stride = (int64_t)gridDim.x * kWavesPerBlock;
   I *labels = reinterpret_cast<I *>(a.labels);
+  unsigned char *stage = stage_all + wave * (kLabelStage * kWave);
+  auto uniform = [](float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+  };
+  float hs[kScalarConsts ? KP : 1], ss[kScalarConsts ? DB + 1 : 1];
+  if constexpr (kScalarConsts) {
+#pragma unroll
+    for (int c = 0; c < KP; ++c) hs[c] = uniform(Hl[c]);
+    if (UPDATE) {
+#pragma unroll
+      for (int f = 0; f <= DB; ++f) ss[f] = uniform(Sc[f]);
+    }
+  }
 
   // Samples of super-block b: row f is read through a buffer descriptor
   // that covers the block's valid samples of that row and nothing else, so
@@ -598,25 +630,33 @@ __global__ void __launch_bounds__(smallk_threads<DB>())
     }
   };
 
-  auto body = [&](int64_t sb, const float (&xr)[DB], float ws) {
+  // slot: the super-block's place in the stage
+  auto body = [&](int64_t sb, int slot, const float (&xr)[DB], float ws) {
     const int64_t s = sb * kWave + lane;
     const bool valid = s < n;
-    f32x4 acc[G];   // the chains start from -hn (read from LDS, not held)
+    f32x4 acc[G];   // the chains start from -hn
 #pragma unroll
-    for (int g = 0; g < G; ++g)
-      acc[g] = *reinterpret_cast<const f32x4 *>(Hl + 4 * g);
+    for (int g = 0; g < G; ++g) {
+      if constexpr (kScalarConsts) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[g][r] = hs[4 * g + r];
+      } else {
+        acc[g] = *reinterpret_cast<const f32x4 *>(Hl + 4 * g);
+      }
+    }
 #pragma unroll
     for (int f4 = 0; f4 < DB; f4 += 4) {
-      f32x4 av[G];
-#pragma unroll
-      for (int g = 0; g < G; ++g)
-        av[g] = *reinterpret_cast<const f32x4 *>(Ca + (4 * g + j4) * DB + f4);
+      // only lanes c < KP supply A operands (the broadcast reads no other)
+      f32x4 av = {};
+      if (lane < KP) av = *reinterpret_cast<const f32x4 *>(Ca + lane * DB + f4);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-#pragma unroll
-        for (int g = 0; g < G; ++g)
-          acc[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(av[g][e], xr[f4 + e],
-                                                      acc[g], 0, 0, 0);
+        acc[0] = mfma4_group<0>(av[e], xr[f4 + e], acc[0]);
+        if constexpr (G > 1) acc[1] = mfma4_group<1>(av[e], xr[f4 + e], acc[1]);
+        if constexpr (G > 2) {
+          acc[2] = mfma4_group<2>(av[e], xr[f4 + e], acc[2]);
+          acc[3] = mfma4_group<3>(av[e], xr[f4 + e], acc[3]);
+        }
       }
     }
     // largest acc (smallest score hn - c.x) over the KP rows, NaNs passed
@@ -631,7 +671,7 @@ __global__ void __launch_bounds__(smallk_threads<DB>())
 #pragma unroll
     for (int c = KP - 1; c >= 0; --c) lab = (acc[c / 4][c % 4] == m) ? c : lab;
     lab = (-m < __builtin_huge_valf()) ? lab : 0;
-    if (valid) labels[s] = (I)lab;
+    stage[slot * kWave + lane] = (unsigned char)lab;
 
     if (UPDATE && valid) {
       const int base = lab * (DB + 1) * R + (lane & (R - 1));
@@ -639,14 +679,21 @@ __global__ void __launch_bounds__(smallk_threads<DB>())
         u64 *Sp = S64 + base;
 #pragma unroll
         for (int f4 = 0; f4 < DB; f4 += 4) {
-          const f32x4 sc = *reinterpret_cast<const f32x4 *>(Sc + f4);
+          f32x4 sc;
+          if constexpr (kScalarConsts) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) sc[e] = ss[f4 + e];
+          } else {
+            sc = *reinterpret_cast<const f32x4 *>(Sc + f4);
+          }
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const int qv = __float2int_rn((xr[f4 + e] * ws) * sc[e]);
             atomicAdd(Sp + (f4 + e) * R, (u64)(long long)qv);
           }
         }
-        atomicAdd(Sp + DB * R, (u64)(long long)__float2int_rn(ws * Sc[DB]));
+        const float scw = kScalarConsts ? ss[DB] : Sc[DB];
+        atomicAdd(Sp + DB * R, (u64)(long long)__float2int_rn(ws * scw));
       } else {
         float *Sp = S32 + base;
 #pragma unroll
@@ -656,18 +703,39 @@ __global__ void __launch_bounds__(smallk_threads<DB>())
     }
   };
 
-  // two register buffers, alternating: the next super-block's samples are
-  // in flight while this one is scored
+  // the labels of super-blocks first, first + stride, ... (cnt of them),
+  // each lane its own column of the stage (no exchange between lanes)
+  auto flush = [&](int64_t first, int cnt) {
+    for (int j = 0; j < cnt; ++j) {
+      const int64_t s = (first + j * sb_stride) * kWave + lane;
+      if (s < n) labels[s] = (I)stage[j * kWave + lane];
+    }
+  };
+  // two register buffers: the next super-block is in flight while this one
+  // is scored (a third would need more loads in flight than the memory
+  // counter can track)
   float xa[DB], xb[DB], wa = 0.f, wb = 0.f;
   int64_t sb = (int64_t)blockIdx.x * kWavesPerBlock + wave;
+  int64_t staged_first = sb;
+  int slot = 0;
+  auto next_slot = [&](int64_t done) {
+    if (++slot == kLabelStage) {
+      flush(staged_first, kLabelStage);
+      staged_first = done + sb_stride;
+      slot = 0;
+    }
+  };
   if (sb < n_super) load(sb, xa, wa);
   for (; sb < n_super; sb += 2 * sb_stride) {
     if (sb + sb_stride < n_super) load(sb + sb_stride, xb, wb);
-    body(sb, xa, wa);
+    body(sb, slot, xa, wa);
+    next_slot(sb);
     if (sb + sb_stride >= n_super) break;
     if (sb + 2 * sb_stride < n_super) load(sb + 2 * sb_stride, xa, wa);
-    body(sb + sb_stride, xb, wb);
+    body(sb + sb_stride, slot, xb, wb);
+    next_slot(sb + sb_stride);
   }
+  flush(staged_first, slot);
 
   if (UPDATE) {
     // raw partials into this block's slot, in the [f][label] layout of the
@@ -878,9 +946,17 @@ inline int smallk_groups(int64_t k) { return k <= 4 ? 1 : k <= 8 ? 2 : 4; }
 inline size_t smallk_lds(int64_t d, int64_t k, bool update, int copies) {
   const int db = smallk_db(d);
   const int kp = 4 * smallk_groups(k);   // the kernel's KP rows
+  const int threads = db <= 32 ? smallk_threads<32>() : smallk_threads<64>();
   size_t b = sizeof(float) * ((size_t)kp * db + kp + db + 4);
   if (update) b += sizeof(u64) * (size_t)k * (db + 1) * copies;
-  return b;
+  return b + (size_t)(threads / kWave) * kLabelStage * kWave;   // label stage
+}
+
+// The kernel's copies of the sums (smallk_copies<DB, G>).
+inline int smallk_copies_of(int64_t d, int64_t k) {
+  const int g = smallk_groups(k);
+  return smallk_db(d) <= 32 ? (g == 4 ? smallk_copies<32, 4>() : smallk_copies<32, 2>())
+                            : (g == 4 ? smallk_copies<64, 4>() : smallk_copies<64, 2>());
 }
 
 // KMD_NO_SMALLK=1 sends k <= 16 to the 32-row tile kernel (the two compute
@@ -914,12 +990,10 @@ template <typename I, bool UPDATE>
 MfmaLaunch plan_smallk(int64_t n, int64_t d, int64_t k) {
   MfmaLaunch L;
   // the kernel's lane-keyed copies of the sums (fixed per bucket)
-  L.copies = UPDATE ? (smallk_db(d) <= 32 ? smallk_copies<32>()
-                                         : smallk_copies<64>())
-                    : 1;
+  L.copies = UPDATE ? smallk_copies_of(d, k) : 1;
   L.lds = smallk_lds(d, k, UPDATE, L.copies);
   if (L.lds > kLdsPerCu) return L;
-  L.threads = smallk_db(d) <= 32 ? 1024 : 512;
+  L.threads = smallk_db(d) <= 32 ? smallk_threads<32>() : smallk_threads<64>();
   L.s = 1;
   L.fn = pick_smallk<I, UPDATE>(d, k);
   if (L.fn == nullptr) return L;

@@ -362,6 +362,27 @@ def test_small_k_kernel_equals_tile_kernel(kd, torch_dev, monkeypatch, shape):
     assert_array_equal(got_a[0], co.assign(Xg, Cg))   # integer data: exact scores
 
 
+def test_small_k_labels_past_one_stage(kd, torch_dev, monkeypatch):
+    """The few-centroid kernel keeps each wave's labels in LDS for up to 64
+    of its super-blocks; n = 17,000,011 gives the waves 77 or more, so the
+    stage is written out mid-run as well as at the end.  Fit and assignment
+    equal the tile kernel's bit for bit."""
+    n, d, k = 17_000_011, 4, 5
+    rng = np.random.default_rng(12)
+    centers = rng.uniform(-5, 5, size=(k, d))
+    Xt = np.ascontiguousarray(
+        (centers[rng.integers(0, k, n)] + rng.standard_normal((n, d))).T.astype(np.float32))
+    init = np.ascontiguousarray(centers.T.astype(np.float32))
+    w = np.ones(n, np.float32)
+    got = run_driver_ex(kd, Xt, w, init, 0.0, 3, "fp32")
+    got_a = assign_labels(kd, Xt, init, torch.int64)
+    monkeypatch.setenv("KMD_NO_SMALLK", "1")
+    ref = run_driver_ex(kd, Xt, w, init, 0.0, 3, "fp32")
+    ref_a = assign_labels(kd, Xt, init, torch.int64)
+    same_fit(got, ref)
+    assert_array_equal(got_a, ref_a)
+
+
 @pytest.mark.parametrize("k,nan_cols", [(8, [0, 1, 2, 3]),
                                         (20, [0, 1, 2, 3, 8, 9, 10, 11, 16, 17, 18, 19])])
 @pytest.mark.parametrize("smallk", ["1", "0"])
