@@ -562,41 +562,12 @@ __global__ void __launch_bounds__(smallk_threads<DB>())
   unsigned char *stage_all = reinterpret_cast<unsigned char *>(S64 + n_sums);
   const bool fixed = UPDATE && a.nonfinite[0] == 0;
 
-  for (int i = tid; i < KP * DB; i += NT) {
-    const int c = i / DB, f = i % DB;
-    Ca[i] = (f < d && c < k) ? a.C[(int64_t)f * k + c] : 0.f;
-  }
-  for (int c = tid; c < KP; c += NT)
-    Hl[c] = (c < k) ? -a.hn[c] : -__builtin_huge_valf();
-  if (UPDATE) {
-    // Sc[f] = the scale of feature f < d, 0 for padded rows; Sc[DB] = the
-    // weights' scale
-    for (int f = tid; f < DB + 4; f += NT)
-      Sc[f] = !fixed ? 0.f : f < d ? a.scale[f] : f == DB ? a.scale[d] : 0.f;
-    for (int i = tid; i < n_sums; i += NT) S64[i] = 0ull;
-  }
-  __syncthreads();
-
   const int lane = tid & (kWave - 1);
   // uniform in the wave (the compiler cannot see that of tid / 64), so the
   // per-row buffer descriptors below stay in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
   const int64_t n_super = ceil_div<int64_t>(n, kWave);
   const int64_t sb_stride = (int64_t)gridDim.x * kWavesPerBlock;
-  I *labels = reinterpret_cast<I *>(a.labels);
-  unsigned char *stage = stage_all + wave * (kLabelStage * kWave);
-  auto uniform = [](float v) {
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-  };
-  float hs[kScalarConsts ? KP : 1], ss[kScalarConsts ? DB + 1 : 1];
-  if constexpr (kScalarConsts) {
-#pragma unroll
-    for (int c = 0; c < KP; ++c) hs[c] = uniform(Hl[c]);
-    if (UPDATE) {
-#pragma unroll
-      for (int f = 0; f <= DB; ++f) ss[f] = uniform(Sc[f]);
-    }
-  }
 
   // Samples of super-block b: row f is read through a buffer descriptor
   // that covers the block's valid samples of that row and nothing else, so
@@ -629,6 +600,45 @@ __global__ void __launch_bounds__(smallk_threads<DB>())
           __builtin_amdgcn_raw_buffer_load_b32(r, lane * 4, 0, 0));
     }
   };
+
+  // two register buffers: the next super-block is in flight while this one
+  // is scored (a third would need more loads in flight than the memory
+  // counter can track); the first one's loads are issued before the block
+  // sets up its LDS
+  float xa[DB], xb[DB], wa = 0.f, wb = 0.f;
+  int64_t sb = (int64_t)blockIdx.x * kWavesPerBlock + wave;
+  if (sb < n_super) load(sb, xa, wa);
+
+  for (int i = tid; i < KP * DB; i += NT) {
+    const int c = i / DB, f = i % DB;
+    Ca[i] = (f < d && c < k) ? a.C[(int64_t)f * k + c] : 0.f;
+  }
+  for (int c = tid; c < KP; c += NT)
+    Hl[c] = (c < k) ? -a.hn[c] : -__builtin_huge_valf();
+  if (UPDATE) {
+    // Sc[f] = the scale of feature f < d, 0 for padded rows; Sc[DB] = the
+    // weights' scale
+    for (int f = tid; f < DB + 4; f += NT)
+      Sc[f] = !fixed ? 0.f : f < d ? a.scale[f] : f == DB ? a.scale[d] : 0.f;
+    for (int i = tid; i < n_sums; i += NT) S64[i] = 0ull;
+  }
+  __syncthreads();
+
+  I *labels = reinterpret_cast<I *>(a.labels);
+  unsigned char *stage = stage_all + wave * (kLabelStage * kWave);
+  auto uniform = [](float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+  };
+  float hs[kScalarConsts ? KP : 1], ss[kScalarConsts ? DB + 1 : 1];
+  if constexpr (kScalarConsts) {
+#pragma unroll
+    for (int c = 0; c < KP; ++c) hs[c] = uniform(Hl[c]);
+    if (UPDATE) {
+#pragma unroll
+      for (int f = 0; f <= DB; ++f) ss[f] = uniform(Sc[f]);
+    }
+  }
+
 
   // slot: the super-block's place in the stage
   auto body = [&](int64_t sb, int slot, const float (&xr)[DB], float ws) {
@@ -711,11 +721,6 @@ __global__ void __launch_bounds__(smallk_threads<DB>())
       if (s < n) labels[s] = (I)stage[j * kWave + lane];
     }
   };
-  // two register buffers: the next super-block is in flight while this one
-  // is scored (a third would need more loads in flight than the memory
-  // counter can track)
-  float xa[DB], xb[DB], wa = 0.f, wb = 0.f;
-  int64_t sb = (int64_t)blockIdx.x * kWavesPerBlock + wave;
   int64_t staged_first = sb;
   int slot = 0;
   auto next_slot = [&](int64_t done) {
@@ -725,7 +730,6 @@ __global__ void __launch_bounds__(smallk_threads<DB>())
       slot = 0;
     }
   };
-  if (sb < n_super) load(sb, xa, wa);
   for (; sb < n_super; sb += 2 * sb_stride) {
     if (sb + sb_stride < n_super) load(sb + sb_stride, xb, wb);
     body(sb, slot, xa, wa);
