@@ -294,11 +294,18 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_workspace(int64_t n,
                                                             const T *w,
                                                             int precision,
                                                             hipStream_t s) {
+  // With k <= 16 the step is bound by reading X, not by the cross term:
+  // every precision runs the FP32 few-centroid kernel there (its labels and
+  // sums are what the mixed paths reproduce, at a fraction of their time).
+  const bool smallk = std::is_same<T, float>::value && !force_tiled() &&
+                      tuned_path() != 2 && fast_smallk_serves(d, k);
   // the FP8 cross term is built on the tiled kernels only
-  const bool tiled_first = force_tiled() || tuned_path() == 2 ||
-                           (precision == KMD_PREC_FP8 && std::is_same<T, float>::value &&
-                            tiled_supported(d, k));
-  if (precision != KMD_PREC_FP32 && precision != KMD_PREC_FP8 && !tiled_first) {
+  const bool tiled_first = !smallk && (force_tiled() || tuned_path() == 2 ||
+                                       (precision == KMD_PREC_FP8 &&
+                                        std::is_same<T, float>::value &&
+                                        tiled_supported(d, k)));
+  if (precision != KMD_PREC_FP32 && precision != KMD_PREC_FP8 && !tiled_first &&
+      !smallk) {
     auto ws = make_mixed_workspace<T, I>(
         n, d, k, precision == KMD_PREC_BF16X3 ? 3 : 1, X, w, s);
     if (ws) return ws;

@@ -1140,6 +1140,12 @@ bool fast_fused_supported(int64_t d, int64_t k) {
          plan_launch<int64_t, true, false>(1, d, k).fn != nullptr;
 }
 
+// The driver's step for these float shapes runs the few-centroid kernel.
+bool fast_smallk_serves(int64_t d, int64_t k) {
+  return d >= 1 && d <= kMaxD && k >= 1 && k <= kSmallK && smallk_enabled() &&
+         plan_smallk<int32_t, true>(1, d, k).fn != nullptr;
+}
+
 template <typename I>
 int fast_fused_blocks(int64_t n, int64_t d, int64_t k) {
   return plan_launch<I, true, true>(n, d, k).blocks;

@@ -68,6 +68,9 @@ bool fast_fused_supported(int64_t d, int64_t k);
 // that use store_flush can size one private copy per block).
 template <typename I>
 int fast_fused_blocks(int64_t n, int64_t d, int64_t k);
+// k <= 16 (float): the driver's fused step runs the one-sample-per-lane
+// kernel (KMD_NO_SMALLK=1 turns it off)
+bool fast_smallk_serves(int64_t d, int64_t k);
 
 // The reference's fused step on the MFMA kernel: FP32 block sums in LDS,
 // flushed into the caller's private copies with FP32 atomics (no scales, no

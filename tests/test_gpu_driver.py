@@ -362,6 +362,28 @@ def test_small_k_kernel_equals_tile_kernel(kd, torch_dev, monkeypatch, shape):
     assert_array_equal(got_a[0], co.assign(Xg, Cg))   # integer data: exact scores
 
 
+def test_small_k_runs_fp32_for_every_precision(kd, torch_dev, monkeypatch):
+    """With k <= 16 the step is bound by reading X: every precision runs the
+    FP32 few-centroid kernel (path "fp32-mfma").  With KMD_NO_SMALLK=1 the
+    BF16, BF16x3 and FP8 kernels run instead, and every one of these fits
+    equals the FP32 fit bit for bit."""
+    n, d, k = 60_000, 16, 8
+    Xt, init = lo.gaussian_blobs(n, d, k, seed=23, std=1.5)
+    w = np.random.default_rng(2).uniform(0.5, 2.0, n).astype(np.float32)
+    ref = run_driver_ex(kd, Xt, w, init, 1e-4, 12, "fp32")
+    assert ref[4]["path"] == "fp32-mfma"
+    for prec in ("bf16", "bf16x3", "fp8"):
+        got = run_driver_ex(kd, Xt, w, init, 1e-4, 12, prec)
+        assert got[4]["path"] == "fp32-mfma"
+        same_fit(got, ref)
+    monkeypatch.setenv("KMD_NO_SMALLK", "1")
+    for prec, path in (("bf16", "bf16+rescore"), ("bf16x3", "bf16x3+rescore"),
+                       ("fp8", "tiled-fp8+rescore")):
+        got = run_driver_ex(kd, Xt, w, init, 1e-4, 12, prec)
+        assert got[4]["path"] == path
+        same_fit(got, ref)
+
+
 def test_small_k_labels_past_one_stage(kd, torch_dev, monkeypatch):
     """The few-centroid kernel keeps each wave's labels in LDS for up to 64
     of its super-blocks; n = 17,000,011 gives the waves 77 or more, so the
