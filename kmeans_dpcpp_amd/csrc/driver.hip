@@ -294,7 +294,7 @@ static std::unique_ptr<LloydWorkspace<T, I>> make_workspace(int64_t n,
                                                             const T *w,
                                                             int precision,
                                                             hipStream_t s) {
-  // With k <= 16 the step is bound by reading X, not by the cross term:
+  // With k <= 32 the step is bound by reading X, not by the cross term:
   // every precision runs the FP32 few-centroid kernel there (its labels and
   // sums are what the mixed paths reproduce, at a fraction of their time).
   const bool smallk = std::is_same<T, float>::value && !force_tiled() &&

@@ -483,9 +483,10 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
   }
 }
 
-// ---- few centroids (k <= 16) -----------------------------------------------
+// ---- few centroids (k <= 32) -----------------------------------------------
 //
 // With k <= 16 the 32-row tile above computes 2-8 x the products it needs,
+// up to k = 32 it needs an exchange between lanes for the argmin,
 // and its per-block sums take most of a wave's 64 atomic adds into a handful
 // of addresses.  This kernel gives each lane one sample and all of its
 // centroids instead:
@@ -507,17 +508,21 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
 // labels, so with R = 32 no two lanes of an instruction share an address
 // except lanes l and l + 32, and the atomics run at the LDS's full width
 // instead of serialising on the few labels.
-constexpr int kSmallK = 16;
+constexpr int kSmallK = 32;
 
 // Per feature-row bucket DB and centroid groups G: threads per block,
 // copies of the sums (R k (DB + 1) int64 beside the centroids and the label
 // stage must fit the LDS; the offsets f R of a lane's adds are then
 // compile-time immediates) and super-blocks of labels a wave stages in LDS.
-template <int DB>
-constexpr int smallk_threads() { return DB <= 32 ? 1024 : 512; }
 template <int DB, int G>
-constexpr int smallk_copies() { return DB <= 32 ? (G == 4 ? 16 : 32) : (G == 4 ? 8 : 16); }
-// Labels are held in LDS, one byte per sample (k <= 16), and written out
+constexpr int smallk_threads() {
+  return DB <= 32 ? ((DB == 32 && G == 8) ? 768 : 1024) : 512;
+}
+template <int DB, int G>
+constexpr int smallk_copies() {
+  return DB <= 32 ? (G == 8 ? 8 : G == 4 ? 16 : 32) : (G == 8 ? 4 : G == 4 ? 8 : 16);
+}
+// Labels are held in LDS, one byte per sample (k <= 32), and written out
 // when kLabelStage super-blocks have been scored and at the end: HBM writes
 // interleaved with the stream of reads cost far more than their bytes (at
 // n = 10 M, d = 32 the label stores of every super-block added 15-40 us to
@@ -535,12 +540,18 @@ template <int Gi>
 __device__ inline f32x4 mfma4_group(float a, float x, f32x4 acc) {
   return __builtin_amdgcn_mfma_f32_4x4x1f32(a, x, acc, 4, Gi, 0);
 }
+// one feature's products for groups Gi .. G - 1
+template <int G, int Gi = 0>
+__device__ inline void mfma4_groups(float a, float x, f32x4 (&acc)[G]) {
+  acc[Gi] = mfma4_group<Gi>(a, x, acc[Gi]);
+  if constexpr (Gi + 1 < G) mfma4_groups<G, Gi + 1>(a, x, acc);
+}
 
 template <int DB, int G, typename I, bool UPDATE>
-__global__ void __launch_bounds__(smallk_threads<DB>())
+__global__ void __launch_bounds__((smallk_threads<DB, G>()))
     smallk_lloyd_krn(FastArgs a) {
   if (a.skip != nullptr && *a.skip != 0) return;
-  constexpr int NT = smallk_threads<DB>();
+  constexpr int NT = smallk_threads<DB, G>();
   constexpr int R = smallk_copies<DB, G>();
   constexpr int kWavesPerBlock = NT / kWave;
   constexpr int KP = 4 * G;
@@ -660,14 +671,7 @@ __global__ void __launch_bounds__(smallk_threads<DB>())
       f32x4 av = {};
       if (lane < KP) av = *reinterpret_cast<const f32x4 *>(Ca + lane * DB + f4);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        acc[0] = mfma4_group<0>(av[e], xr[f4 + e], acc[0]);
-        if constexpr (G > 1) acc[1] = mfma4_group<1>(av[e], xr[f4 + e], acc[1]);
-        if constexpr (G > 2) {
-          acc[2] = mfma4_group<2>(av[e], xr[f4 + e], acc[2]);
-          acc[3] = mfma4_group<3>(av[e], xr[f4 + e], acc[3]);
-        }
-      }
+      for (int e = 0; e < 4; ++e) mfma4_groups<G>(av[e], xr[f4 + e], acc);
     }
     // largest acc (smallest score hn - c.x) over the KP rows, NaNs passed
     // over, -inf when there is none; then its lowest centroid
@@ -945,12 +949,19 @@ inline int smallk_db(int64_t d) {
 }
 
 // Groups of 4 centroids the kernel computes (its template G): 1, 2 or 4.
-inline int smallk_groups(int64_t k) { return k <= 4 ? 1 : k <= 8 ? 2 : 4; }
+inline int smallk_groups(int64_t k) { return k <= 4 ? 1 : k <= 8 ? 2 : k <= 16 ? 4 : 8; }
+
+// The kernel's threads per block (smallk_threads<DB, G>).
+inline int smallk_threads_of(int64_t d, int64_t k) {
+  const int db = smallk_db(d), g = smallk_groups(k);
+  if (db == 32 && g == 8) return smallk_threads<32, 8>();
+  return db <= 32 ? smallk_threads<16, 1>() : smallk_threads<64, 1>();
+}
 
 inline size_t smallk_lds(int64_t d, int64_t k, bool update, int copies) {
   const int db = smallk_db(d);
   const int kp = 4 * smallk_groups(k);   // the kernel's KP rows
-  const int threads = db <= 32 ? smallk_threads<32>() : smallk_threads<64>();
+  const int threads = smallk_threads_of(d, k);
   size_t b = sizeof(float) * ((size_t)kp * db + kp + db + 4);
   if (update) b += sizeof(u64) * (size_t)k * (db + 1) * copies;
   return b + (size_t)(threads / kWave) * kLabelStage * kWave;   // label stage
@@ -959,11 +970,14 @@ inline size_t smallk_lds(int64_t d, int64_t k, bool update, int copies) {
 // The kernel's copies of the sums (smallk_copies<DB, G>).
 inline int smallk_copies_of(int64_t d, int64_t k) {
   const int g = smallk_groups(k);
-  return smallk_db(d) <= 32 ? (g == 4 ? smallk_copies<32, 4>() : smallk_copies<32, 2>())
-                            : (g == 4 ? smallk_copies<64, 4>() : smallk_copies<64, 2>());
+  if (smallk_db(d) <= 32)
+    return g == 8 ? smallk_copies<32, 8>() : g == 4 ? smallk_copies<32, 4>()
+                                                    : smallk_copies<32, 2>();
+  return g == 8 ? smallk_copies<64, 8>() : g == 4 ? smallk_copies<64, 4>()
+                                                  : smallk_copies<64, 2>();
 }
 
-// KMD_NO_SMALLK=1 sends k <= 16 to the 32-row tile kernel (the two compute
+// KMD_NO_SMALLK=1 sends k <= 32 to the 32-row tile kernel (the two compute
 // the same labels and sums; the switch exists to compare them).
 inline bool smallk_enabled() {
   const char *e = std::getenv("KMD_NO_SMALLK");
@@ -978,7 +992,8 @@ KernFn pick_smallk(int64_t d, int64_t k) {
   case DB:                                                     \
     return g == 1   ? &smallk_lloyd_krn<DB, 1, I, UPDATE>      \
            : g == 2 ? &smallk_lloyd_krn<DB, 2, I, UPDATE>      \
-                    : &smallk_lloyd_krn<DB, 4, I, UPDATE>;
+           : g == 4 ? &smallk_lloyd_krn<DB, 4, I, UPDATE>      \
+                    : &smallk_lloyd_krn<DB, 8, I, UPDATE>;
   switch (db) {
     KMD_SK_CASE(8)
     KMD_SK_CASE(16)
@@ -997,7 +1012,7 @@ MfmaLaunch plan_smallk(int64_t n, int64_t d, int64_t k) {
   L.copies = UPDATE ? smallk_copies_of(d, k) : 1;
   L.lds = smallk_lds(d, k, UPDATE, L.copies);
   if (L.lds > kLdsPerCu) return L;
-  L.threads = smallk_db(d) <= 32 ? smallk_threads<32>() : smallk_threads<64>();
+  L.threads = smallk_threads_of(d, k);
   L.s = 1;
   L.fn = pick_smallk<I, UPDATE>(d, k);
   if (L.fn == nullptr) return L;

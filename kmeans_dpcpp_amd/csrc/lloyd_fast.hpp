@@ -68,7 +68,7 @@ bool fast_fused_supported(int64_t d, int64_t k);
 // that use store_flush can size one private copy per block).
 template <typename I>
 int fast_fused_blocks(int64_t n, int64_t d, int64_t k);
-// k <= 16 (float): the driver's fused step runs the one-sample-per-lane
+// k <= 32 (float): the driver's fused step runs the one-sample-per-lane
 // kernel (KMD_NO_SMALLK=1 turns it off)
 bool fast_smallk_serves(int64_t d, int64_t k);
 

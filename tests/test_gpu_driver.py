@@ -226,6 +226,10 @@ def test_relocation_with_next_iteration_enqueued_ahead(kd, torch_dev, monkeypatc
     w = np.ones(n, np.float32)
     if tiled:
         monkeypatch.setenv("KMD_FORCE_TILED", "1")
+    elif prec != "fp32":
+        # the LDS-resident mixed kernels (for k <= 32 every precision runs
+        # the FP32 few-centroid kernel otherwise; the fp32 case covers it)
+        monkeypatch.setenv("KMD_NO_SMALLK", "1")
     got = run_driver_ex(kd, Xt, w, init, 1e-7, 30, prec)
     assert got[4]["path"] == ("tiled-" if tiled else "") + (
         "fp32" + ("" if tiled else "-mfma") if prec == "fp32" else f"{prec}+rescore")
@@ -334,9 +338,11 @@ def assign_labels(kd, Xt, C, itype=torch.int32):
 @pytest.mark.parametrize("shape", [(100_003, 32, 8), (70_001, 7, 3), (50_000, 16, 16),
                                    (40_001, 33, 5), (30_000, 64, 13), (20_011, 1, 1),
                                    (9_999, 48, 9), (65, 2, 4), (3, 5, 2), (20_000, 32, 10),
-                                   (12_345, 40, 12), (8_000, 64, 11)])
+                                   (12_345, 40, 12), (8_000, 64, 11), (60_000, 32, 32),
+                                   (30_011, 24, 17), (20_000, 64, 28), (15_000, 9, 24),
+                                   (40_000, 16, 20)])
 def test_small_k_kernel_equals_tile_kernel(kd, torch_dev, monkeypatch, shape):
-    """k <= 16 runs on the kernel that gives each lane one sample and all its
+    """k <= 32 runs on the kernel that gives each lane one sample and all its
     centroids (v_mfma_f32_4x4x1f32); KMD_NO_SMALLK=1 runs the same fit on
     the 32-row tile kernel.  Both chains of FP32 MFMAs round like fmaf over
     ascending features from -hn, the argmin rule is the same and the sums
@@ -363,7 +369,7 @@ def test_small_k_kernel_equals_tile_kernel(kd, torch_dev, monkeypatch, shape):
 
 
 def test_small_k_runs_fp32_for_every_precision(kd, torch_dev, monkeypatch):
-    """With k <= 16 the step is bound by reading X: every precision runs the
+    """With k <= 32 the step is bound by reading X: every precision runs the
     FP32 few-centroid kernel (path "fp32-mfma").  With KMD_NO_SMALLK=1 the
     BF16, BF16x3 and FP8 kernels run instead, and every one of these fits
     equals the FP32 fit bit for bit."""
@@ -411,10 +417,10 @@ def test_small_k_labels_past_one_stage(kd, torch_dev, monkeypatch):
 def test_nan_centroids_are_passed_over(kd, torch_dev, monkeypatch, k, nan_cols, smallk):
     """Centroids with NaN coordinates score NaN for every sample; the
     reference's scan (strict <) never takes a NaN, so each sample goes to its
-    best finite centroid.  k = 20 puts the NaN centroids on every row that
-    one half of the 32-row tile holds (rows 0-3, 8-11, 16-19), the case in
-    which that half has no candidate at all; k = 8 does the same for the
-    few-centroid kernel and, with KMD_NO_SMALLK=1, for the tile kernel."""
+    best finite centroid.  On the 32-row tile kernel (KMD_NO_SMALLK=1) k = 20
+    puts the NaN centroids on every row that one half of the tile holds (rows
+    0-3, 8-11, 16-19), the case in which that half has no candidate at all;
+    both k run on the few-centroid kernel too."""
     monkeypatch.setenv("KMD_NO_SMALLK", "0" if smallk == "1" else "1")
     rng = np.random.default_rng(3)
     d, n = 6, 4096
