@@ -3,7 +3,8 @@ oracle (the C restatement of the reference's assignment, FP64 NumPy updates
 and inertia over the GPU's own labels) and against the single-fit driver:
 
 * plain Lloyd, n = 10,000,000, d = 32, k = 256 (blobs: centres in
-  [-20, 20], per-blob std in [0.5, 2]);
+  [-20, 20], per-blob std in [0.5, 2]), and the same data shape with k = 8
+  and 32;
 * n = 2,000,000, d = 128, k = 1024 anisotropic blobs, FP32 and the BF16
   cross term with exact re-score;
 * 512 independent fits in one batched call;
@@ -79,6 +80,32 @@ def test_c1_plain_fp32_full_size(kd, torch_dev):
     sums = to_np(cp).astype(np.float64).sum(axis=0)
     assert np.abs(sums - ref * cnt).max() <= 1e-5 * np.abs(ref * cnt).max()
     # inertia of the returned labels and centroids
+    assert abs(inr3 - fp64_inertia(Xt, w, C3n, L3)) <= 1e-5 * abs(inr3)
+
+
+@pytest.mark.parametrize("k", [8, 32])
+def test_few_centroids_full_size(kd, torch_dev, k):
+    """n = 10,000,000, d = 32 with k = 8 and 32 (the few-centroid kernel of
+    the HBM-bound regime), pinned like the k = 256 fit above; a BF16 request
+    runs the same kernel and returns the same bits."""
+    n, d = 10_000_000, 32
+    X_d, init_d = bench.make_data(n, d, k, bench.SEED, torch_dev)
+    w_d = torch.ones(n, dtype=torch.float32, device=torch_dev)
+    lab2, C2, it2, _, _ = drive(kd, X_d, w_d, init_d, 2)
+    lab3, C3, it3, inr3, st = drive(kd, X_d, w_d, init_d, 3)
+    assert (it2, it3) == (2, 3) and st["path"] == "fp32-mfma"
+    labb, Cb, _, inrb, stb = drive(kd, X_d, w_d, init_d, 3, "bf16")
+    assert stb["path"] == "fp32-mfma"
+    assert torch.equal(labb, lab3) and torch.equal(Cb.view(torch.int32), C3.view(torch.int32))
+    assert inrb == inr3
+    Xt = to_np(X_d)
+    w = np.ones(n)
+    C2n, C3n, L2, L3 = to_np(C2), to_np(C3), to_np(lab2), to_np(lab3)
+    m2 = check_label_mismatches(Xt, C2n, L2, co.assign(Xt, C2n), limit=n // 10_000)
+    m3 = check_label_mismatches(Xt, C3n, L3, co.assign(Xt, C3n), limit=n // 10_000)
+    print(f"k={k} label mismatches vs restatement (near-ties): {m2}, {m3}")
+    ref, _ = fp64_update(Xt, w, L2, k)
+    assert rel_err(C3n, ref) <= 1e-5
     assert abs(inr3 - fp64_inertia(Xt, w, C3n, L3)) <= 1e-5 * abs(inr3)
 
 
