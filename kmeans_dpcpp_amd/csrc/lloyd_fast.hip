@@ -505,9 +505,10 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
 // Update.  The int64 fixed-point sums live in LDS as [label][feature][copy]
 // with the copy index (lane % R) fastest: in one ds_add_u64 the 64 lanes
 // add into words whose offsets differ by their copy index, whatever their
-// labels, so with R = 32 no two lanes of an instruction share an address
-// except lanes l and l + 32, and the atomics run at the LDS's full width
-// instead of serialising on the few labels.
+// labels.  With R = 32 (k <= 8 at d <= 32) no two lanes of an instruction
+// share an address except lanes l and l + 32, and the atomics run at the
+// LDS's full width instead of serialising on the few labels; larger k or d
+// leave room for fewer copies (R = 4-16), each shared by 64 / R lanes.
 constexpr int kSmallK = 32;
 
 // Per feature-row bucket DB and centroid groups G: threads per block,
