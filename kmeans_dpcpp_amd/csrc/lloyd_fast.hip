@@ -548,9 +548,31 @@ __device__ inline void mfma4_groups(float a, float x, f32x4 (&acc)[G]) {
   if constexpr (Gi + 1 < G) mfma4_groups<G, Gi + 1>(a, x, acc);
 }
 
-template <int DB, int G, typename I, bool UPDATE>
+// window_add for the few-centroid layout: the LDS word and the caller's
+// FP32 element of an out-of-window value are addressed separately.  `real`
+// is false for a padded feature row (x = 0, so v is 0, or NaN under an
+// infinite weight: never inside a window), whose value has no element in
+// the caller's copy and is dropped.
+__device__ inline void window_add_at(u64 *word, float *fb, int64_t fb_off,
+                                     float v, int lo, int W, bool real) {
+  const unsigned b = __float_as_uint(v);
+  const int sh = (int)((b >> 23) & 0xffu) - lo;
+  if ((unsigned)sh <= (unsigned)W) {
+    const u64 m = (u64)((b & 0x7fffffu) | 0x800000u) << sh;
+    atomicAdd(word, (b >> 31) ? (u64)0 - m : m);
+  } else if (v != 0.f && real) {
+    atomic_add(fb + fb_off, v);
+  }
+}
+
+// UPDATE && !WINDOW: the driver's step (fixed-point sums, or FP32 for
+// non-finite inputs; raw partials into the block's slot).  WINDOW: the
+// reference's fused step (exact binade-window sums, no pre-pass; totals and
+// out-of-window values added into the caller's private copies).
+template <int DB, int G, typename I, bool UPDATE, bool WINDOW = false>
 __global__ void __launch_bounds__((smallk_threads<DB, G>()))
     smallk_lloyd_krn(FastArgs a) {
+  static_assert(!WINDOW || UPDATE, "the window sums belong to the update");
   if (a.skip != nullptr && *a.skip != 0) return;
   constexpr int NT = smallk_threads<DB, G>();
   constexpr int R = smallk_copies<DB, G>();
@@ -572,7 +594,9 @@ __global__ void __launch_bounds__((smallk_threads<DB, G>()))
   const int n_sums = UPDATE ? k * (DB + 1) * R : 0;
   // [wave][kLabelStage][64] labels of the wave's staged super-blocks
   unsigned char *stage_all = reinterpret_cast<unsigned char *>(S64 + n_sums);
-  const bool fixed = UPDATE && a.nonfinite[0] == 0;
+  const bool fixed = UPDATE && !WINDOW && a.nonfinite[0] == 0;
+  int *Ew = reinterpret_cast<int *>(Sc);   // WINDOW: lowest binade per row
+  const int W = a.window;
 
   const int lane = tid & (kWave - 1);
   // uniform in the wave (the compiler cannot see that of tid / 64), so the
@@ -630,11 +654,30 @@ __global__ void __launch_bounds__((smallk_threads<DB, G>()))
   if (UPDATE) {
     // Sc[f] = the scale of feature f < d, 0 for padded rows; Sc[DB] = the
     // weights' scale
-    for (int f = tid; f < DB + 4; f += NT)
-      Sc[f] = !fixed ? 0.f : f < d ? a.scale[f] : f == DB ? a.scale[d] : 0.f;
+    for (int f = tid; f < DB + 4; f += NT) {
+      if constexpr (WINDOW)
+        Ew[f] = 0;
+      else
+        Sc[f] = !fixed ? 0.f : f < d ? a.scale[f] : f == DB ? a.scale[d] : 0.f;
+    }
     for (int i = tid; i < n_sums; i += NT) S64[i] = 0ull;
   }
   __syncthreads();
+  if constexpr (WINDOW) {
+    // the window of feature f ends kWinMargin binades above the largest
+    // centroid coordinate of f, that of the weights (slot DB) at 2^kWinTopW
+    const int64_t dk = (int64_t)d * k;
+    for (int64_t i = tid; i < dk; i += NT) {
+      const int e = (int)((__float_as_uint(a.C[i]) >> 23) & 0xffu);
+      if (e != 0xff) atomicMax(&Ew[i / k], e);
+    }
+    __syncthreads();
+    for (int f = tid; f <= DB; f += NT) {
+      const int top = f < d ? Ew[f] + kWinMargin : 127 + kWinTopW;
+      Ew[f] = min(max(top, W + 1), 254) - W;
+    }
+    __syncthreads();
+  }
 
   I *labels = reinterpret_cast<I *>(a.labels);
   unsigned char *stage = stage_all + wave * (kLabelStage * kWave);
@@ -647,9 +690,10 @@ __global__ void __launch_bounds__((smallk_threads<DB, G>()))
     for (int c = 0; c < KP; ++c) hs[c] = uniform(Hl[c]);
     if (UPDATE) {
 #pragma unroll
-      for (int f = 0; f <= DB; ++f) ss[f] = uniform(Sc[f]);
+      for (int f = 0; f <= DB; ++f) ss[f] = uniform(Sc[f]);   // WINDOW: Ew bits
     }
   }
+  const int64_t copy = UPDATE ? blockIdx.x % a.n_copies : 0;
 
 
   // slot: the super-block's place in the stage
@@ -690,7 +734,18 @@ __global__ void __launch_bounds__((smallk_threads<DB, G>()))
 
     if (UPDATE && valid) {
       const int base = lab * (DB + 1) * R + (lane & (R - 1));
-      if (fixed) {
+      if constexpr (WINDOW) {
+        u64 *Sp = S64 + base;
+        float *cg = reinterpret_cast<float *>(a.copies) + copy * (int64_t)d * k;
+#pragma unroll
+        for (int f = 0; f < DB; ++f) {
+          const int lo = kScalarConsts ? __float_as_int(ss[f]) : Ew[f];
+          window_add_at(Sp + f * R, cg, (int64_t)f * k + lab, xr[f] * ws, lo, W,
+                        f < d);
+        }
+        const int low = kScalarConsts ? __float_as_int(ss[DB]) : Ew[DB];
+        window_add_at(Sp + DB * R, a.sizes_copies + copy * k, lab, ws, low, W, true);
+      } else if (fixed) {
         u64 *Sp = S64 + base;
 #pragma unroll
         for (int f4 = 0; f4 < DB; f4 += 4) {
@@ -748,10 +803,29 @@ __global__ void __launch_bounds__((smallk_threads<DB, G>()))
 
   if (UPDATE) {
     // raw partials into this block's slot, in the [f][label] layout of the
-    // other fused-step kernels (f == d: the weight sums)
+    // other fused-step kernels (f == d: the weight sums); WINDOW: the
+    // totals into the caller's private copy
     __syncthreads();
     const int dk1 = (d + 1) * k;
-    if (fixed) {
+    if constexpr (WINDOW) {
+      float *cg = reinterpret_cast<float *>(a.copies) + copy * (int64_t)d * k;
+      float *sg = a.sizes_copies + copy * k;
+      for (int i = tid; i < dk1; i += NT) {
+        const int f = i / k, c = i % k;
+        const int fs = f < d ? f : DB;
+        const u64 *src = S64 + (c * (DB + 1) + fs) * R;
+        u64 t = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) t += src[r];
+        const float v = (float)ldexp((double)(long long)t, Ew[fs] - 150);
+        if (v != 0.f) {
+          if (f < d)
+            atomic_add(&cg[(int64_t)f * k + c], v);
+          else
+            atomic_add(&sg[c], v);
+        }
+      }
+    } else if (fixed) {
       u64 *cp = reinterpret_cast<u64 *>(a.copies) + (a.slot0 + blockIdx.x) * dk1;
       for (int i = tid; i < dk1; i += NT) {
         const int f = i / k, c = i % k;
@@ -931,6 +1005,7 @@ struct MfmaLaunch {
   int blocks = 0;
   size_t lds = 0;
   int copies = 1;  // copies of the LDS sums (UPDATE)
+  bool smallk = false;   // the few-centroid kernel (64-sample super-blocks)
 };
 
 template <typename I, bool UPDATE, bool STORE>
@@ -985,16 +1060,16 @@ inline bool smallk_enabled() {
   return !(e != nullptr && e[0] == '1' && e[1] == '\0');
 }
 
-template <typename I, bool UPDATE>
+template <typename I, bool UPDATE, bool WINDOW>
 KernFn pick_smallk(int64_t d, int64_t k) {
   const int g = smallk_groups(k);
   const int db = smallk_db(d);
-#define KMD_SK_CASE(DB)                                        \
-  case DB:                                                     \
-    return g == 1   ? &smallk_lloyd_krn<DB, 1, I, UPDATE>      \
-           : g == 2 ? &smallk_lloyd_krn<DB, 2, I, UPDATE>      \
-           : g == 4 ? &smallk_lloyd_krn<DB, 4, I, UPDATE>      \
-                    : &smallk_lloyd_krn<DB, 8, I, UPDATE>;
+#define KMD_SK_CASE(DB)                                                \
+  case DB:                                                             \
+    return g == 1   ? &smallk_lloyd_krn<DB, 1, I, UPDATE, WINDOW>      \
+           : g == 2 ? &smallk_lloyd_krn<DB, 2, I, UPDATE, WINDOW>      \
+           : g == 4 ? &smallk_lloyd_krn<DB, 4, I, UPDATE, WINDOW>      \
+                    : &smallk_lloyd_krn<DB, 8, I, UPDATE, WINDOW>;
   switch (db) {
     KMD_SK_CASE(8)
     KMD_SK_CASE(16)
@@ -1006,16 +1081,17 @@ KernFn pick_smallk(int64_t d, int64_t k) {
 #undef KMD_SK_CASE
 }
 
-template <typename I, bool UPDATE>
+template <typename I, bool UPDATE, bool WINDOW = false>
 MfmaLaunch plan_smallk(int64_t n, int64_t d, int64_t k) {
   MfmaLaunch L;
+  L.smallk = true;
   // the kernel's lane-keyed copies of the sums (fixed per bucket)
   L.copies = UPDATE ? smallk_copies_of(d, k) : 1;
   L.lds = smallk_lds(d, k, UPDATE, L.copies);
   if (L.lds > kLdsPerCu) return L;
   L.threads = smallk_threads_of(d, k);
   L.s = 1;
-  L.fn = pick_smallk<I, UPDATE>(d, k);
+  L.fn = pick_smallk<I, UPDATE, WINDOW>(d, k);
   if (L.fn == nullptr) return L;
   allow_full_lds(reinterpret_cast<const void *>(L.fn));
   int per_cu = 0;
@@ -1038,14 +1114,12 @@ template <typename I, bool UPDATE, bool STORE>
 MfmaLaunch plan_launch(int64_t n, int64_t d, int64_t k) {
   MfmaLaunch L;
   if (d < 1 || d > kMaxD || k < 1 || k > (1 << 20)) return L;
-  // the driver's step (store flush) and the assignment; the reference-API
-  // step (atomic flush, binade windows) stays on the tile kernel
-  if constexpr (!UPDATE || STORE) {
-    if (k <= kSmallK && smallk_enabled()) {
-      L = plan_smallk<I, UPDATE>(n, d, k);
-      if (L.fn != nullptr) return L;
-      L = MfmaLaunch{};
-    }
+  // few centroids: the driver's step (store flush), the assignment and the
+  // reference-API step (atomic flush, binade windows)
+  if (k <= kSmallK && smallk_enabled()) {
+    L = plan_smallk<I, UPDATE, UPDATE && !STORE>(n, d, k);
+    if (L.fn != nullptr) return L;
+    L = MfmaLaunch{};
   }
   L.copies = UPDATE ? sum_copies(d, k) : 1;
   L.lds = lds_bytes(d, k, UPDATE, L.copies);
@@ -1211,8 +1285,14 @@ void launch_fast_fused(int64_t n, int64_t d, int64_t k, int64_t n_copies,
     a.nonfinite = nullptr;
     // window width from the most samples one block adds into one sum:
     // 2^(24 + W) * count < 2^63
-    const int64_t per_block =
+    int64_t per_block =
         ceil_div<int64_t>(ceil_div<int64_t>(n, 32 * L.s), L.blocks) * 32 * L.s;
+    if (L.smallk) {
+      // waves of a block each take every (blocks x waves)-th super-block of 64
+      const int64_t waves = L.threads / kWave;
+      per_block = ceil_div<int64_t>(ceil_div<int64_t>(n, kWave), L.blocks * waves) *
+                  waves * kWave;
+    }
     int bits = 0;
     while (bits < 40 && (int64_t(1) << bits) < per_block) ++bits;
     a.window = std::max(1, std::min(30, 38 - bits));

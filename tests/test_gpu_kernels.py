@@ -492,6 +492,37 @@ def test_fused_step_values_outside_the_sum_window(kd, torch_dev):
     assert np.isinf(s2[4, lab2[9]]) or np.isnan(s2[4, lab2[9]])
 
 
+def test_fused_step_padded_rows_with_infinite_weight(kd, torch_dev):
+    """d = 5: the few-centroid kernel holds the features in rows padded to 8
+    (x = 0 there).  A sample of infinite weight makes its own cluster's sums
+    and size non-finite, as the reference's FP32 atomics would, and no other
+    cluster's: the padded rows' 0 x inf has no element of the caller's
+    copies to go to."""
+    n, d, k = 20_000, 5, 8
+    Xt, init = lo.gaussian_blobs(n, d, k, seed=13, dtype=np.float32)
+    w = np.ones(n, np.float32)
+    w[123] = np.inf
+    hn = co.half_norm(init)
+    n_copies = 4
+    ids = to_dev(np.empty(n, dtype=np.int32))
+    cp = to_dev(np.zeros((n_copies, d, k), np.float32))
+    scp = to_dev(np.zeros((n_copies, k), np.float32))
+    kd.fused_lloyd_single_step(to_dev(Xt), to_dev(w), to_dev(init), to_dev(hn), ids, cp, scp,
+                               8, 128, q())[0].wait()
+    lab = to_np(ids).astype(np.int64)
+    ref_ids = to_dev(np.empty(n, dtype=np.int32))
+    kd.assignment(to_dev(Xt), to_dev(init), to_dev(hn), ref_ids, 8, 128, q())
+    sync()
+    assert_array_equal(lab, to_np(ref_ids))
+    c = lab[123]
+    sums = to_np(cp).astype(np.float64).sum(axis=0)
+    sizes = to_np(scp).astype(np.float64).sum(axis=0)
+    others = np.arange(k) != c
+    assert np.isfinite(sums[:, others]).all() and np.isfinite(sizes[others]).all()
+    assert np.isinf(sizes[c])
+    assert (~np.isfinite(sums[:, c]) | (Xt[:, 123] == 0)).all()
+
+
 def test_compute_number_of_private_copies(kd, torch_dev):
     arr = torch.empty(1, device=torch_dev)
     l2 = torch.cuda.get_device_properties(0).L2_cache_size
