@@ -11,6 +11,7 @@ namespace kmd {
 namespace {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned long long u64x8 __attribute__((ext_vector_type(8)));
 
 // acc[q] = -hn[tile] + C[:, tile] . X[:, columns of block q] for one
 // 32-centroid tile: the chain starts from the negated half norms (the C
@@ -111,7 +112,11 @@ __device__ inline float tile_max(const f32x16 &acc) {
 // The same first minimum in (tile, row) order as tile_argmin, with less
 // vector work per tile: the tile's best (largest acc = smallest score) by a
 // max3 chain, and on a strict improvement the tile index and a copy of its
-// 16 accumulators; the row is resolved once per sample by best_row.
+// 16 accumulators; the row is resolved once per sample by best_row.  The
+// copy is eight v_mov_b64 under the lanes' improvement mask rather than 16
+// selects (the compiler splits a 64-bit copy back into 32-bit moves, hence
+// the asm); it runs after tile_max has read the same registers, so the
+// MFMA chain's results are complete by then.
 template <int S>
 __device__ inline void tile_track_best(int ct, const f32x16 (&acc)[S],
                                        float (&bestm)[S], int (&bestt)[S],
@@ -122,8 +127,17 @@ __device__ inline void tile_track_best(int ct, const f32x16 (&acc)[S],
     const bool upd = m > bestm[q];   // ties keep the earlier tile
     bestm[q] = upd ? m : bestm[q];
     bestt[q] = upd ? ct : bestt[q];
+    u64x8 b = __builtin_bit_cast(u64x8, besta[q]);
+    const u64x8 a = __builtin_bit_cast(u64x8, acc[q]);
+    if (upd) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) besta[q][r] = upd ? acc[q][r] : besta[q][r];
+      for (int i = 0; i < 8; ++i) {
+        unsigned long long v;
+        asm volatile("v_mov_b64 %0, %1" : "=v"(v) : "v"(a[i]));
+        b[i] = v;
+      }
+    }
+    besta[q] = __builtin_bit_cast(f32x16, b);
   }
 }
 
