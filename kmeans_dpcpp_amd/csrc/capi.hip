@@ -5,8 +5,6 @@
 // message kept in a thread-local buffer for kmd_last_error().
 #include <algorithm>
 #include <cmath>
-#include <cstdlib>
-#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -52,176 +50,6 @@ int64_t device_l2_bytes(int device) {
   KMD_HIP_CHECK(
       hipDeviceGetAttribute(&bytes, hipDeviceAttributeL2CacheSize, device));
   return bytes;
-}
-
-// ---- device block cache of the stream-ordered temporaries ---------------
-//
-// A fit allocates its workspace (a second label buffer, per-sample
-// distances, sum copies, queues) and releases it when it returns.  Handed
-// back to the runtime, those releases cost about half a millisecond of host
-// time per fit after its last kernel (hipFreeAsync of the large blocks) and
-// the next fit maps the memory again; kept here, the next temporary that
-// fits takes a block instead.  A released block carries an event recorded on
-// its stream after its last use: it is handed out again at once when that
-// event has completed, otherwise behind a hipStreamWaitEvent on the new
-// user's stream, so reuse is ordered across streams as well.  At most
-// KMD_CACHE_MIB MiB (default 2048; 0 turns the cache off) are held per
-// device: finished blocks are returned to the runtime, largest first, to
-// make room, and when an allocation fails the device's finished blocks are
-// returned and the allocation is tried once more.  Streams being captured
-// into a graph bypass the cache (plain hipMallocAsync / hipFreeAsync).
-namespace {
-
-struct CachedBlock {
-  void *ptr;
-  hipEvent_t done;  // recorded on the releasing stream after its last use
-};
-
-struct DeviceCache {
-  std::multimap<size_t, CachedBlock> blocks;  // by allocated size
-  size_t held = 0;
-};
-
-std::mutex g_cache_mu;
-std::vector<DeviceCache> g_cache;
-
-size_t cache_limit() {
-  static const size_t limit = [] {
-    const char *e = std::getenv("KMD_CACHE_MIB");
-    const long long mib = e ? std::atoll(e) : 2048;
-    return mib > 0 ? (size_t)mib << 20 : (size_t)0;
-  }();
-  return limit;
-}
-
-bool stream_capturing(hipStream_t s) {
-  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(s, &st) != hipSuccess) return true;
-  return st != hipStreamCaptureStatusNone;
-}
-
-// Allocated size of a request: whole 2 MiB pages from 1 MiB up, 4 KiB
-// granules below, so requests of nearly the same size share blocks.
-size_t block_size(size_t bytes) {
-  const size_t g = bytes >= ((size_t)1 << 20) ? ((size_t)2 << 20) : 4096;
-  return (bytes + g - 1) / g * g;
-}
-
-// Takes finished blocks of `c` out, largest first, until `need` more bytes
-// fit under the limit (or all of them when `all`); the caller returns them
-// to the runtime.  Called with g_cache_mu held.
-void take_finished(DeviceCache &c, size_t need, bool all,
-                   std::vector<CachedBlock> &out) {
-  for (auto it = c.blocks.end(); it != c.blocks.begin();) {
-    if (!all && c.held + need <= cache_limit()) break;
-    --it;
-    if (hipEventQuery(it->second.done) == hipSuccess) {
-      out.push_back(it->second);
-      c.held -= it->first;
-      it = c.blocks.erase(it);
-    }
-  }
-}
-
-void return_blocks(const std::vector<CachedBlock> &out, hipStream_t s) {
-  for (const CachedBlock &b : out) {
-    (void)hipEventDestroy(b.done);
-    (void)hipFreeAsync(b.ptr, s);
-  }
-}
-
-}  // namespace
-
-void *cache_alloc(size_t bytes, hipStream_t s, size_t *held) {
-  *held = 0;
-  void *p = nullptr;
-  if (cache_limit() == 0 || stream_capturing(s)) {
-    KMD_HIP_CHECK(hipMallocAsync(&p, bytes, s));
-    return p;
-  }
-  int dev = 0;
-  KMD_HIP_CHECK(hipStreamGetDevice(s, &dev));
-  const size_t want = block_size(bytes);
-  bool found = false;
-  size_t got = 0;
-  CachedBlock b{};
-  {
-    std::lock_guard<std::mutex> lk(g_cache_mu);
-    if ((int)g_cache.size() <= dev) g_cache.resize(dev + 1);
-    DeviceCache &c = g_cache[dev];
-    // the smallest cached block that fits, unless it is over twice the size
-    auto it = c.blocks.lower_bound(want);
-    if (it != c.blocks.end() && it->first <= 2 * want) {
-      got = it->first;
-      b = it->second;
-      c.held -= got;
-      c.blocks.erase(it);
-      found = true;
-    }
-  }
-  if (found) {
-    const hipError_t q = hipEventQuery(b.done);
-    hipError_t e = hipSuccess;
-    if (q == hipErrorNotReady)
-      e = hipStreamWaitEvent(s, b.done, 0);
-    else if (q != hipSuccess)
-      e = q;
-    (void)hipEventDestroy(b.done);
-    if (e != hipSuccess) {
-      (void)hipFreeAsync(b.ptr, s);
-      KMD_HIP_CHECK(e);
-    }
-    *held = got;
-    return b.ptr;
-  }
-  hipError_t e = hipMallocAsync(&p, want, s);
-  if (e == hipErrorOutOfMemory) {
-    (void)hipGetLastError();
-    std::vector<CachedBlock> out;
-    {
-      std::lock_guard<std::mutex> lk(g_cache_mu);
-      take_finished(g_cache[dev], 0, true, out);
-    }
-    return_blocks(out, s);
-    KMD_HIP_CHECK(hipStreamSynchronize(s));
-    e = hipMallocAsync(&p, want, s);
-  }
-  KMD_HIP_CHECK(e);
-  *held = want;
-  return p;
-}
-
-void cache_release(void *p, size_t held, hipStream_t s) noexcept {
-  if (!p) return;
-  int dev = 0;
-  hipEvent_t ev = nullptr;
-  if (held == 0 || cache_limit() == 0 || held > cache_limit() ||
-      stream_capturing(s) || hipStreamGetDevice(s, &dev) != hipSuccess ||
-      hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventRecord(ev, s) != hipSuccess) {
-    if (ev) (void)hipEventDestroy(ev);
-    (void)hipFreeAsync(p, s);
-    return;
-  }
-  bool kept = false;
-  std::vector<CachedBlock> out;
-  try {
-    std::lock_guard<std::mutex> lk(g_cache_mu);
-    if ((int)g_cache.size() <= dev) g_cache.resize(dev + 1);
-    DeviceCache &c = g_cache[dev];
-    if (c.held + held > cache_limit()) take_finished(c, held, false, out);
-    if (c.held + held <= cache_limit()) {
-      c.blocks.emplace(held, CachedBlock{p, ev});
-      c.held += held;
-      kept = true;
-    }
-  } catch (...) {
-  }
-  return_blocks(out, s);
-  if (!kept) {
-    (void)hipEventDestroy(ev);
-    (void)hipFreeAsync(p, s);
-  }
 }
 
 template <typename T, typename I>

@@ -75,45 +75,32 @@ inline int apply_blocks_cap(int per_cu) {
   return (cap > 0 && per_cu > cap) ? cap : per_cu;
 }
 
-// Stream-ordered device blocks kept for reuse (capi.hip).  cache_alloc
-// returns a block of at least `bytes` for work enqueued on `s` after the
-// call, and its allocated size in *held (0: not from the cache: a plain
-// hipMallocAsync block, as while `s` is being captured into a graph).
-// cache_release takes it back once the work enqueued on `s` before the call
-// is done with it; it never throws.
-void *cache_alloc(size_t bytes, hipStream_t s, size_t *held);
-void cache_release(void *p, size_t held, hipStream_t s) noexcept;
-
-// Stream-ordered temporary device buffer.  Released in stream order when
-// the object is destroyed, so work enqueued on the same stream before
-// destruction may still use it; the block goes back to the device cache
-// for the next temporary rather than to the runtime.
+// Stream-ordered temporary device buffer (hipMallocAsync / hipFreeAsync).
+// Freed in stream order when the object is destroyed, so work enqueued on the
+// same stream before destruction may still use it.
 class TempBuffer {
  public:
   TempBuffer(size_t bytes, hipStream_t s) : stream_(s), bytes_(bytes) {
-    if (bytes_ > 0) ptr_ = cache_alloc(bytes_, stream_, &held_);
+    if (bytes_ > 0) KMD_HIP_CHECK(hipMallocAsync(&ptr_, bytes_, stream_));
   }
   ~TempBuffer() {
-    if (ptr_) cache_release(ptr_, held_, stream_);
+    if (ptr_) (void)hipFreeAsync(ptr_, stream_);
   }
   TempBuffer(const TempBuffer &) = delete;
   TempBuffer &operator=(const TempBuffer &) = delete;
   TempBuffer(TempBuffer &&o) noexcept
-      : stream_(o.stream_), bytes_(o.bytes_), held_(o.held_), ptr_(o.ptr_) {
+      : stream_(o.stream_), bytes_(o.bytes_), ptr_(o.ptr_) {
     o.ptr_ = nullptr;
     o.bytes_ = 0;
-    o.held_ = 0;
   }
   TempBuffer &operator=(TempBuffer &&o) noexcept {
     if (this != &o) {
-      if (ptr_) cache_release(ptr_, held_, stream_);
+      if (ptr_) (void)hipFreeAsync(ptr_, stream_);
       stream_ = o.stream_;
       bytes_ = o.bytes_;
-      held_ = o.held_;
       ptr_ = o.ptr_;
       o.ptr_ = nullptr;
       o.bytes_ = 0;
-      o.held_ = 0;
     }
     return *this;
   }
@@ -126,7 +113,6 @@ class TempBuffer {
  private:
   hipStream_t stream_;
   size_t bytes_;
-  size_t held_ = 0;  // size of the block (0: a plain stream-ordered block)
   void *ptr_ = nullptr;
 };
 

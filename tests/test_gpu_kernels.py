@@ -174,30 +174,6 @@ def test_merge_sort_nan_last(kd, torch_dev):
     assert_array_equal(got[:-3], np.sort(x[~np.isnan(x)]))
 
 
-def test_cached_temporaries_reused_across_streams(kd, torch_dev):
-    """The sort's n-element temporary is released when the call returns,
-    while its merge passes are still queued; the next call, on another
-    stream, takes the same block from the library's device block cache and
-    must wait for them.  Every output equals its own stable sort."""
-    n = 4_000_003
-    rng = np.random.default_rng(7)
-    xs_h = [rng.standard_normal(n).astype(np.float32) for _ in range(4)]
-    exp = [np.sort(x, kind="stable") for x in xs_h]
-    xs = [to_dev(x) for x in xs_h]
-    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
-    outs = [torch.empty_like(x) for x in xs]
-    sync()
-    for rep in range(3):
-        for i, x in enumerate(xs):
-            s = streams[i % 2]
-            with torch.cuda.stream(s):
-                outs[i].fill_(float("nan"))
-                kd.iterative_merge_sort(x, outs[i], s)
-        sync()
-        for i in range(len(xs)):
-            assert_array_equal(to_np(outs[i]), exp[i], err_msg=f"rep {rep} call {i}")
-
-
 @pytest.mark.parametrize("dt", FDT)
 @pytest.mark.parametrize("it", IDT)
 def test_select_samples_far_from_centroid(kd, golden, torch_dev, dt, it):
