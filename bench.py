@@ -191,6 +191,33 @@ def fp64_errors(X_t, w, labels_prev, centroids, labels, inertia):
             "empty_before_last_update": int((~ok).sum())}
 
 
+def sampled_fp64_errors(fits, stream):
+    """fp64_errors of finished fits, given as (X_t, w, init, n_iter, centroids,
+    labels, inertia): the labels of the iteration before the last come from
+    the plain FP32 driver rerun for n_iter - 1 iterations from the same init
+    (batched, served and mixed-precision fits equal the plain driver's bit for
+    bit).  Returns the largest errors over the fits and how many were
+    checked."""
+    import torch
+    import kmeans_dpcpp_amd as kd
+    worst_c, worst_i, empty, checked = 0.0, 0.0, 0, 0
+    for X_t, w, init, n_iter, centroids, labels, inertia in fits:
+        if n_iter < 1:
+            continue
+        lab_prev = torch.empty_like(labels)
+        c_a, c_b = init.clone(), torch.empty_like(init)
+        kd.kmeans_lloyd_driver_ex(X_t, w, c_a, lab_prev, c_b, 0.0, False, int(n_iter) - 1,
+                                  "fp32", stream)
+        torch.cuda.synchronize()
+        e = fp64_errors(X_t, w, lab_prev, centroids, labels, inertia)
+        worst_c = max(worst_c, e["centroid_rel_err"])
+        worst_i = max(worst_i, e["inertia_rel_err"])
+        empty += e["empty_before_last_update"]
+        checked += 1
+    return {"fits_checked": checked, "centroid_rel_err_max": worst_c,
+            "inertia_rel_err_max": worst_i, "empty_before_last_update": empty}
+
+
 def bench_c2(args):
     """Compute-bound single fit: the mixed-precision driver against plain FP32
     on the same data, with an FP64 check of the last centroid update."""
@@ -355,6 +382,10 @@ def run_c3(n_fits, precision):
                            "label_mismatch": int((lab != bf[3]).sum()),
                            "max_centroid_diff": float((out - bf[4]).abs().max())})
     same = not differ
+    sample = range(0, len(fits), max(1, len(fits) // 8))
+    err = sampled_fp64_errors(
+        [(data[i][0], data[i][1], data[i][2], int(res[i][0]), fits[i][4], fits[i][3],
+          float(res[i][1])) for i in sample], stream)
     return {
         "fits_per_s": round(len(fits) / el, 3),
         "ms": round(1e3 * el, 3),
@@ -371,6 +402,7 @@ def run_c3(n_fits, precision):
         "identical_to_single_fit": bool(same),
         "fits_differing": differ[:5],
         "n_fits_differing": len(differ),
+        "error_vs_fp64": err,
     }
 
 
@@ -478,6 +510,12 @@ def run_c4(n_requests, rate, large_precision, small_precision):
             torch.equal(res.view(torch.int32), r.centroids.view(torch.int32)))
     el_plain = time.perf_counter() - t0
     plain_lat.sort()
+    small = [i for i, r in enumerate(out) if r.batched]
+    pick = [i for i, r in enumerate(out) if not r.batched] + \
+        small[::max(1, len(small) // 8)][:8]
+    err = sampled_fp64_errors(
+        [(reqs[i][1], reqs[i][2], reqs[i][3], out[i].n_iter, out[i].centroids,
+          out[i].labels, out[i].inertia) for i in pick], stream)
 
     def pct(v, p):
         return v[min(len(v) - 1, int(round(p / 100 * (len(v) - 1))))]
@@ -501,6 +539,7 @@ def run_c4(n_requests, rate, large_precision, small_precision):
         "engine_ticks": srv.ticks,
         "peak_resident_gb": round(srv.peak_resident / 1e9, 2),
         "identical_to_plain": bool(same),
+        "error_vs_fp64": err,
     }
 
 
