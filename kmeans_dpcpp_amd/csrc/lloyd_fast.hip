@@ -530,6 +530,8 @@ constexpr int smallk_copies() {
 // a 0.25 ms step), so a wave writes its labels in one burst after its last
 // load wherever n allows.
 constexpr int kLabelStage = 64;
+// cache-policy operand of the sample loads: non-temporal (read once per step)
+constexpr int kLoadNT = 2;
 // Word 3 of a raw buffer descriptor on gfx9-family devices (untyped dword
 // access, no swizzle).
 constexpr int kRsrcWord3 = 0x00020000;
@@ -626,14 +628,14 @@ __global__ void __launch_bounds__((smallk_threads<DB, G>()))
       const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
           const_cast<float *>(p), (short)0, f < dd ? bytes : 0, kRsrcWord3);
       x[f] = __uint_as_float(
-          __builtin_amdgcn_raw_buffer_load_b32(r, lane * 4, 0, 0));
+          __builtin_amdgcn_raw_buffer_load_b32(r, lane * 4, 0, kLoadNT));
       if (f + 1 < dd) p += nn;
     }
     if (UPDATE) {
       const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
           const_cast<float *>(a.w + s0), (short)0, bytes, kRsrcWord3);
       wv = __uint_as_float(
-          __builtin_amdgcn_raw_buffer_load_b32(r, lane * 4, 0, 0));
+          __builtin_amdgcn_raw_buffer_load_b32(r, lane * 4, 0, kLoadNT));
     }
   };
 
@@ -778,7 +780,7 @@ __global__ void __launch_bounds__((smallk_threads<DB, G>()))
   auto flush = [&](int64_t first, int cnt) {
     for (int j = 0; j < cnt; ++j) {
       const int64_t s = (first + j * sb_stride) * kWave + lane;
-      if (s < n) labels[s] = (I)stage[j * kWave + lane];
+      if (s < n) __builtin_nontemporal_store((I)stage[j * kWave + lane], labels + s);
     }
   };
   int64_t staged_first = sb;
