@@ -347,6 +347,10 @@ __device__ inline double kpp_dist(double xsq, double csq, double xc) {
 }
 
 // One center, scoring pass (pure streaming; the pick is the next launch).
+// cache-policy operand of the LDS-DMA row loads: non-temporal (X is read
+// once per center and is larger than the caches; D^2 is what is reused)
+constexpr int kLoadNT = 2;
+
 // Row 0 of the staged points is center c - 1 (the first center when
 // c == 0): D^2 <- min(D^2, dist0) is written to d2[c & 1].  Rows 1..T are
 // the candidates of center c: m_t = min(D^2, dist_t) gives the segment and
@@ -440,7 +444,8 @@ __global__ void __launch_bounds__(kSNT, 4) kpp_score_krn(KppArgs a, int c, int64
             const int f = (st * kStage + r < fcn) ? st * kStage + r : fcn - 1;
             __builtin_amdgcn_global_load_lds(
                 (const void *)(gbase + (int64_t)f * n),
-                (__attribute__((address_space(3))) void *)(buf + r * kSeg), 16, 0, 0);
+                (__attribute__((address_space(3))) void *)(buf + r * kSeg), 16, 0,
+                kLoadNT);
           }
         };
         issue(0);
