@@ -78,17 +78,19 @@ __device__ inline void load_x(const float *__restrict__ X, int64_t n, int d,
       const float *row = X + (int64_t)f * n + s_first;
       if (whole) {
         if constexpr (S == 2) {
-          const float2 v = *reinterpret_cast<const float2 *>(row);
+          typedef float f32x2 __attribute__((ext_vector_type(2)));
+          const f32x2 v =
+              __builtin_nontemporal_load(reinterpret_cast<const f32x2 *>(row));
           xr[ks][0] = v.x;
           xr[ks][1] = v.y;
         } else {
 #pragma unroll
-          for (int q = 0; q < S; ++q) xr[ks][q] = row[q];
+          for (int q = 0; q < S; ++q) xr[ks][q] = __builtin_nontemporal_load(row + q);
         }
       } else {
 #pragma unroll
         for (int q = 0; q < S; ++q)
-          xr[ks][q] = (s_first + q < n) ? row[q] : 0.f;
+          xr[ks][q] = (s_first + q < n) ? __builtin_nontemporal_load(row + q) : 0.f;
       }
     } else {
 #pragma unroll
@@ -261,7 +263,8 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
     const int64_t pf = b * kSuper + (int64_t)S * col;
     load_x<KS, S, FULL>(X, n, d, pf, half, vec, xn);
 #pragma unroll
-    for (int q = 0; q < S; ++q) wn[q] = (UPDATE && pf + q < n) ? w[pf + q] : 0.f;
+    for (int q = 0; q < S; ++q)
+      wn[q] = (UPDATE && pf + q < n) ? __builtin_nontemporal_load(w + pf + q) : 0.f;
   };
   if (!GATHER && (int64_t)blockIdx.x * kWavesPerBlock + wave < n_super)
     prefetch((int64_t)blockIdx.x * kWavesPerBlock + wave);
@@ -378,7 +381,7 @@ __global__ void __launch_bounds__(NT) mfma_lloyd_krn(FastArgs a) {
     if (half == 0) {
 #pragma unroll
       for (int q = 0; q < S; ++q)
-        if (valid[q]) labels[sidx[q]] = (I)besti[q];
+        if (valid[q]) __builtin_nontemporal_store((I)besti[q], labels + sidx[q]);
     }
     if (UPDATE) {
 #pragma unroll
